@@ -1,0 +1,182 @@
+"""Benchmark of the fused D3Q27 periodic-shift step on one MI355X.
+
+Workload (the sphere configuration): 256 x 128 x 128 channel, icosphere of
+diameter 32 cells (5120 triangles) voxelized through the STL path, velocity
+inlet U = 0.04 (moment-based), pressure outlet, periodic y (copies) and z
+(implicit in the shift), Re = 1000 (nu = 1.28e-3), Smagorinsky Cs = 0.17.
+A step is one updateGrid pass: applyStreaming, applyBCPeriodicY,
+applyLocalCellUpdate (bulk + boundary kernels).  Inputs are resident in HBM
+before timing starts.
+
+value = MLUPS of the FP32-storage step (cellCount * steps / wall time between
+device synchronisations).  FP16 / BF16 deviation-storage variants are
+reported under "variants".  roofline.achieved = algorithmic bytes of the bulk
+kernel per launch (216 B FP32 / 108 B 16-bit per updated cell x bulk cells)
+divided by its mean duration from HIP events on its stream.
+
+cpu_baseline: the project's FP32 CPU oracle (OpenMP) on the same workload for
+a bounded number of steps on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "MLUPS + % HBM roofline, D3Q27 fused step: plain FP32 vs FP16-store/batched/MLP"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_UPDATE = {"f32": 216, "f16": 108, "bf16": 108}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_sphere_grid(P, cases, storage: str, shape=(256, 128, 128), u_inlet=0.04, reynolds=1000.0, cs=0.17):
+    nx, ny, nz = shape
+    nu = cases.sphere_nu(u_inlet, cases.SPHERE_DIAMETER, reynolds)
+    info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=nu, res=1.0)
+    tri = cases.icosphere(4, cases.SPHERE_DIAMETER / 2, cases.SPHERE_CENTER)
+    stl = P.STLStruct.from_triangles(tri)
+    mask = P.BoolArray(nx * ny * nz)
+    P.applyMarkersInsideSTL(mask, stl, True, info)
+    solid = mask.get().astype(bool)
+    flags = cases.channel_flags(shape, solid)
+    g = P.GridStruct(info, storage)
+    g.set_flags(flags)
+    g.set_given_values(cases.channel_given_values(u_inlet))
+    g.set_smagorinsky(cs)
+    g.set_periodic_copies(y=True)
+    P.fill_equilibrium(g)
+    return g, flags, int(solid.sum()), nu
+
+
+def time_steps(P, g, steps: int, warmup: int):
+    P.updateGrid([g], warmup)
+    P.synchronize()
+    t0 = time.perf_counter()
+    P.updateGrid([g], steps)
+    P.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_time(P, g, steps: int):
+    g.profile(True)
+    P.updateGrid([g], steps)
+    total_ms, launches = g.profile_read()
+    g.profile(False)
+    return total_ms / max(1, launches), launches
+
+
+def cpu_baseline(cases, flags, nu, cs, shape, budget_s: float):
+    import oracle as O
+
+    nx, ny, nz = shape
+    og = O.Grid(nx, ny, nz, dtype=np.float32, nu=nu, cs=cs, flags=flags,
+                bcvals=cases.channel_given_values(0.04))
+    og.fill_equilibrium()
+    og.step(periodic_y=True)  # warm caches / threads
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        og.step(periodic_y=True)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nx * ny * nz * steps / el / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"{steps} FP32 oracle steps of the same 256x128x128 sphere workload ({el:.1f} s, OpenMP)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--storages", default="f32,f16,bf16")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "multi-GPU sharding is not part of it.", file=sys.stderr)
+        sys.exit(2)
+
+    import periodicshiftlbm_amd as P
+    from periodicshiftlbm_amd import cases
+
+    P.require_device()
+    shape = (256, 128, 128)
+    ncell = int(np.prod(shape))
+    results = {}
+    flags_keep = None
+    nu_keep = None
+    for storage in args.storages.split(","):
+        log(f"[bench] storage={storage} shape={shape}")
+        g, flags, nsolid, nu = build_sphere_grid(P, cases, storage, shape)
+        flags_keep, nu_keep = flags, nu
+        el = time_steps(P, g, args.steps, args.warmup)
+        mlups = ncell * args.steps / el / 1e6
+        bulk, nbc, skipped = g.census()
+        ksteps = min(args.steps, 200)
+        avg_ms, launches = kernel_time(P, g, ksteps)
+        bytes_per_launch = BYTES_PER_UPDATE[storage] * bulk
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        lo, hi = cases.sphere_box()
+        fx = g.momentum_exchange(lo, hi)[0]
+        results[storage] = {
+            "mlups": mlups, "ms_per_step": el * 1e3 / args.steps, "bulk_kernel_ms": avg_ms,
+            "bulk_cells": bulk, "boundary_cells": nbc, "solid_cells": nsolid,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+            "drag_force_x_after_run": fx,
+        }
+        log(f"  {mlups:.1f} MLUPS, bulk kernel {avg_ms * 1e3:.1f} us ({achieved:.0f} GB/s, "
+            f"{achieved / HBM_PEAK_GBS * 100:.1f}% of HBM peak)")
+        g.close()
+
+    head = results.get("f32") or next(iter(results.values()))
+    head_storage = "f32" if "f32" in results else next(iter(results))
+    out = {
+        "metric": METRIC,
+        "value": head["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head_storage,
+        "data": "synthetic",
+        "config": {"workload": "sphere channel 256x128x128, icosphere D=32 (5120 triangles) voxelized by the STL "
+                               "path, velocity inlet U=0.04, pressure outlet, periodic y/z, Re=1000, Cs=0.17; "
+                               "step = streaming + periodic-y copy + fused bulk + boundary kernels",
+                   "cells": ncell, "storage": head_storage},
+        "roofline": dict(head["roofline"], kernel="k_ps_bulk", bytes_per_cell=BYTES_PER_UPDATE[head_storage],
+                         cells_per_launch=head["bulk_cells"], avg_launch_ms=head["bulk_kernel_ms"]),
+        "variants": {k: {"mlups": v["mlups"], "ms_per_step": v["ms_per_step"], "roofline_frac": v["roofline"]["frac"],
+                         "achieved_GBs": v["roofline"]["achieved"], "bulk_kernel_ms": v["bulk_kernel_ms"]}
+                     for k, v in results.items()},
+    }
+    if "f32" in results and "f16" in results:
+        out["variants"]["f16_over_f32"] = results["f16"]["mlups"] / results["f32"]["mlups"]
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s)")
+        out["cpu_baseline"] = cpu_baseline(cases, flags_keep, nu_keep, 0.17, shape, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

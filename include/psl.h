@@ -1,0 +1,186 @@
+/*
+ * psl.h -- C ABI of the MI355X-native PeriodicShiftLBM D3Q27 step.
+ *
+ * Plain C: opaque handles, plain pointers and sizes.  Every function returns
+ * 0 on success or a negative PSL_E* code; psl_last_error() returns a
+ * thread-local message for the last failure.  Device memory is owned by the
+ * library; host arrays passed in are only read (or written) during the call.
+ * Calls on one grid are ordered on that grid's HIP stream; only functions that
+ * return data to the host synchronise it.  One grid must not be driven from
+ * two threads at once; distinct grids are independent.
+ *
+ * Each entry point names the reference interface it replaces
+ * (stepanmuller/PeriodicShiftLBM, paths relative to the repository root).
+ */
+#ifndef PSL_H
+#define PSL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSL_ABI_VERSION 1
+
+enum {
+  PSL_OK = 0,
+  PSL_E_ARG = -1,       /* bad argument / size */
+  PSL_E_HIP = -2,       /* HIP runtime error */
+  PSL_E_STATE = -3,     /* call not valid in the current state */
+  PSL_E_IO = -4,        /* file could not be read */
+  PSL_E_NOMEM = -5,     /* allocation failed */
+  PSL_E_GEOMETRY = -6   /* mesh / interface check failed */
+};
+
+/* storage of the distributions */
+enum {
+  PSL_F32 = 0,      /* f_q in FP32 (216 B per cell update) */
+  PSL_F16DEV = 1,   /* f_q - w_q in FP16, collided in FP32 (108 B) */
+  PSL_BF16DEV = 2   /* f_q - w_q in BF16, collided in FP32 (108 B) */
+};
+
+/* cell flag byte: kind in bits 0-2, periodic x/y/z in bits 3-5 (that outer
+ * normal component is ignored), given-value slot in bits 6-7.
+ * Replaces MarkerStruct (include/types.h:79-80) and the getMarkers callback. */
+enum {
+  PSL_FLUID = 0,
+  PSL_BOUNCEBACK = 1,
+  PSL_GIVEN_RHO = 2,   /* givenRho: restoreUxUyUz + applyMBBC */
+  PSL_GIVEN_U = 3,     /* givenUxUyUz: restoreRho + applyMBBC */
+  PSL_GIVEN_NONE = 4,  /* neither: restoreRhoUxUyUz + applyMBBC */
+  PSL_GIVEN_RHOU = 5,  /* both: applyMBBC */
+  PSL_GHOST = 6,       /* Marker.ghost: not updated */
+  PSL_COVERED = 7,     /* inside the finer grid's sub-box: not updated */
+  PSL_PERIODIC_X = 8,
+  PSL_PERIODIC_Y = 16,
+  PSL_PERIODIC_Z = 32
+};
+#define PSL_FLAG(kind, slot) ((uint8_t)((kind) | ((slot) << 6)))
+
+/* Mirrors InfoStruct (include/types.h:73-75). */
+typedef struct psl_info {
+  float gridID;
+  float res;
+  int cellCountX, cellCountY, cellCountZ, cellCount;
+  float ox, oy, oz;
+  float nu;
+  float dtPhys;
+  int iSubgridStart, jSubgridStart, kSubgridStart;
+  int iSubgridEnd, jSubgridEnd, kSubgridEnd;
+  int cellCountFineToCoarse, cellCountCoarseToFine;
+  float pRegulator, iRegulator;
+} psl_info;
+
+typedef struct psl_grid psl_grid;
+typedef struct psl_stl psl_stl;
+typedef struct psl_mask psl_mask;
+
+const char* psl_last_error(void);
+int psl_abi_version(void);
+/* HIP device count visible to the library (0 when none). */
+int psl_device_count(int* count);
+
+/* ---------------------------------------------------------------- grids */
+/* GridStruct allocation (include/types.h:77; flowAroundSphere/flowAroundSphereMainSingle.cu:180-181):
+ * fArray(27, cellCount) and shifter = 0. cellCount is recomputed from the counts. */
+int psl_grid_create(const psl_info* info, int storage, psl_grid** out);
+int psl_grid_destroy(psl_grid* grid);
+int psl_grid_get_info(const psl_grid* grid, psl_info* out);
+/* Sub-grid bounds and nu may change after creation (the driver writes Info fields
+ * between levels, flowAroundSphereMainSingle.cu:199-204). */
+int psl_grid_set_info(psl_grid* grid, const psl_info* info);
+int psl_grid_set_stream(psl_grid* grid, void* hip_stream);
+/* Per-cell flags (N bytes, host), or NULL for all fluid: the host-evaluated
+ * getMarkers / bouncebackMarkerArray (include/applyLocalCellUpdate.h:5-23). */
+int psl_grid_set_flags(psl_grid* grid, const uint8_t* host_flags);
+int psl_grid_get_flags(const psl_grid* grid, uint8_t* host_flags);
+/* getGivenRhoUxUyUz table: 4 slots x (rho, ux, uy, uz) (flowAroundSphereMainSingle.cu:69-77). */
+int psl_grid_set_bc_values(psl_grid* grid, const float* values16);
+/* getSmagorinskyConstant (flowAroundSphereMainSingle.cu:79-82): a constant, or a
+ * per-cell host array of N floats when per_cell != NULL. */
+int psl_grid_set_smagorinsky(psl_grid* grid, float cs, const float* per_cell);
+/* Bit 0: apply applyBCPeriodicX, bit 1: applyBCPeriodicY inside psl_update_grid. */
+int psl_grid_set_periodic_copies(psl_grid* grid, int mask);
+
+/* fillEquilibriumFromFunction (include/fillEquilibrium.h:1-22): f = feq(rho, u) from
+ * host arrays of N values; NULL arrays mean rho = 1 / u = 0. */
+int psl_fill_equilibrium(psl_grid* grid, const float* rho, const float* ux, const float* uy, const float* uz);
+
+/* applyStreaming(GridStruct&) (include/applyStreaming.h:11-31): shifter update only. */
+int psl_apply_streaming(psl_grid* grid);
+/* applyLocalCellUpdate(GridStruct&) (include/applyLocalCellUpdate.h:1-74). */
+int psl_apply_local_cell_update(psl_grid* grid);
+/* applyBCPeriodicX / applyBCPeriodicY (include/boundaryConditions/periodicXY.h:38-128). */
+int psl_apply_bc_periodic_x(psl_grid* grid);
+int psl_apply_bc_periodic_y(psl_grid* grid);
+
+/* applyCoarseFineGridCommunication (include/gridRefinementFunctions.h:433-526). */
+int psl_coarse_fine_communication(psl_grid* coarse, psl_grid* fine);
+/* fillCoarseGridFromFine (include/gridRefinementFunctions.h:528-536). */
+int psl_fill_coarse_from_fine(psl_grid* coarse, psl_grid* fine);
+/* updateGrid recursion (flowAroundSphere/flowAroundSphereMainSingle.cu:158-167),
+ * nsteps coarse steps over levels[0..nlevels-1]. */
+int psl_update_grid(psl_grid* const* levels, int nlevels, int nsteps);
+
+/* Shifter (include/types.h:77 GridStruct.shifter). */
+int psl_get_shifter(const psl_grid* grid, int* out27);
+int psl_set_shifter(psl_grid* grid, const int* in27);
+/* Distributions in logical order: host[q*N + cell] = f_q(cell) as FP32. */
+int psl_get_f(psl_grid* grid, float* host);
+int psl_set_f(psl_grid* grid, const float* host);
+/* rho, ux, uy, uz per cell (getRhoUxUyUz, include/cellFunctions.h:135-155); any may be NULL. */
+int psl_get_macroscopic(psl_grid* grid, float* rho, float* ux, float* uy, float* uz);
+/* Device pointer of fArray (storage element type), for callers that keep data resident. */
+int psl_grid_device_f(psl_grid* grid, void** dev_ptr, size_t* bytes);
+
+/* Momentum exchange over bounceback cells of the box [lo, hi): out = sum rho (u - u_bounced),
+ * accumulated in FP64 (getSphereDrag, flowAroundSphereMainSingle.cu:99-156; getLift,
+ * NACA0012/NACA0012Main.cu:96-133).  lo/hi may be NULL for the whole grid. */
+int psl_reduce_momentum_exchange(psl_grid* grid, const int* lo3, const int* hi3, double* out3);
+/* Total mass sum f over updated cells (not ghost / covered / bounceback), FP64. */
+int psl_reduce_mass(psl_grid* grid, double* out);
+
+/* Wait for all work queued on the device (hipDeviceSynchronize). */
+int psl_device_synchronize(void);
+/* Bulk-kernel timing: while enabled, every fused bulk update of this grid is
+ * bracketed by HIP events on the grid's stream; read returns the summed kernel
+ * time in ms and the launch count since enabling (this synchronises). */
+int psl_grid_profile_enable(psl_grid* grid, int enable);
+int psl_grid_profile_read(psl_grid* grid, double* total_ms, int* launches);
+/* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
+int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
+
+/* ------------------------------------------------------------- geometry */
+/* readSTL (include/STLFunctions.h:3-89). */
+int psl_stl_read(const char* path, psl_stl** out);
+/* STL from n triangles of 9 floats (ax ay az bx by bz cx cy cz). */
+int psl_stl_from_triangles(const float* tri9, int n, psl_stl** out);
+int psl_stl_destroy(psl_stl* stl);
+/* triangleCount and bounds xmin ymin zmin xmax ymax zmax. */
+int psl_stl_info(const psl_stl* stl, int* triangle_count, float* bounds6);
+int psl_stl_set_origin(psl_stl* stl, float ox, float oy, float oz);
+int psl_stl_get_triangles(const psl_stl* stl, float* tri9);
+/* checkSTLEdges (include/STLFunctions.h:92-182): triangles whose edges are not shared
+ * exactly twice; per-edge counts (3 per triangle) when counts != NULL. */
+int psl_stl_check_edges(const psl_stl* stl, int* bad_triangles, int* counts);
+
+/* BoolArrayType of cellCount entries on the device. */
+int psl_mask_create(int64_t n, int value, psl_mask** out);
+int psl_mask_destroy(psl_mask* m);
+int psl_mask_get(const psl_mask* m, uint8_t* host);
+int psl_mask_set(psl_mask* m, const uint8_t* host);
+/* multiplyBoolArrays / sumBoolArrays / invertBoolArray (include/STLFunctions.h:811-846). */
+int psl_mask_and(const psl_mask* a, const psl_mask* b, psl_mask* out);
+int psl_mask_or(const psl_mask* a, const psl_mask* b, psl_mask* out);
+int psl_mask_not(psl_mask* m);
+/* applyMarkersInsideSTL (include/STLFunctions.h:277-479). */
+int psl_markers_inside_stl(psl_mask* mask, const psl_stl* stl, int inside_value, const psl_info* info);
+/* Set the kind of every cell with mask != 0 to PSL_BOUNCEBACK (Grid.bouncebackMarkerArray). */
+int psl_grid_apply_bounceback_mask(psl_grid* grid, const psl_mask* mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PSL_H */

@@ -1,0 +1,515 @@
+"""MI355X-native D3Q27 periodic-shift LBM step (HIP, gfx950) behind a C ABI.
+
+Python mirror of the reference's operator interface (stepanmuller/PeriodicShiftLBM
+include/*.h): the same names, argument meanings and call order, implemented
+over libpsl_amd.so (include/psl.h) through ctypes.  Device memory, streams
+and kernels belong to the HIP library; numpy only carries host data in and out.
+
+Reference                                   here
+-----------------------------------------   -------------------------------------------
+InfoStruct (include/types.h:73-75)          InfoStruct
+GridStruct (include/types.h:77)             GridStruct (owns a psl_grid handle)
+getMarkers / bouncebackMarkerArray          GridStruct.set_markers(getMarkers) -> flag bytes
+getGivenRhoUxUyUz                           GridStruct.set_given_values(...)
+getSmagorinskyConstant                      GridStruct.set_smagorinsky(cs or array)
+fillEquilibriumFromFunction                 fillEquilibriumFromFunction(grid, getInitialRhoUxUyUz)
+applyStreaming / applyLocalCellUpdate       applyStreaming / applyLocalCellUpdate
+applyBCPeriodicX / applyBCPeriodicY         applyBCPeriodicX / applyBCPeriodicY
+applyCoarseFineGridCommunication            applyCoarseFineGridCommunication
+fillCoarseGridFromFine                      fillCoarseGridFromFine
+updateGrid recursion                        updateGrid(grids, steps)
+readSTL / checkSTLEdges                     readSTL / checkSTLEdges
+applyMarkersInsideSTL, bool array ops       applyMarkersInsideSTL, multiplyBoolArrays, ...
+getSphereDrag / getLift                     momentumExchange (FP64)
+
+There is no CPU fallback: without the built library and a HIP device every
+compute call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpsl_amd.so")
+
+F32, F16DEV, BF16DEV = 0, 1, 2
+STORAGE_NAMES = {"f32": F32, "f16": F16DEV, "bf16": BF16DEV}
+
+FLUID, BOUNCEBACK, GIVEN_RHO, GIVEN_U, GIVEN_NONE, GIVEN_RHOU, GHOST, COVERED = range(8)
+PERIODIC_X, PERIODIC_Y, PERIODIC_Z = 8, 16, 32
+
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1])
+CY = np.array([0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1, -1, 1])
+CZ = np.array([0, 0, 0, -1, 1, 0, 0, -1, 1, 1, -1, 0, 0, -1, 1, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, -1, 1])
+W = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
+
+
+class PslError(RuntimeError):
+    pass
+
+
+class _Info(ctypes.Structure):
+    _fields_ = [("gridID", ctypes.c_float), ("res", ctypes.c_float),
+                ("cellCountX", ctypes.c_int), ("cellCountY", ctypes.c_int), ("cellCountZ", ctypes.c_int),
+                ("cellCount", ctypes.c_int),
+                ("ox", ctypes.c_float), ("oy", ctypes.c_float), ("oz", ctypes.c_float),
+                ("nu", ctypes.c_float), ("dtPhys", ctypes.c_float),
+                ("iSubgridStart", ctypes.c_int), ("jSubgridStart", ctypes.c_int), ("kSubgridStart", ctypes.c_int),
+                ("iSubgridEnd", ctypes.c_int), ("jSubgridEnd", ctypes.c_int), ("kSubgridEnd", ctypes.c_int),
+                ("cellCountFineToCoarse", ctypes.c_int), ("cellCountCoarseToFine", ctypes.c_int),
+                ("pRegulator", ctypes.c_float), ("iRegulator", ctypes.c_float)]
+
+
+@dataclass
+class InfoStruct:
+    """include/types.h:73-75."""
+    gridID: float = 0.0
+    res: float = 1.0
+    cellCountX: int = 0
+    cellCountY: int = 0
+    cellCountZ: int = 0
+    cellCount: int = 0
+    ox: float = 0.0
+    oy: float = 0.0
+    oz: float = 0.0
+    nu: float = 1.0
+    dtPhys: float = 1.0
+    iSubgridStart: int = 0
+    jSubgridStart: int = 0
+    kSubgridStart: int = 0
+    iSubgridEnd: int = 0
+    jSubgridEnd: int = 0
+    kSubgridEnd: int = 0
+    cellCountFineToCoarse: int = 0
+    cellCountCoarseToFine: int = 0
+    pRegulator: float = 0.0
+    iRegulator: float = 0.0
+
+    def to_c(self) -> _Info:
+        c = _Info()
+        for name, _ in _Info._fields_:
+            setattr(c, name, getattr(self, name))
+        c.cellCount = self.cellCountX * self.cellCountY * self.cellCountZ
+        return c
+
+    @staticmethod
+    def from_c(c: _Info) -> "InfoStruct":
+        return InfoStruct(**{name: getattr(c, name) for name, _ in _Info._fields_})
+
+
+# ---------------------------------------------------------------- library
+_lib = None
+
+# (name, restype, argtypes); every symbol declared in include/psl.h
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_SIGS = [
+    ("psl_last_error", ctypes.c_char_p, []),
+    ("psl_abi_version", _I, []),
+    ("psl_device_count", _I, [_P]),
+    ("psl_grid_create", _I, [_P, _I, _P]),
+    ("psl_grid_destroy", _I, [_P]),
+    ("psl_grid_get_info", _I, [_P, _P]),
+    ("psl_grid_set_info", _I, [_P, _P]),
+    ("psl_grid_set_stream", _I, [_P, _P]),
+    ("psl_grid_set_flags", _I, [_P, _P]),
+    ("psl_grid_get_flags", _I, [_P, _P]),
+    ("psl_grid_set_bc_values", _I, [_P, _P]),
+    ("psl_grid_set_smagorinsky", _I, [_P, ctypes.c_float, _P]),
+    ("psl_grid_set_periodic_copies", _I, [_P, _I]),
+    ("psl_fill_equilibrium", _I, [_P, _P, _P, _P, _P]),
+    ("psl_apply_streaming", _I, [_P]),
+    ("psl_apply_local_cell_update", _I, [_P]),
+    ("psl_apply_bc_periodic_x", _I, [_P]),
+    ("psl_apply_bc_periodic_y", _I, [_P]),
+    ("psl_coarse_fine_communication", _I, [_P, _P]),
+    ("psl_fill_coarse_from_fine", _I, [_P, _P]),
+    ("psl_update_grid", _I, [_P, _I, _I]),
+    ("psl_get_shifter", _I, [_P, _P]),
+    ("psl_set_shifter", _I, [_P, _P]),
+    ("psl_get_f", _I, [_P, _P]),
+    ("psl_set_f", _I, [_P, _P]),
+    ("psl_get_macroscopic", _I, [_P, _P, _P, _P, _P]),
+    ("psl_grid_device_f", _I, [_P, _P, _P]),
+    ("psl_reduce_momentum_exchange", _I, [_P, _P, _P, _P]),
+    ("psl_reduce_mass", _I, [_P, _P]),
+    ("psl_stl_read", _I, [ctypes.c_char_p, _P]),
+    ("psl_stl_from_triangles", _I, [_P, _I, _P]),
+    ("psl_stl_destroy", _I, [_P]),
+    ("psl_stl_info", _I, [_P, _P, _P]),
+    ("psl_stl_set_origin", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    ("psl_stl_get_triangles", _I, [_P, _P]),
+    ("psl_stl_check_edges", _I, [_P, _P, _P]),
+    ("psl_mask_create", _I, [ctypes.c_int64, _I, _P]),
+    ("psl_mask_destroy", _I, [_P]),
+    ("psl_mask_get", _I, [_P, _P]),
+    ("psl_mask_set", _I, [_P, _P]),
+    ("psl_mask_and", _I, [_P, _P, _P]),
+    ("psl_mask_or", _I, [_P, _P, _P]),
+    ("psl_mask_not", _I, [_P]),
+    ("psl_markers_inside_stl", _I, [_P, _P, _I, _P]),
+    ("psl_grid_apply_bounceback_mask", _I, [_P, _P]),
+    ("psl_device_synchronize", _I, []),
+    ("psl_grid_profile_enable", _I, [_P, _I]),
+    ("psl_grid_profile_read", _I, [_P, _P, _P]),
+    ("psl_grid_cell_census", _I, [_P, _P, _P, _P]),
+]
+
+
+def lib():
+    """Load libpsl_amd.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PslError(f"{LIB_PATH} is missing: build it with `python periodicshiftlbm_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().psl_last_error()
+        raise PslError(f"psl error {rc}: {msg.decode() if msg else ''}")
+
+
+def device_count() -> int:
+    c = ctypes.c_int(0)
+    _check(lib().psl_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def require_device():
+    if device_count() < 1:
+        raise PslError("no HIP device visible: the MI355X path has no CPU fallback")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------- grids
+class GridStruct:
+    """include/types.h:77 -- owns the device fArray, shifter and flags."""
+
+    def __init__(self, info: InfoStruct, storage: int | str = F32):
+        if isinstance(storage, str):
+            storage = STORAGE_NAMES[storage]
+        require_device()
+        self.storage = storage
+        self._h = ctypes.c_void_p()
+        ci = info.to_c()
+        _check(lib().psl_grid_create(ctypes.byref(ci), storage, ctypes.byref(self._h)))
+        self.Info = InfoStruct.from_c(ci)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().psl_grid_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def n(self) -> int:
+        return self.Info.cellCountX * self.Info.cellCountY * self.Info.cellCountZ
+
+    def push_info(self):
+        ci = self.Info.to_c()
+        _check(lib().psl_grid_set_info(self._h, ctypes.byref(ci)))
+
+    # -- markers / boundary data (host-evaluated driver callbacks)
+    def set_flags(self, flags: Optional[np.ndarray]):
+        if flags is None:
+            _check(lib().psl_grid_set_flags(self._h, None))
+            return
+        f = np.ascontiguousarray(flags, np.uint8).reshape(-1)
+        if f.size != self.n:
+            raise ValueError("flags must hold one byte per cell")
+        _check(lib().psl_grid_set_flags(self._h, f.ctypes.data))
+
+    def get_flags(self) -> np.ndarray:
+        out = np.empty(self.n, np.uint8)
+        _check(lib().psl_grid_get_flags(self._h, out.ctypes.data))
+        return out
+
+    def ijk(self):
+        I = self.Info
+        k, j, i = np.meshgrid(np.arange(I.cellCountZ), np.arange(I.cellCountY), np.arange(I.cellCountX), indexing="ij")
+        return i.reshape(-1), j.reshape(-1), k.reshape(-1)
+
+    def set_markers(self, getMarkers: Callable, bounceback: Optional[np.ndarray] = None):
+        """Evaluate a vectorised getMarkers(i, j, k, bounceback, Info) -> flag bytes."""
+        i, j, k = self.ijk()
+        bb = np.zeros(self.n, bool) if bounceback is None else np.asarray(bounceback, bool).reshape(-1)
+        self.set_flags(np.asarray(getMarkers(i, j, k, bb, self.Info), np.uint8))
+
+    def set_given_values(self, values):
+        """getGivenRhoUxUyUz table: up to 4 slots of (rho, ux, uy, uz)."""
+        v = np.zeros((4, 4), np.float32)
+        a = np.asarray(values, np.float32).reshape(-1, 4)
+        v[: a.shape[0]] = a
+        _check(lib().psl_grid_set_bc_values(self._h, v.ctypes.data))
+
+    def set_smagorinsky(self, cs):
+        if np.isscalar(cs):
+            _check(lib().psl_grid_set_smagorinsky(self._h, float(cs), None))
+        else:
+            a = np.ascontiguousarray(cs, np.float32).reshape(-1)
+            _check(lib().psl_grid_set_smagorinsky(self._h, 0.0, a.ctypes.data))
+
+    def set_periodic_copies(self, x: bool = False, y: bool = False):
+        _check(lib().psl_grid_set_periodic_copies(self._h, int(x) | (int(y) << 1)))
+
+    # -- state
+    @property
+    def shifter(self) -> np.ndarray:
+        out = np.zeros(27, np.int32)
+        _check(lib().psl_get_shifter(self._h, out.ctypes.data))
+        return out
+
+    @shifter.setter
+    def shifter(self, v):
+        a = np.ascontiguousarray(v, np.int32)
+        _check(lib().psl_set_shifter(self._h, a.ctypes.data))
+
+    def get_f(self) -> np.ndarray:
+        """(27, N) FP32 in logical cell order."""
+        out = np.empty((27, self.n), np.float32)
+        _check(lib().psl_get_f(self._h, out.ctypes.data))
+        return out
+
+    def set_f(self, f: np.ndarray):
+        a = np.ascontiguousarray(f, np.float32)
+        if a.shape != (27, self.n):
+            raise ValueError("f must have shape (27, N)")
+        _check(lib().psl_set_f(self._h, a.ctypes.data))
+
+    def macroscopic(self):
+        n = self.n
+        r, a, b, c = (np.empty(n, np.float32) for _ in range(4))
+        _check(lib().psl_get_macroscopic(self._h, r.ctypes.data, a.ctypes.data, b.ctypes.data, c.ctypes.data))
+        return r, a, b, c
+
+    def device_f(self):
+        p = ctypes.c_void_p()
+        nbytes = ctypes.c_size_t()
+        _check(lib().psl_grid_device_f(self._h, ctypes.byref(p), ctypes.byref(nbytes)))
+        return p.value, nbytes.value
+
+    def set_stream(self, stream_ptr: int):
+        _check(lib().psl_grid_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
+
+    def momentum_exchange(self, lo=None, hi=None) -> np.ndarray:
+        out = np.zeros(3, np.float64)
+        lo_a = None if lo is None else np.ascontiguousarray(lo, np.int32)
+        hi_a = None if hi is None else np.ascontiguousarray(hi, np.int32)
+        _check(lib().psl_reduce_momentum_exchange(self._h, _ptr(lo_a), _ptr(hi_a), out.ctypes.data))
+        return out
+
+    def profile(self, enable: bool):
+        _check(lib().psl_grid_profile_enable(self._h, int(enable)))
+
+    def profile_read(self):
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        _check(lib().psl_grid_profile_read(self._h, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def census(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().psl_grid_cell_census(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def mass(self) -> float:
+        out = ctypes.c_double()
+        _check(lib().psl_reduce_mass(self._h, ctypes.byref(out)))
+        return out.value
+
+
+# ------------------------------------------------------------ operators
+def fillEquilibriumFromFunction(grid: GridStruct, getInitialRhoUxUyUz: Optional[Callable] = None):
+    """include/fillEquilibrium.h:1-22 with a vectorised host callback
+    getInitialRhoUxUyUz(i, j, k, flags, Info) -> (rho, ux, uy, uz)."""
+    if getInitialRhoUxUyUz is None:
+        _check(lib().psl_fill_equilibrium(grid.handle, None, None, None, None))
+        return
+    i, j, k = grid.ijk()
+    vals = getInitialRhoUxUyUz(i, j, k, grid.get_flags(), grid.Info)
+    arrs = [None if v is None else np.ascontiguousarray(np.broadcast_to(v, (grid.n,)), np.float32) for v in vals]
+    _check(lib().psl_fill_equilibrium(grid.handle, *[_ptr(a) for a in arrs]))
+
+
+def fill_equilibrium(grid: GridStruct, rho=None, ux=None, uy=None, uz=None):
+    arrs = [None if v is None else np.ascontiguousarray(np.broadcast_to(v, (grid.n,)), np.float32)
+            for v in (rho, ux, uy, uz)]
+    _check(lib().psl_fill_equilibrium(grid.handle, *[_ptr(a) for a in arrs]))
+
+
+def synchronize():
+    _check(lib().psl_device_synchronize())
+
+
+def applyStreaming(grid: GridStruct):
+    _check(lib().psl_apply_streaming(grid.handle))
+
+
+def applyLocalCellUpdate(grid: GridStruct):
+    _check(lib().psl_apply_local_cell_update(grid.handle))
+
+
+def applyBCPeriodicX(grid: GridStruct):
+    _check(lib().psl_apply_bc_periodic_x(grid.handle))
+
+
+def applyBCPeriodicY(grid: GridStruct):
+    _check(lib().psl_apply_bc_periodic_y(grid.handle))
+
+
+def applyCoarseFineGridCommunication(coarse: GridStruct, fine: GridStruct):
+    _check(lib().psl_coarse_fine_communication(coarse.handle, fine.handle))
+
+
+def fillCoarseGridFromFine(coarse: GridStruct, fine: GridStruct):
+    _check(lib().psl_fill_coarse_from_fine(coarse.handle, fine.handle))
+
+
+def updateGrid(grids: Sequence[GridStruct], steps: int = 1):
+    arr = (ctypes.c_void_p * len(grids))(*[g.handle.value for g in grids])
+    _check(lib().psl_update_grid(arr, len(grids), steps))
+
+
+# ------------------------------------------------------------ geometry
+class STLStruct:
+    """include/types.h:84-119 (host and device copies)."""
+
+    def __init__(self, handle: ctypes.c_void_p):
+        self._h = handle
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().psl_stl_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def triangleCount(self) -> int:
+        c = ctypes.c_int()
+        _check(lib().psl_stl_info(self._h, ctypes.byref(c), None))
+        return c.value
+
+    @property
+    def bounds(self) -> np.ndarray:
+        b = np.zeros(6, np.float32)
+        _check(lib().psl_stl_info(self._h, None, b.ctypes.data))
+        return b
+
+    def set_origin(self, ox, oy, oz):
+        _check(lib().psl_stl_set_origin(self._h, ox, oy, oz))
+
+    def triangles(self) -> np.ndarray:
+        out = np.zeros((self.triangleCount, 9), np.float32)
+        _check(lib().psl_stl_get_triangles(self._h, out.ctypes.data))
+        return out
+
+    @staticmethod
+    def from_triangles(tri) -> "STLStruct":
+        t = np.ascontiguousarray(tri, np.float32).reshape(-1, 9)
+        require_device()
+        h = ctypes.c_void_p()
+        _check(lib().psl_stl_from_triangles(t.ctypes.data, t.shape[0], ctypes.byref(h)))
+        return STLStruct(h)
+
+
+def readSTL(path: str) -> STLStruct:
+    require_device()
+    h = ctypes.c_void_p()
+    _check(lib().psl_stl_read(path.encode(), ctypes.byref(h)))
+    return STLStruct(h)
+
+
+def checkSTLEdges(stl: STLStruct):
+    bad = ctypes.c_int()
+    counts = np.zeros((stl.triangleCount, 3), np.int32)
+    _check(lib().psl_stl_check_edges(stl.handle, ctypes.byref(bad), counts.ctypes.data))
+    return bad.value, counts
+
+
+class BoolArray:
+    """BoolArrayType on the device."""
+
+    def __init__(self, n: int, value: bool = False):
+        require_device()
+        self.n = n
+        self._h = ctypes.c_void_p()
+        _check(lib().psl_mask_create(n, int(value), ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().psl_mask_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.n, np.uint8)
+        _check(lib().psl_mask_get(self._h, out.ctypes.data))
+        return out
+
+    def set(self, host):
+        a = np.ascontiguousarray(host, np.uint8).reshape(-1)
+        _check(lib().psl_mask_set(self._h, a.ctypes.data))
+
+
+def applyMarkersInsideSTL(markerArray: BoolArray, stl: STLStruct, insideMarkerValue: bool, Info: InfoStruct):
+    ci = Info.to_c()
+    _check(lib().psl_markers_inside_stl(markerArray.handle, stl.handle, int(insideMarkerValue), ctypes.byref(ci)))
+
+
+def multiplyBoolArrays(a: BoolArray, b: BoolArray, out: BoolArray):
+    _check(lib().psl_mask_and(a.handle, b.handle, out.handle))
+
+
+def sumBoolArrays(a: BoolArray, b: BoolArray, out: BoolArray):
+    _check(lib().psl_mask_or(a.handle, b.handle, out.handle))
+
+
+def invertBoolArray(a: BoolArray):
+    _check(lib().psl_mask_not(a.handle))
+
+
+def applyBouncebackMask(grid: GridStruct, mask: BoolArray):
+    _check(lib().psl_grid_apply_bounceback_mask(grid.handle, mask.handle))

@@ -1,0 +1,153 @@
+"""Workload builders: the driver-side callbacks of the benchmark configurations.
+
+These play the role of the reference drivers' getMarkers / getGivenRhoUxUyUz /
+getInitialRhoUxUyUz callbacks (flowAroundSphere/flowAroundSphereMainSingle.cu:41-92),
+evaluated on the host into flag bytes and initial fields.
+
+* Taylor-Green vortex on a fully periodic 64^3 box: the reference realises x/y
+  periodicity with a duplicated boundary layer and applyBCPeriodicX/Y copies
+  (include/boundaryConditions/periodicXY.h:28-31), so the grid is 65 x 65 x 64
+  with period 64 in every direction (z is periodic through the shift itself).
+* Flow past a sphere in a 256 x 128 x 128 channel: icosphere of diameter 32
+  voxelized through the STL path, velocity inlet (given u) at i = 0, pressure
+  outlet (given rho) at i = Nx-1, periodic y (copies) and z (implicit).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+FLUID, BOUNCEBACK, GIVEN_RHO, GIVEN_U, GIVEN_NONE, GIVEN_RHOU, GHOST, COVERED = range(8)
+PERIODIC_X, PERIODIC_Y, PERIODIC_Z = 8, 16, 32
+
+
+# ------------------------------------------------------------ Taylor-Green
+def tgv_shape(n: int = 64):
+    return (n + 1, n + 1, n)
+
+
+def tgv_fields(n: int = 64, u0: float = 0.05):
+    """rho, ux, uy, uz (flattened, cell = k*Nx*Ny + j*Nx + i) on the (n+1, n+1, n) grid."""
+    nx, ny, nz = tgv_shape(n)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x = 2 * np.pi * i / n
+    y = 2 * np.pi * j / n
+    z = 2 * np.pi * k / n
+    ux = u0 * np.sin(x) * np.cos(y) * np.cos(z)
+    uy = -u0 * np.cos(x) * np.sin(y) * np.cos(z)
+    uz = np.zeros_like(ux)
+    p = (u0 * u0 / 16.0) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+    rho = 1.0 + 3.0 * p
+    return [a.reshape(-1) for a in (rho, ux, uy, uz)]
+
+
+def kinetic_energy_enstrophy(rho, ux, uy, uz, n: int = 64):
+    """Kinetic energy 1/2 sum rho |u|^2 and enstrophy 1/2 sum |curl u|^2 (central
+    differences) over the n^3 unique cells of the (n+1, n+1, n) periodic grid."""
+    nx, ny, nz = tgv_shape(n)
+    sh = (nz, ny, nx)
+    r, a, b, c = (np.asarray(v, np.float64).reshape(sh)[:, :n, :n] for v in (rho, ux, uy, uz))
+    ke = 0.5 * float(np.sum(r * (a * a + b * b + c * c)))
+
+    def d(f, axis):
+        return 0.5 * (np.roll(f, -1, axis) - np.roll(f, 1, axis))
+
+    # axes: 0 = z, 1 = y, 2 = x
+    wx = d(c, 1) - d(b, 0)
+    wy = d(a, 0) - d(c, 2)
+    wz = d(b, 2) - d(a, 1)
+    ens = 0.5 * float(np.sum(wx * wx + wy * wy + wz * wz))
+    return ke, ens
+
+
+# ------------------------------------------------------------ icosphere
+def icosphere(subdivisions: int = 4, radius: float = 16.0, center=(64.0, 64.0, 64.0)) -> np.ndarray:
+    """Triangles (T, 9) of an icosphere with outward orientation (20 * 4^s triangles)."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts) * radius + np.array(center)
+    # vertices as float32 first so shared edges compare equal exactly
+    V32 = V.astype(np.float32)
+    tri = np.stack([V32[[f[0] for f in faces]], V32[[f[1] for f in faces]], V32[[f[2] for f in faces]]], axis=1)
+    return tri.reshape(-1, 9)
+
+
+def write_binary_stl(path: str, tri: np.ndarray):
+    tri = np.ascontiguousarray(tri, np.float32).reshape(-1, 9)
+    rec = np.zeros(tri.shape[0], dtype=[("n", "<f4", 3), ("v", "<f4", 9), ("a", "<u2")])
+    rec["v"] = tri
+    with open(path, "wb") as fh:
+        fh.write(b"periodicshiftlbm_amd icosphere".ljust(80, b" "))
+        fh.write(np.uint32(tri.shape[0]).tobytes())
+        fh.write(rec.tobytes())
+
+
+# ------------------------------------------------------------ sphere channel
+SPHERE_SHAPE = (256, 128, 128)
+SPHERE_CENTER = (64.0, 64.0, 64.0)
+SPHERE_DIAMETER = 32.0
+
+
+def sphere_nu(u_inlet: float = 0.04, diameter: float = SPHERE_DIAMETER, reynolds: float = 1000.0) -> float:
+    return u_inlet * diameter / reynolds
+
+
+def channel_flags(shape, solid: np.ndarray | None = None) -> np.ndarray:
+    """Flag bytes of the sphere channel (cell = k*Nx*Ny + j*Nx + i).
+
+    slot 0 = inlet (given u), slot 1 = outlet (given rho); y/z faces are periodic
+    (the inlet/outlet cells on those faces keep the x normal only)."""
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    flags = np.full((nz, ny, nx), FLUID, np.uint8)
+    per = PERIODIC_Y | PERIODIC_Z
+    flags[i == 0] = GIVEN_U | per | (0 << 6)
+    flags[i == nx - 1] = GIVEN_RHO | per | (1 << 6)
+    flags = flags.reshape(-1)
+    if solid is not None:
+        s = np.asarray(solid, bool).reshape(-1)
+        flags[s] = BOUNCEBACK
+    return flags
+
+
+def channel_given_values(u_inlet: float = 0.04):
+    return np.array([[1.0, u_inlet, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0]],
+                    np.float32)
+
+
+def sphere_box(center=SPHERE_CENTER, diameter=SPHERE_DIAMETER):
+    """Bounding box [lo, hi) of the solid for the drag reduction
+    (flowAroundSphereMainSingle.cu:105-110)."""
+    r = diameter / 2
+    lo = [int(c - r) for c in center]
+    hi = [int(c + r) + 2 for c in center]
+    return lo, hi
+
+
+def drag_coefficient(fx: float, u_inlet: float = 0.04, diameter: float = SPHERE_DIAMETER) -> float:
+    """C_D = F / (1/2 rho U^2 pi D^2 / 4) with the momentum-exchange force in lattice units;
+    the reference's sign convention gives C_D = -8 F / (rho U^2 pi D^2) for F = sum rho (u - u_bb)
+    (flowAroundSphereMainSingle.cu:250)."""
+    return -8.0 * fx / (u_inlet * u_inlet * math.pi * diameter * diameter)

@@ -1,0 +1,963 @@
+// Grid lifecycle, the fused periodic-shift step kernels, periodic copies,
+// equilibrium fill, field access and FP64 reductions.
+//
+// Data layout in HBM (one allocation per grid): fArray(q, slot) at
+// f[q * N + slot], N = cellCount, cell = k*Nx*Ny + j*Nx + i, and population q
+// of cell c lives at slot (c + shift[q]) mod N (include/types.h:42-45,
+// include/applyStreaming.h:33-41).  Streaming only changes shift[q]
+// (include/applyStreaming.h:11-31), so one step is a read-modify-write of
+// 27 slots per cell with no cross-cell dependency: every slot is owned by
+// exactly one cell (paper section 1.1.2, eqs 1.5-1.6).
+//
+// Kernels of one step:
+//   k_ps_bulk      fluid + bounceback cells, one thread per cell, the hot path;
+//   k_ps_boundary  box-face boundary cells from a compact list (restore +
+//                  MBBC + collision); disjoint from the bulk cells, so the two
+//                  launches together equal the reference's single update;
+//   k_periodic     applyBCPeriodicX/Y surface copies (before the update).
+#include "psl_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace psl {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+// --------------------------------------------------------------- addressing
+// Per-wave addressing of the 27 shifted streams.  For population q the
+// lanes of a wave either all stay below the wrap threshold N - shift[q], all
+// lie above it, or (for at most one wave per q) straddle it.  In the first two
+// cases the base pointer is wave-uniform and the per-lane offset cell*size is
+// shared by all 27 populations; the rare straddling waves take a per-lane path.
+template <class St>
+struct Streams {
+  typedef typename St::T T;
+  const T* f;
+  uint32_t n;
+  uint32_t w_first, w_last;  // first / last valid cell of this wave (uniform)
+  bool straddle;
+
+  __device__ __forceinline__ Streams(const T* f_, uint32_t n_, uint32_t wave_first, const Shift27& sh) : f(f_), n(n_) {
+    w_first = wave_first;
+    w_last = min(wave_first + 63u, n_ - 1u);
+    bool s = false;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const uint32_t thr = n_ - (uint32_t)sh.s[q];  // cells >= thr wrap
+      s |= (w_first < thr) && (w_last >= thr);
+    }
+    straddle = s;
+  }
+  // wave-uniform element offset of population q for the fast path
+  __device__ __forceinline__ int64_t uniform_base(int q, const Shift27& sh) const {
+    const uint32_t thr = n - (uint32_t)sh.s[q];
+    const int64_t off = (int64_t)q * n + sh.s[q];
+    return w_first >= thr ? off - (int64_t)n : off;
+  }
+  __device__ __forceinline__ int64_t lane_index(int q, uint32_t cell, const Shift27& sh) const {
+    uint32_t s = cell + (uint32_t)sh.s[q];
+    if (s >= n) s -= n;
+    return (int64_t)q * n + s;
+  }
+};
+
+template <class St>
+__device__ __forceinline__ void load27(const Streams<St>& S, const typename St::T* f, uint32_t cell, const Shift27& sh,
+                                       float (&v)[27]) {
+  typedef typename St::T T;
+  if (!S.straddle) {
+    const uint32_t off = cell * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const char* base = (const char*)(f + S.uniform_base(q, sh));
+      v[q] = St::load(*(const T*)(base + off));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = St::load(f[S.lane_index(q, cell, sh)]);
+  }
+}
+
+template <class St>
+__device__ __forceinline__ void store27(const Streams<St>& S, typename St::T* f, uint32_t cell, const Shift27& sh,
+                                        const float (&v)[27]) {
+  typedef typename St::T T;
+  if (!S.straddle) {
+    const uint32_t off = cell * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      char* base = (char*)(f + S.uniform_base(q, sh));
+      *(T*)(base + off) = St::store(v[q]);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[S.lane_index(q, cell, sh)] = St::store(v[q]);
+  }
+}
+
+__device__ __forceinline__ uint32_t wave_first_cell(uint32_t block_first) {
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  return block_first + wave * 64u;
+}
+
+struct StepParams {
+  Shift27 sh;
+  uint32_t n;
+  float omega;  // 1/tau, used when cs == 0 and there is no per-cell cs
+  float tau;
+  float cs;
+  const float* cs_cell;
+  const uint8_t* flags;
+};
+
+// ------------------------------------------------------------- bulk update
+// Fluid and bounceback cells: include/applyLocalCellUpdate.h:1-74 minus the
+// box-face branch (46-67), which k_ps_boundary handles.
+template <class St, bool FLAGS, bool LES>
+__global__ __launch_bounds__(256) void k_ps_bulk(typename St::T* __restrict__ f, StepParams p) {
+  const uint32_t block_first = blockIdx.x * 256u;
+  const uint32_t cell = block_first + threadIdx.x;
+  if (cell >= p.n) return;
+  int kind = KIND_FLUID;
+  if (FLAGS) kind = p.flags[cell] & 7;
+  const Streams<St> S(f, p.n, wave_first_cell(block_first), p.sh);
+  if (kind > KIND_BOUNCEBACK) return;
+  float v[27];
+  load27<St>(S, f, cell, p.sh, v);
+  if (kind == KIND_BOUNCEBACK) {
+    // the swap commutes with "f - w" (opposite directions share a weight)
+    bounceback(v);
+  } else {
+    Relax rx{p.omega, p.tau, p.cs};
+    float cs = p.cs;
+    if (LES && p.cs_cell) cs = p.cs_cell[cell];
+    collide<St::W_OFF>(v, rx, LES ? cs : 0.f);
+  }
+  store27<St>(S, f, cell, p.sh, v);
+}
+
+// ---------------------------------------------------------- boundary update
+// include/applyLocalCellUpdate.h:40-69 for listed box-face cells.
+template <class St>
+__global__ __launch_bounds__(256) void k_ps_boundary(typename St::T* __restrict__ f, StepParams p,
+                                                     const uint32_t* __restrict__ list, uint32_t nlist, int nx,
+                                                     int ny, int nz, BcTable bc) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= nlist) return;
+  const uint32_t cell = list[t];
+  const int fl = p.flags[cell];
+  const int kind = fl & 7;
+  const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
+  const int k = (int)(cell / nxy);
+  const uint32_t rem = cell - (uint32_t)k * nxy;
+  const int j = (int)(rem / (uint32_t)nx);
+  const int i = (int)(rem - (uint32_t)j * (uint32_t)nx);
+  auto idx = [&](int q) -> int64_t {
+    uint32_t s = cell + (uint32_t)p.sh.s[q];
+    if (s >= p.n) s -= p.n;
+    return (int64_t)q * p.n + s;
+  };
+  float v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    v[q] = St::load(f[idx(q)]);
+    if (St::W_OFF) v[q] += weight(q);
+  }
+  const int code = outer_normal_code(i, j, k, nx, ny, nz, fl);
+  // given (rho, u) of slot 0..3, selected without dynamic indexing (keeps the table in SGPRs)
+  const int slot = (fl >> 6) & 3;
+  float g[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    g[c] = slot == 0 ? bc.v[c] : slot == 1 ? bc.v[4 + c] : slot == 2 ? bc.v[8 + c] : bc.v[12 + c];
+  boundary_cell(v, kind, code, g[0], g[1], g[2], g[3]);
+  Relax rx{p.omega, p.tau, p.cs};
+  const float cs = p.cs_cell ? p.cs_cell[cell] : p.cs;
+  collide<false>(v, rx, cs);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const float o = St::W_OFF ? v[q] - weight(q) : v[q];
+    f[idx(q)] = St::store(o);
+  }
+}
+
+// ------------------------------------------------------------ periodic copy
+// include/boundaryConditions/periodicXY.h:38-128: +c populations right->left,
+// -c populations left->right, raw storage words.
+__constant__ int c_pos_x[9] = {1, 7, 9, 12, 16, 20, 22, 23, 26};
+__constant__ int c_neg_x[9] = {2, 8, 10, 11, 15, 19, 21, 24, 25};
+__constant__ int c_pos_y[9] = {6, 12, 13, 15, 17, 19, 22, 24, 26};
+__constant__ int c_neg_y[9] = {5, 11, 14, 16, 18, 20, 21, 23, 25};
+
+template <class T>
+__global__ __launch_bounds__(256) void k_periodic(T* __restrict__ f, Shift27 sh, uint32_t n, int nx, int ny, int nz,
+                                                  int axis) {
+  const int a = blockIdx.x * 256 + threadIdx.x;  // j (axis x) or i (axis y)
+  const int k = blockIdx.y;
+  const int na = axis == 0 ? ny : nx;
+  if (a >= na || k >= nz) return;
+  const uint32_t plane = (uint32_t)k * (uint32_t)nx * (uint32_t)ny;
+  uint32_t left, right;
+  if (axis == 0) {
+    left = plane + (uint32_t)a * nx;
+    right = left + nx - 1;
+  } else {
+    left = plane + a;
+    right = plane + (uint32_t)(ny - 1) * nx + a;
+  }
+  const int* pos = axis == 0 ? c_pos_x : c_pos_y;
+  const int* neg = axis == 0 ? c_neg_x : c_neg_y;
+  auto slot = [&](uint32_t c, int q) -> int64_t {
+    uint32_t s = c + (uint32_t)sh.s[q];
+    if (s >= n) s -= n;
+    return (int64_t)q * n + s;
+  };
+  for (int t = 0; t < 9; ++t) {
+    const int q = pos[t];
+    f[slot(left, q)] = f[slot(right, q)];
+  }
+  for (int t = 0; t < 9; ++t) {
+    const int q = neg[t];
+    f[slot(right, q)] = f[slot(left, q)];
+  }
+}
+
+// --------------------------------------------------------- equilibrium fill
+// include/fillEquilibrium.h:1-22.  Written at the shifted slot, which is the
+// reference's unshifted index whenever the shifter is zero (its only use).
+template <class St>
+__global__ __launch_bounds__(256) void k_fill_eq(typename St::T* __restrict__ f, Shift27 sh, uint32_t n,
+                                                 const float* rho, const float* ux, const float* uy, const float* uz) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= n) return;
+  const float r = rho ? rho[cell] : 1.f;
+  const float a = ux ? ux[cell] : 0.f, b = uy ? uy[cell] : 0.f, c = uz ? uz[cell] : 0.f;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    uint32_t s = cell + (uint32_t)sh.s[q];
+    if (s >= n) s -= n;
+    float e = feq(q, r, a, b, c);
+    if (St::W_OFF) e -= weight(q);
+    f[(int64_t)q * n + s] = St::store(e);
+  }
+}
+
+// --------------------------------------------------------------- field I/O
+template <class St>
+__global__ __launch_bounds__(256) void k_get_f(const typename St::T* __restrict__ f, Shift27 sh, uint32_t n,
+                                               float* __restrict__ out) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= n) return;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    uint32_t s = cell + (uint32_t)sh.s[q];
+    if (s >= n) s -= n;
+    float v = St::load(f[(int64_t)q * n + s]);
+    if (St::W_OFF) v += weight(q);
+    out[(int64_t)q * n + cell] = v;
+  }
+}
+
+template <class St>
+__global__ __launch_bounds__(256) void k_set_f(typename St::T* __restrict__ f, Shift27 sh, uint32_t n,
+                                               const float* __restrict__ in) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= n) return;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    uint32_t s = cell + (uint32_t)sh.s[q];
+    if (s >= n) s -= n;
+    float v = in[(int64_t)q * n + cell];
+    if (St::W_OFF) v -= weight(q);
+    f[(int64_t)q * n + s] = St::store(v);
+  }
+}
+
+template <class St>
+__device__ __forceinline__ void load_full(const typename St::T* f, const Shift27& sh, uint32_t n, uint32_t cell,
+                                          float (&v)[27]) {
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    uint32_t s = cell + (uint32_t)sh.s[q];
+    if (s >= n) s -= n;
+    v[q] = St::load(f[(int64_t)q * n + s]);
+    if (St::W_OFF) v[q] += weight(q);
+  }
+}
+
+template <class St>
+__global__ __launch_bounds__(256) void k_macro(const typename St::T* __restrict__ f, Shift27 sh, uint32_t n,
+                                               float* rho, float* ux, float* uy, float* uz) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= n) return;
+  float v[27];
+  load_full<St>(f, sh, n, cell, v);
+  float r, a, b, c;
+  rho_u(v, r, a, b, c);
+  if (rho) rho[cell] = r;
+  if (ux) ux[cell] = a;
+  if (uy) uy[cell] = b;
+  if (uz) uz[cell] = c;
+}
+
+// --------------------------------------------------------------- reductions
+// Deterministic FP64 reductions: per-block partials in a fixed order, then
+// one block sums the partials.
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o, 64);
+  return x;
+}
+
+template <int K>
+__device__ __forceinline__ void block_sum_store(double (&x)[K], double* out) {
+  __shared__ double sm[4][K];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    const double s = wave_sum(x[c]);
+    if (lane == 0) sm[wave][c] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) out[threadIdx.x] = ((sm[0][threadIdx.x] + sm[1][threadIdx.x]) + sm[2][threadIdx.x]) + sm[3][threadIdx.x];
+}
+
+// getSphereDrag / getLift momentum exchange (flowAroundSphereMainSingle.cu:115-146).
+template <class St>
+__global__ __launch_bounds__(256) void k_momentum_exchange(const typename St::T* __restrict__ f, Shift27 sh, uint32_t n,
+                                                           const uint8_t* __restrict__ flags, int nx, int ny, int lo0,
+                                                           int lo1, int lo2, int sx, int sy, int sz,
+                                                           double* partials) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t total = (uint32_t)sx * sy * sz;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (t < total && flags) {
+    const int kr = t / (sx * sy);
+    const int rem = t - kr * sx * sy;
+    const int jr = rem / sx, ir = rem - jr * sx;
+    const uint32_t cell = (uint32_t)(kr + lo2) * nx * ny + (uint32_t)(jr + lo1) * nx + (ir + lo0);
+    if ((flags[cell] & 7) == KIND_BOUNCEBACK) {
+      float v[27];
+      load_full<St>(f, sh, n, cell, v);
+      float r, a, b, c;
+      rho_u(v, r, a, b, c);
+      bounceback(v);
+      float r2, a2, b2, c2;
+      rho_u(v, r2, a2, b2, c2);
+      acc[0] = (double)(r * (a - a2));
+      acc[1] = (double)(r * (b - b2));
+      acc[2] = (double)(r * (c - c2));
+    }
+  }
+  block_sum_store<3>(acc, partials + 3 * (size_t)blockIdx.x);
+}
+
+template <class St>
+__global__ __launch_bounds__(256) void k_mass(const typename St::T* __restrict__ f, Shift27 sh, uint32_t n,
+                                              const uint8_t* __restrict__ flags, double* partials) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  double acc[1] = {0.0};
+  if (cell < n) {
+    const int kind = flags ? (flags[cell] & 7) : KIND_FLUID;
+    if (kind != KIND_GHOST && kind != KIND_COVERED && kind != KIND_BOUNCEBACK) {
+      double m = 0.0;
+#pragma unroll
+      for (int q = 0; q < 27; ++q) {
+        uint32_t s = cell + (uint32_t)sh.s[q];
+        if (s >= n) s -= n;
+        m += (double)St::load(f[(int64_t)q * n + s]);
+      }
+      if (St::W_OFF) m += 1.0;
+      acc[0] = m;
+    }
+  }
+  block_sum_store<1>(acc, partials + blockIdx.x);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_sum_partials(const double* partials, uint32_t nblocks, double* out) {
+  double acc[K];
+#pragma unroll
+  for (int c = 0; c < K; ++c) acc[c] = 0.0;
+  for (uint32_t b = threadIdx.x; b < nblocks; b += 256)
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[c] += partials[(size_t)K * b + c];
+  block_sum_store<K>(acc, out);
+}
+
+// ------------------------------------------------------------ flag helpers
+// Covered region of a grid that carries a finer sub-box
+// (include/applyLocalCellUpdate.h:25-30).
+__global__ void k_flags_overlay(uint8_t* flags, uint32_t n, int nx, int ny, int is, int ie, int js, int je, int ks,
+                                int ke) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= n) return;
+  const uint32_t nxy = (uint32_t)nx * ny;
+  const int k = cell / nxy;
+  const uint32_t rem = cell - (uint32_t)k * nxy;
+  const int j = rem / nx, i = rem - j * nx;
+  if ((i > is + 1 && i < ie - 2) && (j > js + 1 && j < je - 2) && (k > ks + 1 && k < ke - 2))
+    flags[cell] = (uint8_t)((flags[cell] & ~7) | KIND_COVERED);
+}
+
+__global__ void k_apply_bb_mask(uint8_t* flags, const uint8_t* mask, uint32_t n) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= n) return;
+  if (mask[cell] && (flags[cell] & 7) != KIND_COVERED) flags[cell] = (uint8_t)((flags[cell] & ~7) | KIND_BOUNCEBACK);
+}
+
+// ------------------------------------------------------------ host helpers
+static inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+
+static inline Shift27 shift_of(const psl_grid* g) {
+  Shift27 s;
+  for (int q = 0; q < 27; ++q) s.s[q] = g->shift[q];
+  return s;
+}
+
+static inline uint32_t ncells(const psl_grid* g) { return (uint32_t)g->info.cellCount; }
+
+int ensure_partials(psl_grid* g, size_t n) {
+  if (g->n_partials >= n && g->partials) return PSL_OK;
+  if (g->partials) PSL_HIP_TRY(hipFree(g->partials));
+  g->partials = nullptr;
+  PSL_HIP_TRY(hipMalloc(&g->partials, n * sizeof(double)));
+  g->n_partials = n;
+  return PSL_OK;
+}
+
+static bool has_subgrid(const psl_info& I) {
+  return (I.iSubgridEnd - I.iSubgridStart) > 4 && (I.jSubgridEnd - I.jSubgridStart) > 4 &&
+         (I.kSubgridEnd - I.kSubgridStart) > 4;
+}
+
+// Upload the user flags (or all-fluid), overlay the covered sub-box, and
+// rebuild the compact list of box-face boundary cells.
+int grid_rebuild_flags(psl_grid* g) {
+  const uint32_t n = ncells(g);
+  const bool need = !g->host_flags.empty() || has_subgrid(g->info);
+  if (!need) {
+    g->census[0] = n;
+    g->census[1] = g->census[2] = 0;
+    if (g->flags) PSL_HIP_TRY(hipFree(g->flags));
+    if (g->bc_list) PSL_HIP_TRY(hipFree(g->bc_list));
+    g->flags = nullptr;
+    g->bc_list = nullptr;
+    g->n_bc = 0;
+    return PSL_OK;
+  }
+  if (!g->flags) PSL_HIP_TRY(hipMalloc(&g->flags, n));
+  if (g->host_flags.empty()) PSL_HIP_TRY(hipMemsetAsync(g->flags, 0, n, g->stream));
+  else PSL_HIP_TRY(hipMemcpyAsync(g->flags, g->host_flags.data(), n, hipMemcpyHostToDevice, g->stream));
+  const psl_info& I = g->info;
+  if (has_subgrid(I)) {
+    k_flags_overlay<<<blocks_for(n), 256, 0, g->stream>>>(g->flags, n, I.cellCountX, I.cellCountY, I.iSubgridStart,
+                                                          I.iSubgridEnd, I.jSubgridStart, I.jSubgridEnd,
+                                                          I.kSubgridStart, I.kSubgridEnd);
+    PSL_LAUNCH_CHECK();
+  }
+  // boundary list from the final flags
+  std::vector<uint8_t> fin(n);
+  PSL_HIP_TRY(hipMemcpyAsync(fin.data(), g->flags, n, hipMemcpyDeviceToHost, g->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  std::vector<uint32_t> list;
+  int64_t skipped = 0;
+  for (uint32_t c = 0; c < n; ++c) {
+    const int kind = fin[c] & 7;
+    if (kind >= KIND_GIVEN_RHO && kind <= KIND_GIVEN_RHOU) list.push_back(c);
+    else if (kind >= KIND_GHOST) ++skipped;
+  }
+  g->census[1] = (int64_t)list.size();
+  g->census[2] = skipped;
+  g->census[0] = (int64_t)n - g->census[1] - skipped;
+  if (g->bc_list) PSL_HIP_TRY(hipFree(g->bc_list));
+  g->bc_list = nullptr;
+  g->n_bc = (uint32_t)list.size();
+  if (!list.empty()) {
+    PSL_HIP_TRY(hipMalloc(&g->bc_list, list.size() * sizeof(uint32_t)));
+    PSL_HIP_TRY(hipMemcpy(g->bc_list, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  return PSL_OK;
+}
+
+static void drop_profile_events(psl_grid* g) {
+  for (auto& e : g->prof_events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  g->prof_events.clear();
+}
+
+template <class St>
+static int launch_local_update(psl_grid* g) {
+  typedef typename St::T T;
+  StepParams p;
+  p.sh = shift_of(g);
+  p.n = ncells(g);
+  const float tau = 3.f * g->info.nu + 0.5f;
+  p.tau = tau;
+  p.omega = 1.f / tau;
+  p.cs = g->cs;
+  p.cs_cell = g->cs_cell;
+  p.flags = g->flags;
+  const bool les = g->cs != 0.f || g->cs_cell != nullptr;
+  T* f = (T*)g->f;
+  const uint32_t nb = blocks_for(p.n);
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (g->profile) {
+    PSL_HIP_TRY(hipEventCreate(&ev0));
+    PSL_HIP_TRY(hipEventCreate(&ev1));
+    PSL_HIP_TRY(hipEventRecord(ev0, g->stream));
+  }
+  if (g->flags) {
+    if (les) k_ps_bulk<St, true, true><<<nb, 256, 0, g->stream>>>(f, p);
+    else k_ps_bulk<St, true, false><<<nb, 256, 0, g->stream>>>(f, p);
+  } else {
+    if (les) k_ps_bulk<St, false, true><<<nb, 256, 0, g->stream>>>(f, p);
+    else k_ps_bulk<St, false, false><<<nb, 256, 0, g->stream>>>(f, p);
+  }
+  PSL_LAUNCH_CHECK();
+  if (g->profile) {
+    PSL_HIP_TRY(hipEventRecord(ev1, g->stream));
+    g->prof_events.emplace_back(ev0, ev1);
+  }
+  if (g->n_bc) {
+    k_ps_boundary<St><<<blocks_for(g->n_bc), 256, 0, g->stream>>>(f, p, g->bc_list, g->n_bc, g->info.cellCountX,
+                                                                  g->info.cellCountY, g->info.cellCountZ, g->bcvals);
+    PSL_LAUNCH_CHECK();
+  }
+  return PSL_OK;
+}
+
+template <template <class> class Fn, class... A>
+static int dispatch_storage(int storage, A... args) {
+  switch (storage) {
+    case PSL_F32: return Fn<StF32>::run(args...);
+    case PSL_F16DEV: return Fn<StF16>::run(args...);
+    case PSL_BF16DEV: return Fn<StBF16>::run(args...);
+  }
+  set_error("unknown storage");
+  return PSL_E_ARG;
+}
+
+template <class St> struct LocalUpdateFn { static int run(psl_grid* g) { return launch_local_update<St>(g); } };
+
+template <class St> struct FillEqFn {
+  static int run(psl_grid* g, const float* r, const float* a, const float* b, const float* c) {
+    const uint32_t n = ncells(g);
+    k_fill_eq<St><<<blocks_for(n), 256, 0, g->stream>>>((typename St::T*)g->f, shift_of(g), n, r, a, b, c);
+    PSL_LAUNCH_CHECK();
+    return PSL_OK;
+  }
+};
+
+template <class St> struct GetFFn {
+  static int run(psl_grid* g, float* d) {
+    const uint32_t n = ncells(g);
+    k_get_f<St><<<blocks_for(n), 256, 0, g->stream>>>((const typename St::T*)g->f, shift_of(g), n, d);
+    PSL_LAUNCH_CHECK();
+    return PSL_OK;
+  }
+};
+template <class St> struct SetFFn {
+  static int run(psl_grid* g, const float* d) {
+    const uint32_t n = ncells(g);
+    k_set_f<St><<<blocks_for(n), 256, 0, g->stream>>>((typename St::T*)g->f, shift_of(g), n, d);
+    PSL_LAUNCH_CHECK();
+    return PSL_OK;
+  }
+};
+template <class St> struct MacroFn {
+  static int run(psl_grid* g, float* r, float* a, float* b, float* c) {
+    const uint32_t n = ncells(g);
+    k_macro<St><<<blocks_for(n), 256, 0, g->stream>>>((const typename St::T*)g->f, shift_of(g), n, r, a, b, c);
+    PSL_LAUNCH_CHECK();
+    return PSL_OK;
+  }
+};
+template <class St> struct MomentumFn {
+  static int run(psl_grid* g, const int* lo, const int* hi, uint32_t nb) {
+    const psl_info& I = g->info;
+    k_momentum_exchange<St><<<nb, 256, 0, g->stream>>>((const typename St::T*)g->f, shift_of(g), ncells(g), g->flags,
+                                                       I.cellCountX, I.cellCountY, lo[0], lo[1], lo[2], hi[0] - lo[0],
+                                                       hi[1] - lo[1], hi[2] - lo[2], g->partials);
+    PSL_LAUNCH_CHECK();
+    return PSL_OK;
+  }
+};
+template <class St> struct MassFn {
+  static int run(psl_grid* g, uint32_t nb) {
+    k_mass<St><<<nb, 256, 0, g->stream>>>((const typename St::T*)g->f, shift_of(g), ncells(g), g->flags, g->partials);
+    PSL_LAUNCH_CHECK();
+    return PSL_OK;
+  }
+};
+
+static int periodic_copy(psl_grid* g, int axis) {
+  const psl_info& I = g->info;
+  const int na = axis == 0 ? I.cellCountY : I.cellCountX;
+  dim3 grid(blocks_for(na), I.cellCountZ);
+  if (g->storage == PSL_F32)
+    k_periodic<float><<<grid, 256, 0, g->stream>>>((float*)g->f, shift_of(g), ncells(g), I.cellCountX, I.cellCountY,
+                                                   I.cellCountZ, axis);
+  else
+    k_periodic<uint16_t><<<grid, 256, 0, g->stream>>>((uint16_t*)g->f, shift_of(g), ncells(g), I.cellCountX,
+                                                      I.cellCountY, I.cellCountZ, axis);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+}  // namespace psl
+
+using namespace psl;
+
+// ======================================================================= C ABI
+extern "C" {
+
+const char* psl_last_error(void) { return g_last_error.c_str(); }
+int psl_abi_version(void) { return PSL_ABI_VERSION; }
+
+int psl_device_count(int* count) {
+  PSL_CHECK_ARG(count, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *count = e == hipSuccess ? c : 0;
+  return PSL_OK;
+}
+
+int psl_grid_create(const psl_info* info, int storage, psl_grid** out) {
+  PSL_CHECK_ARG(info && out, "null argument");
+  PSL_CHECK_ARG(storage == PSL_F32 || storage == PSL_F16DEV || storage == PSL_BF16DEV, "unknown storage");
+  PSL_CHECK_ARG(info->cellCountX > 0 && info->cellCountY > 0 && info->cellCountZ > 0, "cell counts must be positive");
+  const uint64_t n = (uint64_t)info->cellCountX * info->cellCountY * info->cellCountZ;
+  PSL_CHECK_ARG(n < (1ull << 31), "cellCount must be below 2^31");
+  psl_grid* g = new psl_grid();
+  g->info = *info;
+  g->info.cellCount = (int)n;
+  g->storage = storage;
+  const size_t bytes = 27ull * n * storage_bytes(storage);
+  hipError_t e = hipMalloc(&g->f, bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc fArray: ") + hipGetErrorString(e));
+    delete g;
+    return PSL_E_NOMEM;
+  }
+  e = hipMalloc(&g->result, 4 * sizeof(double));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc result: ") + hipGetErrorString(e));
+    (void)hipFree(g->f);
+    delete g;
+    return PSL_E_NOMEM;
+  }
+  int rc = grid_rebuild_flags(g);
+  if (rc) {
+    psl_grid_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return PSL_OK;
+}
+
+int psl_grid_destroy(psl_grid* g) {
+  if (!g) return PSL_OK;
+  (void)hipDeviceSynchronize();
+  drop_profile_events(g);
+  if (g->f) (void)hipFree(g->f);
+  if (g->flags) (void)hipFree(g->flags);
+  if (g->bc_list) (void)hipFree(g->bc_list);
+  if (g->cs_cell) (void)hipFree(g->cs_cell);
+  if (g->partials) (void)hipFree(g->partials);
+  if (g->result) (void)hipFree(g->result);
+  delete g;
+  return PSL_OK;
+}
+
+int psl_grid_get_info(const psl_grid* g, psl_info* out) {
+  PSL_CHECK_ARG(g && out, "null argument");
+  *out = g->info;
+  return PSL_OK;
+}
+
+int psl_grid_set_info(psl_grid* g, const psl_info* info) {
+  PSL_CHECK_ARG(g && info, "null argument");
+  PSL_CHECK_ARG(info->cellCountX == g->info.cellCountX && info->cellCountY == g->info.cellCountY &&
+                    info->cellCountZ == g->info.cellCountZ,
+                "cell counts of an existing grid cannot change");
+  const bool sub_changed = memcmp(&info->iSubgridStart, &g->info.iSubgridStart, 6 * sizeof(int)) != 0;
+  const int n = g->info.cellCount;
+  g->info = *info;
+  g->info.cellCount = n;
+  if (sub_changed) return grid_rebuild_flags(g);
+  return PSL_OK;
+}
+
+int psl_grid_set_stream(psl_grid* g, void* s) {
+  PSL_CHECK_ARG(g, "null grid");
+  g->stream = (hipStream_t)s;
+  return PSL_OK;
+}
+
+int psl_grid_set_flags(psl_grid* g, const uint8_t* host_flags) {
+  PSL_CHECK_ARG(g, "null grid");
+  if (host_flags) g->host_flags.assign(host_flags, host_flags + g->info.cellCount);
+  else g->host_flags.clear();
+  return grid_rebuild_flags(g);
+}
+
+int psl_grid_get_flags(const psl_grid* g, uint8_t* host) {
+  PSL_CHECK_ARG(g && host, "null argument");
+  if (!g->flags) {
+    memset(host, 0, (size_t)g->info.cellCount);
+    return PSL_OK;
+  }
+  PSL_HIP_TRY(hipMemcpyAsync(host, g->flags, (size_t)g->info.cellCount, hipMemcpyDeviceToHost, g->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  return PSL_OK;
+}
+
+int psl_grid_set_bc_values(psl_grid* g, const float* v) {
+  PSL_CHECK_ARG(g && v, "null argument");
+  memcpy(g->bcvals.v, v, sizeof(g->bcvals.v));
+  return PSL_OK;
+}
+
+int psl_grid_set_smagorinsky(psl_grid* g, float cs, const float* per_cell) {
+  PSL_CHECK_ARG(g, "null grid");
+  g->cs = cs;
+  if (per_cell) {
+    if (!g->cs_cell) PSL_HIP_TRY(hipMalloc(&g->cs_cell, (size_t)g->info.cellCount * sizeof(float)));
+    PSL_HIP_TRY(hipMemcpy(g->cs_cell, per_cell, (size_t)g->info.cellCount * sizeof(float), hipMemcpyHostToDevice));
+  } else if (g->cs_cell) {
+    PSL_HIP_TRY(hipFree(g->cs_cell));
+    g->cs_cell = nullptr;
+  }
+  return PSL_OK;
+}
+
+int psl_grid_set_periodic_copies(psl_grid* g, int mask) {
+  PSL_CHECK_ARG(g, "null grid");
+  g->periodic_copies = mask & 3;
+  return PSL_OK;
+}
+
+int psl_fill_equilibrium(psl_grid* g, const float* rho, const float* ux, const float* uy, const float* uz) {
+  PSL_CHECK_ARG(g, "null grid");
+  const size_t n = (size_t)g->info.cellCount;
+  float* d[4] = {nullptr, nullptr, nullptr, nullptr};
+  const float* h[4] = {rho, ux, uy, uz};
+  int rc = PSL_OK;
+  for (int a = 0; a < 4 && rc == PSL_OK; ++a) {
+    if (!h[a]) continue;
+    if (hipMalloc(&d[a], n * sizeof(float)) != hipSuccess ||
+        hipMemcpy(d[a], h[a], n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("fill_equilibrium: upload failed");
+      rc = PSL_E_HIP;
+    }
+  }
+  if (rc == PSL_OK) rc = dispatch_storage<FillEqFn>(g->storage, g, (const float*)d[0], (const float*)d[1],
+                                                    (const float*)d[2], (const float*)d[3]);
+  if (hipStreamSynchronize(g->stream) != hipSuccess && rc == PSL_OK) {
+    set_error("fill_equilibrium: sync failed");
+    rc = PSL_E_HIP;
+  }
+  for (int a = 0; a < 4; ++a)
+    if (d[a]) (void)hipFree(d[a]);
+  return rc;
+}
+
+int psl_apply_streaming(psl_grid* g) {
+  PSL_CHECK_ARG(g, "null grid");
+  const int64_t N = g->info.cellCount;
+  const int64_t nx = g->info.cellCountX, nxy = nx * g->info.cellCountY;
+  for (int q = 0; q < 27; ++q) {
+    int64_t s = g->shift[q];
+    s -= CX[q];
+    s -= nx * CY[q];
+    s -= nxy * CZ[q];
+    // two wraps allow cellCountZ = 1 (include/applyStreaming.h:24-27)
+    if (s < 0) s += N;
+    if (s < 0) s += N;
+    if (s >= N) s -= N;
+    if (s >= N) s -= N;
+    g->shift[q] = (int)s;
+  }
+  return PSL_OK;
+}
+
+int psl_apply_local_cell_update(psl_grid* g) {
+  PSL_CHECK_ARG(g, "null grid");
+  return dispatch_storage<LocalUpdateFn>(g->storage, g);
+}
+
+int psl_apply_bc_periodic_x(psl_grid* g) {
+  PSL_CHECK_ARG(g, "null grid");
+  return periodic_copy(g, 0);
+}
+
+int psl_apply_bc_periodic_y(psl_grid* g) {
+  PSL_CHECK_ARG(g, "null grid");
+  return periodic_copy(g, 1);
+}
+
+int psl_get_shifter(const psl_grid* g, int* out) {
+  PSL_CHECK_ARG(g && out, "null argument");
+  memcpy(out, g->shift, sizeof(g->shift));
+  return PSL_OK;
+}
+
+int psl_set_shifter(psl_grid* g, const int* in) {
+  PSL_CHECK_ARG(g && in, "null argument");
+  for (int q = 0; q < 27; ++q) PSL_CHECK_ARG(in[q] >= 0 && in[q] < g->info.cellCount, "shift out of range");
+  memcpy(g->shift, in, sizeof(g->shift));
+  return PSL_OK;
+}
+
+int psl_get_f(psl_grid* g, float* host) {
+  PSL_CHECK_ARG(g && host, "null argument");
+  const size_t bytes = 27ull * g->info.cellCount * sizeof(float);
+  float* d = nullptr;
+  PSL_HIP_TRY(hipMalloc(&d, bytes));
+  int rc = dispatch_storage<GetFFn>(g->storage, g, d);
+  if (rc == PSL_OK && hipMemcpyAsync(host, d, bytes, hipMemcpyDeviceToHost, g->stream) != hipSuccess) rc = PSL_E_HIP;
+  if (hipStreamSynchronize(g->stream) != hipSuccess && rc == PSL_OK) rc = PSL_E_HIP;
+  (void)hipFree(d);
+  if (rc == PSL_E_HIP) set_error("psl_get_f: copy failed");
+  return rc;
+}
+
+int psl_set_f(psl_grid* g, const float* host) {
+  PSL_CHECK_ARG(g && host, "null argument");
+  const size_t bytes = 27ull * g->info.cellCount * sizeof(float);
+  float* d = nullptr;
+  PSL_HIP_TRY(hipMalloc(&d, bytes));
+  int rc = PSL_OK;
+  if (hipMemcpy(d, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("psl_set_f: upload failed");
+    rc = PSL_E_HIP;
+  }
+  if (rc == PSL_OK) rc = dispatch_storage<SetFFn>(g->storage, g, (const float*)d);
+  if (hipStreamSynchronize(g->stream) != hipSuccess && rc == PSL_OK) rc = PSL_E_HIP;
+  (void)hipFree(d);
+  return rc;
+}
+
+int psl_get_macroscopic(psl_grid* g, float* rho, float* ux, float* uy, float* uz) {
+  PSL_CHECK_ARG(g, "null grid");
+  const size_t n = (size_t)g->info.cellCount;
+  float* d = nullptr;
+  PSL_HIP_TRY(hipMalloc(&d, 4 * n * sizeof(float)));
+  int rc = dispatch_storage<MacroFn>(g->storage, g, d, d + n, d + 2 * n, d + 3 * n);
+  float* h[4] = {rho, ux, uy, uz};
+  for (int a = 0; a < 4 && rc == PSL_OK; ++a)
+    if (h[a] && hipMemcpyAsync(h[a], d + a * n, n * sizeof(float), hipMemcpyDeviceToHost, g->stream) != hipSuccess)
+      rc = PSL_E_HIP;
+  if (hipStreamSynchronize(g->stream) != hipSuccess && rc == PSL_OK) rc = PSL_E_HIP;
+  (void)hipFree(d);
+  if (rc == PSL_E_HIP) set_error("psl_get_macroscopic: copy failed");
+  return rc;
+}
+
+int psl_grid_device_f(psl_grid* g, void** dev_ptr, size_t* bytes) {
+  PSL_CHECK_ARG(g && dev_ptr, "null argument");
+  *dev_ptr = g->f;
+  if (bytes) *bytes = 27ull * g->info.cellCount * storage_bytes(g->storage);
+  return PSL_OK;
+}
+
+int psl_reduce_momentum_exchange(psl_grid* g, const int* lo3, const int* hi3, double* out3) {
+  PSL_CHECK_ARG(g && out3, "null argument");
+  const psl_info& I = g->info;
+  int lo[3] = {0, 0, 0}, hi[3] = {I.cellCountX, I.cellCountY, I.cellCountZ};
+  if (lo3) for (int a = 0; a < 3; ++a) lo[a] = std::max(0, lo3[a]);
+  if (hi3) {
+    hi[0] = std::min(I.cellCountX, hi3[0]);
+    hi[1] = std::min(I.cellCountY, hi3[1]);
+    hi[2] = std::min(I.cellCountZ, hi3[2]);
+  }
+  out3[0] = out3[1] = out3[2] = 0.0;
+  if (!g->flags || hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2]) return PSL_OK;
+  const uint64_t total = (uint64_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+  const uint32_t nb = blocks_for(total);
+  int rc = ensure_partials(g, 3ull * nb);
+  if (rc) return rc;
+  rc = dispatch_storage<MomentumFn>(g->storage, g, (const int*)lo, (const int*)hi, nb);
+  if (rc) return rc;
+  k_sum_partials<3><<<1, 256, 0, g->stream>>>(g->partials, nb, g->result);
+  PSL_LAUNCH_CHECK();
+  PSL_HIP_TRY(hipMemcpyAsync(out3, g->result, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  return PSL_OK;
+}
+
+int psl_reduce_mass(psl_grid* g, double* out) {
+  PSL_CHECK_ARG(g && out, "null argument");
+  const uint32_t nb = blocks_for(ncells(g));
+  int rc = ensure_partials(g, nb);
+  if (rc) return rc;
+  rc = dispatch_storage<MassFn>(g->storage, g, nb);
+  if (rc) return rc;
+  k_sum_partials<1><<<1, 256, 0, g->stream>>>(g->partials, nb, g->result);
+  PSL_LAUNCH_CHECK();
+  PSL_HIP_TRY(hipMemcpyAsync(out, g->result, sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  return PSL_OK;
+}
+
+int psl_device_synchronize(void) {
+  PSL_HIP_TRY(hipDeviceSynchronize());
+  return PSL_OK;
+}
+
+int psl_grid_profile_enable(psl_grid* g, int enable) {
+  PSL_CHECK_ARG(g, "null grid");
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  drop_profile_events(g);
+  g->profile = enable != 0;
+  return PSL_OK;
+}
+
+int psl_grid_profile_read(psl_grid* g, double* total_ms, int* launches) {
+  PSL_CHECK_ARG(g && total_ms, "null argument");
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  double t = 0.0;
+  for (auto& e : g->prof_events) {
+    float ms = 0.f;
+    PSL_HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+    t += ms;
+  }
+  *total_ms = t;
+  if (launches) *launches = (int)g->prof_events.size();
+  drop_profile_events(g);
+  return PSL_OK;
+}
+
+int psl_grid_cell_census(const psl_grid* g, int64_t* bulk, int64_t* boundary, int64_t* skipped) {
+  PSL_CHECK_ARG(g, "null grid");
+  if (bulk) *bulk = g->census[0];
+  if (boundary) *boundary = g->census[1];
+  if (skipped) *skipped = g->census[2];
+  return PSL_OK;
+}
+
+int psl_grid_apply_bounceback_mask(psl_grid* g, const psl_mask* m) {
+  PSL_CHECK_ARG(g && m, "null argument");
+  PSL_CHECK_ARG(m->n == g->info.cellCount, "mask size does not match the grid");
+  const uint32_t n = ncells(g);
+  // materialise host flags so that later set_info rebuilds keep the cells
+  if (g->host_flags.empty()) g->host_flags.assign(n, 0);
+  int rc = grid_rebuild_flags(g);
+  if (rc) return rc;
+  k_apply_bb_mask<<<blocks_for(n), 256, 0, g->stream>>>(g->flags, m->d, n);
+  PSL_LAUNCH_CHECK();
+  std::vector<uint8_t> hm(n);
+  PSL_HIP_TRY(hipMemcpyAsync(hm.data(), m->d, n, hipMemcpyDeviceToHost, g->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  for (uint32_t c = 0; c < n; ++c)
+    if (hm[c]) g->host_flags[c] = (uint8_t)((g->host_flags[c] & ~7) | KIND_BOUNCEBACK);
+  return grid_rebuild_flags(g);
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// Library-internal declarations shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/psl.h"
+#include "psl_math.h"
+
+namespace psl {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+
+#define PSL_HIP_TRY(expr)                                                                       \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess) {                                                                     \
+      ::psl::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                      \
+      return PSL_E_HIP;                                                                         \
+    }                                                                                           \
+  } while (0)
+
+#define PSL_CHECK_ARG(cond, msg)        \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::psl::set_error(msg);            \
+      return PSL_E_ARG;                 \
+    }                                   \
+  } while (0)
+
+#define PSL_LAUNCH_CHECK()                                                                      \
+  do {                                                                                          \
+    hipError_t _e = hipGetLastError();                                                          \
+    if (_e != hipSuccess) {                                                                     \
+      ::psl::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));                 \
+      return PSL_E_HIP;                                                                         \
+    }                                                                                           \
+  } while (0)
+
+// ------------------------------------------------------------- storage types
+// FP32 stores f; the 16-bit storages hold d = f - w as raw 16-bit words.
+struct StF32 {
+  typedef float T;
+  static constexpr bool W_OFF = false;
+  __device__ __forceinline__ static float load(T x) { return x; }
+  __device__ __forceinline__ static T store(float v) { return v; }
+};
+struct StF16 {
+  typedef uint16_t T;
+  static constexpr bool W_OFF = true;
+  __device__ __forceinline__ static float load(T x) { return __half2float(__ushort_as_half(x)); }
+  __device__ __forceinline__ static T store(float v) { return __half_as_ushort(__float2half_rn(v)); }
+};
+struct StBF16 {
+  typedef uint16_t T;
+  static constexpr bool W_OFF = true;
+  __device__ __forceinline__ static float load(T x) { return __uint_as_float((uint32_t)x << 16); }
+  __device__ __forceinline__ static T store(float v) { return __builtin_bit_cast(uint16_t, (__bf16)v); }
+};
+
+inline size_t storage_bytes(int storage) { return storage == PSL_F32 ? 4 : 2; }
+
+// 27 shift offsets passed by value: they live in SGPRs inside the kernels.
+struct Shift27 {
+  int s[27];
+};
+
+struct BcTable {
+  float v[16];  // 4 slots x (rho, ux, uy, uz)
+};
+
+}  // namespace psl
+
+// ------------------------------------------------------------------ handles
+struct psl_grid {
+  psl_info info;
+  int storage = PSL_F32;
+  void* f = nullptr;              // 27 * N storage elements, q-major
+  int shift[27] = {0};            // authoritative shifter (host)
+  uint8_t* flags = nullptr;       // device, N bytes, or null when all fluid
+  std::vector<uint8_t> host_flags;  // user flags before the covered overlay (empty: all fluid)
+  uint32_t* bc_list = nullptr;    // device list of box-face boundary cells
+  uint32_t n_bc = 0;
+  psl::BcTable bcvals{};
+  float cs = 0.f;
+  float* cs_cell = nullptr;       // device, N floats, or null
+  int periodic_copies = 0;        // bit 0: x, bit 1: y
+  hipStream_t stream = nullptr;
+  double* partials = nullptr;     // reduction scratch
+  size_t n_partials = 0;
+  double* result = nullptr;       // reduction result (device, 4 doubles)
+  bool profile = false;           // bracket bulk launches with events
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+  int64_t census[3] = {0, 0, 0};  // bulk, boundary-list, skipped cells
+};
+
+struct psl_mask {
+  uint8_t* d = nullptr;
+  int64_t n = 0;
+};
+
+struct psl_stl {
+  std::vector<float> tri;  // 9 floats per triangle (host copy)
+  float* d_tri = nullptr;  // device copy
+  int count = 0;
+  float xmin, ymin, zmin, xmax, ymax, zmax;
+  float ox = 0.f, oy = 0.f, oz = 0.f;
+};
+
+namespace psl {
+// implemented in psl_grid.hip
+int grid_rebuild_flags(psl_grid* g);
+int ensure_partials(psl_grid* g, size_t n);
+}  // namespace psl

@@ -1,0 +1,218 @@
+"""GPU parity of the fused periodic-shift step against the CPU oracle.
+
+Every test drives libpsl_amd.so through its C ABI (periodicshiftlbm_amd) and
+the oracle (oracle/libpsl_oracle.so) through the same sequence of reference
+operations on the same inputs, then compares distributions in logical order.
+
+Tolerance (FP32 storage): |f_gpu - f_oracle| <= 1e-5 * |f_oracle| element-wise.
+The GPU collision is pure FP32 with a restructured operation order; the
+FP32 oracle mirrors the reference's own mixed FP32/FP64 expressions, so the
+two agree to accumulated FP32 roundoff.
+"""
+import numpy as np
+import pytest
+
+from periodicshiftlbm_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _rel_err(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)) / np.maximum(np.abs(b), 1e-3)))
+
+
+def _pair(P, O, shape, nu, cs=0.0, flags=None, bcvals=None, storage="f32", oracle_dtype=np.float32):
+    nx, ny, nz = shape
+    info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=nu)
+    g = P.GridStruct(info, storage)
+    og = O.Grid(nx, ny, nz, dtype=oracle_dtype, nu=nu, cs=cs, flags=flags, bcvals=bcvals)
+    g.set_flags(flags)
+    if bcvals is not None:
+        g.set_given_values(bcvals)
+    g.set_smagorinsky(cs)
+    return g, og
+
+
+def _init_same(g, og, rho, ux, uy, uz):
+    og.fill_equilibrium(rho, ux, uy, uz)
+    g.set_f(og.logical_f().astype(np.float32))
+
+
+def test_collision_matches_oracle_per_cell(psl, oracle_mod):
+    """One update with zero shift = collision of every cell (plain and LES)."""
+    P, O = psl, oracle_mod
+    rng = np.random.default_rng(1)
+    shape = (16, 8, 8)
+    n = 16 * 8 * 8
+    for cs in (0.0, 0.17):
+        g, og = _pair(P, O, shape, nu=0.005, cs=cs)
+        rho = rng.uniform(0.95, 1.05, n)
+        u = rng.uniform(-0.08, 0.08, (3, n))
+        _init_same(g, og, rho, *u)
+        # add non-equilibrium parts so the LES norm is non-trivial
+        f0 = og.logical_f()
+        f0 = (f0 * (1 + rng.normal(0, 0.01, f0.shape))).astype(np.float32)
+        og.f[:] = f0
+        g.set_f(f0)
+        print(f"[collision] cs={cs} shape={shape}", flush=True)
+        P.applyLocalCellUpdate(g)
+        og.local_update()
+        err = _rel_err(g.get_f(), og.logical_f())
+        print(f"  max rel err {err:.3e}")
+        assert err < RTOL
+
+
+def test_streaming_shift_matches_oracle(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    for shape in [(5, 4, 3), (7, 1, 1), (64, 64, 1)]:
+        info = P.InfoStruct(cellCountX=shape[0], cellCountY=shape[1], cellCountZ=shape[2])
+        g = P.GridStruct(info)
+        s = np.zeros(27, np.int32)
+        for _ in range(13):
+            P.applyStreaming(g)
+            s = O.stream_shift(s, *shape)
+            assert np.array_equal(g.shifter, s)
+
+
+def test_tgv_periodic_matches_oracle(psl, oracle_mod):
+    """Taylor-Green 64^3 periodic box (65 x 65 x 64 with copies), 40 steps."""
+    P, O = psl, oracle_mod
+    n = 64
+    shape = cases.tgv_shape(n)
+    g, og = _pair(P, O, shape, nu=0.005)
+    rho, ux, uy, uz = cases.tgv_fields(n)
+    _init_same(g, og, rho, ux, uy, uz)
+    g.set_periodic_copies(x=True, y=True)
+    steps = 40
+    print(f"[tgv] shape={shape} steps={steps}", flush=True)
+    P.updateGrid([g], steps)
+    for _ in range(steps):
+        og.step(periodic_x=True, periodic_y=True)
+    fg, fo = g.get_f(), og.logical_f()
+    err = _rel_err(fg, fo)
+    print(f"  max rel err f {err:.3e}")
+    assert err < RTOL
+    r, a, b, c = g.macroscopic()
+    ke_g, en_g = cases.kinetic_energy_enstrophy(r, a, b, c, n)
+    ro = O.rho_u(np.ascontiguousarray(fo.T.astype(np.float64)))
+    ke_o, en_o = cases.kinetic_energy_enstrophy(ro[:, 0], ro[:, 1], ro[:, 2], ro[:, 3], n)
+    print(f"  KE {ke_g:.9e} vs {ke_o:.9e}; enstrophy {en_g:.9e} vs {en_o:.9e}")
+    assert abs(ke_g - ke_o) <= 1e-5 * abs(ke_o)
+    assert abs(en_g - en_o) <= 1e-5 * abs(en_o)
+
+
+def _channel(shape, radius, center):
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    solid = ((i - center[0]) ** 2 + (j - center[1]) ** 2 + (k - center[2]) ** 2) <= radius * radius
+    return cases.channel_flags(shape, solid.reshape(-1))
+
+
+def test_sphere_channel_matches_oracle(psl, oracle_mod):
+    """Inlet / outlet moment-based BCs, bounceback sphere, Smagorinsky, periodic y copies."""
+    P, O = psl, oracle_mod
+    shape = (48, 24, 24)
+    flags = _channel(shape, 5, (12, 12, 12))
+    bc = cases.channel_given_values(0.04)
+    g, og = _pair(P, O, shape, nu=0.0048, cs=0.17, flags=flags, bcvals=bc)
+    n = np.prod(shape)
+    _init_same(g, og, np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n))
+    g.set_periodic_copies(y=True)
+    steps = 30
+    print(f"[sphere-channel] shape={shape} steps={steps}", flush=True)
+    P.updateGrid([g], steps)
+    for _ in range(steps):
+        og.step(periodic_y=True)
+    err = _rel_err(g.get_f(), og.logical_f())
+    print(f"  max rel err {err:.3e}")
+    assert err < RTOL
+    lo, hi = [6, 6, 6], [19, 19, 19]
+    fg = g.momentum_exchange(lo, hi)
+    fo = og.momentum_exchange(lo, hi)
+    print(f"  momentum exchange gpu {fg} oracle {fo}")
+    assert np.allclose(fg, fo, rtol=1e-4, atol=1e-9)
+
+
+def test_all_boundary_normals_match_oracle(psl, oracle_mod):
+    """Every box face, edge and corner with restore/MBBC of all four kinds."""
+    P, O = psl, oracle_mod
+    rng = np.random.default_rng(7)
+    shape = (10, 9, 8)
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    face = (i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)
+    kinds = rng.choice([cases.GIVEN_RHO, cases.GIVEN_U, cases.GIVEN_NONE, cases.GIVEN_RHOU], size=face.shape)
+    slots = rng.integers(0, 4, size=face.shape)
+    flags = np.where(face, kinds | (slots << 6), cases.FLUID).astype(np.uint8).reshape(-1)
+    bc = np.array([[1.0, 0.02, 0.0, 0.0], [1.01, 0.0, 0.01, -0.01], [0.99, -0.01, 0.0, 0.02], [1.0, 0.0, 0.0, 0.0]],
+                  np.float32)
+    g, og = _pair(P, O, shape, nu=0.02, cs=0.1, flags=flags, bcvals=bc)
+    n = np.prod(shape)
+    _init_same(g, og, np.ones(n), np.full(n, 0.01), np.zeros(n), np.zeros(n))
+    steps = 8
+    print(f"[all-normals] shape={shape} steps={steps}", flush=True)
+    for _ in range(steps):
+        P.applyStreaming(g)
+        P.applyLocalCellUpdate(g)
+        og.step()
+    err = _rel_err(g.get_f(), og.logical_f())
+    print(f"  max rel err {err:.3e}")
+    assert err < RTOL
+
+
+def test_periodic_copies_match_oracle(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    rng = np.random.default_rng(3)
+    shape = (9, 7, 5)
+    g, og = _pair(P, O, shape, nu=0.01)
+    f0 = rng.uniform(0.0, 0.3, (27, np.prod(shape))).astype(np.float32)
+    og.f[:] = f0
+    g.set_f(f0)
+    for _ in range(3):
+        P.applyStreaming(g)
+        og.stream()
+        P.applyBCPeriodicX(g)
+        og.periodic(0)
+        P.applyBCPeriodicY(g)
+        og.periodic(1)
+    assert np.array_equal(g.get_f(), og.logical_f())
+
+
+def test_mass_reduction_fp64(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    n = 32
+    shape = cases.tgv_shape(n)
+    g, og = _pair(P, O, shape, nu=0.01)
+    rho, ux, uy, uz = cases.tgv_fields(n)
+    _init_same(g, og, rho, ux, uy, uz)
+    g.set_periodic_copies(x=True, y=True)
+    P.updateGrid([g], 5)
+    m_gpu = g.mass()
+    f = g.get_f().astype(np.float64)
+    assert abs(m_gpu - float(f.sum())) <= 1e-9 * abs(m_gpu)
+
+
+@pytest.mark.parametrize("storage,bound", [("f16", 2e-4), ("bf16", 2e-3)])
+def test_reduced_storage_error_vs_fp64(psl, oracle_mod, storage, bound):
+    """f - w stored in FP16 / BF16, collided in FP32: error against the FP64 oracle."""
+    P, O = psl, oracle_mod
+    n = 32
+    shape = cases.tgv_shape(n)
+    g, og = _pair(P, O, shape, nu=0.01, storage=storage, oracle_dtype=np.float64)
+    rho, ux, uy, uz = cases.tgv_fields(n)
+    og.fill_equilibrium(rho, ux, uy, uz)
+    g.set_f(og.logical_f().astype(np.float32))
+    g.set_periodic_copies(x=True, y=True)
+    steps = 40
+    print(f"[{storage}] shape={shape} steps={steps}", flush=True)
+    P.updateGrid([g], steps)
+    for _ in range(steps):
+        og.step(periodic_x=True, periodic_y=True)
+    err = float(np.max(np.abs(g.get_f().astype(np.float64) - og.logical_f())))
+    r, a, b, c = g.macroscopic()
+    ro = O.rho_u(np.ascontiguousarray(og.logical_f().T))
+    uerr = float(np.max(np.abs(a - ro[:, 1])))
+    print(f"  max |f - f64| = {err:.3e}, max |ux - ux64| = {uerr:.3e}")
+    assert err < bound
