@@ -252,15 +252,15 @@ def emit_device(data) -> str:
         for i in range(4):
             L.append(f"  const T q{i} = {_lincomb(list(d['rhou_full'].row(i)), fk)};")
         L.append("  rho = q0;")
-        L.append("  const T inv_rho = T(1) / rho;")
-        L.append("  ux = q1 * inv_rho; uy = q2 * inv_rho; uz = q3 * inv_rho;")
+        # true divisions, as in the reference: the restored velocities come from
+        # cancelling sums and a reciprocal would add a rounding step
+        L.append("  ux = q1 / rho; uy = q2 / rho; uz = q3 / rho;")
         L.append("}")
         # ---- restore u given rho
         L.append("template <typename T> PSL_HD PSL_INLINE void bc_restore_u_%d(const T (&f)[27], T rho, T& ux, T& uy, T& uz)" % code)
         L.append("{")
-        L.append("  const T inv_rho = T(1) / rho;")
         for i in range(3):
-            L.append(f"  const T s{i} = ({_lincomb(list(d['u_scmfk'].row(i)), fk)}) * inv_rho - {_c(d['u_scq'][i])};")
+            L.append(f"  const T s{i} = ({_lincomb(list(d['u_scmfk'].row(i)), fk)}) / rho - {_c(d['u_scq'][i])};")
         for a, name in enumerate(["ux", "uy", "uz"]):
             L.append(f"  {name} = {_lincomb(list(d['u_inv'].row(a)), ['s0', 's1', 's2'])};")
         L.append("}")

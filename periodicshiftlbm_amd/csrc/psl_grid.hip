@@ -28,84 +28,19 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 // --------------------------------------------------------------- addressing
-// Per-wave addressing of the 27 shifted streams.  For population q the
-// lanes of a wave either all stay below the wrap threshold N - shift[q], all
-// lie above it, or (for at most one wave per q) straddle it.  In the first two
-// cases the base pointer is wave-uniform and the per-lane offset cell*size is
-// shared by all 27 populations; the rare straddling waves take a per-lane path.
-template <class St>
-struct Streams {
-  typedef typename St::T T;
-  const T* f;
-  uint32_t n;
-  uint32_t w_first, w_last;  // first / last valid cell of this wave (uniform)
-  bool straddle;
-
-  __device__ __forceinline__ Streams(const T* f_, uint32_t n_, uint32_t wave_first, const Shift27& sh) : f(f_), n(n_) {
-    w_first = wave_first;
-    w_last = min(wave_first + 63u, n_ - 1u);
-    bool s = false;
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const uint32_t thr = n_ - (uint32_t)sh.s[q];  // cells >= thr wrap
-      s |= (w_first < thr) && (w_last >= thr);
-    }
-    straddle = s;
-  }
-  // wave-uniform element offset of population q for the fast path
-  __device__ __forceinline__ int64_t uniform_base(int q, const Shift27& sh) const {
-    const uint32_t thr = n - (uint32_t)sh.s[q];
-    const int64_t off = (int64_t)q * n + sh.s[q];
-    return w_first >= thr ? off - (int64_t)n : off;
-  }
-  __device__ __forceinline__ int64_t lane_index(int q, uint32_t cell, const Shift27& sh) const {
-    uint32_t s = cell + (uint32_t)sh.s[q];
-    if (s >= n) s -= n;
-    return (int64_t)q * n + s;
-  }
-};
-
-template <class St>
-__device__ __forceinline__ void load27(const Streams<St>& S, const typename St::T* f, uint32_t cell, const Shift27& sh,
-                                       float (&v)[27]) {
-  typedef typename St::T T;
-  if (!S.straddle) {
-    const uint32_t off = cell * (uint32_t)sizeof(T);
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const char* base = (const char*)(f + S.uniform_base(q, sh));
-      v[q] = St::load(*(const T*)(base + off));
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = St::load(f[S.lane_index(q, cell, sh)]);
-  }
-}
-
-template <class St>
-__device__ __forceinline__ void store27(const Streams<St>& S, typename St::T* f, uint32_t cell, const Shift27& sh,
-                                        const float (&v)[27]) {
-  typedef typename St::T T;
-  if (!S.straddle) {
-    const uint32_t off = cell * (uint32_t)sizeof(T);
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      char* base = (char*)(f + S.uniform_base(q, sh));
-      *(T*)(base + off) = St::store(v[q]);
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 27; ++q) f[S.lane_index(q, cell, sh)] = St::store(v[q]);
-  }
-}
-
-__device__ __forceinline__ uint32_t wave_first_cell(uint32_t block_first) {
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  return block_first + wave * 64u;
-}
-
+// Population q of cell c lives at element (c + shift[q]) mod N of row q.  For
+// each q the host passes two row bases, lo[q] = row + shift[q] (cells below
+// the wrap threshold thr[q] = N - shift[q]) and hi[q] = lo[q] - N (cells at
+// or above it), so that the element is base + c either way.  A wave whose 64
+// cells lie on one side of every threshold picks each base with one scalar
+// compare/select and addresses all 27 populations with the same per-lane byte
+// offset c * sizeof(T) (SGPR base + VGPR offset loads, no per-q vector
+// arithmetic).  At most one wave per q straddles a threshold; those waves
+// select the base per lane.
 struct StepParams {
-  Shift27 sh;
+  const void* lo[27];
+  const void* hi[27];
+  uint32_t thr[27];
   uint32_t n;
   float omega;  // 1/tau, used when cs == 0 and there is no per-cell cs
   float tau;
@@ -114,20 +49,81 @@ struct StepParams {
   const uint8_t* flags;
 };
 
+__device__ __forceinline__ uint32_t wave_first_cell(uint32_t block_first) {
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  return block_first + wave * 64u;
+}
+
+// True when some population's wrap threshold falls inside this wave: one
+// lane per population tests it and a ballot makes the answer wave-uniform.
+__device__ __forceinline__ bool wave_straddles(const StepParams& p, uint32_t w_first) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t w_last = min(w_first + 63u, p.n - 1u);
+  bool s = false;
+  if (lane < 27u) {
+    const uint32_t t = p.thr[lane];
+    s = (w_first < t) && (w_last >= t);
+  }
+  return __ballot(s) != 0;
+}
+
+template <class St>
+__device__ __forceinline__ void load27(const StepParams& p, uint32_t w_first, bool straddle, uint32_t cell,
+                                       float (&v)[27]) {
+  typedef typename St::T T;
+  if (!straddle) {
+    const uint32_t off = cell * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const char* base = (const char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
+      v[q] = St::load(*(const T*)(base + off));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+      v[q] = St::load(base[cell]);
+    }
+  }
+}
+
+template <class St>
+__device__ __forceinline__ void store27(const StepParams& p, uint32_t w_first, bool straddle, uint32_t cell,
+                                        const float (&v)[27]) {
+  typedef typename St::T T;
+  if (!straddle) {
+    const uint32_t off = cell * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      char* base = (char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
+      *(T*)(base + off) = St::store(v[q]);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+      base[cell] = St::store(v[q]);
+    }
+  }
+}
+
 // ------------------------------------------------------------- bulk update
 // Fluid and bounceback cells: include/applyLocalCellUpdate.h:1-74 minus the
 // box-face branch (46-67), which k_ps_boundary handles.
 template <class St, bool FLAGS, bool LES>
-__global__ __launch_bounds__(256) void k_ps_bulk(typename St::T* __restrict__ f, StepParams p) {
+__global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
   const uint32_t block_first = blockIdx.x * 256u;
   const uint32_t cell = block_first + threadIdx.x;
+  const uint32_t w_first = wave_first_cell(block_first);
+  if (w_first >= p.n) return;  // whole wave past the grid (wave-uniform)
+  // all 64 lanes take part in the ballot, so every population is tested
+  const bool straddle = wave_straddles(p, w_first);
   if (cell >= p.n) return;
   int kind = KIND_FLUID;
   if (FLAGS) kind = p.flags[cell] & 7;
-  const Streams<St> S(f, p.n, wave_first_cell(block_first), p.sh);
   if (kind > KIND_BOUNCEBACK) return;
   float v[27];
-  load27<St>(S, f, cell, p.sh, v);
+  load27<St>(p, w_first, straddle, cell, v);
   if (kind == KIND_BOUNCEBACK) {
     // the swap commutes with "f - w" (opposite directions share a weight)
     bounceback(v);
@@ -137,15 +133,15 @@ __global__ __launch_bounds__(256) void k_ps_bulk(typename St::T* __restrict__ f,
     if (LES && p.cs_cell) cs = p.cs_cell[cell];
     collide<St::W_OFF>(v, rx, LES ? cs : 0.f);
   }
-  store27<St>(S, f, cell, p.sh, v);
+  store27<St>(p, w_first, straddle, cell, v);
 }
 
 // ---------------------------------------------------------- boundary update
 // include/applyLocalCellUpdate.h:40-69 for listed box-face cells.
 template <class St>
-__global__ __launch_bounds__(256) void k_ps_boundary(typename St::T* __restrict__ f, StepParams p,
-                                                     const uint32_t* __restrict__ list, uint32_t nlist, int nx,
-                                                     int ny, int nz, BcTable bc) {
+__global__ __launch_bounds__(256) void k_ps_boundary(StepParams p, const uint32_t* __restrict__ list, uint32_t nlist,
+                                                     int nx, int ny, int nz, BcTable bc) {
+  typedef typename St::T T;
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   if (t >= nlist) return;
   const uint32_t cell = list[t];
@@ -156,19 +152,15 @@ __global__ __launch_bounds__(256) void k_ps_boundary(typename St::T* __restrict_
   const uint32_t rem = cell - (uint32_t)k * nxy;
   const int j = (int)(rem / (uint32_t)nx);
   const int i = (int)(rem - (uint32_t)j * (uint32_t)nx);
-  auto idx = [&](int q) -> int64_t {
-    uint32_t s = cell + (uint32_t)p.sh.s[q];
-    if (s >= p.n) s -= p.n;
-    return (int64_t)q * p.n + s;
-  };
   float v[27];
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
-    v[q] = St::load(f[idx(q)]);
+    const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+    v[q] = St::load(base[cell]);
     if (St::W_OFF) v[q] += weight(q);
   }
   const int code = outer_normal_code(i, j, k, nx, ny, nz, fl);
-  // given (rho, u) of slot 0..3, selected without dynamic indexing (keeps the table in SGPRs)
+  // given (rho, u) of slot 0..3, selected without dynamic indexing
   const int slot = (fl >> 6) & 3;
   float g[4];
 #pragma unroll
@@ -180,8 +172,9 @@ __global__ __launch_bounds__(256) void k_ps_boundary(typename St::T* __restrict_
   collide<false>(v, rx, cs);
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
+    T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
     const float o = St::W_OFF ? v[q] - weight(q) : v[q];
-    f[idx(q)] = St::store(o);
+    base[cell] = St::store(o);
   }
 }
 
@@ -495,8 +488,14 @@ template <class St>
 static int launch_local_update(psl_grid* g) {
   typedef typename St::T T;
   StepParams p;
-  p.sh = shift_of(g);
   p.n = ncells(g);
+  T* f = (T*)g->f;
+  for (int q = 0; q < 27; ++q) {
+    T* row = f + (size_t)q * p.n + g->shift[q];
+    p.lo[q] = row;
+    p.hi[q] = row - p.n;
+    p.thr[q] = p.n - (uint32_t)g->shift[q];
+  }
   const float tau = 3.f * g->info.nu + 0.5f;
   p.tau = tau;
   p.omega = 1.f / tau;
@@ -504,7 +503,6 @@ static int launch_local_update(psl_grid* g) {
   p.cs_cell = g->cs_cell;
   p.flags = g->flags;
   const bool les = g->cs != 0.f || g->cs_cell != nullptr;
-  T* f = (T*)g->f;
   const uint32_t nb = blocks_for(p.n);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (g->profile) {
@@ -513,11 +511,11 @@ static int launch_local_update(psl_grid* g) {
     PSL_HIP_TRY(hipEventRecord(ev0, g->stream));
   }
   if (g->flags) {
-    if (les) k_ps_bulk<St, true, true><<<nb, 256, 0, g->stream>>>(f, p);
-    else k_ps_bulk<St, true, false><<<nb, 256, 0, g->stream>>>(f, p);
+    if (les) k_ps_bulk<St, true, true><<<nb, 256, 0, g->stream>>>(p);
+    else k_ps_bulk<St, true, false><<<nb, 256, 0, g->stream>>>(p);
   } else {
-    if (les) k_ps_bulk<St, false, true><<<nb, 256, 0, g->stream>>>(f, p);
-    else k_ps_bulk<St, false, false><<<nb, 256, 0, g->stream>>>(f, p);
+    if (les) k_ps_bulk<St, false, true><<<nb, 256, 0, g->stream>>>(p);
+    else k_ps_bulk<St, false, false><<<nb, 256, 0, g->stream>>>(p);
   }
   PSL_LAUNCH_CHECK();
   if (g->profile) {
@@ -525,7 +523,7 @@ static int launch_local_update(psl_grid* g) {
     g->prof_events.emplace_back(ev0, ev1);
   }
   if (g->n_bc) {
-    k_ps_boundary<St><<<blocks_for(g->n_bc), 256, 0, g->stream>>>(f, p, g->bc_list, g->n_bc, g->info.cellCountX,
+    k_ps_boundary<St><<<blocks_for(g->n_bc), 256, 0, g->stream>>>(p, g->bc_list, g->n_bc, g->info.cellCountX,
                                                                   g->info.cellCountY, g->info.cellCountZ, g->bcvals);
     PSL_LAUNCH_CHECK();
   }

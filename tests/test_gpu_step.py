@@ -132,7 +132,9 @@ def test_sphere_channel_matches_oracle(psl, oracle_mod):
     fg = g.momentum_exchange(lo, hi)
     fo = og.momentum_exchange(lo, hi)
     print(f"  momentum exchange gpu {fg} oracle {fo}")
-    assert np.allclose(fg, fo, rtol=1e-4, atol=1e-9)
+    # components are compared relative to the force magnitude: the transverse
+    # components are sums of cancelling per-cell terms
+    assert np.all(np.abs(fg - fo) <= 1e-5 * np.max(np.abs(fo)))
 
 
 def test_all_boundary_normals_match_oracle(psl, oracle_mod):
@@ -160,6 +162,35 @@ def test_all_boundary_normals_match_oracle(psl, oracle_mod):
     err = _rel_err(g.get_f(), og.logical_f())
     print(f"  max rel err {err:.3e}")
     assert err < RTOL
+
+
+@pytest.mark.parametrize("kind", ["GIVEN_RHO", "GIVEN_U", "GIVEN_NONE", "GIVEN_RHOU"])
+def test_boundary_cells_single_update(psl, oracle_mod, kind):
+    """One update with zero shift: every box-face cell (all 26 outer normals)
+    restores / imposes its moments and collides in place; compared per cell."""
+    P, O = psl, oracle_mod
+    rng = np.random.default_rng(11)
+    shape = (6, 6, 6)
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    face = ((i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)).reshape(-1)
+    kv = getattr(cases, kind)
+    flags = np.where(face, kv | (1 << 6), cases.FLUID).astype(np.uint8)
+    bc = np.array([[1.0, 0.0, 0.0, 0.0], [1.01, 0.02, -0.01, 0.015], [1.0, 0, 0, 0], [1.0, 0, 0, 0]], np.float32)
+    g, og = _pair(P, O, shape, nu=0.02, cs=0.1, flags=flags, bcvals=bc)
+    n = np.prod(shape)
+    _init_same(g, og, rng.uniform(0.98, 1.02, n), *rng.uniform(-0.03, 0.03, (3, n)))
+    P.applyLocalCellUpdate(g)
+    og.local_update()
+    a, b = g.get_f(), og.logical_f()
+    e = np.max(np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-3), axis=0)
+    bad = np.nonzero(e > RTOL)[0]
+    codes = sorted({int((np.sign(i.reshape(-1)[c] - (nx - 1) / 2) * (i.reshape(-1)[c] in (0, nx - 1)) + 5) * 100
+                        + (np.sign(j.reshape(-1)[c] - (ny - 1) / 2) * (j.reshape(-1)[c] in (0, ny - 1)) + 5) * 10
+                        + (np.sign(k.reshape(-1)[c] - (nz - 1) / 2) * (k.reshape(-1)[c] in (0, nz - 1)) + 5))
+                    for c in bad})
+    print(f"  [{kind}] max rel err {e.max():.3e}; failing normal codes {codes}")
+    assert e.max() < RTOL
 
 
 def test_periodic_copies_match_oracle(psl, oracle_mod):
