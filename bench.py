@@ -104,6 +104,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f32,f16,bf16")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (10 times) for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -114,6 +116,14 @@ def main():
     from periodicshiftlbm_amd import cases
 
     P.require_device()
+    if args.copy_calibration_mb:
+        nbytes = args.copy_calibration_mb << 20
+        a, b = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
+        for _ in range(10):
+            P.device_copy(b, a, nbytes)
+        P.synchronize()
+        log(f"[bench] copied {nbytes} bytes 10 times")
+        return
     shape = (256, 128, 128)
     ncell = int(np.prod(shape))
     results = {}

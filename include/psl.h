@@ -149,8 +149,38 @@ int psl_device_synchronize(void);
  * time in ms and the launch count since enabling (this synchronises). */
 int psl_grid_profile_enable(psl_grid* grid, int enable);
 int psl_grid_profile_read(psl_grid* grid, double* total_ms, int* launches);
+/* Plain device copy of `bytes` bytes (16-byte vector loads/stores): a kernel of
+ * known traffic, used to calibrate hardware byte counters against the step kernels. */
+int psl_device_copy(void* dst, const void* src, size_t bytes);
+/* Device buffer helpers for callers without a device allocator. */
+int psl_device_alloc(size_t bytes, void** out);
+int psl_device_free(void* ptr);
 /* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
 int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
+
+/* --------------------------------------------- esotwist (DIAD) grids */
+/* DIADGridStruct (include/types.h:166-171) from host cell tables: ijk3 = 3 arrays of n
+ * (i, j, k), nbr7 = 7 arrays of n neighbour cell indices (+i, +j, +k, +ij, +ik, +jk, +ijk). */
+typedef struct psl_diad psl_diad;
+int psl_diad_create(const psl_info* info, int storage, int64_t n, const int* ijk3, const int* nbr7, psl_diad** out);
+int psl_diad_destroy(psl_diad* grid);
+int psl_diad_set_flags(psl_diad* grid, const uint8_t* host_flags);
+int psl_diad_set_bc_values(psl_diad* grid, const float* values16);
+int psl_diad_set_smagorinsky(psl_diad* grid, float cs);
+int psl_diad_set_stream(psl_diad* grid, void* hip_stream);
+/* fillEquilibriumFromFunction(DIADGridStruct&) (include/fillEquilibrium.h:25-70). */
+int psl_diad_fill_equilibrium(psl_diad* grid, const float* rho, const float* ux, const float* uy, const float* uz);
+/* applyStreaming(DIADGridStruct&) (include/applyStreaming.h:44-47): flips the parity. */
+int psl_diad_apply_streaming(psl_diad* grid);
+/* applyLocalCellUpdate(DIADGridStruct&) (include/applyLocalCellUpdate.h:77-171). */
+int psl_diad_apply_local_cell_update(psl_diad* grid);
+int psl_diad_get_flipper(const psl_diad* grid, int* out);
+/* Post-collision populations per cell through the write maps of the current parity
+ * (getEsotwistWriteIndex, include/applyStreaming.h:138-192): host[q*n + cell]. */
+int psl_diad_get_f(psl_diad* grid, float* host);
+int psl_diad_set_f(psl_diad* grid, const float* host);
+/* Cell tables of a dense nx*ny*nz box; periodic_mask bits 0/1/2 wrap x/y/z. */
+int psl_diad_dense_box(int nx, int ny, int nz, int periodic_mask, int* ijk3, int* nbr7);
 
 /* ------------------------------------------------------------- geometry */
 /* readSTL (include/STLFunctions.h:3-89). */

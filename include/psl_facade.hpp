@@ -1,0 +1,200 @@
+// psl_facade.hpp -- header-only C++ facade with the reference's spellings over
+// the C ABI of include/psl.h.
+//
+// A host driver written against the reference's include/ API
+// (flowAroundSphere/flowAroundSphereMainSingle.cu) keeps its call sequence:
+//
+//   GridStruct grid(info);                 // include/types.h:77
+//   grid.setMarkers(getMarkers);           // getMarkers callback, evaluated on the host
+//   grid.setGivenRhoUxUyUz(getGiven);      // getGivenRhoUxUyUz callback
+//   fillEquilibriumFromFunction(grid, getInitialRhoUxUyUz);
+//   for (...) { applyStreaming(grid); applyLocalCellUpdate(grid); }
+//
+// The per-cell callbacks of the reference are __cuda_callable__ functions
+// compiled into its kernels.  Here they are ordinary host functions evaluated
+// once per cell into the flag bytes / value table that the HIP kernels read;
+// call setMarkers again whenever their inputs change.
+#pragma once
+
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "psl.h"
+
+namespace psl_facade {
+
+inline void check(int rc) {
+  if (rc != PSL_OK) throw std::runtime_error(std::string("psl: ") + psl_last_error());
+}
+
+// MarkerStruct (include/types.h:79-80)
+struct MarkerStruct {
+  bool fluid = false, bounceback = false, movingBounceback = false, forcedVelocity = false;
+  bool periodicX = false, periodicY = false, periodicZ = false, givenRho = false, givenUxUyUz = false;
+  bool nonReflectiveOutlet = false, refinement = false, ghost = false;
+};
+
+using InfoStruct = psl_info;
+
+// Flag byte of a resolved marker (the order of tests in
+// include/applyLocalCellUpdate.h:17-67).
+inline uint8_t flag_of(const MarkerStruct& m, int slot = 0) {
+  int kind;
+  if (m.ghost) kind = PSL_GHOST;
+  else if (m.bounceback) kind = PSL_BOUNCEBACK;
+  else if (m.fluid) kind = PSL_FLUID;
+  else if (m.givenRho && !m.givenUxUyUz) kind = PSL_GIVEN_RHO;
+  else if (!m.givenRho && m.givenUxUyUz) kind = PSL_GIVEN_U;
+  else if (!m.givenRho && !m.givenUxUyUz) kind = PSL_GIVEN_NONE;
+  else kind = PSL_GIVEN_RHOU;
+  int f = kind | (slot << 6);
+  if (m.periodicX) f |= PSL_PERIODIC_X;
+  if (m.periodicY) f |= PSL_PERIODIC_Y;
+  if (m.periodicZ) f |= PSL_PERIODIC_Z;
+  return (uint8_t)f;
+}
+
+class GridStruct {
+ public:
+  InfoStruct Info{};
+
+  explicit GridStruct(const InfoStruct& info, int storage = PSL_F32) {
+    check(psl_grid_create(&info, storage, &h_));
+    check(psl_grid_get_info(h_, &Info));
+  }
+  GridStruct(const GridStruct&) = delete;
+  GridStruct& operator=(const GridStruct&) = delete;
+  ~GridStruct() { psl_grid_destroy(h_); }
+
+  psl_grid* handle() const { return h_; }
+  int64_t cellCount() const { return (int64_t)Info.cellCountX * Info.cellCountY * Info.cellCountZ; }
+
+  // push Info changes (sub-grid bounds, nu) to the library
+  void pushInfo() { check(psl_grid_set_info(h_, &Info)); }
+
+  // getMarkers(i, j, k, Marker, Info); the bounceback array (if any) is applied first,
+  // as Grid.bouncebackMarkerArray is read before the callback in the reference.
+  template <class GetMarkers>
+  void setMarkers(GetMarkers getMarkers, const std::vector<uint8_t>* bounceback = nullptr,
+                  std::function<int(int, int, int)> slot = nullptr) {
+    std::vector<uint8_t> flags((size_t)cellCount());
+    for (int k = 0; k < Info.cellCountZ; ++k)
+      for (int j = 0; j < Info.cellCountY; ++j)
+        for (int i = 0; i < Info.cellCountX; ++i) {
+          const size_t c = ((size_t)k * Info.cellCountY + j) * Info.cellCountX + i;
+          MarkerStruct m;
+          if (bounceback) m.bounceback = (*bounceback)[c] != 0;
+          getMarkers(i, j, k, m, Info);
+          flags[c] = flag_of(m, slot ? slot(i, j, k) : 0);
+        }
+    check(psl_grid_set_flags(h_, flags.data()));
+  }
+
+  // getGivenRhoUxUyUz for the (up to 4) value slots used by the markers
+  void setGivenValues(const float (&v)[16]) { check(psl_grid_set_bc_values(h_, v)); }
+  void setSmagorinsky(float cs) { check(psl_grid_set_smagorinsky(h_, cs, nullptr)); }
+  void setSmagorinsky(const std::vector<float>& perCell) {
+    check(psl_grid_set_smagorinsky(h_, 0.f, perCell.data()));
+  }
+  void setPeriodicCopies(bool x, bool y) { check(psl_grid_set_periodic_copies(h_, (x ? 1 : 0) | (y ? 2 : 0))); }
+
+ private:
+  psl_grid* h_ = nullptr;
+};
+
+// fillEquilibriumFromFunction (include/fillEquilibrium.h:1-22) with
+// getInitialRhoUxUyUz(i, j, k, rho, ux, uy, uz, Marker, Info).
+template <class GetInitial>
+inline void fillEquilibriumFromFunction(GridStruct& g, GetInitial getInitial) {
+  const size_t n = (size_t)g.cellCount();
+  std::vector<float> r(n, 1.f), a(n, 0.f), b(n, 0.f), c(n, 0.f);
+  const InfoStruct& I = g.Info;
+  for (int k = 0; k < I.cellCountZ; ++k)
+    for (int j = 0; j < I.cellCountY; ++j)
+      for (int i = 0; i < I.cellCountX; ++i) {
+        const size_t id = ((size_t)k * I.cellCountY + j) * I.cellCountX + i;
+        MarkerStruct m;
+        getInitial(i, j, k, r[id], a[id], b[id], c[id], m, I);
+      }
+  check(psl_fill_equilibrium(g.handle(), r.data(), a.data(), b.data(), c.data()));
+}
+
+inline void applyStreaming(GridStruct& g) { check(psl_apply_streaming(g.handle())); }
+inline void applyLocalCellUpdate(GridStruct& g) { check(psl_apply_local_cell_update(g.handle())); }
+inline void applyBCPeriodicX(GridStruct& g) { check(psl_apply_bc_periodic_x(g.handle())); }
+inline void applyBCPeriodicY(GridStruct& g) { check(psl_apply_bc_periodic_y(g.handle())); }
+inline void applyCoarseFineGridCommunication(GridStruct& c, GridStruct& f) {
+  check(psl_coarse_fine_communication(c.handle(), f.handle()));
+}
+inline void fillCoarseGridFromFine(GridStruct& c, GridStruct& f) { check(psl_fill_coarse_from_fine(c.handle(), f.handle())); }
+
+// updateGrid recursion (flowAroundSphereMainSingle.cu:158-167)
+inline void updateGrid(std::vector<GridStruct*>& grids, int steps = 1) {
+  std::vector<psl_grid*> h;
+  for (auto* g : grids) h.push_back(g->handle());
+  check(psl_update_grid(h.data(), (int)h.size(), steps));
+}
+
+// getSphereDrag-style momentum exchange over bounceback cells of a box (FP64)
+inline void momentumExchange(GridStruct& g, const int (&lo)[3], const int (&hi)[3], double (&out)[3]) {
+  check(psl_reduce_momentum_exchange(g.handle(), lo, hi, out));
+}
+
+// ------------------------------------------------------------- geometry
+class STLStruct {
+ public:
+  explicit STLStruct(const std::string& path) { check(psl_stl_read(path.c_str(), &h_)); }
+  STLStruct(const float* tri9, int n) { check(psl_stl_from_triangles(tri9, n, &h_)); }
+  STLStruct(const STLStruct&) = delete;
+  STLStruct& operator=(const STLStruct&) = delete;
+  ~STLStruct() { psl_stl_destroy(h_); }
+  psl_stl* handle() const { return h_; }
+  int triangleCount() const {
+    int c = 0;
+    check(psl_stl_info(h_, &c, nullptr));
+    return c;
+  }
+  void setOrigin(float ox, float oy, float oz) { check(psl_stl_set_origin(h_, ox, oy, oz)); }
+
+ private:
+  psl_stl* h_ = nullptr;
+};
+
+class BoolArrayType {
+ public:
+  BoolArrayType(int64_t n, bool value) { check(psl_mask_create(n, value ? 1 : 0, &h_)); n_ = n; }
+  BoolArrayType(const BoolArrayType&) = delete;
+  BoolArrayType& operator=(const BoolArrayType&) = delete;
+  ~BoolArrayType() { psl_mask_destroy(h_); }
+  psl_mask* handle() const { return h_; }
+  std::vector<uint8_t> host() const {
+    std::vector<uint8_t> v((size_t)n_);
+    check(psl_mask_get(h_, v.data()));
+    return v;
+  }
+
+ private:
+  psl_mask* h_ = nullptr;
+  int64_t n_ = 0;
+};
+
+inline int checkSTLEdges(const STLStruct& s) {
+  int bad = 0;
+  check(psl_stl_check_edges(s.handle(), &bad, nullptr));
+  return bad;
+}
+inline void applyMarkersInsideSTL(BoolArrayType& m, const STLStruct& s, bool inside, const InfoStruct& info) {
+  check(psl_markers_inside_stl(m.handle(), s.handle(), inside ? 1 : 0, &info));
+}
+inline void multiplyBoolArrays(BoolArrayType& a, BoolArrayType& b, BoolArrayType& r) {
+  check(psl_mask_and(a.handle(), b.handle(), r.handle()));
+}
+inline void sumBoolArrays(BoolArrayType& a, BoolArrayType& b, BoolArrayType& r) {
+  check(psl_mask_or(a.handle(), b.handle(), r.handle()));
+}
+inline void invertBoolArray(BoolArrayType& a) { check(psl_mask_not(a.handle())); }
+
+}  // namespace psl_facade

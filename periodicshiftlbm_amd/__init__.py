@@ -158,6 +158,22 @@ _SIGS = [
     ("psl_grid_profile_enable", _I, [_P, _I]),
     ("psl_grid_profile_read", _I, [_P, _P, _P]),
     ("psl_grid_cell_census", _I, [_P, _P, _P, _P]),
+    ("psl_device_copy", _I, [_P, _P, ctypes.c_size_t]),
+    ("psl_device_alloc", _I, [ctypes.c_size_t, _P]),
+    ("psl_device_free", _I, [_P]),
+    ("psl_diad_create", _I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
+    ("psl_diad_destroy", _I, [_P]),
+    ("psl_diad_set_flags", _I, [_P, _P]),
+    ("psl_diad_set_bc_values", _I, [_P, _P]),
+    ("psl_diad_set_smagorinsky", _I, [_P, ctypes.c_float]),
+    ("psl_diad_set_stream", _I, [_P, _P]),
+    ("psl_diad_fill_equilibrium", _I, [_P, _P, _P, _P, _P]),
+    ("psl_diad_apply_streaming", _I, [_P]),
+    ("psl_diad_apply_local_cell_update", _I, [_P]),
+    ("psl_diad_get_flipper", _I, [_P, _P]),
+    ("psl_diad_get_f", _I, [_P, _P]),
+    ("psl_diad_set_f", _I, [_P, _P]),
+    ("psl_diad_dense_box", _I, [_I, _I, _I, _I, _P, _P]),
 ]
 
 
@@ -343,6 +359,81 @@ class GridStruct:
         return out.value
 
 
+class DIADGridStruct:
+    """include/types.h:166-171 -- esotwist grid over an explicit cell list."""
+
+    def __init__(self, info: InfoStruct, ijk3: np.ndarray, nbr7: np.ndarray, storage: int | str = F32):
+        if isinstance(storage, str):
+            storage = STORAGE_NAMES[storage]
+        require_device()
+        ijk3 = np.ascontiguousarray(ijk3, np.int32)
+        nbr7 = np.ascontiguousarray(nbr7, np.int32)
+        self.n = ijk3.shape[1]
+        self.Info = info
+        self.storage = storage
+        self._h = ctypes.c_void_p()
+        ci = info.to_c()
+        _check(lib().psl_diad_create(ctypes.byref(ci), storage, self.n, ijk3.ctypes.data, nbr7.ctypes.data,
+                                     ctypes.byref(self._h)))
+
+    @staticmethod
+    def dense_box(nx, ny, nz, periodic=(True, True, True)):
+        n = nx * ny * nz
+        ijk = np.zeros((3, n), np.int32)
+        nbr = np.zeros((7, n), np.int32)
+        mask = int(periodic[0]) | (int(periodic[1]) << 1) | (int(periodic[2]) << 2)
+        _check(lib().psl_diad_dense_box(nx, ny, nz, mask, ijk.ctypes.data, nbr.ctypes.data))
+        return ijk, nbr
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().psl_diad_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_flags(self, flags):
+        a = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+        _check(lib().psl_diad_set_flags(self._h, _ptr(a)))
+
+    def set_given_values(self, values):
+        v = np.zeros((4, 4), np.float32)
+        a = np.asarray(values, np.float32).reshape(-1, 4)
+        v[: a.shape[0]] = a
+        _check(lib().psl_diad_set_bc_values(self._h, v.ctypes.data))
+
+    def set_smagorinsky(self, cs: float):
+        _check(lib().psl_diad_set_smagorinsky(self._h, float(cs)))
+
+    def fill_equilibrium(self, rho=None, ux=None, uy=None, uz=None):
+        arrs = [None if v is None else np.ascontiguousarray(np.broadcast_to(v, (self.n,)), np.float32)
+                for v in (rho, ux, uy, uz)]
+        _check(lib().psl_diad_fill_equilibrium(self._h, *[_ptr(a) for a in arrs]))
+
+    @property
+    def esotwistFlipper(self) -> bool:
+        v = ctypes.c_int()
+        _check(lib().psl_diad_get_flipper(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def get_f(self) -> np.ndarray:
+        out = np.empty((27, self.n), np.float32)
+        _check(lib().psl_diad_get_f(self._h, out.ctypes.data))
+        return out
+
+    def set_f(self, f):
+        a = np.ascontiguousarray(f, np.float32)
+        _check(lib().psl_diad_set_f(self._h, a.ctypes.data))
+
+
 # ------------------------------------------------------------ operators
 def fillEquilibriumFromFunction(grid: GridStruct, getInitialRhoUxUyUz: Optional[Callable] = None):
     """include/fillEquilibrium.h:1-22 with a vectorised host callback
@@ -366,12 +457,49 @@ def synchronize():
     _check(lib().psl_device_synchronize())
 
 
-def applyStreaming(grid: GridStruct):
-    _check(lib().psl_apply_streaming(grid.handle))
+class DeviceBuffer:
+    """A raw device allocation (used for counter calibration runs)."""
+
+    def __init__(self, nbytes: int):
+        require_device()
+        self.nbytes = nbytes
+        self._p = ctypes.c_void_p()
+        _check(lib().psl_device_alloc(nbytes, ctypes.byref(self._p)))
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def close(self):
+        if getattr(self, "_p", None) and self._p.value:
+            lib().psl_device_free(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
-def applyLocalCellUpdate(grid: GridStruct):
-    _check(lib().psl_apply_local_cell_update(grid.handle))
+def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
+    _check(lib().psl_device_copy(dst.ptr, src.ptr, nbytes))
+
+
+def applyStreaming(grid):
+    """applyStreaming(GridStruct&) / applyStreaming(DIADGridStruct&)."""
+    if isinstance(grid, DIADGridStruct):
+        _check(lib().psl_diad_apply_streaming(grid.handle))
+    else:
+        _check(lib().psl_apply_streaming(grid.handle))
+
+
+def applyLocalCellUpdate(grid):
+    """applyLocalCellUpdate(GridStruct&) / applyLocalCellUpdate(DIADGridStruct&)."""
+    if isinstance(grid, DIADGridStruct):
+        _check(lib().psl_diad_apply_local_cell_update(grid.handle))
+    else:
+        _check(lib().psl_apply_local_cell_update(grid.handle))
 
 
 def applyBCPeriodicX(grid: GridStruct):

@@ -1,0 +1,366 @@
+// Esotwist streaming on directly addressed (DIAD) grids.
+//
+// include/applyStreaming.h:44-192 and include/applyLocalCellUpdate.h:77-171.
+// A DIAD grid stores only existing cells; cell c carries its (i, j, k) and the
+// indices of its +i, +j, +k, +ij, +ik, +jk, +ijk neighbours.  Population q of
+// a cell is read from and written to the cell itself or one of those
+// neighbours (tables ESO_READ / ESO_WRITE), and a parity bit flips between
+// steps: with the flipper set, reads use q and writes the opposite direction,
+// otherwise the reverse.  Streaming is therefore only the parity flip; the
+// update is in place.  Bounceback cells are skipped and reflect implicitly.
+//
+// Layout: fArray(q, cell) at f[q * n + cell]; ijk[3][n]; nbr[7][n].
+#include "psl_internal.h"
+
+#include <cstring>
+
+namespace psl {
+
+// neighbour slot per population: 0 = cell, 1 = i, 2 = j, 3 = k, 4 = ij, 5 = ik, 6 = jk, 7 = ijk
+constexpr int ESO_READ[27] = {0, 0, 1, 3, 0, 2, 0, 3, 1, 0, 5, 4, 0, 3, 2, 1, 2, 0, 6, 5, 2, 4, 3, 6, 1, 7, 0};
+constexpr int ESO_WRITE[27] = {0, 1, 0, 0, 3, 0, 2, 1, 3, 5, 0, 0, 4, 2, 3, 2, 1, 6, 0, 2, 5, 3, 4, 1, 6, 0, 7};
+__host__ __device__ constexpr int opp(int q) { return q == 0 ? 0 : ((q & 1) ? q + 1 : q - 1); }
+
+struct EsoParams {
+  uint32_t n;
+  int nx, ny, nz;
+  float omega, tau, cs;
+  const uint8_t* flags;
+  const int* ijk;  // 3 x n
+  const int* nbr;  // 7 x n
+  BcTable bc;
+};
+
+template <class St>
+__device__ __forceinline__ void eso_slots(const EsoParams& p, uint32_t cell, uint32_t (&slot)[8]) {
+  slot[0] = cell;
+#pragma unroll
+  for (int t = 0; t < 7; ++t) slot[t + 1] = (uint32_t)p.nbr[(size_t)t * p.n + cell];
+}
+
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_update(typename St::T* __restrict__ f, EsoParams p) {
+  typedef typename St::T T;
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= p.n) return;
+  const int fl = p.flags ? p.flags[cell] : 0;
+  int kind = fl & 7;
+  if (kind == KIND_BOUNCEBACK) return;  // reflected implicitly by the maps
+  if (kind == KIND_GHOST || kind == KIND_COVERED) kind = KIND_GIVEN_NONE;
+  uint32_t slot[8];
+  eso_slots<St>(p, cell, slot);
+  float v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int fi = FLIP ? q : opp(q);
+    v[q] = St::load(f[(size_t)fi * p.n + slot[ESO_READ[q]]]);
+  }
+  Relax rx{p.omega, p.tau, p.cs};
+  if (kind == KIND_FLUID) {
+    collide<St::W_OFF>(v, rx, p.cs);
+  } else {
+    if (St::W_OFF) {
+#pragma unroll
+      for (int q = 0; q < 27; ++q) v[q] += weight(q);
+    }
+    const int i = p.ijk[cell], j = p.ijk[(size_t)p.n + cell], k = p.ijk[2 * (size_t)p.n + cell];
+    const int code = outer_normal_code(i, j, k, p.nx, p.ny, p.nz, fl);
+    const int s = (fl >> 6) & 3;
+    float g[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) g[c] = s == 0 ? p.bc.v[c] : s == 1 ? p.bc.v[4 + c] : s == 2 ? p.bc.v[8 + c] : p.bc.v[12 + c];
+    boundary_cell(v, kind, code, g[0], g[1], g[2], g[3]);
+    collide<false>(v, rx, p.cs);
+    if (St::W_OFF) {
+#pragma unroll
+      for (int q = 0; q < 27; ++q) v[q] -= weight(q);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int fi = FLIP ? opp(q) : q;
+    f[(size_t)fi * p.n + slot[ESO_WRITE[q]]] = St::store(v[q]);
+  }
+}
+
+// fillEquilibriumFromFunction(DIADGridStruct&) (include/fillEquilibrium.h:25-70):
+// feq written through the write maps with the current parity.
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_fill(typename St::T* __restrict__ f, EsoParams p, const float* rho,
+                                                  const float* ux, const float* uy, const float* uz) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= p.n) return;
+  uint32_t slot[8];
+  eso_slots<St>(p, cell, slot);
+  const float r = rho ? rho[cell] : 1.f, a = ux ? ux[cell] : 0.f, b = uy ? uy[cell] : 0.f, c = uz ? uz[cell] : 0.f;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    float e = feq(q, r, a, b, c);
+    if (St::W_OFF) e -= weight(q);
+    const int fi = FLIP ? opp(q) : q;
+    f[(size_t)fi * p.n + slot[ESO_WRITE[q]]] = St::store(e);
+  }
+}
+
+// Post-collision populations of every cell, read back through the write maps
+// of the current parity (the state a following streaming step would move).
+template <class St, bool FLIP, bool SET>
+__global__ __launch_bounds__(256) void k_eso_io(typename St::T* __restrict__ f, EsoParams p, float* buf) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= p.n) return;
+  uint32_t slot[8];
+  eso_slots<St>(p, cell, slot);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int fi = FLIP ? opp(q) : q;
+    const size_t at = (size_t)fi * p.n + slot[ESO_WRITE[q]];
+    if (SET) {
+      float v = buf[(size_t)q * p.n + cell];
+      if (St::W_OFF) v -= weight(q);
+      f[at] = St::store(v);
+    } else {
+      float v = St::load(f[at]);
+      if (St::W_OFF) v += weight(q);
+      buf[(size_t)q * p.n + cell] = v;
+    }
+  }
+}
+
+}  // namespace psl
+
+struct psl_diad {
+  psl_info info;
+  int storage = PSL_F32;
+  int64_t n = 0;
+  void* f = nullptr;
+  int* ijk = nullptr;
+  int* nbr = nullptr;
+  uint8_t* flags = nullptr;
+  psl::BcTable bcvals{};
+  float cs = 0.f;
+  bool flipper = false;
+  hipStream_t stream = nullptr;
+};
+
+using namespace psl;
+
+static EsoParams eso_params(const psl_diad* d) {
+  EsoParams p;
+  p.n = (uint32_t)d->n;
+  p.nx = d->info.cellCountX;
+  p.ny = d->info.cellCountY;
+  p.nz = d->info.cellCountZ;
+  const float tau = 3.f * d->info.nu + 0.5f;
+  p.tau = tau;
+  p.omega = 1.f / tau;
+  p.cs = d->cs;
+  p.flags = d->flags;
+  p.ijk = d->ijk;
+  p.nbr = d->nbr;
+  p.bc = d->bcvals;
+  return p;
+}
+
+template <class St>
+static int eso_update(psl_diad* d) {
+  const EsoParams p = eso_params(d);
+  const uint32_t nb = (uint32_t)((d->n + 255) / 256);
+  if (d->flipper) k_eso_update<St, true><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p);
+  else k_eso_update<St, false><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+template <class St>
+static int eso_fill(psl_diad* d, const float* r, const float* a, const float* b, const float* c) {
+  const EsoParams p = eso_params(d);
+  const uint32_t nb = (uint32_t)((d->n + 255) / 256);
+  if (d->flipper) k_eso_fill<St, true><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p, r, a, b, c);
+  else k_eso_fill<St, false><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p, r, a, b, c);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+template <class St, bool SET>
+static int eso_io(psl_diad* d, float* buf) {
+  const EsoParams p = eso_params(d);
+  const uint32_t nb = (uint32_t)((d->n + 255) / 256);
+  if (d->flipper) k_eso_io<St, true, SET><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p, buf);
+  else k_eso_io<St, false, SET><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p, buf);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+#define PSL_ESO_DISPATCH(fn, ...)                                         \
+  (d->storage == PSL_F32 ? fn<StF32>(__VA_ARGS__)                         \
+                         : d->storage == PSL_F16DEV ? fn<StF16>(__VA_ARGS__) : fn<StBF16>(__VA_ARGS__))
+
+extern "C" {
+
+int psl_diad_create(const psl_info* info, int storage, int64_t n, const int* ijk3, const int* nbr7, psl_diad** out) {
+  PSL_CHECK_ARG(info && out && ijk3 && nbr7 && n > 0, "bad DIAD grid arguments");
+  PSL_CHECK_ARG(n < (1ll << 31), "cell count must be below 2^31");
+  PSL_CHECK_ARG(storage == PSL_F32 || storage == PSL_F16DEV || storage == PSL_BF16DEV, "unknown storage");
+  for (int64_t t = 0; t < 7 * n; ++t)
+    PSL_CHECK_ARG(nbr7[t] >= 0 && nbr7[t] < n, "neighbour index out of range");
+  psl_diad* d = new psl_diad();
+  d->info = *info;
+  d->storage = storage;
+  d->n = n;
+  const size_t fb = 27ull * n * storage_bytes(storage);
+  if (hipMalloc(&d->f, fb) != hipSuccess || hipMalloc(&d->ijk, 3ull * n * sizeof(int)) != hipSuccess ||
+      hipMalloc(&d->nbr, 7ull * n * sizeof(int)) != hipSuccess ||
+      hipMemcpy(d->ijk, ijk3, 3ull * n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d->nbr, nbr7, 7ull * n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("DIAD allocation / upload failed");
+    if (d->f) (void)hipFree(d->f);
+    if (d->ijk) (void)hipFree(d->ijk);
+    if (d->nbr) (void)hipFree(d->nbr);
+    delete d;
+    return PSL_E_NOMEM;
+  }
+  *out = d;
+  return PSL_OK;
+}
+
+int psl_diad_destroy(psl_diad* d) {
+  if (!d) return PSL_OK;
+  (void)hipDeviceSynchronize();
+  if (d->f) (void)hipFree(d->f);
+  if (d->ijk) (void)hipFree(d->ijk);
+  if (d->nbr) (void)hipFree(d->nbr);
+  if (d->flags) (void)hipFree(d->flags);
+  delete d;
+  return PSL_OK;
+}
+
+int psl_diad_set_flags(psl_diad* d, const uint8_t* host) {
+  PSL_CHECK_ARG(d, "null grid");
+  if (!host) {
+    if (d->flags) PSL_HIP_TRY(hipFree(d->flags));
+    d->flags = nullptr;
+    return PSL_OK;
+  }
+  if (!d->flags) PSL_HIP_TRY(hipMalloc(&d->flags, (size_t)d->n));
+  PSL_HIP_TRY(hipMemcpy(d->flags, host, (size_t)d->n, hipMemcpyHostToDevice));
+  return PSL_OK;
+}
+
+int psl_diad_set_bc_values(psl_diad* d, const float* v16) {
+  PSL_CHECK_ARG(d && v16, "null argument");
+  memcpy(d->bcvals.v, v16, sizeof(d->bcvals.v));
+  return PSL_OK;
+}
+
+int psl_diad_set_smagorinsky(psl_diad* d, float cs) {
+  PSL_CHECK_ARG(d, "null grid");
+  d->cs = cs;
+  return PSL_OK;
+}
+
+int psl_diad_set_stream(psl_diad* d, void* s) {
+  PSL_CHECK_ARG(d, "null grid");
+  d->stream = (hipStream_t)s;
+  return PSL_OK;
+}
+
+int psl_diad_fill_equilibrium(psl_diad* d, const float* rho, const float* ux, const float* uy, const float* uz) {
+  PSL_CHECK_ARG(d, "null grid");
+  const size_t n = (size_t)d->n;
+  float* dv[4] = {nullptr, nullptr, nullptr, nullptr};
+  const float* hv[4] = {rho, ux, uy, uz};
+  int rc = PSL_OK;
+  for (int a = 0; a < 4 && rc == PSL_OK; ++a)
+    if (hv[a] && (hipMalloc(&dv[a], n * sizeof(float)) != hipSuccess ||
+                  hipMemcpy(dv[a], hv[a], n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess))
+      rc = PSL_E_HIP;
+  if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_fill, d, dv[0], dv[1], dv[2], dv[3]);
+  if (hipStreamSynchronize(d->stream) != hipSuccess && rc == PSL_OK) rc = PSL_E_HIP;
+  for (int a = 0; a < 4; ++a)
+    if (dv[a]) (void)hipFree(dv[a]);
+  if (rc == PSL_E_HIP) set_error("DIAD fill_equilibrium failed");
+  return rc;
+}
+
+int psl_diad_apply_streaming(psl_diad* d) {
+  PSL_CHECK_ARG(d, "null grid");
+  d->flipper = !d->flipper;
+  return PSL_OK;
+}
+
+int psl_diad_apply_local_cell_update(psl_diad* d) {
+  PSL_CHECK_ARG(d, "null grid");
+  return PSL_ESO_DISPATCH(eso_update, d);
+}
+
+int psl_diad_get_flipper(const psl_diad* d, int* out) {
+  PSL_CHECK_ARG(d && out, "null argument");
+  *out = d->flipper ? 1 : 0;
+  return PSL_OK;
+}
+
+int psl_diad_get_f(psl_diad* d, float* host) {
+  PSL_CHECK_ARG(d && host, "null argument");
+  const size_t bytes = 27ull * d->n * sizeof(float);
+  float* buf = nullptr;
+  PSL_HIP_TRY(hipMalloc(&buf, bytes));
+  int rc;
+  switch (d->storage) {
+    case PSL_F32: rc = eso_io<StF32, false>(d, buf); break;
+    case PSL_F16DEV: rc = eso_io<StF16, false>(d, buf); break;
+    default: rc = eso_io<StBF16, false>(d, buf); break;
+  }
+  if (rc == PSL_OK && hipMemcpyAsync(host, buf, bytes, hipMemcpyDeviceToHost, d->stream) != hipSuccess) rc = PSL_E_HIP;
+  if (hipStreamSynchronize(d->stream) != hipSuccess && rc == PSL_OK) rc = PSL_E_HIP;
+  (void)hipFree(buf);
+  if (rc == PSL_E_HIP) set_error("DIAD get_f failed");
+  return rc;
+}
+
+int psl_diad_set_f(psl_diad* d, const float* host) {
+  PSL_CHECK_ARG(d && host, "null argument");
+  const size_t bytes = 27ull * d->n * sizeof(float);
+  float* buf = nullptr;
+  PSL_HIP_TRY(hipMalloc(&buf, bytes));
+  int rc = hipMemcpy(buf, host, bytes, hipMemcpyHostToDevice) == hipSuccess ? PSL_OK : PSL_E_HIP;
+  if (rc == PSL_OK) {
+    switch (d->storage) {
+      case PSL_F32: rc = eso_io<StF32, true>(d, buf); break;
+      case PSL_F16DEV: rc = eso_io<StF16, true>(d, buf); break;
+      default: rc = eso_io<StBF16, true>(d, buf); break;
+    }
+  }
+  if (hipStreamSynchronize(d->stream) != hipSuccess && rc == PSL_OK) rc = PSL_E_HIP;
+  (void)hipFree(buf);
+  if (rc == PSL_E_HIP) set_error("DIAD set_f failed");
+  return rc;
+}
+
+// Neighbour maps of a dense nx*ny*nz box (cell = (k*ny + j)*nx + i); axes with
+// the periodic bit wrap, the others clamp to the last layer.
+int psl_diad_dense_box(int nx, int ny, int nz, int periodic_mask, int* ijk3, int* nbr7) {
+  PSL_CHECK_ARG(nx > 0 && ny > 0 && nz > 0 && ijk3 && nbr7, "bad box");
+  const int64_t n = (int64_t)nx * ny * nz;
+  auto wrap = [](int a, int m, bool per) { return per ? (a % m) : (a < m ? a : m - 1); };
+  const bool px = periodic_mask & 1, py = periodic_mask & 2, pz = periodic_mask & 4;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const int64_t c = ((int64_t)k * ny + j) * nx + i;
+        ijk3[c] = i;
+        ijk3[n + c] = j;
+        ijk3[2 * n + c] = k;
+        const int i1 = wrap(i + 1, nx, px), j1 = wrap(j + 1, ny, py), k1 = wrap(k + 1, nz, pz);
+        auto id = [&](int a, int b, int cc) { return (int)(((int64_t)cc * ny + b) * nx + a); };
+        nbr7[0 * n + c] = id(i1, j, k);
+        nbr7[1 * n + c] = id(i, j1, k);
+        nbr7[2 * n + c] = id(i, j, k1);
+        nbr7[3 * n + c] = id(i1, j1, k);
+        nbr7[4 * n + c] = id(i1, j, k1);
+        nbr7[5 * n + c] = id(i, j1, k1);
+        nbr7[6 * n + c] = id(i1, j1, k1);
+      }
+  return PSL_OK;
+}
+
+}  // extern "C"

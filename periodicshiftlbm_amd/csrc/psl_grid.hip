@@ -381,6 +381,16 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* partials, ui
   block_sum_store<K>(acc, out);
 }
 
+// ------------------------------------------------------------- plain copy
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n16; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+__global__ void k_copy1(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, size_t n) {
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+
 // ------------------------------------------------------------ flag helpers
 // Covered region of a grid that carries a finer sub-box
 // (include/applyLocalCellUpdate.h:25-30).
@@ -901,6 +911,36 @@ int psl_reduce_mass(psl_grid* g, double* out) {
   PSL_LAUNCH_CHECK();
   PSL_HIP_TRY(hipMemcpyAsync(out, g->result, sizeof(double), hipMemcpyDeviceToHost, g->stream));
   PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  return PSL_OK;
+}
+
+int psl_device_copy(void* dst, const void* src, size_t bytes) {
+  PSL_CHECK_ARG(dst && src, "null pointer");
+  const size_t n16 = bytes / 16;
+  if (((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0 && n16) {
+    const unsigned nb = (unsigned)std::min<size_t>((n16 + 255) / 256, 8192);
+    k_copy16<<<nb, 256>>>((const uint4*)src, (uint4*)dst, n16);
+    PSL_LAUNCH_CHECK();
+    const size_t done = n16 * 16;
+    if (bytes > done) {
+      k_copy1<<<1, 256>>>((const uint8_t*)src + done, (uint8_t*)dst + done, bytes - done);
+      PSL_LAUNCH_CHECK();
+    }
+  } else if (bytes) {
+    k_copy1<<<(unsigned)std::min<size_t>((bytes + 255) / 256, 8192), 256>>>((const uint8_t*)src, (uint8_t*)dst, bytes);
+    PSL_LAUNCH_CHECK();
+  }
+  return PSL_OK;
+}
+
+int psl_device_alloc(size_t bytes, void** out) {
+  PSL_CHECK_ARG(out, "null argument");
+  PSL_HIP_TRY(hipMalloc(out, bytes));
+  return PSL_OK;
+}
+
+int psl_device_free(void* ptr) {
+  if (ptr) PSL_HIP_TRY(hipFree(ptr));
   return PSL_OK;
 }
 

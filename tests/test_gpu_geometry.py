@@ -1,0 +1,184 @@
+"""GPU parity of block refinement transfers, the level recursion and the STL
+voxelizer against the CPU oracle.
+
+Refinement (FP32): distributions within 1e-5 relative after the transfers and
+after several coarse steps of the two-level recursion.
+Voxelizer: masks bit-exact; edge counts exact.
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from periodicshiftlbm_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _rel_err(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)) / np.maximum(np.abs(b), 1e-3)))
+
+
+def _two_level(P, O, nu=0.01, cs=0.0):
+    """Coarse 24 x 20 x 18 periodic box (copies in x, y) with a nested fine box."""
+    nxc, nyc, nzc = 24, 20, 18
+    sub = (6, 5, 4, 18, 15, 13)
+    nxf, nyf, nzf = 2 * (sub[3] - sub[0]), 2 * (sub[4] - sub[1]), 2 * (sub[5] - sub[2])
+    ci = P.InfoStruct(cellCountX=nxc, cellCountY=nyc, cellCountZ=nzc, nu=nu, res=1.0,
+                      iSubgridStart=sub[0], jSubgridStart=sub[1], kSubgridStart=sub[2],
+                      iSubgridEnd=sub[3], jSubgridEnd=sub[4], kSubgridEnd=sub[5])
+    nu_f = 2 * nu  # same physical viscosity on the finer grid (dt and dx halved)
+    fi = P.InfoStruct(gridID=1, cellCountX=nxf, cellCountY=nyf, cellCountZ=nzf, nu=nu_f, res=0.5,
+                      ox=sub[0] - 0.25, oy=sub[1] - 0.25, oz=sub[2] - 0.25)
+    gc = P.GridStruct(ci, "f32")
+    gf = P.GridStruct(fi, "f32")
+    k, j, i = np.meshgrid(np.arange(nzf), np.arange(nyf), np.arange(nxf), indexing="ij")
+    ghost = (i == 0) | (i == nxf - 1) | (j == 0) | (j == nyf - 1) | (k == 0) | (k == nzf - 1)
+    fflags = np.where(ghost, cases.GHOST, cases.FLUID).astype(np.uint8).reshape(-1)
+    gf.set_flags(fflags)
+    gc.set_periodic_copies(x=True, y=True)
+    gc.set_smagorinsky(cs)
+    gf.set_smagorinsky(cs)
+    oc = O.Grid(nxc, nyc, nzc, dtype=np.float32, nu=nu, cs=cs, sub=sub)
+    of = O.Grid(nxf, nyf, nzf, dtype=np.float32, nu=nu_f, cs=cs, flags=fflags)
+
+    def field(nx, ny, nz, x0, y0, z0, h):
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        x = 2 * np.pi * (x0 + i * h) / 23
+        y = 2 * np.pi * (y0 + j * h) / 19
+        z = 2 * np.pi * (z0 + k * h) / 18
+        ux = 0.03 * np.sin(x) * np.cos(y) * np.cos(z)
+        uy = -0.03 * np.cos(x) * np.sin(y) * np.cos(z)
+        return np.ones(nx * ny * nz), ux.reshape(-1), uy.reshape(-1), np.zeros(nx * ny * nz)
+
+    oc.fill_equilibrium(*field(nxc, nyc, nzc, 0, 0, 0, 1.0))
+    of.fill_equilibrium(*field(nxf, nyf, nzf, sub[0] - 0.25, sub[1] - 0.25, sub[2] - 0.25, 0.5))
+    gc.set_f(oc.logical_f())
+    gf.set_f(of.logical_f())
+    return gc, gf, oc, of
+
+
+def test_coarse_fine_communication_matches_oracle(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    gc, gf, oc, of = _two_level(P, O)
+    print("[refine] coarse-fine communication", flush=True)
+    P.applyCoarseFineGridCommunication(gc, gf)
+    O.coarse_fine_comm(oc, of)
+    ec, ef = _rel_err(gc.get_f(), oc.logical_f()), _rel_err(gf.get_f(), of.logical_f())
+    print(f"  coarse {ec:.3e} fine {ef:.3e}")
+    assert ec < RTOL and ef < RTOL
+    P.fillCoarseGridFromFine(gc, gf)
+    O.fill_coarse_from_fine(oc, of)
+    assert _rel_err(gc.get_f(), oc.logical_f()) < RTOL
+
+
+def test_two_level_recursion_matches_oracle(psl, oracle_mod):
+    """updateGrid recursion (flowAroundSphereMainSingle.cu:158-167) over two levels."""
+    P, O = psl, oracle_mod
+    gc, gf, oc, of = _two_level(P, O, nu=0.01, cs=0.12)
+    steps = 4
+    print(f"[refine] two-level recursion, {steps} coarse steps", flush=True)
+    P.updateGrid([gc, gf], steps)
+    for _ in range(steps):
+        oc.step(periodic_x=True, periodic_y=True)
+        for _ in range(2):
+            of.step()
+        O.coarse_fine_comm(oc, of)
+    ec, ef = _rel_err(gc.get_f(), oc.logical_f()), _rel_err(gf.get_f(), of.logical_f())
+    print(f"  coarse {ec:.3e} fine {ef:.3e}")
+    assert ec < RTOL and ef < RTOL
+    # covered coarse cells were skipped, as in the reference
+    bulk, nbc, skipped = gc.census()
+    assert skipped == (12 - 4) * (10 - 4) * (9 - 4)
+
+
+def test_voxelizer_matches_oracle_bit_exact(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    tri = cases.icosphere(3, 7.3, (15.2, 13.7, 12.9))
+    info = P.InfoStruct(cellCountX=32, cellCountY=28, cellCountZ=26, res=1.0)
+    stl = P.STLStruct.from_triangles(tri)
+    m = P.BoolArray(32 * 28 * 26)
+    print("[stl] icosphere subdivision 3", flush=True)
+    P.applyMarkersInsideSTL(m, stl, True, info)
+    got = m.get()
+    want = O.markers_inside_stl(tri, (0, 0, 0), (0, 0, 0), 1.0, 32, 28, 26)
+    assert np.array_equal(got, want)
+    assert got.sum() > 1000
+    # inverted inside value, grid origin and resolution other than 1
+    info2 = P.InfoStruct(cellCountX=40, cellCountY=36, cellCountZ=30, res=0.75, ox=1.5, oy=-0.25, oz=0.5)
+    stl.set_origin(0.5, 0.25, -0.5)
+    P.applyMarkersInsideSTL(m2 := P.BoolArray(40 * 36 * 30), stl, False, info2)
+    want2 = O.markers_inside_stl(tri, (0.5, 0.25, -0.5), (1.5, -0.25, 0.5), 0.75, 40, 36, 30, inside_value=0)
+    assert np.array_equal(m2.get(), want2)
+
+
+def test_sphere_config_voxelization(psl, oracle_mod):
+    """The sphere configuration: icosphere subdivision 4 (5120 triangles), D = 32."""
+    P, O = psl, oracle_mod
+    tri = cases.icosphere(4, cases.SPHERE_DIAMETER / 2, cases.SPHERE_CENTER)
+    assert tri.shape == (5120, 9)
+    stl = P.STLStruct.from_triangles(tri)
+    bad, counts = P.checkSTLEdges(stl)
+    obad, ocounts = O.check_stl_edges(tri)
+    assert bad == 0 and obad == 0 and np.array_equal(counts, ocounts)
+    shape = (128, 128, 128)
+    info = P.InfoStruct(cellCountX=shape[0], cellCountY=shape[1], cellCountZ=shape[2], res=1.0)
+    m = P.BoolArray(int(np.prod(shape)))
+    P.applyMarkersInsideSTL(m, stl, True, info)
+    got = m.get()
+    want = O.markers_inside_stl(tri, (0, 0, 0), (0, 0, 0), 1.0, *shape)
+    assert np.array_equal(got, want)
+    vol = got.sum()
+    print(f"  solid cells {vol}, sphere volume {4 / 3 * np.pi * 16 ** 3:.0f}")
+    assert abs(vol / (4 / 3 * np.pi * 16 ** 3) - 1) < 0.02
+
+
+def test_stl_file_round_trip_and_mask_ops(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    tri = cases.icosphere(2, 5.0, (8.0, 8.0, 8.0))
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "ico.stl")
+        cases.write_binary_stl(path, tri)
+        stl = P.readSTL(path)
+        assert stl.triangleCount == tri.shape[0]
+        assert np.array_equal(stl.triangles(), tri)
+        b = stl.bounds
+        assert np.allclose(b[:3], tri.reshape(-1, 3).min(0)) and np.allclose(b[3:], tri.reshape(-1, 3).max(0))
+    with pytest.raises(P.PslError):
+        P.readSTL("/nonexistent/file.stl")
+    n = 16 ** 3
+    a, bm, out = P.BoolArray(n), P.BoolArray(n), P.BoolArray(n)
+    rng = np.random.default_rng(0)
+    ha, hb = rng.integers(0, 2, n).astype(np.uint8), rng.integers(0, 2, n).astype(np.uint8)
+    a.set(ha)
+    bm.set(hb)
+    P.multiplyBoolArrays(a, bm, out)
+    assert np.array_equal(out.get(), ha & hb)
+    P.sumBoolArrays(a, bm, out)
+    assert np.array_equal(out.get(), ha | hb)
+    P.invertBoolArray(out)
+    assert np.array_equal(out.get(), 1 - (ha | hb))
+    # open mesh reported
+    stl_open = P.STLStruct.from_triangles(tri[1:])
+    bad, _ = P.checkSTLEdges(stl_open)
+    assert bad == 3
+
+
+def test_bounceback_mask_applies_to_flags(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    shape = (20, 18, 16)
+    info = P.InfoStruct(cellCountX=20, cellCountY=18, cellCountZ=16, nu=0.01)
+    g = P.GridStruct(info)
+    flags = cases.channel_flags(shape)
+    g.set_flags(flags)
+    tri = cases.icosphere(2, 4.0, (10.0, 9.0, 8.0))
+    m = P.BoolArray(int(np.prod(shape)))
+    P.applyMarkersInsideSTL(m, P.STLStruct.from_triangles(tri), True, info)
+    P.applyBouncebackMask(g, m)
+    got = g.get_flags()
+    want = flags.copy()
+    want[m.get().astype(bool)] = cases.BOUNCEBACK
+    assert np.array_equal(got, want)
