@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f32,f16,bf16")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times) for counter calibration")
     args = ap.parse_args()
@@ -133,6 +134,12 @@ def main():
         log(f"[bench] storage={storage} shape={shape}")
         g, flags, nsolid, nu = build_sphere_grid(P, cases, storage, shape)
         flags_keep, nu_keep = flags, nu
+        tune_times = None
+        if not args.no_autotune:
+            best, tt = g.autotune(10)
+            tune_times = {int(v): float(t) for v, t in enumerate(tt)}
+            P.fill_equilibrium(g)  # restart from rest after tuning
+            log(f"  autotune: variant {best} ({', '.join(f'{v}:{t * 1e3:.1f}us' for v, t in tune_times.items())})")
         el = time_steps(P, g, args.steps, args.warmup)
         mlups = ncell * args.steps / el / 1e6
         bulk, nbc, skipped = g.census()
@@ -148,6 +155,7 @@ def main():
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None},
             "drag_force_x_after_run": fx,
+            "tuning": g.tuning, "autotune_ms": tune_times,
         }
         log(f"  {mlups:.1f} MLUPS, bulk kernel {avg_ms * 1e3:.1f} us ({achieved:.0f} GB/s, "
             f"{achieved / HBM_PEAK_GBS * 100:.1f}% of HBM peak)")
@@ -175,7 +183,8 @@ def main():
         "roofline": dict(head["roofline"], kernel="k_ps_bulk", bytes_per_cell=BYTES_PER_UPDATE[head_storage],
                          cells_per_launch=head["bulk_cells"], avg_launch_ms=head["bulk_kernel_ms"]),
         "variants": {k: {"mlups": v["mlups"], "ms_per_step": v["ms_per_step"], "roofline_frac": v["roofline"]["frac"],
-                         "achieved_GBs": v["roofline"]["achieved"], "bulk_kernel_ms": v["bulk_kernel_ms"]}
+                         "achieved_GBs": v["roofline"]["achieved"], "bulk_kernel_ms": v["bulk_kernel_ms"],
+                         "tuning": v["tuning"], "autotune_ms": v["autotune_ms"]}
                      for k, v in results.items()},
     }
     if "f32" in results and "f16" in results:

@@ -155,6 +155,13 @@ int psl_device_copy(void* dst, const void* src, size_t bytes);
 /* Device buffer helpers for callers without a device allocator. */
 int psl_device_alloc(size_t bytes, void** out);
 int psl_device_free(void* ptr);
+/* Bulk-kernel variants: non-temporal loads / stores, XCD-contiguous block order. */
+enum { PSL_TUNE_NT_LOAD = 1, PSL_TUNE_NT_STORE = 2, PSL_TUNE_XCD_REMAP = 4 };
+int psl_grid_set_tuning(psl_grid* grid, int tuning);
+int psl_grid_get_tuning(const psl_grid* grid, int* tuning);
+/* Autotuner: times all 8 variants on this grid (steps bulk updates each; the state
+ * advances), keeps the fastest; best and times_ms[8] (mean kernel ms) are optional. */
+int psl_grid_autotune(psl_grid* grid, int steps, int* best, double* times_ms);
 /* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
 int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
 

@@ -38,6 +38,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpsl_amd.so")
 
 F32, F16DEV, BF16DEV = 0, 1, 2
+TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_XCD_REMAP = 1, 2, 4
 STORAGE_NAMES = {"f32": F32, "f16": F16DEV, "bf16": BF16DEV}
 
 FLUID, BOUNCEBACK, GIVEN_RHO, GIVEN_U, GIVEN_NONE, GIVEN_RHOU, GHOST, COVERED = range(8)
@@ -158,6 +159,9 @@ _SIGS = [
     ("psl_grid_profile_enable", _I, [_P, _I]),
     ("psl_grid_profile_read", _I, [_P, _P, _P]),
     ("psl_grid_cell_census", _I, [_P, _P, _P, _P]),
+    ("psl_grid_set_tuning", _I, [_P, _I]),
+    ("psl_grid_get_tuning", _I, [_P, _P]),
+    ("psl_grid_autotune", _I, [_P, _I, _P, _P]),
     ("psl_device_copy", _I, [_P, _P, ctypes.c_size_t]),
     ("psl_device_alloc", _I, [ctypes.c_size_t, _P]),
     ("psl_device_free", _I, [_P]),
@@ -347,6 +351,23 @@ class GridStruct:
         n = ctypes.c_int()
         _check(lib().psl_grid_profile_read(self._h, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
+
+    @property
+    def tuning(self) -> int:
+        v = ctypes.c_int()
+        _check(lib().psl_grid_get_tuning(self._h, ctypes.byref(v)))
+        return v.value
+
+    @tuning.setter
+    def tuning(self, v: int):
+        _check(lib().psl_grid_set_tuning(self._h, int(v)))
+
+    def autotune(self, steps: int = 20):
+        """Time the 8 bulk-kernel variants on this grid and keep the fastest (state advances)."""
+        best = ctypes.c_int()
+        times = np.zeros(8, np.float64)
+        _check(lib().psl_grid_autotune(self._h, steps, ctypes.byref(best), times.ctypes.data))
+        return best.value, times
 
     def census(self):
         a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -42,6 +42,7 @@ struct StepParams {
   const void* hi[27];
   uint32_t thr[27];
   uint32_t n;
+  uint32_t xcd_remap;  // non-zero: consecutive cell ranges stay on one XCD
   float omega;  // 1/tau, used when cs == 0 and there is no per-cell cs
   float tau;
   float cs;
@@ -67,52 +68,79 @@ __device__ __forceinline__ bool wave_straddles(const StepParams& p, uint32_t w_f
   return __ballot(s) != 0;
 }
 
-template <class St>
+// Tuning bits of the bulk kernel (chosen per grid by psl_grid_autotune).
+enum : int { TUNE_NT_LOAD = 1, TUNE_NT_STORE = 2 };
+
+template <class T, bool NT>
+__device__ __forceinline__ T ld(const T* p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <class T, bool NT>
+__device__ __forceinline__ void st(T* p, T v) {
+  if (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <class St, int TUNE>
 __device__ __forceinline__ void load27(const StepParams& p, uint32_t w_first, bool straddle, uint32_t cell,
                                        float (&v)[27]) {
   typedef typename St::T T;
+  constexpr bool NT = (TUNE & TUNE_NT_LOAD) != 0;
   if (!straddle) {
     const uint32_t off = cell * (uint32_t)sizeof(T);
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       const char* base = (const char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      v[q] = St::load(*(const T*)(base + off));
+      v[q] = St::load(ld<T, NT>((const T*)(base + off)));
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      v[q] = St::load(base[cell]);
+      v[q] = St::load(ld<T, NT>(base + cell));
     }
   }
 }
 
-template <class St>
+template <class St, int TUNE>
 __device__ __forceinline__ void store27(const StepParams& p, uint32_t w_first, bool straddle, uint32_t cell,
                                         const float (&v)[27]) {
   typedef typename St::T T;
+  constexpr bool NT = (TUNE & TUNE_NT_STORE) != 0;
   if (!straddle) {
     const uint32_t off = cell * (uint32_t)sizeof(T);
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       char* base = (char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      *(T*)(base + off) = St::store(v[q]);
+      st<T, NT>((T*)(base + off), St::store(v[q]));
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      base[cell] = St::store(v[q]);
+      st<T, NT>(base + cell, St::store(v[q]));
     }
   }
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
+// each with its own L2.  The PS streams are misaligned, so the cache line at
+// every block boundary of every stream is shared by neighbouring blocks;
+// remapping gives each XCD one contiguous range of blocks, so those lines are
+// fetched into one L2 only.  The map is a bijection of [0, nb).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t xcd = b & 7u, idx = b >> 3, per = nb >> 3, rem = nb & 7u;
+  return xcd * per + min(xcd, rem) + idx;
 }
 
 // ------------------------------------------------------------- bulk update
 // Fluid and bounceback cells: include/applyLocalCellUpdate.h:1-74 minus the
 // box-face branch (46-67), which k_ps_boundary handles.
-template <class St, bool FLAGS, bool LES>
+template <class St, bool FLAGS, bool LES, int TUNE>
 __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
-  const uint32_t block_first = blockIdx.x * 256u;
+  const uint32_t block = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t block_first = block * 256u;
   const uint32_t cell = block_first + threadIdx.x;
   const uint32_t w_first = wave_first_cell(block_first);
   if (w_first >= p.n) return;  // whole wave past the grid (wave-uniform)
@@ -123,7 +151,7 @@ __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
   if (FLAGS) kind = p.flags[cell] & 7;
   if (kind > KIND_BOUNCEBACK) return;
   float v[27];
-  load27<St>(p, w_first, straddle, cell, v);
+  load27<St, TUNE>(p, w_first, straddle, cell, v);
   if (kind == KIND_BOUNCEBACK) {
     // the swap commutes with "f - w" (opposite directions share a weight)
     bounceback(v);
@@ -133,7 +161,7 @@ __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
     if (LES && p.cs_cell) cs = p.cs_cell[cell];
     collide<St::W_OFF>(v, rx, LES ? cs : 0.f);
   }
-  store27<St>(p, w_first, straddle, cell, v);
+  store27<St, TUNE>(p, w_first, straddle, cell, v);
 }
 
 // ---------------------------------------------------------- boundary update
@@ -494,6 +522,17 @@ static void drop_profile_events(psl_grid* g) {
   g->prof_events.clear();
 }
 
+template <class St, int TUNE>
+static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, const StepParams& p) {
+  if (flags) {
+    if (les) k_ps_bulk<St, true, true, TUNE><<<nb, 256, 0, stream>>>(p);
+    else k_ps_bulk<St, true, false, TUNE><<<nb, 256, 0, stream>>>(p);
+  } else {
+    if (les) k_ps_bulk<St, false, true, TUNE><<<nb, 256, 0, stream>>>(p);
+    else k_ps_bulk<St, false, false, TUNE><<<nb, 256, 0, stream>>>(p);
+  }
+}
+
 template <class St>
 static int launch_local_update(psl_grid* g) {
   typedef typename St::T T;
@@ -520,12 +559,13 @@ static int launch_local_update(psl_grid* g) {
     PSL_HIP_TRY(hipEventCreate(&ev1));
     PSL_HIP_TRY(hipEventRecord(ev0, g->stream));
   }
-  if (g->flags) {
-    if (les) k_ps_bulk<St, true, true><<<nb, 256, 0, g->stream>>>(p);
-    else k_ps_bulk<St, true, false><<<nb, 256, 0, g->stream>>>(p);
-  } else {
-    if (les) k_ps_bulk<St, false, true><<<nb, 256, 0, g->stream>>>(p);
-    else k_ps_bulk<St, false, false><<<nb, 256, 0, g->stream>>>(p);
+  p.xcd_remap = (g->tuning & PSL_TUNE_XCD_REMAP) ? 1u : 0u;
+  const int nt = g->tuning & (TUNE_NT_LOAD | TUNE_NT_STORE);
+  switch (nt) {
+    case 0: launch_bulk<St, 0>(g->flags != nullptr, les, nb, g->stream, p); break;
+    case 1: launch_bulk<St, 1>(g->flags != nullptr, les, nb, g->stream, p); break;
+    case 2: launch_bulk<St, 2>(g->flags != nullptr, les, nb, g->stream, p); break;
+    default: launch_bulk<St, 3>(g->flags != nullptr, les, nb, g->stream, p); break;
   }
   PSL_LAUNCH_CHECK();
   if (g->profile) {
@@ -969,6 +1009,52 @@ int psl_grid_profile_read(psl_grid* g, double* total_ms, int* launches) {
   *total_ms = t;
   if (launches) *launches = (int)g->prof_events.size();
   drop_profile_events(g);
+  return PSL_OK;
+}
+
+int psl_grid_set_tuning(psl_grid* g, int tuning) {
+  PSL_CHECK_ARG(g, "null grid");
+  PSL_CHECK_ARG((tuning & ~(PSL_TUNE_NT_LOAD | PSL_TUNE_NT_STORE | PSL_TUNE_XCD_REMAP)) == 0, "unknown tuning bits");
+  g->tuning = tuning;
+  return PSL_OK;
+}
+
+int psl_grid_get_tuning(const psl_grid* g, int* tuning) {
+  PSL_CHECK_ARG(g && tuning, "null argument");
+  *tuning = g->tuning;
+  return PSL_OK;
+}
+
+// Time every tuning variant of the bulk kernel on this grid (each for `steps`
+// updates with the grid's own data; the distributions advance by
+// 8 * steps updates) and keep the fastest.  times_ms (optional, 8 entries)
+// receives the mean bulk-kernel time of each variant.
+int psl_grid_autotune(psl_grid* g, int steps, int* best, double* times_ms) {
+  PSL_CHECK_ARG(g && steps > 0, "bad autotune arguments");
+  const int saved = g->tuning;
+  double best_t = 1e300;
+  int best_v = saved;
+  for (int v = 0; v < 8; ++v) {
+    g->tuning = v;
+    int rc = psl_grid_profile_enable(g, 1);
+    for (int s = 0; s < steps && rc == PSL_OK; ++s) rc = psl_apply_local_cell_update(g);
+    double ms = 0.0;
+    int launches = 0;
+    if (rc == PSL_OK) rc = psl_grid_profile_read(g, &ms, &launches);
+    g->profile = false;
+    if (rc) {
+      g->tuning = saved;
+      return rc;
+    }
+    const double mean = ms / std::max(1, launches);
+    if (times_ms) times_ms[v] = mean;
+    if (mean < best_t) {
+      best_t = mean;
+      best_v = v;
+    }
+  }
+  g->tuning = best_v;
+  if (best) *best = best_v;
   return PSL_OK;
 }
 

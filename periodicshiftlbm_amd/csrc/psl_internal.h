@@ -98,6 +98,7 @@ struct psl_grid {
   bool profile = false;           // bracket bulk launches with events
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
   int64_t census[3] = {0, 0, 0};  // bulk, boundary-list, skipped cells
+  int tuning = PSL_TUNE_XCD_REMAP;  // bulk-kernel variant (psl_grid_set_tuning / autotune)
 };
 
 struct psl_mask {
