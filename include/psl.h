@@ -165,6 +165,26 @@ int psl_grid_autotune(psl_grid* grid, int steps, int* best, double* times_ms);
 /* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
 int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
 
+/* ------------------------------------------------------- batched grids */
+/* Many independent grids advanced together (no reference counterpart: the
+ * reference steps one grid at a time).  Each grid keeps its own flags, given
+ * values, Cs and storage (all grids share the storage); one step of all grids
+ * is a single replay of a captured hipGraph: a device shift update, one bulk
+ * launch over the blocks of every grid and one launch over all boundary lists.
+ * Grids must not use per-cell Cs, periodic copies or sub-grids.  The grids'
+ * own handles stay valid (get_f, reductions, single-grid steps). */
+typedef struct psl_batch psl_batch;
+int psl_batch_create(psl_grid* const* grids, int count, psl_batch** out);
+int psl_batch_destroy(psl_batch* batch);
+int psl_batch_step(psl_batch* batch, int nsteps);
+/* order 0: grids in the given order, 1: largest grid first (block order of the launch). */
+int psl_batch_set_tuning(psl_batch* batch, int tuning, int order);
+int psl_batch_get_tuning(const psl_batch* batch, int* tuning, int* order);
+/* Times the 8 variants x 2 orders (steps steps each; the grids advance) and keeps the
+ * fastest; times_ms[16] receives ms per step. */
+int psl_batch_autotune(psl_batch* batch, int steps, int* best_tuning, int* best_order, double* times_ms);
+int psl_batch_census(const psl_batch* batch, int64_t* cells, int* blocks, int* boundary_cells);
+
 /* --------------------------------------------- esotwist (DIAD) grids */
 /* DIADGridStruct (include/types.h:166-171) from host cell tables: ijk3 = 3 arrays of n
  * (i, j, k), nbr7 = 7 arrays of n neighbour cell indices (+i, +j, +k, +ij, +ik, +jk, +ijk). */

@@ -165,6 +165,13 @@ _SIGS = [
     ("psl_device_copy", _I, [_P, _P, ctypes.c_size_t]),
     ("psl_device_alloc", _I, [ctypes.c_size_t, _P]),
     ("psl_device_free", _I, [_P]),
+    ("psl_batch_create", _I, [_P, _I, _P]),
+    ("psl_batch_destroy", _I, [_P]),
+    ("psl_batch_step", _I, [_P, _I]),
+    ("psl_batch_set_tuning", _I, [_P, _I, _I]),
+    ("psl_batch_get_tuning", _I, [_P, _P, _P]),
+    ("psl_batch_autotune", _I, [_P, _I, _P, _P, _P]),
+    ("psl_batch_census", _I, [_P, _P, _P, _P]),
     ("psl_diad_create", _I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     ("psl_diad_destroy", _I, [_P]),
     ("psl_diad_set_flags", _I, [_P, _P]),
@@ -378,6 +385,50 @@ class GridStruct:
         out = ctypes.c_double()
         _check(lib().psl_reduce_mass(self._h, ctypes.byref(out)))
         return out.value
+
+
+class Batch:
+    """Many GridStructs advanced together by one hipGraph replay per step."""
+
+    def __init__(self, grids: Sequence[GridStruct]):
+        self.grids = list(grids)  # keep the grids alive
+        arr = (ctypes.c_void_p * len(grids))(*[g.handle.value for g in grids])
+        self._h = ctypes.c_void_p()
+        _check(lib().psl_batch_create(arr, len(grids), ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().psl_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1):
+        _check(lib().psl_batch_step(self._h, n))
+
+    def set_tuning(self, tuning: int, order: int = 0):
+        _check(lib().psl_batch_set_tuning(self._h, tuning, order))
+
+    @property
+    def tuning(self):
+        t, o = ctypes.c_int(), ctypes.c_int()
+        _check(lib().psl_batch_get_tuning(self._h, ctypes.byref(t), ctypes.byref(o)))
+        return t.value, o.value
+
+    def autotune(self, steps: int = 10):
+        t, o = ctypes.c_int(), ctypes.c_int()
+        times = np.zeros(16, np.float64)
+        _check(lib().psl_batch_autotune(self._h, steps, ctypes.byref(t), ctypes.byref(o), times.ctypes.data))
+        return t.value, o.value, times
+
+    def census(self):
+        c, b, n = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().psl_batch_census(self._h, ctypes.byref(c), ctypes.byref(b), ctypes.byref(n)))
+        return c.value, b.value, n.value
 
 
 class DIADGridStruct:

@@ -28,7 +28,7 @@ struct EsoParams {
   const uint8_t* flags;
   const int* ijk;  // 3 x n
   const int* nbr;  // 7 x n
-  BcTable bc;
+  const float4* bc;  // 4 given-value slots (device)
 };
 
 template <class St>
@@ -65,11 +65,8 @@ __global__ __launch_bounds__(256) void k_eso_update(typename St::T* __restrict__
     }
     const int i = p.ijk[cell], j = p.ijk[(size_t)p.n + cell], k = p.ijk[2 * (size_t)p.n + cell];
     const int code = outer_normal_code(i, j, k, p.nx, p.ny, p.nz, fl);
-    const int s = (fl >> 6) & 3;
-    float g[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) g[c] = s == 0 ? p.bc.v[c] : s == 1 ? p.bc.v[4 + c] : s == 2 ? p.bc.v[8 + c] : p.bc.v[12 + c];
-    boundary_cell(v, kind, code, g[0], g[1], g[2], g[3]);
+    const float4 g = p.bc[(fl >> 6) & 3];  // indexed load, no select chain
+    boundary_cell(v, kind, code, g.x, g.y, g.z, g.w);
     collide<false>(v, rx, p.cs);
     if (St::W_OFF) {
 #pragma unroll
@@ -136,7 +133,7 @@ struct psl_diad {
   int* ijk = nullptr;
   int* nbr = nullptr;
   uint8_t* flags = nullptr;
-  psl::BcTable bcvals{};
+  float4* bc_dev = nullptr;
   float cs = 0.f;
   bool flipper = false;
   hipStream_t stream = nullptr;
@@ -157,7 +154,7 @@ static EsoParams eso_params(const psl_diad* d) {
   p.flags = d->flags;
   p.ijk = d->ijk;
   p.nbr = d->nbr;
-  p.bc = d->bcvals;
+  p.bc = d->bc_dev;
   return p;
 }
 
@@ -208,12 +205,14 @@ int psl_diad_create(const psl_info* info, int storage, int64_t n, const int* ijk
   d->storage = storage;
   d->n = n;
   const size_t fb = 27ull * n * storage_bytes(storage);
-  if (hipMalloc(&d->f, fb) != hipSuccess || hipMalloc(&d->ijk, 3ull * n * sizeof(int)) != hipSuccess ||
+  if (hipMalloc(&d->f, fb) != hipSuccess || hipMalloc(&d->bc_dev, 4 * sizeof(float4)) != hipSuccess ||
+      hipMemset(d->bc_dev, 0, 4 * sizeof(float4)) != hipSuccess || hipMalloc(&d->ijk, 3ull * n * sizeof(int)) != hipSuccess ||
       hipMalloc(&d->nbr, 7ull * n * sizeof(int)) != hipSuccess ||
       hipMemcpy(d->ijk, ijk3, 3ull * n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(d->nbr, nbr7, 7ull * n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
     set_error("DIAD allocation / upload failed");
     if (d->f) (void)hipFree(d->f);
+    if (d->bc_dev) (void)hipFree(d->bc_dev);
     if (d->ijk) (void)hipFree(d->ijk);
     if (d->nbr) (void)hipFree(d->nbr);
     delete d;
@@ -230,6 +229,7 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->ijk) (void)hipFree(d->ijk);
   if (d->nbr) (void)hipFree(d->nbr);
   if (d->flags) (void)hipFree(d->flags);
+  if (d->bc_dev) (void)hipFree(d->bc_dev);
   delete d;
   return PSL_OK;
 }
@@ -248,7 +248,7 @@ int psl_diad_set_flags(psl_diad* d, const uint8_t* host) {
 
 int psl_diad_set_bc_values(psl_diad* d, const float* v16) {
   PSL_CHECK_ARG(d && v16, "null argument");
-  memcpy(d->bcvals.v, v16, sizeof(d->bcvals.v));
+  PSL_HIP_TRY(hipMemcpy(d->bc_dev, v16, 4 * sizeof(float4), hipMemcpyHostToDevice));
   return PSL_OK;
 }
 

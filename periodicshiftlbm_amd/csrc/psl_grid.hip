@@ -12,10 +12,12 @@
 // Kernels of one step:
 //   k_ps_bulk      fluid + bounceback cells, one thread per cell, the hot path;
 //   k_ps_boundary  box-face boundary cells from a compact list (restore +
-//                  MBBC + collision); disjoint from the bulk cells, so the two
-//                  launches together equal the reference's single update;
+//                  MBBC + collision), on a side stream concurrently with the
+//                  bulk: the cell sets are disjoint, so the two launches
+//                  together equal the reference's single update;
 //   k_periodic     applyBCPeriodicX/Y surface copies (before the update).
 #include "psl_internal.h"
+#include "psl_stream.h"
 
 #include <algorithm>
 #include <cmath>
@@ -27,116 +29,63 @@ namespace psl {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
-// --------------------------------------------------------------- addressing
-// Population q of cell c lives at element (c + shift[q]) mod N of row q.  For
-// each q the host passes two row bases, lo[q] = row + shift[q] (cells below
-// the wrap threshold thr[q] = N - shift[q]) and hi[q] = lo[q] - N (cells at
-// or above it), so that the element is base + c either way.  A wave whose 64
-// cells lie on one side of every threshold picks each base with one scalar
-// compare/select and addresses all 27 populations with the same per-lane byte
-// offset c * sizeof(T) (SGPR base + VGPR offset loads, no per-q vector
-// arithmetic).  At most one wave per q straddles a threshold; those waves
-// select the base per lane.
-struct StepParams {
-  const void* lo[27];
-  const void* hi[27];
-  uint32_t thr[27];
-  uint32_t n;
-  uint32_t xcd_remap;  // non-zero: consecutive cell ranges stay on one XCD
-  float omega;  // 1/tau, used when cs == 0 and there is no per-cell cs
-  float tau;
-  float cs;
-  const float* cs_cell;
-  const uint8_t* flags;
+// ---------------------------------------------------------------- the step
+// The local cell update (include/applyLocalCellUpdate.h:1-74) is two kernels
+// over disjoint cell sets: k_ps_boundary takes the listed box-face cells (the
+// branch at lines 46-67), k_ps_bulk the fluid and bounceback cells in cell
+// order.  After applyStreaming every cell owns its 27 slots, so the two never
+// touch the same element and run concurrently on two streams: the short,
+// latency-bound list hides under the bandwidth-bound bulk.  (One kernel with
+// both paths needs ~100 VGPRs against ~70 for either alone, which halves the
+// bulk occupancy.)
+struct BoundaryParams {
+  const uint32_t* list;  // box-face cells in cell order
+  uint32_t n;            // list length
+  int nx, ny, nz;
+  const float4* bc;      // 4 given-value slots (rho, ux, uy, uz)
 };
 
-__device__ __forceinline__ uint32_t wave_first_cell(uint32_t block_first) {
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  return block_first + wave * 64u;
-}
-
-// True when some population's wrap threshold falls inside this wave: one
-// lane per population tests it and a ballot makes the answer wave-uniform.
-__device__ __forceinline__ bool wave_straddles(const StepParams& p, uint32_t w_first) {
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t w_last = min(w_first + 63u, p.n - 1u);
-  bool s = false;
-  if (lane < 27u) {
-    const uint32_t t = p.thr[lane];
-    s = (w_first < t) && (w_last >= t);
-  }
-  return __ballot(s) != 0;
-}
-
-// Tuning bits of the bulk kernel (chosen per grid by psl_grid_autotune).
-enum : int { TUNE_NT_LOAD = 1, TUNE_NT_STORE = 2 };
-
-template <class T, bool NT>
-__device__ __forceinline__ T ld(const T* p) {
-  if (NT) return __builtin_nontemporal_load(p);
-  return *p;
-}
-template <class T, bool NT>
-__device__ __forceinline__ void st(T* p, T v) {
-  if (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
-}
-
-template <class St, int TUNE>
-__device__ __forceinline__ void load27(const StepParams& p, uint32_t w_first, bool straddle, uint32_t cell,
-                                       float (&v)[27]) {
+template <class St>
+__device__ __forceinline__ void boundary_update(const StepParams& p, const BoundaryParams& b, uint32_t t) {
   typedef typename St::T T;
-  constexpr bool NT = (TUNE & TUNE_NT_LOAD) != 0;
-  if (!straddle) {
-    const uint32_t off = cell * (uint32_t)sizeof(T);
+  if (t >= b.n) return;
+  const uint32_t cell = b.list[t];
+  const int fl = p.flags[cell];
+  const int kind = fl & 7;
+  const uint32_t nxy = (uint32_t)b.nx * (uint32_t)b.ny;
+  const int k = (int)(cell / nxy);
+  const uint32_t rem = cell - (uint32_t)k * nxy;
+  const int j = (int)(rem / (uint32_t)b.nx);
+  const int i = (int)(rem - (uint32_t)j * (uint32_t)b.nx);
+  float v[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const char* base = (const char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      v[q] = St::load(ld<T, NT>((const T*)(base + off)));
-    }
-  } else {
+  for (int q = 0; q < 27; ++q) {
+    const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+    v[q] = St::load(base[cell]);
+    if (St::W_OFF) v[q] += weight(q);
+  }
+  const int code = outer_normal_code(i, j, k, b.nx, b.ny, b.nz, fl);
+  // given (rho, u) of the cell's slot: one indexed load from the device table
+  // (a select chain over the four slots was miscompiled for divergent slots,
+  // so no branch or select is used here)
+  const float4 g = b.bc[(fl >> 6) & 3];
+  boundary_cell(v, kind, code, g.x, g.y, g.z, g.w);
+  Relax rx{p.omega, p.tau, p.cs};
+  const float cs = p.cs_cell ? p.cs_cell[cell] : p.cs;
+  collide<false>(v, rx, cs);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      v[q] = St::load(ld<T, NT>(base + cell));
-    }
+  for (int q = 0; q < 27; ++q) {
+    T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+    const float o = St::W_OFF ? v[q] - weight(q) : v[q];
+    base[cell] = St::store(o);
   }
 }
 
-template <class St, int TUNE>
-__device__ __forceinline__ void store27(const StepParams& p, uint32_t w_first, bool straddle, uint32_t cell,
-                                        const float (&v)[27]) {
-  typedef typename St::T T;
-  constexpr bool NT = (TUNE & TUNE_NT_STORE) != 0;
-  if (!straddle) {
-    const uint32_t off = cell * (uint32_t)sizeof(T);
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      char* base = (char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      st<T, NT>((T*)(base + off), St::store(v[q]));
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      st<T, NT>(base + cell, St::store(v[q]));
-    }
-  }
+template <class St>
+__global__ __launch_bounds__(256) void k_ps_boundary(StepParams p, BoundaryParams b) {
+  boundary_update<St>(p, b, blockIdx.x * 256u + threadIdx.x);
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8),
-// each with its own L2.  The PS streams are misaligned, so the cache line at
-// every block boundary of every stream is shared by neighbouring blocks;
-// remapping gives each XCD one contiguous range of blocks, so those lines are
-// fetched into one L2 only.  The map is a bijection of [0, nb).
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-  const uint32_t xcd = b & 7u, idx = b >> 3, per = nb >> 3, rem = nb & 7u;
-  return xcd * per + min(xcd, rem) + idx;
-}
-
-// ------------------------------------------------------------- bulk update
-// Fluid and bounceback cells: include/applyLocalCellUpdate.h:1-74 minus the
-// box-face branch (46-67), which k_ps_boundary handles.
 template <class St, bool FLAGS, bool LES, int TUNE>
 __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
   const uint32_t block = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -164,48 +113,6 @@ __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
   store27<St, TUNE>(p, w_first, straddle, cell, v);
 }
 
-// ---------------------------------------------------------- boundary update
-// include/applyLocalCellUpdate.h:40-69 for listed box-face cells.
-template <class St>
-__global__ __launch_bounds__(256) void k_ps_boundary(StepParams p, const uint32_t* __restrict__ list, uint32_t nlist,
-                                                     int nx, int ny, int nz, BcTable bc) {
-  typedef typename St::T T;
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= nlist) return;
-  const uint32_t cell = list[t];
-  const int fl = p.flags[cell];
-  const int kind = fl & 7;
-  const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
-  const int k = (int)(cell / nxy);
-  const uint32_t rem = cell - (uint32_t)k * nxy;
-  const int j = (int)(rem / (uint32_t)nx);
-  const int i = (int)(rem - (uint32_t)j * (uint32_t)nx);
-  float v[27];
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-    v[q] = St::load(base[cell]);
-    if (St::W_OFF) v[q] += weight(q);
-  }
-  const int code = outer_normal_code(i, j, k, nx, ny, nz, fl);
-  // given (rho, u) of slot 0..3, selected without dynamic indexing
-  const int slot = (fl >> 6) & 3;
-  float g[4];
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-    g[c] = slot == 0 ? bc.v[c] : slot == 1 ? bc.v[4 + c] : slot == 2 ? bc.v[8 + c] : bc.v[12 + c];
-  boundary_cell(v, kind, code, g[0], g[1], g[2], g[3]);
-  Relax rx{p.omega, p.tau, p.cs};
-  const float cs = p.cs_cell ? p.cs_cell[cell] : p.cs;
-  collide<false>(v, rx, cs);
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-    const float o = St::W_OFF ? v[q] - weight(q) : v[q];
-    base[cell] = St::store(o);
-  }
-}
-
 // ------------------------------------------------------------ periodic copy
 // include/boundaryConditions/periodicXY.h:38-128: +c populations right->left,
 // -c populations left->right, raw storage words.
@@ -217,10 +124,14 @@ __constant__ int c_neg_y[9] = {5, 11, 14, 16, 18, 20, 21, 23, 25};
 template <class T>
 __global__ __launch_bounds__(256) void k_periodic(T* __restrict__ f, Shift27 sh, uint32_t n, int nx, int ny, int nz,
                                                   int axis) {
-  const int a = blockIdx.x * 256 + threadIdx.x;  // j (axis x) or i (axis y)
-  const int k = blockIdx.y;
+  // one thread per (surface cell, copied population): 18 independent copies
   const int na = axis == 0 ? ny : nx;
-  if (a >= na || k >= nz) return;
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= (uint32_t)na * nz * 18u) return;
+  const int c = (int)(t % 18u);
+  const uint32_t rest = t / 18u;
+  const int a = (int)(rest % (uint32_t)na);
+  const int k = (int)(rest / (uint32_t)na);
   const uint32_t plane = (uint32_t)k * (uint32_t)nx * (uint32_t)ny;
   uint32_t left, right;
   if (axis == 0) {
@@ -230,21 +141,16 @@ __global__ __launch_bounds__(256) void k_periodic(T* __restrict__ f, Shift27 sh,
     left = plane + a;
     right = plane + (uint32_t)(ny - 1) * nx + a;
   }
-  const int* pos = axis == 0 ? c_pos_x : c_pos_y;
-  const int* neg = axis == 0 ? c_neg_x : c_neg_y;
-  auto slot = [&](uint32_t c, int q) -> int64_t {
-    uint32_t s = c + (uint32_t)sh.s[q];
+  const bool pos = c < 9;
+  const int q = pos ? (axis == 0 ? c_pos_x[c] : c_pos_y[c]) : (axis == 0 ? c_neg_x[c - 9] : c_neg_y[c - 9]);
+  auto slot = [&](uint32_t cell) -> int64_t {
+    uint32_t s = cell + (uint32_t)sh.s[q];
     if (s >= n) s -= n;
     return (int64_t)q * n + s;
   };
-  for (int t = 0; t < 9; ++t) {
-    const int q = pos[t];
-    f[slot(left, q)] = f[slot(right, q)];
-  }
-  for (int t = 0; t < 9; ++t) {
-    const int q = neg[t];
-    f[slot(right, q)] = f[slot(left, q)];
-  }
+  // +c populations right -> left, -c populations left -> right
+  if (pos) f[slot(left)] = f[slot(right)];
+  else f[slot(right)] = f[slot(left)];
 }
 
 // --------------------------------------------------------- equilibrium fill
@@ -533,6 +439,15 @@ static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, c
   }
 }
 
+// Side stream and fork/join events of a grid, created on first use.
+static int ensure_side_stream(psl_grid* g) {
+  if (g->side) return PSL_OK;
+  PSL_HIP_TRY(hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
+  PSL_HIP_TRY(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+  PSL_HIP_TRY(hipEventCreateWithFlags(&g->join, hipEventDisableTiming));
+  return PSL_OK;
+}
+
 template <class St>
 static int launch_local_update(psl_grid* g) {
   typedef typename St::T T;
@@ -551,32 +466,48 @@ static int launch_local_update(psl_grid* g) {
   p.cs = g->cs;
   p.cs_cell = g->cs_cell;
   p.flags = g->flags;
+  p.xcd_remap = (g->tuning & PSL_TUNE_XCD_REMAP) ? 1u : 0u;
   const bool les = g->cs != 0.f || g->cs_cell != nullptr;
-  const uint32_t nb = blocks_for(p.n);
+  const bool boundary = g->flags && g->n_bc;
+  if (boundary) {
+    // fork: the boundary list runs on the side stream beside the bulk
+    int rc = ensure_side_stream(g);
+    if (rc != PSL_OK) return rc;
+    PSL_HIP_TRY(hipEventRecord(g->fork, g->stream));
+    PSL_HIP_TRY(hipStreamWaitEvent(g->side, g->fork, 0));
+    BoundaryParams b;
+    b.list = g->bc_list;
+    b.n = g->n_bc;
+    b.nx = g->info.cellCountX;
+    b.ny = g->info.cellCountY;
+    b.nz = g->info.cellCountZ;
+    b.bc = g->bc_dev;
+    k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->side>>>(p, b);
+    PSL_LAUNCH_CHECK();
+    PSL_HIP_TRY(hipEventRecord(g->join, g->side));
+  }
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (g->profile) {
     PSL_HIP_TRY(hipEventCreate(&ev0));
     PSL_HIP_TRY(hipEventCreate(&ev1));
     PSL_HIP_TRY(hipEventRecord(ev0, g->stream));
   }
-  p.xcd_remap = (g->tuning & PSL_TUNE_XCD_REMAP) ? 1u : 0u;
   const int nt = g->tuning & (TUNE_NT_LOAD | TUNE_NT_STORE);
+  const uint32_t nb = blocks_for(p.n);
+  const bool fl = g->flags != nullptr;
   switch (nt) {
-    case 0: launch_bulk<St, 0>(g->flags != nullptr, les, nb, g->stream, p); break;
-    case 1: launch_bulk<St, 1>(g->flags != nullptr, les, nb, g->stream, p); break;
-    case 2: launch_bulk<St, 2>(g->flags != nullptr, les, nb, g->stream, p); break;
-    default: launch_bulk<St, 3>(g->flags != nullptr, les, nb, g->stream, p); break;
+    case 0: launch_bulk<St, 0>(fl, les, nb, g->stream, p); break;
+    case 1: launch_bulk<St, 1>(fl, les, nb, g->stream, p); break;
+    case 2: launch_bulk<St, 2>(fl, les, nb, g->stream, p); break;
+    default: launch_bulk<St, 3>(fl, les, nb, g->stream, p); break;
   }
   PSL_LAUNCH_CHECK();
   if (g->profile) {
     PSL_HIP_TRY(hipEventRecord(ev1, g->stream));
     g->prof_events.emplace_back(ev0, ev1);
   }
-  if (g->n_bc) {
-    k_ps_boundary<St><<<blocks_for(g->n_bc), 256, 0, g->stream>>>(p, g->bc_list, g->n_bc, g->info.cellCountX,
-                                                                  g->info.cellCountY, g->info.cellCountZ, g->bcvals);
-    PSL_LAUNCH_CHECK();
-  }
+  // join: later work on the grid's stream sees the boundary cells too
+  if (boundary) PSL_HIP_TRY(hipStreamWaitEvent(g->stream, g->join, 0));
   return PSL_OK;
 }
 
@@ -647,7 +578,7 @@ template <class St> struct MassFn {
 static int periodic_copy(psl_grid* g, int axis) {
   const psl_info& I = g->info;
   const int na = axis == 0 ? I.cellCountY : I.cellCountX;
-  dim3 grid(blocks_for(na), I.cellCountZ);
+  const uint32_t grid = blocks_for((uint64_t)na * I.cellCountZ * 18u);
   if (g->storage == PSL_F32)
     k_periodic<float><<<grid, 256, 0, g->stream>>>((float*)g->f, shift_of(g), ncells(g), I.cellCountX, I.cellCountY,
                                                    I.cellCountZ, axis);
@@ -694,9 +625,13 @@ int psl_grid_create(const psl_info* info, int storage, psl_grid** out) {
     return PSL_E_NOMEM;
   }
   e = hipMalloc(&g->result, 4 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&g->bc_dev, 4 * sizeof(float4));
+  if (e == hipSuccess) e = hipMemset(g->bc_dev, 0, 4 * sizeof(float4));
   if (e != hipSuccess) {
-    set_error(std::string("hipMalloc result: ") + hipGetErrorString(e));
+    set_error(std::string("hipMalloc grid tables: ") + hipGetErrorString(e));
     (void)hipFree(g->f);
+    if (g->result) (void)hipFree(g->result);
+    if (g->bc_dev) (void)hipFree(g->bc_dev);
     delete g;
     return PSL_E_NOMEM;
   }
@@ -719,6 +654,10 @@ int psl_grid_destroy(psl_grid* g) {
   if (g->cs_cell) (void)hipFree(g->cs_cell);
   if (g->partials) (void)hipFree(g->partials);
   if (g->result) (void)hipFree(g->result);
+  if (g->bc_dev) (void)hipFree(g->bc_dev);
+  if (g->fork) (void)hipEventDestroy(g->fork);
+  if (g->join) (void)hipEventDestroy(g->join);
+  if (g->side) (void)hipStreamDestroy(g->side);
   delete g;
   return PSL_OK;
 }
@@ -769,6 +708,8 @@ int psl_grid_get_flags(const psl_grid* g, uint8_t* host) {
 int psl_grid_set_bc_values(psl_grid* g, const float* v) {
   PSL_CHECK_ARG(g && v, "null argument");
   memcpy(g->bcvals.v, v, sizeof(g->bcvals.v));
+  PSL_HIP_TRY(hipMemcpyAsync(g->bc_dev, g->bcvals.v, sizeof(g->bcvals.v), hipMemcpyHostToDevice, g->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
   return PSL_OK;
 }
 
