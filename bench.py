@@ -9,10 +9,12 @@ applyLocalCellUpdate (bulk + boundary kernels).  Inputs are resident in HBM
 before timing starts.
 
 value = MLUPS of the FP32-storage step (cellCount * steps / wall time between
-device synchronisations).  FP16 / BF16 deviation-storage variants are
-reported under "variants".  roofline.achieved = algorithmic bytes of the bulk
-kernel per launch (216 B FP32 / 108 B 16-bit per updated cell x bulk cells)
-divided by its mean duration from HIP events on its stream.
+device synchronisations).  FP16 / BF16 deviation-storage variants and the
+64-domain batch are reported under "variants".  roofline.achieved =
+algorithmic bytes of the dominant launch (216 B FP32 / 108 B 16-bit per
+updated cell x the cells it updates: bulk cells for k_ps_bulk, bulk + listed
+boundary cells for k_ps_fused) divided by its mean duration from HIP events
+on its stream.
 
 cpu_baseline: the project's FP32 CPU oracle (OpenMP) on the same workload for
 a bounded number of steps on this host's cores.
@@ -75,6 +77,38 @@ def kernel_time(P, g, steps: int):
     return total_ms / max(1, launches), launches
 
 
+def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
+    """64 sphere-in-channel domains of uneven size advanced by one hipGraph
+    replay per step; aggregate MLUPS and per-domain drag coefficient."""
+    doms = cases.batch_domains()
+    grids = [cases.domain_grid(d, storage)[0] for d in doms]
+    b = P.Batch(grids)
+    cells, blocks, nbnd = b.census()
+    tune = None
+    if autotune_steps:
+        t, o, times = b.autotune(autotune_steps)
+        tune = {"tuning": t, "order": o, "ms_per_step": [round(float(x), 4) for x in times]}
+        for g in grids:  # restart from rest after tuning
+            P.fill_equilibrium(g)
+    P.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    P.synchronize()
+    el = time.perf_counter() - t0
+    cd = []
+    for d, g in zip(doms, grids):
+        lo, hi = cases.sphere_box_of(d["center"], d["diameter"])
+        fx = g.momentum_exchange(lo, hi)[0]
+        cd.append(cases.drag_coefficient(fx, d["u_inlet"], d["diameter"]))
+    b.close()
+    for g in grids:
+        g.close()
+    return {"mlups": cells * steps / el / 1e6, "ms_per_step": el * 1e3 / steps, "domains": len(doms),
+            "cells": cells, "boundary_cells": nbnd, "steps": steps, "storage": storage, "autotune": tune,
+            "drag_coefficient": [round(float(c), 5) for c in cd],
+            "L": [d["L"] for d in doms], "re": [round(d["re"], 1) for d in doms]}
+
+
 def cpu_baseline(cases, flags, nu, cs, shape, budget_s: float):
     import oracle as O
 
@@ -105,6 +139,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f32,f16,bf16")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the 64-domain batch (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times) for counter calibration")
     args = ap.parse_args()
@@ -145,19 +180,25 @@ def main():
         bulk, nbc, skipped = g.census()
         ksteps = min(args.steps, 200)
         avg_ms, launches = kernel_time(P, g, ksteps)
-        bytes_per_launch = BYTES_PER_UPDATE[storage] * bulk
+        # the timed launch: k_ps_fused updates the bulk and the boundary list,
+        # k_ps_bulk the bulk only (the list follows in its own launch)
+        fused = bool(g.tuning & P.TUNE_FUSED_BOUNDARY) and nbc > 0
+        kernel = "k_ps_fused" if fused else "k_ps_bulk"
+        cells_per_launch = bulk + (nbc if fused else 0)
+        bytes_per_launch = BYTES_PER_UPDATE[storage] * cells_per_launch
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         lo, hi = cases.sphere_box()
         fx = g.momentum_exchange(lo, hi)[0]
         results[storage] = {
             "mlups": mlups, "ms_per_step": el * 1e3 / args.steps, "bulk_kernel_ms": avg_ms,
+            "kernel": kernel, "cells_per_launch": cells_per_launch,
             "bulk_cells": bulk, "boundary_cells": nbc, "solid_cells": nsolid,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "traffic": None},
             "drag_force_x_after_run": fx,
             "tuning": g.tuning, "autotune_ms": tune_times,
         }
-        log(f"  {mlups:.1f} MLUPS, bulk kernel {avg_ms * 1e3:.1f} us ({achieved:.0f} GB/s, "
+        log(f"  {mlups:.1f} MLUPS, {kernel} {avg_ms * 1e3:.1f} us ({achieved:.0f} GB/s, "
             f"{achieved / HBM_PEAK_GBS * 100:.1f}% of HBM peak)")
         g.close()
 
@@ -180,15 +221,21 @@ def main():
                                "path, velocity inlet U=0.04, pressure outlet, periodic y/z, Re=1000, Cs=0.17; "
                                "step = streaming + periodic-y copy + fused bulk + boundary kernels",
                    "cells": ncell, "storage": head_storage},
-        "roofline": dict(head["roofline"], kernel="k_ps_bulk", bytes_per_cell=BYTES_PER_UPDATE[head_storage],
-                         cells_per_launch=head["bulk_cells"], avg_launch_ms=head["bulk_kernel_ms"]),
+        "roofline": dict(head["roofline"], kernel=head["kernel"], bytes_per_cell=BYTES_PER_UPDATE[head_storage],
+                         cells_per_launch=head["cells_per_launch"], avg_launch_ms=head["bulk_kernel_ms"]),
         "variants": {k: {"mlups": v["mlups"], "ms_per_step": v["ms_per_step"], "roofline_frac": v["roofline"]["frac"],
-                         "achieved_GBs": v["roofline"]["achieved"], "bulk_kernel_ms": v["bulk_kernel_ms"],
+                         "achieved_GBs": v["roofline"]["achieved"], "kernel": v["kernel"],
+                         "kernel_ms": v["bulk_kernel_ms"],
                          "tuning": v["tuning"], "autotune_ms": v["autotune_ms"]}
                      for k, v in results.items()},
     }
     if "f32" in results and "f16" in results:
         out["variants"]["f16_over_f32"] = results["f16"]["mlups"] / results["f32"]["mlups"]
+    if args.batch_steps:
+        log(f"[bench] batch of 64 domains, {args.batch_steps} steps")
+        bl = batch_leg(P, cases, args.batch_steps, 0 if args.no_autotune else 3)
+        log(f"  {bl['mlups']:.1f} MLUPS over {bl['cells']} cells")
+        out["variants"]["batch"] = bl
     if not args.no_cpu:
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(cases, flags_keep, nu_keep, 0.17, shape, args.cpu_seconds)

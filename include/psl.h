@@ -155,12 +155,21 @@ int psl_device_copy(void* dst, const void* src, size_t bytes);
 /* Device buffer helpers for callers without a device allocator. */
 int psl_device_alloc(size_t bytes, void** out);
 int psl_device_free(void* ptr);
-/* Bulk-kernel variants: non-temporal loads / stores, XCD-contiguous block order. */
-enum { PSL_TUNE_NT_LOAD = 1, PSL_TUNE_NT_STORE = 2, PSL_TUNE_XCD_REMAP = 4 };
+/* Local-update variants (bits): non-temporal loads / stores and XCD-contiguous
+ * block order in the bulk kernel; FUSED_BOUNDARY updates the box-face list in
+ * the same launch as the bulk instead of a second launch. */
+enum {
+  PSL_TUNE_NT_LOAD = 1,
+  PSL_TUNE_NT_STORE = 2,
+  PSL_TUNE_XCD_REMAP = 4,
+  PSL_TUNE_FUSED_BOUNDARY = 8,
+  PSL_TUNE_VARIANTS = 16
+};
 int psl_grid_set_tuning(psl_grid* grid, int tuning);
 int psl_grid_get_tuning(const psl_grid* grid, int* tuning);
-/* Autotuner: times all 8 variants on this grid (steps bulk updates each; the state
- * advances), keeps the fastest; best and times_ms[8] (mean kernel ms) are optional. */
+/* Autotuner: times all PSL_TUNE_VARIANTS variants on this grid (`steps` local
+ * updates each after one warm-up; the state advances) and keeps the fastest;
+ * best and times_ms[PSL_TUNE_VARIANTS] (mean ms per local update) are optional. */
 int psl_grid_autotune(psl_grid* grid, int steps, int* best, double* times_ms);
 /* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
 int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
@@ -170,9 +179,12 @@ int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* bou
  * reference steps one grid at a time).  Each grid keeps its own flags, given
  * values, Cs and storage (all grids share the storage); one step of all grids
  * is a single replay of a captured hipGraph: a device shift update, one bulk
- * launch over the blocks of every grid and one launch over all boundary lists.
- * Grids must not use per-cell Cs, periodic copies or sub-grids.  The grids'
- * own handles stay valid (get_f, reductions, single-grid steps). */
+ * launch over the blocks of every grid and one launch over all boundary lists
+ * (or one fused launch).  Every grid's result is bit-identical to stepping it
+ * alone.  Grids need flags and must not use per-cell Cs, periodic copies or
+ * sub-grids.  The grids' own handles stay valid (get_f, reductions,
+ * single-grid steps); changes to a grid's flags, values, nu or Cs are picked
+ * up at the next psl_batch_step.  Destroy the batch before its grids. */
 typedef struct psl_batch psl_batch;
 int psl_batch_create(psl_grid* const* grids, int count, psl_batch** out);
 int psl_batch_destroy(psl_batch* batch);
@@ -180,8 +192,9 @@ int psl_batch_step(psl_batch* batch, int nsteps);
 /* order 0: grids in the given order, 1: largest grid first (block order of the launch). */
 int psl_batch_set_tuning(psl_batch* batch, int tuning, int order);
 int psl_batch_get_tuning(const psl_batch* batch, int* tuning, int* order);
-/* Times the 8 variants x 2 orders (steps steps each; the grids advance) and keeps the
- * fastest; times_ms[16] receives ms per step. */
+/* Times the PSL_TUNE_VARIANTS variants x 2 orders (one warm-up step and `steps`
+ * steps each; the grids advance) and keeps the fastest; times_ms[2 * PSL_TUNE_VARIANTS]
+ * receives ms per step at order * PSL_TUNE_VARIANTS + variant. */
 int psl_batch_autotune(psl_batch* batch, int steps, int* best_tuning, int* best_order, double* times_ms);
 int psl_batch_census(const psl_batch* batch, int64_t* cells, int* blocks, int* boundary_cells);
 

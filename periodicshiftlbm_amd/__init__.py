@@ -38,7 +38,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpsl_amd.so")
 
 F32, F16DEV, BF16DEV = 0, 1, 2
-TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_XCD_REMAP = 1, 2, 4
+TUNE_NT_LOAD, TUNE_NT_STORE, TUNE_XCD_REMAP, TUNE_FUSED_BOUNDARY = 1, 2, 4, 8
+TUNE_VARIANTS = 16
 STORAGE_NAMES = {"f32": F32, "f16": F16DEV, "bf16": BF16DEV}
 
 FLUID, BOUNCEBACK, GIVEN_RHO, GIVEN_U, GIVEN_NONE, GIVEN_RHOU, GHOST, COVERED = range(8)
@@ -370,9 +371,9 @@ class GridStruct:
         _check(lib().psl_grid_set_tuning(self._h, int(v)))
 
     def autotune(self, steps: int = 20):
-        """Time the 8 bulk-kernel variants on this grid and keep the fastest (state advances)."""
+        """Time the local-update variants on this grid and keep the fastest (state advances)."""
         best = ctypes.c_int()
-        times = np.zeros(8, np.float64)
+        times = np.zeros(TUNE_VARIANTS, np.float64)
         _check(lib().psl_grid_autotune(self._h, steps, ctypes.byref(best), times.ctypes.data))
         return best.value, times
 
@@ -421,7 +422,7 @@ class Batch:
 
     def autotune(self, steps: int = 10):
         t, o = ctypes.c_int(), ctypes.c_int()
-        times = np.zeros(16, np.float64)
+        times = np.zeros(2 * TUNE_VARIANTS, np.float64)
         _check(lib().psl_batch_autotune(self._h, steps, ctypes.byref(t), ctypes.byref(o), times.ctypes.data))
         return t.value, o.value, times
 

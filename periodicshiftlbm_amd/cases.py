@@ -11,6 +11,11 @@ evaluated on the host into flag bytes and initial fields.
 * Flow past a sphere in a 256 x 128 x 128 channel: icosphere of diameter 32
   voxelized through the STL path, velocity inlet (given u) at i = 0, pressure
   outlet (given rho) at i = Nx-1, periodic y (copies) and z (implicit).
+* A batch of independent sphere-in-channel domains of uneven size (2L x L x L,
+  L uniform in [32, 96], sphere diameter L/4, Re log-uniform in [100, 5000])
+  drawn from a counter-based generator with a fixed seed.  No reference
+  counterpart: flowAroundSphere/flowAroundSphereMainBatch.cu:179-292 sweeps
+  Reynolds numbers one grid at a time.
 """
 from __future__ import annotations
 
@@ -151,3 +156,73 @@ def drag_coefficient(fx: float, u_inlet: float = 0.04, diameter: float = SPHERE_
     the reference's sign convention gives C_D = -8 F / (rho U^2 pi D^2) for F = sum rho (u - u_bb)
     (flowAroundSphereMainSingle.cu:250)."""
     return -8.0 * fx / (u_inlet * u_inlet * math.pi * diameter * diameter)
+
+
+# ------------------------------------------------------------ domain batch
+_MASK64 = (1 << 64) - 1
+
+
+def counter_uniform(seed: int, counter: int) -> float:
+    """Uniform [0, 1) from (seed, counter): the splitmix64 finaliser applied to
+    seed + (counter + 1) * 0x9E3779B97F4A7C15, top 53 bits."""
+    z = (seed + (counter + 1) * 0x9E3779B97F4A7C15) & _MASK64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+    z ^= z >> 31
+    return (z >> 11) / float(1 << 53)
+
+
+def batch_domains(count: int = 64, seed: int = 20240607, l_min: int = 32, l_max: int = 96,
+                  re_min: float = 100.0, re_max: float = 5000.0, u_inlet: float = 0.04):
+    """Parameters of `count` sphere-in-channel domains: edge L uniform in
+    [l_min, l_max], box 2L x L x L, sphere of diameter L/4 centred at
+    (L/2, L/2, L/2), Re log-uniform in [re_min, re_max], nu = U D / Re."""
+    out = []
+    for d in range(count):
+        L = l_min + int(counter_uniform(seed, 2 * d) * (l_max - l_min + 1))
+        re = re_min * (re_max / re_min) ** counter_uniform(seed, 2 * d + 1)
+        diameter = 0.25 * L
+        out.append({"L": L, "shape": (2 * L, L, L), "re": re, "diameter": diameter,
+                    "center": (0.5 * L, 0.5 * L, 0.5 * L), "nu": u_inlet * diameter / re, "u_inlet": u_inlet})
+    return out
+
+
+def walled_channel_flags(shape, solid: np.ndarray | None = None) -> np.ndarray:
+    """Flag bytes of a batch domain: velocity inlet (slot 0) at i = 0, pressure
+    outlet (slot 1) at i = Nx-1, bounceback walls at j = 0 and j = Ny-1, z
+    periodic through the shift (the inlet/outlet normal ignores z)."""
+    nx, ny, nz = shape
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    flags = np.full((nz, ny, nx), FLUID, np.uint8)
+    flags[i == 0] = GIVEN_U | PERIODIC_Z | (0 << 6)
+    flags[i == nx - 1] = GIVEN_RHO | PERIODIC_Z | (1 << 6)
+    flags[(j == 0) | (j == ny - 1)] = BOUNCEBACK
+    flags = flags.reshape(-1)
+    if solid is not None:
+        flags[np.asarray(solid, bool).reshape(-1)] = BOUNCEBACK
+    return flags
+
+
+def sphere_box_of(center, diameter):
+    """[lo, hi) box around a sphere for the momentum-exchange reduction."""
+    r = diameter / 2
+    return [int(c - r) - 1 for c in center], [int(c + r) + 2 for c in center]
+
+
+def domain_grid(dom: dict, storage: str = "f16", cs: float = 0.17, subdivisions: int = 3):
+    """GridStruct of one batch domain: the sphere is an icosphere voxelized by
+    the STL path, the grid starts at rest (rho = 1, u = 0)."""
+    from . import (BoolArray, GridStruct, InfoStruct, STLStruct, applyMarkersInsideSTL, fill_equilibrium)
+
+    nx, ny, nz = dom["shape"]
+    info = InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=dom["nu"], res=1.0)
+    stl = STLStruct.from_triangles(icosphere(subdivisions, dom["diameter"] / 2, dom["center"]))
+    mask = BoolArray(nx * ny * nz)
+    applyMarkersInsideSTL(mask, stl, True, info)
+    flags = walled_channel_flags(dom["shape"], mask.get().astype(bool))
+    g = GridStruct(info, storage)
+    g.set_flags(flags)
+    g.set_given_values(channel_given_values(dom["u_inlet"]))
+    g.set_smagorinsky(cs)
+    fill_equilibrium(g)
+    return g, flags

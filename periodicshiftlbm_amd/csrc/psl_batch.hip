@@ -6,15 +6,19 @@
 // list, given values, storage) and a batch advances all of them together:
 //
 //   k_batch_advance   applyStreaming of every grid on the device: the shift
-//                     update and the per-population base / threshold tables;
-//   k_batch_bulk      one launch over the concatenated blocks of all grids
-//                     (block -> grid table), the same per-cell path as k_ps_bulk;
-//   k_batch_boundary  one launch over the concatenated box-face lists.
+//                     update and the per-population bases of the step, written
+//                     into each grid's StepParams;
+//   k_batch_bulk      one launch over the concatenated bulk blocks of all
+//                     grids (block -> grid table);
+//   k_batch_boundary  one launch over the concatenated boundary-list blocks;
+//   k_batch_fused     both in one launch (a tuning variant).
 //
-// One step is captured once in a hipGraph and replayed, so a step costs one
-// graph launch whatever the number of grids.  The block order of the grids
-// (as given, or largest first) and the bulk-kernel variant are chosen by
-// psl_batch_autotune.
+// The per-cell work is bulk_update / boundary_update, the same functions on
+// the same StepParams / BoundaryParams as the single-grid kernels, so every
+// grid's result is bit-identical to stepping it alone.  One step is captured
+// once in a hipGraph and replayed, so a step costs one graph launch whatever
+// the number of grids.  The block order of the grids (as given, or largest
+// first) and the kernel variant are chosen by psl_batch_autotune.
 #include "psl_internal.h"
 #include "psl_stream.h"
 
@@ -24,124 +28,64 @@
 
 namespace psl {
 
-struct DomainDesc {
-  char* f;
-  const uint8_t* flags;
-  const uint32_t* bc_list;
-  uint32_t n, n_bc;
-  int nx, ny, nz;
-  uint32_t block_first;
-  float omega, tau, cs;
-  int delta[27];  // shift decrement per step: c_x + Nx c_y + Nx Ny c_z
-  const float4* bc;  // the grid's given-value slots (device)
-};
-
-// Per-grid tables of the current step (written by k_batch_advance).
-struct DomainTable {
-  const void* lo[27];
-  const void* hi[27];
-  uint32_t thr[27];
-  uint32_t n;
-};
-
 // applyStreaming of every grid (include/applyStreaming.h:11-31) and the
 // per-population stream bases of the step.
 template <class St>
-__global__ void k_batch_advance(const DomainDesc* __restrict__ desc, int* __restrict__ shift,
-                                DomainTable* __restrict__ table, int nd) {
+__global__ void k_batch_advance(StepParams* __restrict__ params, const void* const* __restrict__ fbase,
+                                const int* __restrict__ delta, int* __restrict__ shift, int nd) {
   typedef typename St::T T;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nd * 27) return;
   const int d = t / 27, q = t - d * 27;
-  const DomainDesc& D = desc[d];
-  const int64_t N = D.n;
+  StepParams& P = params[d];
+  const int64_t N = P.n;
   int64_t s = shift[t];
-  s -= D.delta[q];
+  s -= delta[t];
   if (s < 0) s += N;
   if (s < 0) s += N;
   if (s >= N) s -= N;
   if (s >= N) s -= N;
   shift[t] = (int)s;
-  T* row = (T*)D.f + (size_t)q * D.n + s;
-  table[d].lo[q] = row;
-  table[d].hi[q] = row - D.n;
-  table[d].thr[q] = D.n - (uint32_t)s;
-  if (q == 0) table[d].n = D.n;
+  T* row = (T*)fbase[d] + (size_t)q * P.n + s;
+  P.lo[q] = row;
+  P.hi[q] = row - P.n;
+  P.thr[q] = P.n - (uint32_t)s;
 }
 
 template <class St, int TUNE>
-__global__ __launch_bounds__(256) void k_batch_bulk(const DomainDesc* __restrict__ desc,
-                                                    const DomainTable* __restrict__ table,
-                                                    const uint32_t* __restrict__ block_domain, uint32_t xcd_remap) {
-  const uint32_t b = xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+__global__ __launch_bounds__(256) void k_batch_bulk(const StepParams* __restrict__ params,
+                                                    const uint32_t* __restrict__ block_domain,
+                                                    const uint32_t* __restrict__ block_first, uint32_t remap) {
+  const uint32_t b = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint32_t d = block_domain[b];
-  const DomainDesc& D = desc[d];
-  const DomainTable& T = table[d];
-  const uint32_t block_first = (b - D.block_first) * 256u;
-  const uint32_t cell = block_first + threadIdx.x;
-  const uint32_t w_first = wave_first_cell(block_first);
-  if (w_first >= D.n) return;
-  const bool straddle = wave_straddles(T, w_first);
-  if (cell >= D.n) return;
-  const int kind = D.flags ? (D.flags[cell] & 7) : KIND_FLUID;
-  if (kind > KIND_BOUNCEBACK) return;
-  float v[27];
-  load27<St, TUNE>(T, w_first, straddle, cell, v);
-  if (kind == KIND_BOUNCEBACK) {
-    bounceback(v);
-  } else {
-    Relax rx{D.omega, D.tau, D.cs};
-    collide<St::W_OFF>(v, rx, D.cs);
-  }
-  store27<St, TUNE>(T, w_first, straddle, cell, v);
+  bulk_update<St, true, true, TUNE>(params[d], b - block_first[d]);
 }
 
 template <class St>
-__global__ __launch_bounds__(256) void k_batch_boundary(const DomainDesc* __restrict__ desc,
-                                                        const DomainTable* __restrict__ table,
-                                                        const uint32_t* __restrict__ bnd_domain,
-                                                        const uint32_t* __restrict__ bnd_cell, uint32_t total) {
-  typedef typename St::T T;
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= total) return;
-  const uint32_t d = bnd_domain[t];
-  const uint32_t cell = bnd_cell[t];
-  const DomainDesc& D = desc[d];
-  const DomainTable& Tb = table[d];
-  const int fl = D.flags[cell];
-  const int kind = fl & 7;
-  const uint32_t nxy = (uint32_t)D.nx * (uint32_t)D.ny;
-  const int k = (int)(cell / nxy);
-  const uint32_t rem = cell - (uint32_t)k * nxy;
-  const int j = (int)(rem / (uint32_t)D.nx);
-  const int i = (int)(rem - (uint32_t)j * (uint32_t)D.nx);
-  float v[27];
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    const T* base = (const T*)(cell >= Tb.thr[q] ? Tb.hi[q] : Tb.lo[q]);
-    v[q] = St::load(base[cell]);
-    if (St::W_OFF) v[q] += weight(q);
-  }
-  const int code = outer_normal_code(i, j, k, D.nx, D.ny, D.nz, fl);
-  const float4 g = D.bc[(fl >> 6) & 3];
-  boundary_cell(v, kind, code, g.x, g.y, g.z, g.w);
-  Relax rx{D.omega, D.tau, D.cs};
-  collide<false>(v, rx, D.cs);
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    T* base = (T*)(cell >= Tb.thr[q] ? Tb.hi[q] : Tb.lo[q]);
-    const float o = St::W_OFF ? v[q] - weight(q) : v[q];
-    base[cell] = St::store(o);
-  }
+__global__ __launch_bounds__(256) void k_batch_boundary(const StepParams* __restrict__ params,
+                                                        const BoundaryParams* __restrict__ bparams,
+                                                        const uint32_t* __restrict__ bblock_domain,
+                                                        const uint32_t* __restrict__ bblock_first) {
+  const uint32_t d = bblock_domain[blockIdx.x];
+  boundary_update<St>(params[d], bparams[d], (blockIdx.x - bblock_first[d]) * 256u + threadIdx.x);
 }
 
-// device copy of the boundary list of one grid, tagged with its index
-__global__ void k_tag_list(const uint32_t* __restrict__ src, uint32_t n, uint32_t d, uint32_t* __restrict__ dom,
-                           uint32_t* __restrict__ cell) {
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= n) return;
-  dom[t] = d;
-  cell[t] = src[t];
+// Boundary blocks first (their count nbb is a multiple of 8, padded with
+// blocks past the end of the last list), then the bulk blocks.
+template <class St, int TUNE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_batch_fused(
+    const StepParams* __restrict__ params, const BoundaryParams* __restrict__ bparams,
+    const uint32_t* __restrict__ bblock_domain, const uint32_t* __restrict__ bblock_first, uint32_t nbb,
+    const uint32_t* __restrict__ block_domain, const uint32_t* __restrict__ block_first, uint32_t remap) {
+  if (blockIdx.x < nbb) {
+    const uint32_t d = bblock_domain[blockIdx.x];
+    boundary_update<St>(params[d], bparams[d], (blockIdx.x - bblock_first[d]) * 256u + threadIdx.x);
+    return;
+  }
+  const uint32_t hw = blockIdx.x - nbb;
+  const uint32_t b = remap ? xcd_block(hw, gridDim.x - nbb) : hw;
+  const uint32_t d = block_domain[b];
+  bulk_update<St, true, true, TUNE>(params[d], b - block_first[d]);
 }
 
 }  // namespace psl
@@ -154,16 +98,23 @@ struct psl_batch {
   int tuning = PSL_TUNE_XCD_REMAP;
   int order = 0;  // 0: as given, 1: largest grid first
   hipStream_t stream = nullptr;
-  DomainDesc* d_desc = nullptr;
-  DomainTable* d_table = nullptr;
+  StepParams* d_params = nullptr;
+  BoundaryParams* d_bparams = nullptr;
+  const void** d_fbase = nullptr;
+  int* d_delta = nullptr;
   int* d_shift = nullptr;
   uint32_t* d_block_domain = nullptr;
-  uint32_t* d_bnd_domain = nullptr;
-  uint32_t* d_bnd_cell = nullptr;
-  uint32_t total_blocks = 0, total_bnd = 0;
+  uint32_t* d_block_first = nullptr;
+  uint32_t* d_bblock_domain = nullptr;
+  uint32_t* d_bblock_first = nullptr;
+  uint32_t total_blocks = 0, total_bblocks = 0, total_bnd = 0;
   int64_t total_cells = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // what the captured graph was built from
+  std::vector<StepParams> h_params;
+  std::vector<BoundaryParams> h_bparams;
+  std::vector<const void*> h_fbase;
 };
 
 namespace {
@@ -173,31 +124,56 @@ void free_batch_device(psl_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
-  void* ptrs[] = {b->d_desc, b->d_table, b->d_shift, b->d_block_domain, b->d_bnd_domain, b->d_bnd_cell};
+  void* ptrs[] = {b->d_params,       b->d_bparams,      b->d_fbase,         b->d_delta,         b->d_shift,
+                  b->d_block_domain, b->d_block_first, b->d_bblock_domain, b->d_bblock_first};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  b->d_desc = nullptr;
-  b->d_table = nullptr;
+  b->d_params = nullptr;
+  b->d_bparams = nullptr;
+  b->d_fbase = nullptr;
+  b->d_delta = nullptr;
   b->d_shift = nullptr;
   b->d_block_domain = nullptr;
-  b->d_bnd_domain = nullptr;
-  b->d_bnd_cell = nullptr;
+  b->d_block_first = nullptr;
+  b->d_bblock_domain = nullptr;
+  b->d_bblock_first = nullptr;
 }
 
 template <class St>
 int enqueue_step(psl_batch* b) {
   const int nd = (int)b->grids.size();
-  k_batch_advance<St><<<(nd * 27 + 255) / 256, 256, 0, b->stream>>>(b->d_desc, b->d_shift, b->d_table, nd);
+  k_batch_advance<St><<<(nd * 27 + 255) / 256, 256, 0, b->stream>>>(b->d_params, b->d_fbase, b->d_delta,
+                                                                     b->d_shift, nd);
   const uint32_t remap = (b->tuning & PSL_TUNE_XCD_REMAP) ? 1u : 0u;
-  switch (b->tuning & 3) {
-    case 0: k_batch_bulk<St, 0><<<b->total_blocks, 256, 0, b->stream>>>(b->d_desc, b->d_table, b->d_block_domain, remap); break;
-    case 1: k_batch_bulk<St, 1><<<b->total_blocks, 256, 0, b->stream>>>(b->d_desc, b->d_table, b->d_block_domain, remap); break;
-    case 2: k_batch_bulk<St, 2><<<b->total_blocks, 256, 0, b->stream>>>(b->d_desc, b->d_table, b->d_block_domain, remap); break;
-    default: k_batch_bulk<St, 3><<<b->total_blocks, 256, 0, b->stream>>>(b->d_desc, b->d_table, b->d_block_domain, remap); break;
+  const int nt = b->tuning & (TUNE_NT_LOAD | TUNE_NT_STORE);
+  if ((b->tuning & PSL_TUNE_FUSED_BOUNDARY) && b->total_bnd) {
+    const uint32_t nb = b->total_bblocks + b->total_blocks;
+#define PSL_FUSED(V)                                                                                           \
+  k_batch_fused<St, V><<<nb, 256, 0, b->stream>>>(b->d_params, b->d_bparams, b->d_bblock_domain,               \
+                                                  b->d_bblock_first, b->total_bblocks, b->d_block_domain,      \
+                                                  b->d_block_first, remap)
+    switch (nt) {
+      case 0: PSL_FUSED(0); break;
+      case 1: PSL_FUSED(1); break;
+      case 2: PSL_FUSED(2); break;
+      default: PSL_FUSED(3); break;
+    }
+#undef PSL_FUSED
+  } else {
+#define PSL_BULK(V)                                                                                             \
+  k_batch_bulk<St, V><<<b->total_blocks, 256, 0, b->stream>>>(b->d_params, b->d_block_domain, b->d_block_first, \
+                                                              remap)
+    switch (nt) {
+      case 0: PSL_BULK(0); break;
+      case 1: PSL_BULK(1); break;
+      case 2: PSL_BULK(2); break;
+      default: PSL_BULK(3); break;
+    }
+#undef PSL_BULK
+    if (b->total_bnd)
+      k_batch_boundary<St><<<b->total_bblocks, 256, 0, b->stream>>>(b->d_params, b->d_bparams, b->d_bblock_domain,
+                                                                     b->d_bblock_first);
   }
-  if (b->total_bnd)
-    k_batch_boundary<St><<<(b->total_bnd + 255) / 256, 256, 0, b->stream>>>(b->d_desc, b->d_table, b->d_bnd_domain,
-                                                                             b->d_bnd_cell, b->total_bnd);
   PSL_LAUNCH_CHECK();
   return PSL_OK;
 }
@@ -210,6 +186,41 @@ int enqueue_dispatch(psl_batch* b) {
   }
 }
 
+template <class T>
+int upload(T** dst, const std::vector<T>& src) {
+  PSL_HIP_TRY(hipMalloc((void**)dst, std::max<size_t>(1, src.size()) * sizeof(T)));
+  if (!src.empty()) PSL_HIP_TRY(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PSL_OK;
+}
+
+// Host-side StepParams / BoundaryParams of a grid (the stream bases are
+// filled in on the device each step).
+StepParams step_params_of(const psl_grid* g) {
+  StepParams p;
+  memset(&p, 0, sizeof(p));
+  p.n = (uint32_t)g->info.cellCount;
+  const float tau = 3.f * g->info.nu + 0.5f;
+  p.tau = tau;
+  p.omega = 1.f / tau;
+  p.cs = g->cs;
+  p.cs_cell = nullptr;
+  p.flags = g->flags;
+  return p;
+}
+
+BoundaryParams boundary_params_of(const psl_grid* g) {
+  BoundaryParams q;
+  memset(&q, 0, sizeof(q));
+  q.list = g->bc_list;
+  q.n = g->n_bc;
+  q.blocks = (q.n + 255u) / 256u;
+  q.nx = g->info.cellCountX;
+  q.ny = g->info.cellCountY;
+  q.nz = g->info.cellCountZ;
+  q.bc = g->bc_dev;
+  return q;
+}
+
 // Build the device tables for the current order and capture one step.
 int build_batch(psl_batch* b) {
   free_batch_device(b);
@@ -219,69 +230,61 @@ int build_batch(psl_batch* b) {
   if (b->order == 1)
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return b->grids[x]->info.cellCount > b->grids[y]->info.cellCount; });
-  std::vector<DomainDesc> desc(nd);
-  std::vector<uint32_t> block_domain;
+  std::vector<StepParams> params(nd);
+  std::vector<BoundaryParams> bparams(nd);
+  std::vector<const void*> fbase(nd);
+  std::vector<int> delta((size_t)nd * 27);
+  std::vector<uint32_t> block_domain, block_first(nd), bblock_domain, bblock_first(nd);
   b->total_cells = 0;
-  uint32_t first = 0;
-  std::vector<uint32_t> block_first(nd);
+  b->total_bnd = 0;
   for (int d : order) {
     const psl_grid* g = b->grids[d];
     const uint32_t nbk = (uint32_t)((g->info.cellCount + 255) / 256);
-    block_first[d] = first;
-    for (uint32_t k = 0; k < nbk; ++k) block_domain.push_back((uint32_t)d);
-    first += nbk;
+    block_first[d] = (uint32_t)block_domain.size();
+    block_domain.insert(block_domain.end(), nbk, (uint32_t)d);
+    const uint32_t nbb = (g->n_bc + 255u) / 256u;
+    bblock_first[d] = (uint32_t)bblock_domain.size();
+    bblock_domain.insert(bblock_domain.end(), nbb, (uint32_t)d);
   }
-  b->total_blocks = first;
-  b->total_bnd = 0;
+  // pad the boundary blocks to a multiple of 8, so that the fused kernel's
+  // bulk blocks sit on the XCD the remap expects; a pad block continues the
+  // last grid's list past its end, where boundary_update returns at once
+  if (!bblock_domain.empty()) {
+    const uint32_t last = bblock_domain.back();
+    while (bblock_domain.size() % 8) bblock_domain.push_back(last);
+  }
   for (int d = 0; d < nd; ++d) {
     const psl_grid* g = b->grids[d];
-    DomainDesc& D = desc[d];
-    D.f = (char*)g->f;
-    D.flags = g->flags;
-    D.bc_list = g->bc_list;
-    D.n = (uint32_t)g->info.cellCount;
-    D.n_bc = g->n_bc;
-    D.nx = g->info.cellCountX;
-    D.ny = g->info.cellCountY;
-    D.nz = g->info.cellCountZ;
-    D.block_first = block_first[d];
-    const float tau = 3.f * g->info.nu + 0.5f;
-    D.tau = tau;
-    D.omega = 1.f / tau;
-    D.cs = g->cs;
-    for (int q = 0; q < 27; ++q) D.delta[q] = CX[q] + D.nx * CY[q] + D.nx * D.ny * CZ[q];
-    D.bc = g->bc_dev;
+    params[d] = step_params_of(g);
+    bparams[d] = boundary_params_of(g);
+    fbase[d] = g->f;
+    for (int q = 0; q < 27; ++q)
+      delta[(size_t)d * 27 + q] = CX[q] + g->info.cellCountX * CY[q] + g->info.cellCountX * g->info.cellCountY * CZ[q];
     b->total_bnd += g->n_bc;
     b->total_cells += g->info.cellCount;
   }
-  PSL_HIP_TRY(hipMalloc(&b->d_desc, nd * sizeof(DomainDesc)));
-  PSL_HIP_TRY(hipMalloc(&b->d_table, nd * sizeof(DomainTable)));
-  PSL_HIP_TRY(hipMalloc(&b->d_shift, nd * 27 * sizeof(int)));
-  PSL_HIP_TRY(hipMalloc(&b->d_block_domain, block_domain.size() * sizeof(uint32_t)));
-  PSL_HIP_TRY(hipMemcpy(b->d_desc, desc.data(), nd * sizeof(DomainDesc), hipMemcpyHostToDevice));
-  PSL_HIP_TRY(hipMemcpy(b->d_block_domain, block_domain.data(), block_domain.size() * sizeof(uint32_t),
-                        hipMemcpyHostToDevice));
-  if (b->total_bnd) {
-    PSL_HIP_TRY(hipMalloc(&b->d_bnd_domain, b->total_bnd * sizeof(uint32_t)));
-    PSL_HIP_TRY(hipMalloc(&b->d_bnd_cell, b->total_bnd * sizeof(uint32_t)));
-    uint32_t off = 0;
-    for (int d = 0; d < nd; ++d) {
-      const psl_grid* g = b->grids[d];
-      if (!g->n_bc) continue;
-      k_tag_list<<<(g->n_bc + 255) / 256, 256, 0, b->stream>>>(g->bc_list, g->n_bc, (uint32_t)d,
-                                                                b->d_bnd_domain + off, b->d_bnd_cell + off);
-      PSL_LAUNCH_CHECK();
-      off += g->n_bc;
-    }
-  }
-  PSL_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->total_blocks = (uint32_t)block_domain.size();
+  b->total_bblocks = (uint32_t)bblock_domain.size();
+  b->h_params = params;
+  b->h_bparams = bparams;
+  b->h_fbase = fbase;
+  int rc = upload(&b->d_params, params);
+  if (!rc) rc = upload(&b->d_bparams, bparams);
+  if (!rc) rc = upload(&b->d_fbase, fbase);
+  if (!rc) rc = upload(&b->d_delta, delta);
+  if (!rc) rc = upload(&b->d_block_domain, block_domain);
+  if (!rc) rc = upload(&b->d_block_first, block_first);
+  if (!rc) rc = upload(&b->d_bblock_domain, bblock_domain);
+  if (!rc) rc = upload(&b->d_bblock_first, bblock_first);
+  if (!rc) rc = upload(&b->d_shift, std::vector<int>((size_t)nd * 27, 0));
+  if (rc) return rc;
   // capture one step; the graph's kernels read all state through device tables
   hipStream_t cap = nullptr;
   PSL_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   hipStream_t saved = b->stream;
   b->stream = cap;
   hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-  int rc = PSL_OK;
+  rc = PSL_OK;
   if (e == hipSuccess) rc = enqueue_dispatch(b);
   hipGraph_t graph = nullptr;
   hipError_t e2 = hipStreamEndCapture(cap, &graph);
@@ -297,11 +300,37 @@ int build_batch(psl_batch* b) {
   return PSL_OK;
 }
 
+// True when a grid changed since the graph was captured (new flags, boundary
+// list, given-value table, viscosity or Smagorinsky constant).
+bool batch_stale(const psl_batch* b) {
+  for (size_t d = 0; d < b->grids.size(); ++d) {
+    const psl_grid* g = b->grids[d];
+    const StepParams p = step_params_of(g);
+    const BoundaryParams q = boundary_params_of(g);
+    const StepParams& P = b->h_params[d];
+    const BoundaryParams& Q = b->h_bparams[d];
+    if (p.n != P.n || p.tau != P.tau || p.cs != P.cs || p.flags != P.flags || q.list != Q.list || q.n != Q.n ||
+        q.bc != Q.bc || g->f != b->h_fbase[d] || g->cs_cell || g->periodic_copies)
+      return true;
+  }
+  return false;
+}
+
+// A grid the batch kernels can advance: flags set (the boundary list is
+// built from them), no per-cell Cs, no periodic copies, no sub-grid.
+const char* unbatchable(const psl_grid* g) {
+  if (!g->flags) return "batch grids need flags (psl_grid_set_flags)";
+  if (g->cs_cell) return "batch grids cannot use per-cell Cs";
+  if (g->periodic_copies) return "batch grids cannot use periodic copies";
+  if (g->info.iSubgridEnd - g->info.iSubgridStart > 4) return "batch grids cannot carry a sub-grid";
+  return nullptr;
+}
+
 int upload_shifts(psl_batch* b) {
   const int nd = (int)b->grids.size();
-  std::vector<int> sh(nd * 27);
+  std::vector<int> sh((size_t)nd * 27);
   for (int d = 0; d < nd; ++d)
-    for (int q = 0; q < 27; ++q) sh[d * 27 + q] = b->grids[d]->shift[q];
+    for (int q = 0; q < 27; ++q) sh[(size_t)d * 27 + q] = b->grids[d]->shift[q];
   PSL_HIP_TRY(hipMemcpyAsync(b->d_shift, sh.data(), sh.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
   PSL_HIP_TRY(hipStreamSynchronize(b->stream));
   return PSL_OK;
@@ -316,10 +345,11 @@ int psl_batch_create(psl_grid* const* grids, int count, psl_batch** out) {
   psl_batch* b = new psl_batch();
   for (int d = 0; d < count; ++d) {
     psl_grid* g = grids[d];
-    if (!g || g->storage != grids[0]->storage || g->cs_cell || g->periodic_copies ||
-        g->info.iSubgridEnd - g->info.iSubgridStart > 4) {
+    const char* why = !g ? "null grid"
+                         : g->storage != grids[0]->storage ? "batch grids must share the storage" : unbatchable(g);
+    if (why) {
       delete b;
-      set_error("batch grids must share the storage and use neither per-cell Cs, periodic copies nor sub-grids");
+      set_error(why);
       return PSL_E_ARG;
     }
     b->grids.push_back(g);
@@ -349,6 +379,17 @@ int psl_batch_destroy(psl_batch* b) {
 int psl_batch_step(psl_batch* b, int nsteps) {
   PSL_CHECK_ARG(b && nsteps >= 0, "bad batch step");
   if (nsteps == 0) return PSL_OK;
+  if (batch_stale(b)) {
+    for (const psl_grid* g : b->grids) {
+      const char* why = unbatchable(g);
+      if (why) {
+        set_error(why);
+        return PSL_E_ARG;
+      }
+    }
+    int rc = build_batch(b);
+    if (rc) return rc;
+  }
   int rc = upload_shifts(b);
   if (rc) return rc;
   for (int s = 0; s < nsteps; ++s) PSL_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
@@ -359,7 +400,7 @@ int psl_batch_step(psl_batch* b, int nsteps) {
 
 int psl_batch_set_tuning(psl_batch* b, int tuning, int order) {
   PSL_CHECK_ARG(b, "null batch");
-  PSL_CHECK_ARG((tuning & ~7) == 0 && (order == 0 || order == 1), "bad tuning");
+  PSL_CHECK_ARG(tuning >= 0 && tuning < PSL_TUNE_VARIANTS && (order == 0 || order == 1), "bad tuning");
   PSL_HIP_TRY(hipStreamSynchronize(b->stream));
   b->tuning = tuning;
   b->order = order;
@@ -373,8 +414,9 @@ int psl_batch_get_tuning(const psl_batch* b, int* tuning, int* order) {
   return PSL_OK;
 }
 
-// Time all 16 (variant x grid order) configurations for `steps` graph steps each
-// (the grids advance) and keep the fastest.  times_ms[16] receives ms per step.
+// Time every (variant, grid order) configuration for `steps` graph steps each
+// after one warm-up step (the grids advance) and keep the fastest.
+// times_ms[2 * PSL_TUNE_VARIANTS] receives ms per step at order * 16 + variant.
 int psl_batch_autotune(psl_batch* b, int steps, int* best_tuning, int* best_order, double* times_ms) {
   PSL_CHECK_ARG(b && steps > 0, "bad autotune arguments");
   double best = 1e300;
@@ -383,7 +425,7 @@ int psl_batch_autotune(psl_batch* b, int steps, int* best_tuning, int* best_orde
   PSL_HIP_TRY(hipEventCreate(&e0));
   PSL_HIP_TRY(hipEventCreate(&e1));
   for (int order = 0; order < 2; ++order)
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < PSL_TUNE_VARIANTS; ++t) {
       int rc = psl_batch_set_tuning(b, t, order);
       if (!rc) rc = psl_batch_step(b, 1);  // warm
       if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = PSL_E_HIP;
@@ -398,7 +440,7 @@ int psl_batch_autotune(psl_batch* b, int steps, int* best_tuning, int* best_orde
         return rc;
       }
       const double per = ms / steps;
-      if (times_ms) times_ms[order * 8 + t] = per;
+      if (times_ms) times_ms[order * PSL_TUNE_VARIANTS + t] = per;
       if (per < best) {
         best = per;
         bt = t;

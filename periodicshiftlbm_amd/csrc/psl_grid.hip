@@ -12,9 +12,9 @@
 // Kernels of one step:
 //   k_ps_bulk      fluid + bounceback cells, one thread per cell, the hot path;
 //   k_ps_boundary  box-face boundary cells from a compact list (restore +
-//                  MBBC + collision), on a side stream concurrently with the
-//                  bulk: the cell sets are disjoint, so the two launches
-//                  together equal the reference's single update;
+//                  MBBC + collision); the cell sets are disjoint, so the two
+//                  launches together equal the reference's single update;
+//   k_ps_fused     both in one launch (a tuning variant);
 //   k_periodic     applyBCPeriodicX/Y surface copies (before the update).
 #include "psl_internal.h"
 #include "psl_stream.h"
@@ -30,57 +30,20 @@ static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
 
 // ---------------------------------------------------------------- the step
-// The local cell update (include/applyLocalCellUpdate.h:1-74) is two kernels
-// over disjoint cell sets: k_ps_boundary takes the listed box-face cells (the
-// branch at lines 46-67), k_ps_bulk the fluid and bounceback cells in cell
-// order.  After applyStreaming every cell owns its 27 slots, so the two never
-// touch the same element and run concurrently on two streams: the short,
-// latency-bound list hides under the bandwidth-bound bulk.  (One kernel with
-// both paths needs ~100 VGPRs against ~70 for either alone, which halves the
-// bulk occupancy.)
-struct BoundaryParams {
-  const uint32_t* list;  // box-face cells in cell order
-  uint32_t n;            // list length
-  int nx, ny, nz;
-  const float4* bc;      // 4 given-value slots (rho, ux, uy, uz)
-};
-
-template <class St>
-__device__ __forceinline__ void boundary_update(const StepParams& p, const BoundaryParams& b, uint32_t t) {
-  typedef typename St::T T;
-  if (t >= b.n) return;
-  const uint32_t cell = b.list[t];
-  const int fl = p.flags[cell];
-  const int kind = fl & 7;
-  const uint32_t nxy = (uint32_t)b.nx * (uint32_t)b.ny;
-  const int k = (int)(cell / nxy);
-  const uint32_t rem = cell - (uint32_t)k * nxy;
-  const int j = (int)(rem / (uint32_t)b.nx);
-  const int i = (int)(rem - (uint32_t)j * (uint32_t)b.nx);
-  float v[27];
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-    v[q] = St::load(base[cell]);
-    if (St::W_OFF) v[q] += weight(q);
-  }
-  const int code = outer_normal_code(i, j, k, b.nx, b.ny, b.nz, fl);
-  // given (rho, u) of the cell's slot: one indexed load from the device table
-  // (a select chain over the four slots was miscompiled for divergent slots,
-  // so no branch or select is used here)
-  const float4 g = b.bc[(fl >> 6) & 3];
-  boundary_cell(v, kind, code, g.x, g.y, g.z, g.w);
-  Relax rx{p.omega, p.tau, p.cs};
-  const float cs = p.cs_cell ? p.cs_cell[cell] : p.cs;
-  collide<false>(v, rx, cs);
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-    const float o = St::W_OFF ? v[q] - weight(q) : v[q];
-    base[cell] = St::store(o);
-  }
-}
-
+// The local cell update (include/applyLocalCellUpdate.h:1-74) splits the
+// cells into two disjoint sets: the listed box-face cells (the branch at
+// lines 46-67: restore + MBBC + collision) and the bulk (fluid and
+// bounceback cells, in cell order).  After applyStreaming every cell owns its
+// 27 slots, so the two sets can be updated in any order or concurrently.
+//   k_ps_bulk + k_ps_boundary  two launches, one after the other;
+//   k_ps_fused                 one launch whose leading workgroups take the
+//                              list, so the short, latency-bound list overlaps
+//                              the bandwidth-bound bulk.  The boundary path
+//                              needs ~100 VGPRs against ~70 for the bulk, so
+//                              the kernel is held at 7 waves per SIMD and
+//                              spills only inside the boundary path.
+// The per-cell paths (bulk_update, boundary_update) are shared with the
+// batched kernels, so a batched grid computes bit-identical results.
 template <class St>
 __global__ __launch_bounds__(256) void k_ps_boundary(StepParams p, BoundaryParams b) {
   boundary_update<St>(p, b, blockIdx.x * 256u + threadIdx.x);
@@ -88,29 +51,19 @@ __global__ __launch_bounds__(256) void k_ps_boundary(StepParams p, BoundaryParam
 
 template <class St, bool FLAGS, bool LES, int TUNE>
 __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
-  const uint32_t block = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  const uint32_t block_first = block * 256u;
-  const uint32_t cell = block_first + threadIdx.x;
-  const uint32_t w_first = wave_first_cell(block_first);
-  if (w_first >= p.n) return;  // whole wave past the grid (wave-uniform)
-  // all 64 lanes take part in the ballot, so every population is tested
-  const bool straddle = wave_straddles(p, w_first);
-  if (cell >= p.n) return;
-  int kind = KIND_FLUID;
-  if (FLAGS) kind = p.flags[cell] & 7;
-  if (kind > KIND_BOUNCEBACK) return;
-  float v[27];
-  load27<St, TUNE>(p, w_first, straddle, cell, v);
-  if (kind == KIND_BOUNCEBACK) {
-    // the swap commutes with "f - w" (opposite directions share a weight)
-    bounceback(v);
-  } else {
-    Relax rx{p.omega, p.tau, p.cs};
-    float cs = p.cs;
-    if (LES && p.cs_cell) cs = p.cs_cell[cell];
-    collide<St::W_OFF>(v, rx, LES ? cs : 0.f);
+  bulk_update<St, FLAGS, LES, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+
+template <class St, bool LES, int TUNE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_ps_fused(StepParams p,
+                                                                                          BoundaryParams b) {
+  if (blockIdx.x < b.blocks) {
+    boundary_update<St>(p, b, blockIdx.x * 256u + threadIdx.x);
+    return;
   }
-  store27<St, TUNE>(p, w_first, straddle, cell, v);
+  // b.blocks is a multiple of 8, so hw and blockIdx.x sit on the same XCD
+  const uint32_t hw = blockIdx.x - b.blocks;
+  bulk_update<St, true, LES, TUNE>(p, p.xcd_remap ? xcd_block(hw, gridDim.x - b.blocks) : hw);
 }
 
 // ------------------------------------------------------------ periodic copy
@@ -126,12 +79,12 @@ __global__ __launch_bounds__(256) void k_periodic(T* __restrict__ f, Shift27 sh,
                                                   int axis) {
   // one thread per (surface cell, copied population): 18 independent copies
   const int na = axis == 0 ? ny : nx;
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= (uint32_t)na * nz * 18u) return;
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  if (t >= (uint64_t)na * (uint64_t)nz * 18u) return;
   const int c = (int)(t % 18u);
-  const uint32_t rest = t / 18u;
-  const int a = (int)(rest % (uint32_t)na);
-  const int k = (int)(rest / (uint32_t)na);
+  const uint64_t rest = t / 18u;
+  const int a = (int)(rest % (uint64_t)na);
+  const int k = (int)(rest / (uint64_t)na);
   const uint32_t plane = (uint32_t)k * (uint32_t)nx * (uint32_t)ny;
   uint32_t left, right;
   if (axis == 0) {
@@ -439,13 +392,10 @@ static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, c
   }
 }
 
-// Side stream and fork/join events of a grid, created on first use.
-static int ensure_side_stream(psl_grid* g) {
-  if (g->side) return PSL_OK;
-  PSL_HIP_TRY(hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking));
-  PSL_HIP_TRY(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
-  PSL_HIP_TRY(hipEventCreateWithFlags(&g->join, hipEventDisableTiming));
-  return PSL_OK;
+template <class St, int TUNE>
+static void launch_fused(bool les, uint32_t nb, hipStream_t stream, const StepParams& p, const BoundaryParams& b) {
+  if (les) k_ps_fused<St, true, TUNE><<<nb, 256, 0, stream>>>(p, b);
+  else k_ps_fused<St, false, TUNE><<<nb, 256, 0, stream>>>(p, b);
 }
 
 template <class St>
@@ -468,24 +418,15 @@ static int launch_local_update(psl_grid* g) {
   p.flags = g->flags;
   p.xcd_remap = (g->tuning & PSL_TUNE_XCD_REMAP) ? 1u : 0u;
   const bool les = g->cs != 0.f || g->cs_cell != nullptr;
-  const bool boundary = g->flags && g->n_bc;
-  if (boundary) {
-    // fork: the boundary list runs on the side stream beside the bulk
-    int rc = ensure_side_stream(g);
-    if (rc != PSL_OK) return rc;
-    PSL_HIP_TRY(hipEventRecord(g->fork, g->stream));
-    PSL_HIP_TRY(hipStreamWaitEvent(g->side, g->fork, 0));
-    BoundaryParams b;
-    b.list = g->bc_list;
-    b.n = g->n_bc;
-    b.nx = g->info.cellCountX;
-    b.ny = g->info.cellCountY;
-    b.nz = g->info.cellCountZ;
-    b.bc = g->bc_dev;
-    k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->side>>>(p, b);
-    PSL_LAUNCH_CHECK();
-    PSL_HIP_TRY(hipEventRecord(g->join, g->side));
-  }
+  BoundaryParams b;
+  b.list = g->bc_list;
+  b.n = g->flags ? g->n_bc : 0u;
+  b.blocks = (blocks_for(b.n) + 7u) & ~7u;
+  b.nx = g->info.cellCountX;
+  b.ny = g->info.cellCountY;
+  b.nz = g->info.cellCountZ;
+  b.bc = g->bc_dev;
+  const bool fused = b.n && (g->tuning & PSL_TUNE_FUSED_BOUNDARY);
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (g->profile) {
     PSL_HIP_TRY(hipEventCreate(&ev0));
@@ -494,20 +435,31 @@ static int launch_local_update(psl_grid* g) {
   }
   const int nt = g->tuning & (TUNE_NT_LOAD | TUNE_NT_STORE);
   const uint32_t nb = blocks_for(p.n);
-  const bool fl = g->flags != nullptr;
-  switch (nt) {
-    case 0: launch_bulk<St, 0>(fl, les, nb, g->stream, p); break;
-    case 1: launch_bulk<St, 1>(fl, les, nb, g->stream, p); break;
-    case 2: launch_bulk<St, 2>(fl, les, nb, g->stream, p); break;
-    default: launch_bulk<St, 3>(fl, les, nb, g->stream, p); break;
+  if (fused) {
+    switch (nt) {
+      case 0: launch_fused<St, 0>(les, nb + b.blocks, g->stream, p, b); break;
+      case 1: launch_fused<St, 1>(les, nb + b.blocks, g->stream, p, b); break;
+      case 2: launch_fused<St, 2>(les, nb + b.blocks, g->stream, p, b); break;
+      default: launch_fused<St, 3>(les, nb + b.blocks, g->stream, p, b); break;
+    }
+  } else {
+    const bool fl = g->flags != nullptr;
+    switch (nt) {
+      case 0: launch_bulk<St, 0>(fl, les, nb, g->stream, p); break;
+      case 1: launch_bulk<St, 1>(fl, les, nb, g->stream, p); break;
+      case 2: launch_bulk<St, 2>(fl, les, nb, g->stream, p); break;
+      default: launch_bulk<St, 3>(fl, les, nb, g->stream, p); break;
+    }
   }
   PSL_LAUNCH_CHECK();
   if (g->profile) {
     PSL_HIP_TRY(hipEventRecord(ev1, g->stream));
     g->prof_events.emplace_back(ev0, ev1);
   }
-  // join: later work on the grid's stream sees the boundary cells too
-  if (boundary) PSL_HIP_TRY(hipStreamWaitEvent(g->stream, g->join, 0));
+  if (b.n && !fused) {
+    k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->stream>>>(p, b);
+    PSL_LAUNCH_CHECK();
+  }
   return PSL_OK;
 }
 
@@ -655,9 +607,6 @@ int psl_grid_destroy(psl_grid* g) {
   if (g->partials) (void)hipFree(g->partials);
   if (g->result) (void)hipFree(g->result);
   if (g->bc_dev) (void)hipFree(g->bc_dev);
-  if (g->fork) (void)hipEventDestroy(g->fork);
-  if (g->join) (void)hipEventDestroy(g->join);
-  if (g->side) (void)hipStreamDestroy(g->side);
   delete g;
   return PSL_OK;
 }
@@ -955,7 +904,8 @@ int psl_grid_profile_read(psl_grid* g, double* total_ms, int* launches) {
 
 int psl_grid_set_tuning(psl_grid* g, int tuning) {
   PSL_CHECK_ARG(g, "null grid");
-  PSL_CHECK_ARG((tuning & ~(PSL_TUNE_NT_LOAD | PSL_TUNE_NT_STORE | PSL_TUNE_XCD_REMAP)) == 0, "unknown tuning bits");
+  PSL_CHECK_ARG((tuning & ~(PSL_TUNE_NT_LOAD | PSL_TUNE_NT_STORE | PSL_TUNE_XCD_REMAP | PSL_TUNE_FUSED_BOUNDARY)) == 0,
+                "unknown tuning bits");
   g->tuning = tuning;
   return PSL_OK;
 }
@@ -972,27 +922,38 @@ int psl_grid_get_tuning(const psl_grid* g, int* tuning) {
 // receives the mean bulk-kernel time of each variant.
 int psl_grid_autotune(psl_grid* g, int steps, int* best, double* times_ms) {
   PSL_CHECK_ARG(g && steps > 0, "bad autotune arguments");
+  // whole local updates (bulk + boundary list) are timed, so that the fused
+  // and the two-launch forms compare on equal terms
   const int saved = g->tuning;
   double best_t = 1e300;
   int best_v = saved;
-  for (int v = 0; v < 8; ++v) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PSL_HIP_TRY(hipEventCreate(&e0));
+  PSL_HIP_TRY(hipEventCreate(&e1));
+  int rc = PSL_OK;
+  for (int v = 0; v < PSL_TUNE_VARIANTS && rc == PSL_OK; ++v) {
     g->tuning = v;
-    int rc = psl_grid_profile_enable(g, 1);
+    rc = psl_apply_local_cell_update(g);  // warm
+    if (rc == PSL_OK && hipEventRecord(e0, g->stream) != hipSuccess) rc = PSL_E_HIP;
     for (int s = 0; s < steps && rc == PSL_OK; ++s) rc = psl_apply_local_cell_update(g);
-    double ms = 0.0;
-    int launches = 0;
-    if (rc == PSL_OK) rc = psl_grid_profile_read(g, &ms, &launches);
-    g->profile = false;
-    if (rc) {
-      g->tuning = saved;
-      return rc;
-    }
-    const double mean = ms / std::max(1, launches);
+    if (rc == PSL_OK && hipEventRecord(e1, g->stream) != hipSuccess) rc = PSL_E_HIP;
+    float ms = 0.f;
+    if (rc == PSL_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = PSL_E_HIP;
+    if (rc != PSL_OK) break;
+    const double mean = (double)ms / steps;
     if (times_ms) times_ms[v] = mean;
     if (mean < best_t) {
       best_t = mean;
       best_v = v;
     }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != PSL_OK) {
+    g->tuning = saved;
+    if (rc == PSL_E_HIP) set_error("autotune timing failed");
+    return rc;
   }
   g->tuning = best_v;
   if (best) *best = best_v;

@@ -89,8 +89,6 @@ struct psl_grid {
   uint32_t n_bc = 0;
   psl::BcTable bcvals{};
   float4* bc_dev = nullptr;       // device copy of the given-value table (4 x float4)
-  hipStream_t side = nullptr;     // boundary-list stream (forked from `stream` per step)
-  hipEvent_t fork = nullptr, join = nullptr;
   float cs = 0.f;
   float* cs_cell = nullptr;       // device, N floats, or null
   int periodic_copies = 0;        // bit 0: x, bit 1: y

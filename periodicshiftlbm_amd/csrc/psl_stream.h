@@ -1,4 +1,5 @@
-// Shifted-stream addressing shared by the single-grid and batched step kernels.
+// Shifted-stream addressing and the per-cell update paths shared by the
+// single-grid and batched step kernels.
 #pragma once
 
 #include "psl_internal.h"
@@ -50,15 +51,22 @@ __device__ __forceinline__ bool wave_straddles(const Tab& p, uint32_t w_first) {
 // Tuning bits of the bulk kernel (chosen per grid by psl_grid_autotune).
 enum : int { TUNE_NT_LOAD = 1, TUNE_NT_STORE = 2 };
 
+// The stream bases may come from device memory (batched grids), where the
+// compiler cannot see that they point to global memory; the explicit global
+// address space keeps global_load / global_store instead of flat accesses.
+#define PSL_GLOBAL __attribute__((address_space(1)))
+
 template <class T, bool NT>
 __device__ __forceinline__ T ld(const T* p) {
-  if (NT) return __builtin_nontemporal_load(p);
-  return *p;
+  const PSL_GLOBAL T* g = (const PSL_GLOBAL T*)p;
+  if constexpr (NT) return __builtin_nontemporal_load(g);
+  else return *g;
 }
 template <class T, bool NT>
 __device__ __forceinline__ void st(T* p, T v) {
-  if (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
+  PSL_GLOBAL T* g = (PSL_GLOBAL T*)p;
+  if constexpr (NT) __builtin_nontemporal_store(v, g);
+  else *g = v;
 }
 
 template <class St, int TUNE, class Tab>
@@ -111,6 +119,79 @@ __device__ __forceinline__ void store27(const Tab& p, uint32_t w_first, bool str
 __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   const uint32_t xcd = b & 7u, idx = b >> 3, per = nb >> 3, rem = nb & 7u;
   return xcd * per + min(xcd, rem) + idx;
+}
+
+// ------------------------------------------------------- per-cell updates
+// Box-face cells of one grid (include/applyLocalCellUpdate.h:46-67).
+struct BoundaryParams {
+  const uint32_t* list;  // box-face cells in cell order
+  uint32_t n;            // list length
+  uint32_t blocks;       // workgroups of the list in the fused kernel (multiple of 8)
+  int nx, ny, nz;
+  const float4* bc;      // 4 given-value slots (rho, ux, uy, uz)
+};
+
+template <class St>
+__device__ __forceinline__ void boundary_update(const StepParams& p, const BoundaryParams& b, uint32_t t) {
+  typedef typename St::T T;
+  if (t >= b.n) return;
+  const uint32_t cell = ld<uint32_t, false>(b.list + t);
+  const int fl = ld<uint8_t, false>(p.flags + cell);
+  const int kind = fl & 7;
+  const uint32_t nxy = (uint32_t)b.nx * (uint32_t)b.ny;
+  const int k = (int)(cell / nxy);
+  const uint32_t rem = cell - (uint32_t)k * nxy;
+  const int j = (int)(rem / (uint32_t)b.nx);
+  const int i = (int)(rem - (uint32_t)j * (uint32_t)b.nx);
+  float v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+    v[q] = St::load(ld<T, false>(base + cell));
+    if (St::W_OFF) v[q] += weight(q);
+  }
+  const int code = outer_normal_code(i, j, k, b.nx, b.ny, b.nz, fl);
+  // given (rho, u) of the cell's slot: one indexed load from the device table
+  // (a select chain over the four slots was miscompiled for divergent slots,
+  // so no branch or select is used here)
+  const float* g = (const float*)(b.bc + ((fl >> 6) & 3));
+  boundary_cell(v, kind, code, ld<float, false>(g), ld<float, false>(g + 1), ld<float, false>(g + 2),
+                ld<float, false>(g + 3));
+  Relax rx{p.omega, p.tau, p.cs};
+  const float cs = p.cs_cell ? ld<float, false>(p.cs_cell + cell) : p.cs;
+  collide<false>(v, rx, cs);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+    const float o = St::W_OFF ? v[q] - weight(q) : v[q];
+    st<T, false>(base + cell, St::store(o));
+  }
+}
+
+template <class St, bool FLAGS, bool LES, int TUNE>
+__device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block) {
+  const uint32_t block_first = block * 256u;
+  const uint32_t cell = block_first + threadIdx.x;
+  const uint32_t w_first = wave_first_cell(block_first);
+  if (w_first >= p.n) return;  // whole wave past the grid (wave-uniform)
+  // all 64 lanes take part in the ballot, so every population is tested
+  const bool straddle = wave_straddles(p, w_first);
+  if (cell >= p.n) return;
+  int kind = KIND_FLUID;
+  if (FLAGS) kind = ld<uint8_t, false>(p.flags + cell) & 7;
+  if (kind > KIND_BOUNCEBACK) return;
+  float v[27];
+  load27<St, TUNE>(p, w_first, straddle, cell, v);
+  if (kind == KIND_BOUNCEBACK) {
+    // the swap commutes with "f - w" (opposite directions share a weight)
+    bounceback(v);
+  } else {
+    Relax rx{p.omega, p.tau, p.cs};
+    float cs = p.cs;
+    if (LES && p.cs_cell) cs = ld<float, false>(p.cs_cell + cell);
+    collide<St::W_OFF>(v, rx, LES ? cs : 0.f);
+  }
+  store27<St, TUNE>(p, w_first, straddle, cell, v);
 }
 
 }  // namespace psl

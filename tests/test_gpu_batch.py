@@ -1,0 +1,133 @@
+"""Batched steps over independent grids of uneven size (psl_batch_*).
+
+A batch advances every grid by one hipGraph replay per step, with the same
+per-cell device code as the single-grid kernels.  Each grid must end exactly
+(bit for bit, every storage type, every kernel variant and block order) where
+the same number of single-grid updateGrid steps puts it; the single-grid
+path is pinned against the oracle by the other GPU tests, and one batched
+domain is compared with the oracle directly here (1e-5 relative, FP32).
+Domains are sphere-in-channel boxes 2L x L x L (velocity inlet, pressure
+outlet, bounceback walls in y, z periodic through the shift), one of them
+without Smagorinsky (Cs = 0).
+"""
+import numpy as np
+import pytest
+
+from periodicshiftlbm_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _rel_err(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)) / np.maximum(np.abs(b), 1e-3)))
+
+
+def _domains():
+    return cases.batch_domains(count=5, seed=11, l_min=12, l_max=28)
+
+
+def _grids(doms, storage):
+    # the third domain runs without Smagorinsky
+    return [cases.domain_grid(d, storage, cs=0.0 if i == 2 else 0.17)[0] for i, d in enumerate(doms)]
+
+
+def _assert_identical(doms, batched, single, label):
+    for d, gb, gs in zip(doms, batched, single):
+        assert gb.shifter.tolist() == gs.shifter.tolist()
+        fb, fs = gb.get_f(), gs.get_f()
+        nbad = int(np.count_nonzero(fb != fs))
+        print(f"  {label} L={d['L']} re={d['re']:.0f}: {nbad} differing values, max rel "
+              f"{_rel_err(fb, fs):.3e}", flush=True)
+        assert nbad == 0
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_batch_matches_single_grid_steps(psl, storage):
+    P = psl
+    doms = _domains()
+    print(f"[batch] storage={storage} shapes={[d['shape'] for d in doms]}", flush=True)
+    batched = _grids(doms, storage)
+    single = _grids(doms, storage)
+    b = P.Batch(batched)
+    cells, blocks, nbnd = b.census()
+    assert cells == sum(int(np.prod(d["shape"])) for d in doms)
+    steps = 7
+    b.step(steps)
+    for g in single:
+        P.updateGrid([g], steps)
+    _assert_identical(doms, batched, single, "two-launch")
+    # other variants (fused boundary, non-temporal stores, XCD order) and the
+    # largest-first block order give the same result
+    for tuning, order in ((P.TUNE_FUSED_BOUNDARY | P.TUNE_XCD_REMAP, 0),
+                          (P.TUNE_NT_STORE | P.TUNE_FUSED_BOUNDARY, 1),
+                          (P.TUNE_NT_LOAD | P.TUNE_XCD_REMAP, 1)):
+        b.set_tuning(tuning, order)
+        b.step(3)
+        for g in single:
+            P.updateGrid([g], 3)
+        _assert_identical(doms, batched, single, f"tuning {tuning} order {order}")
+    b.close()
+
+
+def test_batch_domain_matches_oracle(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    d = _domains()[1]
+    g, flags = cases.domain_grid(d, "f32")
+    nx, ny, nz = d["shape"]
+    og = O.Grid(nx, ny, nz, dtype=np.float32, nu=d["nu"], cs=0.17, flags=flags,
+                bcvals=cases.channel_given_values(d["u_inlet"]))
+    og.fill_equilibrium()
+    b = P.Batch([g])
+    steps = 12
+    print(f"[batch-oracle] shape={d['shape']} steps={steps}", flush=True)
+    b.step(steps)
+    for _ in range(steps):
+        og.step()
+    err = _rel_err(g.get_f(), og.logical_f())
+    print(f"  max rel err {err:.3e}")
+    assert err < RTOL
+    b.close()
+
+
+def test_batch_autotune_keeps_results(psl):
+    P = psl
+    doms = _domains()[:3]
+    batched = _grids(doms, "f16")
+    single = _grids(doms, "f16")
+    b = P.Batch(batched)
+    tuning, order, times = b.autotune(2)
+    print(f"[batch-autotune] best tuning {tuning} order {order} ms/step {np.round(times, 4).tolist()}", flush=True)
+    assert np.all(times > 0)
+    # autotune advanced the batch by 2 orders x 16 variants x (1 + 2) steps
+    for g in single:
+        P.updateGrid([g], 2 * P.TUNE_VARIANTS * 3)
+    _assert_identical(doms, batched, single, "after autotune")
+    b.close()
+
+
+def test_single_grid_variants_identical(psl):
+    """Every local-update variant of the single-grid path gives the same bits."""
+    P = psl
+    d = _domains()[1]
+    ref = cases.domain_grid(d, "f32")[0]
+    ref.tuning = 0
+    P.updateGrid([ref], 9)
+    f0 = ref.get_f()
+    for v in range(1, P.TUNE_VARIANTS):
+        g = cases.domain_grid(d, "f32")[0]
+        g.tuning = v
+        P.updateGrid([g], 9)
+        nbad = int(np.count_nonzero(g.get_f() != f0))
+        print(f"  variant {v}: {nbad} differing values", flush=True)
+        assert nbad == 0
+
+
+def test_batch_rejects_mixed_storage(psl):
+    P = psl
+    doms = _domains()[:2]
+    a = cases.domain_grid(doms[0], "f32")[0]
+    c = cases.domain_grid(doms[1], "f16")[0]
+    with pytest.raises(RuntimeError):
+        P.Batch([a, c])
