@@ -132,6 +132,9 @@ int psl_get_f(psl_grid* grid, float* host);
 int psl_set_f(psl_grid* grid, const float* host);
 /* rho, ux, uy, uz per cell (getRhoUxUyUz, include/cellFunctions.h:135-155); any may be NULL. */
 int psl_get_macroscopic(psl_grid* grid, float* rho, float* ux, float* uy, float* uz);
+/* The same fields into a device buffer of 4 x cellCount floats (rho, ux, uy,
+ * uz), ordered on the grid's stream, no host copy. */
+int psl_get_macroscopic_device(psl_grid* grid, float* out_device);
 /* Device pointer of fArray (storage element type), for callers that keep data resident. */
 int psl_grid_device_f(psl_grid* grid, void** dev_ptr, size_t* bytes);
 
@@ -155,6 +158,9 @@ int psl_device_copy(void* dst, const void* src, size_t bytes);
 /* Device buffer helpers for callers without a device allocator. */
 int psl_device_alloc(size_t bytes, void** out);
 int psl_device_free(void* ptr);
+/* Synchronous host <-> device copies for such buffers. */
+int psl_device_upload(void* dst_device, const void* src_host, size_t bytes);
+int psl_device_download(void* dst_host, const void* src_device, size_t bytes);
 /* Local-update variants (bits): non-temporal loads / stores and XCD-contiguous
  * block order in the bulk kernel; FUSED_BOUNDARY updates the box-face list in
  * the same launch as the bulk instead of a second launch. */
@@ -197,6 +203,49 @@ int psl_batch_get_tuning(const psl_batch* batch, int* tuning, int* order);
  * receives ms per step at order * PSL_TUNE_VARIANTS + variant. */
 int psl_batch_autotune(psl_batch* batch, int steps, int* best_tuning, int* best_order, double* times_ms);
 int psl_batch_census(const psl_batch* batch, int64_t* cells, int* blocks, int* boundary_cells);
+
+/* ----------------------------------------------- learned LES closure */
+/* No reference counterpart: replaces the Smagorinsky rate of getOmegaLES
+ * (include/cellFunctions.h:157-198) in the bulk cells by
+ *   tau_eff = 3 (nu + 1e-3 * max(0, y)) + 1/2,
+ * y = network(standardised features of the non-equilibrium second moments);
+ * network 9 -> 32 tanh -> 32 tanh -> 1, evaluated with bf16 matrix cores in
+ * the step.  Box-face boundary-list cells keep Smagorinsky (the fallback).
+ * Flat parameter vector (PSL_MLP_PARAMS floats): W1[9][32], b1[32],
+ * W2[32][32], b2[32], W3[32], b3.  Raw feature rows are 9 floats per cell:
+ * |dev Pi|/rho, tr Pi/rho, dev Pi/|dev Pi| (xx, yy, zz, xy, xz, yz),
+ * 3 sqrt(6) det(dev Pi)/|dev Pi|^3, with Pi = kappa - rho/3 I. */
+enum { PSL_MLP_INPUTS = 9, PSL_MLP_PARAMS = 1409 };
+typedef struct psl_mlp psl_mlp;
+int psl_mlp_create(psl_mlp** out);
+int psl_mlp_destroy(psl_mlp* mlp);
+/* mean / std: input standardisation (NULL keeps the current one). */
+int psl_mlp_set_params(psl_mlp* mlp, const float* params, const float* mean, const float* std);
+int psl_mlp_get_params(const psl_mlp* mlp, float* params, float* mean, float* std);
+/* FP32 training kernels on standardised inputs x_dev (n x 9, device) and
+ * targets t_dev (device): mean squared error and its gradient (host arrays). */
+int psl_mlp_loss_grad(const psl_mlp* mlp, const float* x_dev, const float* t_dev, int n, double* loss,
+                      float* grad);
+/* Adam training (iters steps, mini-batches of `batch` rows drawn from seed) on
+ * raw feature rows raw_dev (n x 9) and targets t_dev (nu_t / 1e-3); the
+ * standardisation is taken from the data.  loss_hist[iters] is optional. */
+int psl_mlp_train(psl_mlp* mlp, const float* raw_dev, const float* t_dev, int n, int batch, int iters, float lr,
+                  uint64_t seed, float* loss_hist);
+/* Network output y of raw feature rows (device arrays) with the step's bf16 path. */
+int psl_mlp_eval(const psl_mlp* mlp, const float* raw_dev, int n, float* y_dev);
+/* Training data from a fine periodic run.  psl_les_filter: box filter of width
+ * r over the unique n^3 region of the fine macroscopic fields (device arrays
+ * in cell order of a fine_nx x fine_ny x * grid) -> coarse[10 * m^3], m = n/r:
+ * rho, Favre u (3), subgrid stress (xx, yy, zz, xy, xz, yz).
+ * psl_les_targets: coarse strain by periodic central differences, target
+ * nu_t* / 1e-3 = max(0, -dev(tau):S / (2 S:S)) / 1e-3, and the raw features
+ * of Pi = -(2/3) rho (3 (nu + nu_t*) + 1/2) S (feat: m^3 x 9). */
+int psl_les_filter(const float* rho, const float* ux, const float* uy, const float* uz, int fine_nx, int fine_ny,
+                   int n, int r, float* coarse);
+int psl_les_targets(const float* coarse, int m, float nu, float* feat, float* target);
+/* Use the network in the grid's bulk update (NULL: back to Smagorinsky).  The
+ * network must outlive its use; the fused-boundary variant is not used. */
+int psl_grid_set_closure(psl_grid* grid, const psl_mlp* mlp);
 
 /* --------------------------------------------- esotwist (DIAD) grids */
 /* DIADGridStruct (include/types.h:166-171) from host cell tables: ijk3 = 3 arrays of n

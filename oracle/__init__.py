@@ -54,6 +54,7 @@ def lib():
 
 def _declare(L):
     P = ctypes.c_void_p
+    L.oracle_velocity_set.argtypes = [P, P, P]
     for suf, R in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
         getattr(L, f"oracle_feq_{suf}").argtypes = [P, P, ctypes.c_longlong]
         getattr(L, f"oracle_rho_u_{suf}").argtypes = [P, P, ctypes.c_longlong]
@@ -242,6 +243,13 @@ def stream_shift(shift, nx, ny, nz):
     shift = np.array(shift, np.int32, copy=True)
     lib().oracle_stream_shift(shift.ctypes.data, nx, ny, nz)
     return shift
+
+
+def velocity_set():
+    """(cx, cy, cz), 27 ints each, in the reference ordering."""
+    c = [np.zeros(27, np.int32) for _ in range(3)]
+    lib().oracle_velocity_set(*[a.ctypes.data for a in c])
+    return c
 
 
 def ray_hit(ax, ay, bx, by, cx, cy) -> bool:

@@ -1030,6 +1030,15 @@ extern "C" {
 
 int oracle_abi_version() { return 1; }
 
+// the velocity set (include/applyStreaming.h:6-8)
+void oracle_velocity_set(int* cx, int* cy, int* cz) {
+  for (int q = 0; q < 27; ++q) {
+    cx[q] = CX[q];
+    cy[q] = CY[q];
+    cz[q] = CZ[q];
+  }
+}
+
 #define ORACLE_PER_CELL(SUF, R)                                                                      \
   void oracle_feq_##SUF(const R* rho_u, R* feq, long long n) {                                       \
     for (long long c = 0; c < n; ++c) {                                                              \

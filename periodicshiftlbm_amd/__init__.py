@@ -173,6 +173,19 @@ _SIGS = [
     ("psl_batch_get_tuning", _I, [_P, _P, _P]),
     ("psl_batch_autotune", _I, [_P, _I, _P, _P, _P]),
     ("psl_batch_census", _I, [_P, _P, _P, _P]),
+    ("psl_device_upload", _I, [_P, _P, ctypes.c_size_t]),
+    ("psl_device_download", _I, [_P, _P, ctypes.c_size_t]),
+    ("psl_mlp_create", _I, [_P]),
+    ("psl_mlp_destroy", _I, [_P]),
+    ("psl_mlp_set_params", _I, [_P, _P, _P, _P]),
+    ("psl_mlp_get_params", _I, [_P, _P, _P, _P]),
+    ("psl_mlp_loss_grad", _I, [_P, _P, _P, _I, _P, _P]),
+    ("psl_mlp_train", _I, [_P, _P, _P, _I, _I, _I, ctypes.c_float, ctypes.c_uint64, _P]),
+    ("psl_mlp_eval", _I, [_P, _P, _I, _P]),
+    ("psl_les_filter", _I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),
+    ("psl_les_targets", _I, [_P, _I, ctypes.c_float, _P, _P]),
+    ("psl_grid_set_closure", _I, [_P, _P]),
+    ("psl_get_macroscopic_device", _I, [_P, _P]),
     ("psl_diad_create", _I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     ("psl_diad_destroy", _I, [_P]),
     ("psl_diad_set_flags", _I, [_P, _P]),
@@ -334,6 +347,17 @@ class GridStruct:
         r, a, b, c = (np.empty(n, np.float32) for _ in range(4))
         _check(lib().psl_get_macroscopic(self._h, r.ctypes.data, a.ctypes.data, b.ctypes.data, c.ctypes.data))
         return r, a, b, c
+
+    def macroscopic_device(self) -> "DeviceBuffer":
+        """rho, ux, uy, uz (4 x N floats) in a new device buffer."""
+        out = DeviceBuffer(4 * self.n * 4)
+        _check(lib().psl_get_macroscopic_device(self._h, out.ptr))
+        return out
+
+    def set_closure(self, mlp):
+        """Use a learned closure (MLP) in the bulk update; None restores Smagorinsky."""
+        self._closure = mlp  # keep the network alive while the grid uses it
+        _check(lib().psl_grid_set_closure(self._h, None if mlp is None else mlp.handle))
 
     def device_f(self):
         p = ctypes.c_void_p()
@@ -543,6 +567,26 @@ class DeviceBuffer:
     def ptr(self):
         return self._p
 
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(max(1, a.nbytes))
+        buf.upload(a)
+        return buf
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        _check(lib().psl_device_upload(self._p, a.ctypes.data, a.nbytes))
+
+    def download(self, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested more bytes than the buffer holds")
+        _check(lib().psl_device_download(out.ctypes.data, self._p, out.nbytes))
+        return out
+
     def close(self):
         if getattr(self, "_p", None) and self._p.value:
             lib().psl_device_free(self._p)
@@ -553,6 +597,92 @@ class DeviceBuffer:
             self.close()
         except Exception:
             pass
+
+
+MLP_INPUTS, MLP_PARAMS, MLP_NU_SCALE = 9, 1409, 1e-3
+
+
+class MLP:
+    """Learned eddy-viscosity closure (psl_mlp_*): 9 -> 32 tanh -> 32 tanh -> 1."""
+
+    def __init__(self, params=None, mean=None, std=None):
+        require_device()
+        self._h = ctypes.c_void_p()
+        _check(lib().psl_mlp_create(ctypes.byref(self._h)))
+        if params is not None:
+            self.set_params(params, mean, std)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().psl_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params, mean=None, std=None):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != MLP_PARAMS:
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        m = None if mean is None else np.ascontiguousarray(mean, np.float32)
+        s = None if std is None else np.ascontiguousarray(std, np.float32)
+        _check(lib().psl_mlp_set_params(self._h, p.ctypes.data, None if m is None else m.ctypes.data,
+                                        None if s is None else s.ctypes.data))
+
+    def get_params(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        m = np.zeros(MLP_INPUTS, np.float32)
+        s = np.zeros(MLP_INPUTS, np.float32)
+        _check(lib().psl_mlp_get_params(self._h, p.ctypes.data, m.ctypes.data, s.ctypes.data))
+        return p, m, s
+
+    def loss_grad(self, x_std: np.ndarray, t: np.ndarray):
+        """FP32 training kernels: mean squared error and gradient on standardised inputs."""
+        x = DeviceBuffer.from_array(np.asarray(x_std, np.float32))
+        tt = DeviceBuffer.from_array(np.asarray(t, np.float32))
+        loss = ctypes.c_double()
+        g = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().psl_mlp_loss_grad(self._h, x.ptr, tt.ptr, int(len(t)), ctypes.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def train(self, raw_dev: DeviceBuffer, t_dev: DeviceBuffer, n: int, batch: int = 4096, iters: int = 500,
+              lr: float = 1e-3, seed: int = 1):
+        hist = np.zeros(iters, np.float32)
+        _check(lib().psl_mlp_train(self._h, raw_dev.ptr, t_dev.ptr, int(n), int(batch), int(iters),
+                                   ctypes.c_float(lr), ctypes.c_uint64(seed), hist.ctypes.data))
+        return hist
+
+    def eval(self, raw: np.ndarray) -> np.ndarray:
+        """Network output of raw feature rows through the step's bf16 matrix-core path."""
+        raw = np.ascontiguousarray(raw, np.float32).reshape(-1, MLP_INPUTS)
+        d = DeviceBuffer.from_array(raw)
+        y = DeviceBuffer(raw.shape[0] * 4)
+        _check(lib().psl_mlp_eval(self._h, d.ptr, raw.shape[0], y.ptr))
+        return y.download(raw.shape[0])
+
+
+def les_filter(rho: DeviceBuffer, ux: DeviceBuffer, uy: DeviceBuffer, uz: DeviceBuffer, fine_nx: int, fine_ny: int,
+               n: int, r: int) -> DeviceBuffer:
+    """Box filter of the fine fields -> device buffer of 10 x m^3 coarse values."""
+    m = n // r
+    out = DeviceBuffer(10 * m ** 3 * 4)
+    _check(lib().psl_les_filter(rho.ptr, ux.ptr, uy.ptr, uz.ptr, fine_nx, fine_ny, n, r, out.ptr))
+    return out
+
+
+def les_targets(coarse: DeviceBuffer, m: int, nu: float):
+    """Raw features (m^3 x 9) and targets (nu_t / 1e-3) from filtered fields."""
+    feat = DeviceBuffer(m ** 3 * MLP_INPUTS * 4)
+    tgt = DeviceBuffer(m ** 3 * 4)
+    _check(lib().psl_les_targets(coarse.ptr, m, ctypes.c_float(nu), feat.ptr, tgt.ptr))
+    return feat, tgt
 
 
 def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):

@@ -27,7 +27,7 @@ EXTRA = {
     # voxelizer masks must be bit-identical to the CPU restatement
     "psl_stl.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["psl_grid.hip", "psl_refine.hip", "psl_stl.hip", "psl_esotwist.hip", "psl_batch.hip"]
+SOURCES = ["psl_grid.hip", "psl_refine.hip", "psl_stl.hip", "psl_esotwist.hip", "psl_batch.hip", "psl_closure.hip"]
 
 
 def hipcc() -> str:

@@ -332,6 +332,7 @@ bool batch_stale(const psl_batch* b) {
 const char* unbatchable(const psl_grid* g) {
   if (!g->flags) return "batch grids need flags (psl_grid_set_flags)";
   if (g->cs_cell) return "batch grids cannot use per-cell Cs";
+  if (g->closure) return "batch grids cannot use the learned closure";
   if (g->periodic_copies) return "batch grids cannot use periodic copies";
   if (g->info.iSubgridEnd - g->info.iSubgridStart > 4) return "batch grids cannot carry a sub-grid";
   return nullptr;

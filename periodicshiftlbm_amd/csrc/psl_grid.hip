@@ -18,6 +18,7 @@
 //   k_periodic     applyBCPeriodicX/Y surface copies (before the update).
 #include "psl_internal.h"
 #include "psl_stream.h"
+#include "psl_closure.h"
 
 #include <algorithm>
 #include <cmath>
@@ -59,6 +60,11 @@ __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
 
 // One workgroup per entry of the fused order (fused_order): boundary-list
 // ranges interleaved with the bulk blocks they border.
+template <class St, bool FLAGS, int TUNE>
+__global__ __launch_bounds__(256) void k_ps_bulk_closure(StepParams p, const MlpNet* __restrict__ net, float nu) {
+  bulk_update_closure<St, FLAGS, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, net, nu);
+}
+
 template <class St, bool LES, int TUNE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_ps_fused(
     StepParams p, BoundaryParams b, const FusedEntry* __restrict__ order) {
@@ -435,6 +441,13 @@ static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, c
 }
 
 template <class St, int TUNE>
+static void launch_closure(bool flags, uint32_t nb, hipStream_t stream, const StepParams& p, const MlpNet* net,
+                           float nu) {
+  if (flags) k_ps_bulk_closure<St, true, TUNE><<<nb, 256, 0, stream>>>(p, net, nu);
+  else k_ps_bulk_closure<St, false, TUNE><<<nb, 256, 0, stream>>>(p, net, nu);
+}
+
+template <class St, int TUNE>
 static void launch_fused(bool les, uint32_t nb, hipStream_t stream, const StepParams& p, const BoundaryParams& b,
                          const FusedEntry* order) {
   if (les) k_ps_fused<St, true, TUNE><<<nb, 256, 0, stream>>>(p, b, order);
@@ -469,7 +482,8 @@ static int launch_local_update(psl_grid* g) {
   b.ny = g->info.cellCountY;
   b.nz = g->info.cellCountZ;
   b.bc = g->bc_dev;
-  const bool fused = b.n && (g->tuning & PSL_TUNE_FUSED_BOUNDARY);
+  const MlpNet* net = psl_mlp_device_net(g->closure);
+  const bool fused = b.n && (g->tuning & PSL_TUNE_FUSED_BOUNDARY) && !net;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (g->profile) {
     PSL_HIP_TRY(hipEventCreate(&ev0));
@@ -478,7 +492,16 @@ static int launch_local_update(psl_grid* g) {
   }
   const int nt = g->tuning & (TUNE_NT_LOAD | TUNE_NT_STORE);
   const uint32_t nb = blocks_for(p.n);
-  if (fused) {
+  if (net) {
+    const bool fl = g->flags != nullptr;
+    const float nu = g->info.nu;
+    switch (nt) {
+      case 0: launch_closure<St, 0>(fl, nb, g->stream, p, net, nu); break;
+      case 1: launch_closure<St, 1>(fl, nb, g->stream, p, net, nu); break;
+      case 2: launch_closure<St, 2>(fl, nb, g->stream, p, net, nu); break;
+      default: launch_closure<St, 3>(fl, nb, g->stream, p, net, nu); break;
+    }
+  } else if (fused) {
     const uint32_t nf = (uint32_t)g->fused_host.size();
     switch (nt) {
       case 0: launch_fused<St, 0>(les, nf, g->stream, p, b, g->fused_dev); break;
@@ -827,6 +850,12 @@ int psl_set_f(psl_grid* g, const float* host) {
   return rc;
 }
 
+int psl_get_macroscopic_device(psl_grid* g, float* out) {
+  PSL_CHECK_ARG(g && out, "null argument");
+  const size_t n = (size_t)g->info.cellCount;
+  return dispatch_storage<MacroFn>(g->storage, g, out, out + n, out + 2 * n, out + 3 * n);
+}
+
 int psl_get_macroscopic(psl_grid* g, float* rho, float* ux, float* uy, float* uz) {
   PSL_CHECK_ARG(g, "null grid");
   const size_t n = (size_t)g->info.cellCount;
@@ -919,6 +948,18 @@ int psl_device_free(void* ptr) {
   return PSL_OK;
 }
 
+int psl_device_upload(void* dst, const void* host, size_t bytes) {
+  PSL_CHECK_ARG(dst && host, "null argument");
+  PSL_HIP_TRY(hipMemcpy(dst, host, bytes, hipMemcpyHostToDevice));
+  return PSL_OK;
+}
+
+int psl_device_download(void* host, const void* src, size_t bytes) {
+  PSL_CHECK_ARG(host && src, "null argument");
+  PSL_HIP_TRY(hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost));
+  return PSL_OK;
+}
+
 int psl_device_synchronize(void) {
   PSL_HIP_TRY(hipDeviceSynchronize());
   return PSL_OK;
@@ -1002,6 +1043,12 @@ int psl_grid_autotune(psl_grid* g, int steps, int* best, double* times_ms) {
   }
   g->tuning = best_v;
   if (best) *best = best_v;
+  return PSL_OK;
+}
+
+int psl_grid_set_closure(psl_grid* g, const psl_mlp* m) {
+  PSL_CHECK_ARG(g, "null grid");
+  g->closure = m;
   return PSL_OK;
 }
 

@@ -79,6 +79,7 @@ struct BcTable {
 
 // ------------------------------------------------------------------ handles
 namespace psl {
+struct MlpNet;
 // One workgroup of a fused update launch: a block of 256 bulk cells
 // (b == FUSED_BULK, a = block index) or the boundary-list entries [a, b).
 struct FusedEntry {
@@ -119,7 +120,11 @@ struct psl_grid {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
   int64_t census[3] = {0, 0, 0};  // bulk, boundary-list, skipped cells
   int tuning = PSL_TUNE_XCD_REMAP;  // bulk-kernel variant (psl_grid_set_tuning / autotune)
+  const psl_mlp* closure = nullptr;  // learned closure (null: Smagorinsky)
 };
+
+// device image of the learned closure of a psl_mlp (psl_closure.hip)
+const psl::MlpNet* psl_mlp_device_net(const psl_mlp* m);
 
 struct psl_mask {
   uint8_t* d = nullptr;

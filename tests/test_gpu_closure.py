@@ -1,0 +1,213 @@
+"""Learned eddy-viscosity closure on the GPU against the CPU closure oracle
+(oracle/closure.py, FP64) and the step oracle.
+
+Tolerances:
+  * FP32 training kernels (loss, gradient, Adam steps) vs FP64: 1e-4 relative
+    to the largest component (FP32 sums over hundreds of samples);
+  * bf16 matrix-core inference vs the FP64 forward: |dy| <= 0.03 max|y| + 0.02
+    (bf16 keeps 8 significant bits of weights and activations);
+  * filter / targets (FP64 filter sums, FP32 strain and features) vs FP64:
+    1e-4 relative;
+  * the closure inside the step: a network with constant output must give
+    the plain step with nu + nu_t (1e-5 relative), and one collision with a
+    random network must equal the oracle collision with the per-cell nu_t the
+    network gives for the oracle's own features (1e-5 relative).
+"""
+import numpy as np
+import pytest
+
+from oracle import closure as C
+from periodicshiftlbm_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _splitmix_indices(seed, step, B, n):
+    """The trainer's mini-batch draw (splitmix64 of seed + (step B + b + 1) gamma)."""
+    M = (1 << 64) - 1
+    out = np.empty(B, np.int64)
+    for b in range(B):
+        z = (seed + (step * B + b + 1) * 0x9E3779B97F4A7C15) & M
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        z ^= z >> 31
+        out[b] = z % n
+    return out
+
+
+def _random_pi(rng, n, scale=1e-3):
+    rho = rng.uniform(0.97, 1.03, n)
+    P = rng.normal(0, scale, (6, n))
+    return rho, P
+
+
+def test_training_gradient_matches_oracle(psl):
+    P = psl
+    rng = np.random.default_rng(11)
+    p = (C.init_params(3) + rng.normal(0, 0.05, C.N_PARAM)).astype(np.float32)
+    x = rng.normal(0, 1, (777, C.N_IN)).astype(np.float32)
+    t = rng.normal(0.5, 1, 777).astype(np.float32)
+    net = P.MLP(p)
+    loss, g = net.loss_grad(x, t)
+    p64, x64, t64 = p.astype(np.float64), x.astype(np.float64), t.astype(np.float64)
+    g_ref = C.mlp_grad(p64, x64, t64)
+    l_ref = C.mlp_loss(p64, x64, t64)
+    err = np.max(np.abs(g - g_ref)) / np.max(np.abs(g_ref))
+    print(f"[mlp-grad] loss {loss:.6e} (oracle {l_ref:.6e}); gradient max rel err {err:.2e}", flush=True)
+    assert abs(loss / l_ref - 1) < 1e-4
+    assert err < 1e-4
+
+
+def test_adam_training_matches_oracle(psl):
+    P = psl
+    rng = np.random.default_rng(12)
+    n, B, iters, lr, seed = 3000, 256, 5, 3e-3, 99
+    rho, Pi = _random_pi(rng, n)
+    raw = C.raw_features(rho, *Pi).astype(np.float32)
+    t = rng.uniform(0, 2, n).astype(np.float32)
+    p0 = C.init_params(4).astype(np.float32)
+    net = P.MLP(p0)
+    hist = net.train(P.DeviceBuffer.from_array(raw), P.DeviceBuffer.from_array(t), n, batch=B, iters=iters, lr=lr,
+                     seed=seed)
+    p_gpu, mean_gpu, std_gpu = net.get_params()
+    # oracle: same standardisation (FP64 moments, then FP32), same batches, FP64 Adam
+    r64 = raw.astype(np.float64)
+    mean = r64.mean(0)
+    std = np.sqrt(np.maximum((r64 * r64).mean(0) - mean * mean, 0))
+    np.testing.assert_allclose(mean_gpu, mean, rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(std_gpu, std, rtol=1e-4)
+    x = ((raw - mean_gpu) * (1.0 / std_gpu).astype(np.float32)).astype(np.float64)
+    p = p0.astype(np.float64)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for it in range(iters):
+        idx = _splitmix_indices(seed, it, B, n)
+        p, m, v = C.adam_step(p, C.mlp_grad(p, x[idx], t[idx].astype(np.float64)), m, v, it + 1, lr)
+    err = np.max(np.abs(p_gpu - p)) / np.max(np.abs(p))
+    print(f"[adam] losses {hist.tolist()}; params max rel err {err:.2e}", flush=True)
+    assert err < 1e-4
+
+
+def test_matrix_core_inference_matches_oracle(psl):
+    P = psl
+    rng = np.random.default_rng(13)
+    n = 1000  # not a multiple of 64: the last wavefront is partly past the rows
+    rho, Pi = _random_pi(rng, n)
+    raw = C.raw_features(rho, *Pi)
+    mean, std = raw.mean(0), raw.std(0)
+    p = C.init_params(5) + rng.normal(0, 0.1, C.N_PARAM)
+    net = P.MLP(p.astype(np.float32), mean.astype(np.float32), std.astype(np.float32))
+    y = net.eval(raw.astype(np.float32))
+    y_ref, _ = C.mlp_forward(p, C.standardise(raw, mean, std))
+    err = np.max(np.abs(y - y_ref))
+    print(f"[mlp-mfma] max |y - y64| {err:.3e}, max |y64| {np.max(np.abs(y_ref)):.3e}", flush=True)
+    assert err <= 0.03 * np.max(np.abs(y_ref)) + 0.02
+
+
+def test_les_filter_and_targets_match_oracle(psl):
+    P = psl
+    rng = np.random.default_rng(14)
+    n, r = 16, 4
+    m = n // r
+    fx, fy, fz = n + 1, n + 1, n  # the periodic layout with duplicated x / y layers
+    k, j, i = np.meshgrid(np.arange(fz), np.arange(fy), np.arange(fx), indexing="ij")
+    th = 2 * np.pi / n
+    rho = 1 + 0.01 * np.cos(th * i) * np.sin(2 * th * k) + 0.002 * rng.normal(size=i.shape)
+    ux = 0.04 * np.sin(th * i + 1) * np.cos(th * j) + 0.005 * rng.normal(size=i.shape)
+    uy = -0.03 * np.cos(th * i) * np.sin(th * j + 2) + 0.005 * rng.normal(size=i.shape)
+    uz = 0.02 * np.sin(th * k + th * i) + 0.005 * rng.normal(size=i.shape)
+    bufs = [P.DeviceBuffer.from_array(a.astype(np.float32).ravel()) for a in (rho, ux, uy, uz)]
+    co = P.les_filter(*bufs, fx, fy, n, r)
+    got = co.download((10, m, m, m))
+    uniq = [a.astype(np.float32).astype(np.float64)[:, :n, :n] for a in (rho, ux, uy, uz)]
+    rb, a, b, c, tau = C.les_filter(*uniq, r)
+    ref = np.concatenate([np.stack([rb, a, b, c]), tau])
+    err = np.max(np.abs(got - ref) / (np.abs(ref) + 1e-6))
+    print(f"[les-filter] max rel err {err:.2e}", flush=True)
+    assert err < 1e-4
+    nu = 2.5e-4
+    feat, tgt = P.les_targets(co, m, nu)
+    f_gpu = feat.download((m, m, m, C.N_IN))
+    t_gpu = tgt.download((m, m, m))
+    S = C.strain_rate(got[1].astype(np.float64), got[2].astype(np.float64), got[3].astype(np.float64))
+    nut, f_ref = C.les_targets(got[0].astype(np.float64), got[4:].astype(np.float64), S, nu)
+    terr = np.max(np.abs(t_gpu - nut / C.NU_SCALE)) / max(np.max(nut / C.NU_SCALE), 1e-12)
+    ferr = np.max(np.abs(f_gpu - f_ref) / (np.abs(f_ref) + 1e-3 * np.max(np.abs(f_ref), axis=(0, 1, 2))))
+    print(f"[les-targets] nu_t* range {nut.min():.2e}..{nut.max():.2e}; target err {terr:.2e}, "
+          f"feature err {ferr:.2e}", flush=True)
+    assert terr < 1e-4
+    assert ferr < 1e-4
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_constant_network_equals_viscosity_shift(psl, storage):
+    P = psl
+    n = 16
+    shape = cases.tgv_shape(n)
+    nu, y0 = 0.004, 2.0
+    rho, ux, uy, uz = cases.tgv_fields(n)
+    p = np.zeros(C.N_PARAM, np.float32)
+    p[-1] = y0  # b3: y = y0 for every cell, nu_t = 1e-3 y0
+    net = P.MLP(p)
+    grids = []
+    for nu_g, closure in ((nu, net), (nu + C.NU_SCALE * y0, None)):
+        info = P.InfoStruct(cellCountX=shape[0], cellCountY=shape[1], cellCountZ=shape[2], nu=nu_g)
+        g = P.GridStruct(info, storage)
+        P.fill_equilibrium(g, rho, ux, uy, uz)
+        g.set_periodic_copies(x=True, y=True)
+        if closure is not None:
+            g.set_closure(closure)
+        P.updateGrid([g], 10)
+        grids.append(g)
+    a, b = grids[0].get_f(), grids[1].get_f()
+    err = float(np.max(np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-3)))
+    print(f"[closure-const-{storage}] max rel err {err:.3e}", flush=True)
+    assert err < 1e-5
+
+
+def test_closure_collision_matches_oracle(psl, oracle_mod):
+    P, O = psl, oracle_mod
+    rng = np.random.default_rng(15)
+    shape = (16, 8, 8)
+    nc = int(np.prod(shape))
+    nu = 3e-4
+    info = P.InfoStruct(cellCountX=shape[0], cellCountY=shape[1], cellCountZ=shape[2], nu=nu)
+    g = P.GridStruct(info, "f32")
+    rho = rng.uniform(0.97, 1.03, nc)
+    u = rng.uniform(-0.05, 0.05, (3, nc))
+    f = O.feq(np.stack([rho, *u], axis=1))  # (nc, 27)
+    f = (f * (1 + rng.normal(0, 0.01, f.shape))).astype(np.float32)
+    g.set_f(np.ascontiguousarray(f.T))
+    # network and standardisation from the features of these cells
+    raw = _features_from_f(f.astype(np.float64))
+    mean, std = raw.mean(0), raw.std(0) + 1e-12
+    p = C.init_params(6) + rng.normal(0, 0.1, C.N_PARAM)
+    p[-1] = 1.0
+    net = P.MLP(p.astype(np.float32), mean.astype(np.float32), std.astype(np.float32))
+    g.set_closure(net)
+    P.applyLocalCellUpdate(g)
+    got = g.get_f().T
+    # expected: oracle collision with nu + nu_t, nu_t from the network on the oracle's features
+    y = net.eval(raw.astype(np.float32))
+    exp = np.empty_like(f64)
+    for cidx in range(nc):
+        nut = C.NU_SCALE * max(float(y[cidx]), 0.0)
+        exp[cidx] = O.collide(f[cidx:cidx + 1].astype(np.float32), nu + nut, 0.0)[0]
+    err = float(np.max(np.abs(got.astype(np.float64) - exp) / np.maximum(np.abs(exp), 1e-3)))
+    print(f"[closure-collision] nu_t range {C.NU_SCALE * max(y.min(), 0):.2e}..{C.NU_SCALE * y.max():.2e}; "
+          f"max rel err {err:.3e}", flush=True)
+    assert err < 1e-5
+
+
+def _features_from_f(f):
+    """Raw closure features of populations (n, 27) in FP64 (central second moments)."""
+    from oracle import velocity_set
+    cx, cy, cz = (np.asarray(a, np.float64) for a in velocity_set())
+    rho = f.sum(1)
+    ux, uy, uz = (f @ cx) / rho, (f @ cy) / rho, (f @ cz) / rho
+    m200, m020, m002 = f @ (cx * cx), f @ (cy * cy), f @ (cz * cz)
+    m110, m101, m011 = f @ (cx * cy), f @ (cx * cz), f @ (cy * cz)
+    k200, k020, k002 = m200 - rho * ux * ux, m020 - rho * uy * uy, m002 - rho * uz * uz
+    k110, k101, k011 = m110 - rho * ux * uy, m101 - rho * ux * uz, m011 - rho * uy * uz
+    third = rho / 3
+    return C.raw_features(rho, k200 - third, k020 - third, k002 - third, k110, k101, k011)
