@@ -1,0 +1,255 @@
+"""Learned-closure workflow: decaying isotropic turbulence on a fine periodic
+grid, filtered training data, training, and coarse LES runs judged against
+the filtered fine run.
+
+No reference counterpart (the reference's LES model is Smagorinsky,
+include/cellFunctions.h:157-198).  All solver work runs through the library:
+the fine and coarse steps, the filter / target kernels (psl_les_filter,
+psl_les_targets), training (psl_mlp_train) and the step with the network
+(psl_grid_set_closure).  The host does the initial random field (FFT) and
+the energy spectra of the coarse velocity fields.
+
+Scaling between the grids (same lattice velocity, acoustic scaling): a coarse
+cell is r fine cells, a coarse step is r fine steps and nu_coarse = nu_fine / r.
+"""
+from __future__ import annotations
+
+import os
+import time
+
+import numpy as np
+
+from . import cases
+
+
+def isotropic_field(n: int, k_peak: float, u_rms: float = 0.04, seed: int = 7):
+    """Random-phase solenoidal velocity on a periodic n^3 box with shell
+    spectrum E(k) ~ k^4 exp(-2 (k / k_peak)^2), scaled to u_rms (per
+    component).  Arrays indexed [k, j, i] (z, y, x), float32."""
+    import scipy.fft as sfft
+
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, 1.0 / n).astype(np.float32)
+    kz, ky, kx = np.meshgrid(k1, k1, k1, indexing="ij")
+    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
+    kk[0, 0, 0] = 1.0
+    amp = np.sqrt(kk ** 4 * np.exp(-2.0 * (kk / k_peak) ** 2) / (4 * np.pi * kk * kk)).astype(np.float32)
+    amp[0, 0, 0] = 0.0
+    comps = []
+    for _ in range(3):
+        c = (rng.standard_normal((n, n, n), dtype=np.float32)
+             + 1j * rng.standard_normal((n, n, n), dtype=np.float32)).astype(np.complex64)
+        comps.append(c * amp)
+    # divergence-free projection u - k (k.u) / k^2
+    dot = (kx * comps[0] + ky * comps[1] + kz * comps[2]) / (kk * kk)
+    comps = [comps[0] - kx * dot, comps[1] - ky * dot, comps[2] - kz * dot]
+    del dot
+    workers = max(1, min(16, os.cpu_count() or 1))
+    out = []
+    for c in comps:
+        out.append(np.ascontiguousarray(sfft.ifftn(c, workers=workers).real.astype(np.float32)))
+    rms = np.sqrt(np.mean([np.mean(u.astype(np.float64) ** 2) for u in out]))
+    return [u * np.float32(u_rms / rms) for u in out]
+
+
+def glorot_params(seed: int) -> np.ndarray:
+    """Initial flat parameters (W1, b1, W2, b2, W3, b3): Glorot-uniform weights,
+    zero biases, from a fixed seed."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((9, 32), (32, 32), (32, 1)):
+        lim = np.sqrt(6.0 / (fan_in + fan_out))
+        parts += [rng.uniform(-lim, lim, fan_in * fan_out), np.zeros(fan_out)]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def _padded(a: np.ndarray) -> np.ndarray:
+    """n^3 periodic field -> the (n, n+1, n+1) array of the duplicated-layer layout."""
+    return np.concatenate([np.concatenate([a, a[:, :1, :]], axis=1),
+                           np.concatenate([a, a[:, :1, :]], axis=1)[:, :, :1]], axis=2)
+
+
+def periodic_grid(P, n: int, nu: float, storage: str, u, cs: float = 0.0, rho=None):
+    """Fully periodic n^3 box (duplicated x / y layers, z through the shift) at feq(rho, u)."""
+    nx, ny, nz = cases.tgv_shape(n)
+    info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=nu)
+    g = P.GridStruct(info, storage)
+    g.set_periodic_copies(x=True, y=True)
+    if cs:
+        g.set_smagorinsky(cs)
+    r = np.ones((n, n, n), np.float32) if rho is None else rho
+    P.fill_equilibrium(g, *[_padded(np.asarray(a, np.float32)).ravel() for a in (r, *u)])
+    return g
+
+
+def unique_velocity(g, n: int):
+    r, a, b, c = g.macroscopic()
+    sh = (n, n + 1, n + 1)
+    return [x.reshape(sh)[:, :n, :n].astype(np.float64) for x in (a, b, c)]
+
+
+def energy_spectrum(ux, uy, uz):
+    """Shell-summed kinetic energy spectrum of a periodic n^3 field, k = 0..n/2."""
+    n = ux.shape[0]
+    k1 = np.fft.fftfreq(n, 1.0 / n)
+    kk = np.sqrt(k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + k1[None, None, :] ** 2)
+    shell = np.minimum(np.rint(kk).astype(int), n // 2).ravel()
+    e = np.zeros(n // 2 + 1)
+    for c in (ux, uy, uz):
+        F = np.fft.fftn(c) / c.size
+        np.add.at(e, shell, 0.5 * (np.abs(F) ** 2).ravel())
+    return e
+
+
+def spectrum_error(e, e_ref, kmax: int) -> float:
+    """Mean |ln(E / E_ref)| over shells 1..kmax."""
+    k = np.arange(1, kmax + 1)
+    return float(np.mean(np.abs(np.log(e[k] / e_ref[k]))))
+
+
+class Workflow:
+    """Fine run -> filtered snapshots (training pairs + reference spectra) ->
+    training -> coarse LES comparisons."""
+
+    def __init__(self, P, n: int = 512, r: int = 4, k_peak: float = 8.0, u_rms: float = 0.04,
+                 nu_fine: float = 5e-4, cs_fine: float = 0.1, storage: str = "f16", seed: int = 7, log=print):
+        self.P, self.n, self.r, self.m = P, n, r, n // r
+        self.k_peak, self.u_rms, self.nu_fine, self.cs_fine = k_peak, u_rms, nu_fine, cs_fine
+        self.nu_coarse = nu_fine / r
+        self.storage, self.seed, self.log = storage, seed, log
+        self.snapshots = {}   # fine step -> (coarse rho, u) host arrays of the filtered field
+        self.feats, self.tgts = [], []  # device buffers per snapshot
+        self.timings = {}
+
+    def run_fine(self, steps: int, every: int, first_sample: int):
+        P, n, r, m = self.P, self.n, self.r, self.m
+        t0 = time.perf_counter()
+        u = isotropic_field(n, self.k_peak, self.u_rms, self.seed)
+        self.log(f"[les] initial field {n}^3 in {time.perf_counter() - t0:.1f} s")
+        g = periodic_grid(P, n, self.nu_fine, self.storage, u, self.cs_fine)
+        del u
+        nx, ny, _ = cases.tgv_shape(n)
+        done = 0
+        t_step = 0.0
+        for s in range(0, steps + 1, every):
+            if s > done:
+                P.synchronize()
+                t1 = time.perf_counter()
+                P.updateGrid([g], s - done)
+                P.synchronize()
+                t_step += time.perf_counter() - t1
+                done = s
+            mac = g.macroscopic_device()
+            N = nx * ny * n
+            views = [_Offset(mac, k * N * 4) for k in range(4)]
+            co = P.les_filter(*views, nx, ny, n, r)
+            coarse = co.download((10, m, m, m))
+            self.snapshots[s] = (coarse[0], coarse[1], coarse[2], coarse[3])
+            if s >= first_sample:
+                feat, tgt = P.les_targets(co, m, self.nu_coarse)
+                self.feats.append(feat)
+                self.tgts.append(tgt)
+            del mac, co
+            self.log(f"[les] fine step {s}: filtered snapshot ({'sampled' if s >= first_sample else 'reference'})")
+        self.timings["fine_mlups"] = g.n * done / max(t_step, 1e-9) / 1e6
+        g.close()
+
+    def training_set(self):
+        P, m = self.P, self.m
+        k = len(self.feats)
+        ns = m ** 3
+        feat = P.DeviceBuffer(k * ns * 9 * 4)
+        tgt = P.DeviceBuffer(k * ns * 4)
+        for i, (f, t) in enumerate(zip(self.feats, self.tgts)):
+            P.device_copy(_Offset(feat, i * ns * 9 * 4), f, ns * 9 * 4)
+            P.device_copy(_Offset(tgt, i * ns * 4), t, ns * 4)
+        return feat, tgt, k * ns
+
+    def train(self, iters: int = 2000, batch: int = 16384, lr: float = 2e-3):
+        P = self.P
+        feat, tgt, ns = self.training_set()
+        net = P.MLP(glorot_params(self.seed))
+        t0 = time.perf_counter()
+        hist = net.train(feat, tgt, ns, batch=batch, iters=iters, lr=lr, seed=self.seed)
+        self.timings["train_s"] = time.perf_counter() - t0
+        t = tgt.download(ns)
+        self.target_stats = {"samples": ns, "mean": float(t.mean()), "std": float(t.std()),
+                             "frac_zero": float(np.mean(t <= 0))}
+        self.log(f"[les] trained on {ns} samples: loss {hist[0]:.4e} -> {hist[-1]:.4e} "
+                 f"(target mean {t.mean():.3f}, std {t.std():.3f})")
+        self.net, self.loss_hist = net, hist
+        return net
+
+    def coarse_run(self, model: str, start: int, cs: float = 0.17):
+        """Coarse LES from the filtered snapshot at fine step `start`; spectra
+        at every later snapshot.  model: "smagorinsky", "mlp" or "none"."""
+        P, m, r = self.P, self.m, self.r
+        rho, ux, uy, uz = self.snapshots[start]
+        g = periodic_grid(P, m, self.nu_coarse, "f32", (ux, uy, uz), cs if model == "smagorinsky" else 0.0, rho)
+        if model == "mlp":
+            g.set_closure(self.net)
+        errs = {}
+        done = start
+        t_step, n_step = 0.0, 0
+        for s in sorted(k for k in self.snapshots if k > start):
+            nc = (s - done) // r
+            P.synchronize()
+            t1 = time.perf_counter()
+            P.updateGrid([g], nc)
+            P.synchronize()
+            t_step += time.perf_counter() - t1
+            n_step += nc
+            done = s
+            e = energy_spectrum(*unique_velocity(g, m))
+            errs[s] = spectrum_error(e, self.reference_spectrum(s), m // 2)
+        g.close()
+        return errs, t_step / max(n_step, 1)
+
+    def reference_spectrum(self, s: int):
+        """Energy spectrum of the filtered fine field at fine step s (cached)."""
+        if not hasattr(self, "_eref"):
+            self._eref = {}
+        if s not in self._eref:
+            _, fx, fy, fz = self.snapshots[s]
+            self._eref[s] = energy_spectrum(fx.astype(np.float64), fy.astype(np.float64), fz.astype(np.float64))
+        return self._eref[s]
+
+    def compare(self, start: int, cs: float = 0.17):
+        """Spectrum errors of the coarse LES runs (mean over the later snapshots)."""
+        out = {}
+        for model in ("none", "smagorinsky", "mlp"):
+            errs, sec = self.coarse_run(model, start, cs)
+            out[model] = {"mean_error": float(np.mean(list(errs.values()))),
+                          "errors": {int(k): round(v, 5) for k, v in errs.items()}, "ms_per_step": sec * 1e3}
+            self.log(f"[les] coarse {model}: mean spectrum error {out[model]['mean_error']:.4f}")
+        return out
+
+    def cost_ratio(self, n: int = 256, steps: int = 20, storage: str = "f32", cs: float = 0.17):
+        """Mean bulk-kernel time with the network / with Smagorinsky on a periodic
+        n^3 box (HIP events around the bulk launch)."""
+        P = self.P
+        u = [np.zeros((n, n, n), np.float32)] * 3
+        times = {}
+        for model in ("smagorinsky", "mlp"):
+            g = periodic_grid(P, n, self.nu_coarse, storage, u, cs)
+            if model == "mlp":
+                g.set_closure(self.net)
+            P.updateGrid([g], 2)
+            g.profile(True)
+            P.updateGrid([g], steps)
+            ms, launches = g.profile_read()
+            g.profile(False)
+            times[model] = ms / max(1, launches)
+            g.close()
+        return times["mlp"] / times["smagorinsky"], times
+
+
+class _Offset:
+    """A view of a DeviceBuffer at a byte offset (for the C ABI)."""
+
+    def __init__(self, buf, offset: int):
+        import ctypes
+
+        self._buf = buf
+        self.nbytes = buf.nbytes - offset
+        self.ptr = ctypes.c_void_p(buf.ptr.value + offset)

@@ -211,3 +211,22 @@ def _features_from_f(f):
     k110, k101, k011 = m110 - rho * ux * uy, m101 - rho * ux * uz, m011 - rho * uy * uz
     third = rho / 3
     return C.raw_features(rho, k200 - third, k020 - third, k002 - third, k110, k101, k011)
+
+
+def test_les_workflow_small(psl):
+    """The learned-closure workflow at reduced size: 128^3 fine decaying
+    turbulence, filtered to 32^3, trained, and coarse LES runs judged by their
+    energy spectra against the filtered fine run.  Shows the gain of the
+    network over running without a model and reports it against Smagorinsky."""
+    from periodicshiftlbm_amd import les
+
+    P = psl
+    wf = les.Workflow(P, n=128, r=4, k_peak=4.0, nu_fine=1e-3, cs_fine=0.1, storage="f16", seed=3,
+                      log=lambda *a: print(*a, flush=True))
+    wf.run_fine(steps=800, every=100, first_sample=100)
+    wf.train(iters=400, batch=8192, lr=2e-3)
+    assert wf.loss_hist[-1] < 0.5 * wf.loss_hist[0]
+    res = wf.compare(start=100)
+    print(f"[les-small] errors: none {res['none']['mean_error']:.4f}, smagorinsky "
+          f"{res['smagorinsky']['mean_error']:.4f}, mlp {res['mlp']['mean_error']:.4f}", flush=True)
+    assert res["mlp"]["mean_error"] < res["none"]["mean_error"]
