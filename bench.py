@@ -109,6 +109,36 @@ def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
             "L": [d["L"] for d in doms], "re": [round(d["re"], 1) for d in doms]}
 
 
+def closure_leg(P, n: int, log):
+    """Learned closure at the decaying-turbulence configuration: n^3 fine run
+    (FP16 storage, Cs 0.1), filtered to (n/4)^3 every 200 steps, network trained
+    on the filtered pairs, coarse LES runs judged by energy spectrum against the
+    filtered fine run, and the bulk-kernel cost with the network against
+    Smagorinsky."""
+    from periodicshiftlbm_amd import les
+
+    wf = les.Workflow(P, n=n, r=4, k_peak=n / 64.0, nu_fine=5e-4, cs_fine=0.1, storage="f16", seed=7, log=log)
+    steps = 3200 if n >= 256 else 800
+    every = 200 if n >= 256 else 100
+    wf.run_fine(steps=steps, every=every, first_sample=every)
+    wf.train(iters=2000, batch=16384, lr=2e-3)
+    res = wf.compare(start=every)
+    ratio32, t32 = wf.cost_ratio(n=256, storage="f32")
+    ratio16, t16 = wf.cost_ratio(n=256, storage="f16")
+    gain = 1.0 - res["mlp"]["mean_error"] / res["smagorinsky"]["mean_error"]
+    log(f"  closure: spectrum error none {res['none']['mean_error']:.4f}, smagorinsky "
+        f"{res['smagorinsky']['mean_error']:.4f}, mlp {res['mlp']['mean_error']:.4f} (gain {gain * 100:.1f} %); "
+        f"cost ratio f32 {ratio32:.2f} f16 {ratio16:.2f}")
+    return {"fine": f"{n}^3", "coarse": f"{n // 4}^3", "fine_steps": steps, "snapshot_every": every,
+            "fine_mlups": wf.timings.get("fine_mlups"), "train_s": wf.timings.get("train_s"),
+            "targets": wf.target_stats, "loss_first": float(wf.loss_hist[0]), "loss_last": float(wf.loss_hist[-1]),
+            "spectrum_error": {k: v["mean_error"] for k, v in res.items()},
+            "spectrum_error_per_snapshot": {k: v["errors"] for k, v in res.items()},
+            "gain_vs_smagorinsky": gain,
+            "cost_ratio_vs_smagorinsky": {"f32": ratio32, "f16": ratio16},
+            "bulk_kernel_ms_256cubed": {"f32": t32, "f16": t16}}
+
+
 def cpu_baseline(cases, flags, nu, cs, shape, budget_s: float):
     import oracle as O
 
@@ -139,7 +169,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--storages", default="f32,f16,bf16")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", type=int, default=None,
+                    help="use this local-update variant (bits: 1 NT loads, 2 NT stores, 4 XCD order, 8 fused "
+                         "boundary) instead of autotuning")
     ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the 64-domain batch (0: skip)")
+    ap.add_argument("--closure-n", type=int, default=512,
+                    help="fine grid edge of the learned-closure leg (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times) for counter calibration")
     args = ap.parse_args()
@@ -170,7 +205,9 @@ def main():
         g, flags, nsolid, nu = build_sphere_grid(P, cases, storage, shape)
         flags_keep, nu_keep = flags, nu
         tune_times = None
-        if not args.no_autotune:
+        if args.tuning is not None:
+            g.tuning = args.tuning
+        elif not args.no_autotune:
             best, tt = g.autotune(10)
             tune_times = {int(v): float(t) for v, t in enumerate(tt)}
             P.fill_equilibrium(g)  # restart from rest after tuning
@@ -233,9 +270,12 @@ def main():
         out["variants"]["f16_over_f32"] = results["f16"]["mlups"] / results["f32"]["mlups"]
     if args.batch_steps:
         log(f"[bench] batch of 64 domains, {args.batch_steps} steps")
-        bl = batch_leg(P, cases, args.batch_steps, 0 if args.no_autotune else 3)
+        bl = batch_leg(P, cases, args.batch_steps, 0 if (args.no_autotune or args.tuning is not None) else 3)
         log(f"  {bl['mlups']:.1f} MLUPS over {bl['cells']} cells")
         out["variants"]["batch"] = bl
+    if args.closure_n:
+        log(f"[bench] learned closure, {args.closure_n}^3 decaying turbulence")
+        out["variants"]["closure"] = closure_leg(P, args.closure_n, log)
     if not args.no_cpu:
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(cases, flags_keep, nu_keep, 0.17, shape, args.cpu_seconds)
