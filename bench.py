@@ -41,6 +41,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "sphere_256x128x128_hbm_counters.md")
+
+
+def measured_traffic(kernel: str, storage: str, tuning: int):
+    """HBM bytes per launch of `kernel` at `storage` and tuning variant from the
+    committed rocprofv3 counter summary (FETCH_SIZE x 2 + WRITE_SIZE x 1, the
+    factors from a copy kernel of known size); None when not measured."""
+    import re
+
+    try:
+        text = open(TRAFFIC_SUMMARY).read()
+    except OSError:
+        return None
+    pat = re.compile(rf"^traffic {re.escape(kernel)} {re.escape(storage)} tuning={tuning} = (\d+)", re.M)
+    m = pat.search(text)
+    return int(m.group(1)) if m else None
+
+
 def build_sphere_grid(P, cases, storage: str, shape=(256, 128, 128), u_inlet=0.04, reynolds=1000.0, cs=0.17):
     nx, ny, nz = shape
     nu = cases.sphere_nu(u_inlet, cases.SPHERE_DIAMETER, reynolds)
@@ -231,7 +249,8 @@ def main():
             "kernel": kernel, "cells_per_launch": cells_per_launch,
             "bulk_cells": bulk, "boundary_cells": nbc, "solid_cells": nsolid,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(kernel, storage, g.tuning)},
             "drag_force_x_after_run": fx,
             "tuning": g.tuning, "autotune_ms": tune_times,
         }
