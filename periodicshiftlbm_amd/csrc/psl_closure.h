@@ -129,6 +129,10 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
   float y_own = 0.f;
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
+    // the per-lane bias / output-weight rows are re-read for each column block
+    // (L1 hits) rather than held in 48 registers across both
+    const MlpNet* nt = net;
+    asm volatile("" : "+s"(nt));
     bf16x8 B;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -139,7 +143,7 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
     }
     f32x16 acc;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = net->b1[lane][r];
+    for (int r = 0; r < 16; ++r) acc[r] = nt->b1[lane][r];
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1, B, acc, 0, 0, 0);
     bf16x8 H0, H1;
 #pragma unroll
@@ -148,12 +152,12 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
       H1[j] = (__bf16)fast_tanh(acc[8 + j]);
     }
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = net->b2[lane][r];
+    for (int r = 0; r < 16; ++r) acc[r] = nt->b2[lane][r];
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A20, H0, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A21, H1, acc, 0, 0, 0);
     float part = 0.f;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) part += fast_tanh(acc[r]) * net->w3[lane][r];
+    for (int r = 0; r < 16; ++r) part += fast_tanh(acc[r]) * nt->w3[lane][r];
     const float y = part + __shfl_xor(part, 32) + net->b3;
     if (nb == half) y_own = y;  // block nb holds cells 32 nb .. 32 nb + 31
   }
