@@ -127,6 +127,31 @@ def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
             "L": [d["L"] for d in doms], "re": [round(d["re"], 1) for d in doms]}
 
 
+def two_level_leg(P, cases, steps: int, storage: str = "f16"):
+    """The sphere at Re = 1e5 on two levels (coarse 256x128x128 + fine 256x192x192
+    around the sphere); MLUPS counts the updated cells of each level times its
+    steps per coarse step (sum_l N_l 2^l without the skipped covered coarse
+    cells and fine ghost cells)."""
+    (gc, gf), (lo, hi) = cases.two_level_sphere(storage)
+    P.updateGrid([gc, gf], 5)
+    P.synchronize()
+    t0 = time.perf_counter()
+    P.updateGrid([gc, gf], steps)
+    P.synchronize()
+    el = time.perf_counter() - t0
+    bc, nc, skipped = gc.census()
+    bf, nf, sf = gf.census()
+    updates = (bc + nc) + 2 * (bf + nf)  # cells actually updated (covered and ghost cells are skipped)
+    force = gf.momentum_exchange(lo, hi)
+    out = {"mlups": updates * steps / el / 1e6, "ms_per_coarse_step": el * 1e3 / steps, "coarse_steps": steps,
+           "coarse_cells": gc.n, "covered_coarse_cells": int(skipped), "fine_cells": gf.n,
+           "fine_ghost_cells": int(sf), "storage": storage,
+           "fine_force_after_run": [float(x) for x in force]}
+    gc.close()
+    gf.close()
+    return out
+
+
 def closure_leg(P, n: int, log):
     """Learned closure at the decaying-turbulence configuration: n^3 fine run
     (FP16 storage, Cs 0.1), filtered to (n/4)^3 every 200 steps, network trained
@@ -191,6 +216,7 @@ def main():
                     help="use this local-update variant (bits: 1 NT loads, 2 NT stores, 4 XCD order, 8 fused "
                          "boundary) instead of autotuning")
     ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the 64-domain batch (0: skip)")
+    ap.add_argument("--two-level-steps", type=int, default=200, help="coarse steps of the two-level sphere (0: skip)")
     ap.add_argument("--closure-n", type=int, default=512,
                     help="fine grid edge of the learned-closure leg (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -292,6 +318,11 @@ def main():
         bl = batch_leg(P, cases, args.batch_steps, 0 if (args.no_autotune or args.tuning is not None) else 3)
         log(f"  {bl['mlups']:.1f} MLUPS over {bl['cells']} cells")
         out["variants"]["batch"] = bl
+    if args.two_level_steps:
+        log(f"[bench] two-level sphere, {args.two_level_steps} coarse steps")
+        tl = two_level_leg(P, cases, args.two_level_steps)
+        log(f"  {tl['mlups']:.1f} MLUPS ({tl['ms_per_coarse_step']:.3f} ms per coarse step)")
+        out["variants"]["two_level"] = tl
     if args.closure_n:
         log(f"[bench] learned closure, {args.closure_n}^3 decaying turbulence")
         out["variants"]["closure"] = closure_leg(P, args.closure_n, log)

@@ -226,3 +226,56 @@ def domain_grid(dom: dict, storage: str = "f16", cs: float = 0.17, subdivisions:
     g.set_smagorinsky(cs)
     fill_equilibrium(g)
     return g, flags
+
+
+# ------------------------------------------------------------ two-level sphere
+# Fine box of the two-level sphere configuration in coarse cells [start, end).
+TWO_LEVEL_SUB = (32, 16, 16, 160, 112, 112)
+
+
+def two_level_sphere(storage: str = "f16", reynolds: float = 1e5, u_inlet: float = 0.04, cs: float = 0.17,
+                     sub=TWO_LEVEL_SUB):
+    """The sphere channel (256 x 128 x 128 coarse cells) with a 2x finer nested
+    box around the sphere (flowAroundSphere/flowAroundSphereMainSingle.cu:188-213
+    sets sub-grid bounds and the fine origin ox_c + iStart res_c - res_f / 2).
+    The fine grid has ghost box faces; the sphere is voxelized on both levels
+    by the STL path.  Returns ([coarse, fine], fine sphere box (lo, hi))."""
+    from . import BoolArray, GridStruct, InfoStruct, STLStruct, applyMarkersInsideSTL, fill_equilibrium
+
+    nxc, nyc, nzc = SPHERE_SHAPE
+    nu_c = u_inlet * SPHERE_DIAMETER / reynolds
+    ci = InfoStruct(cellCountX=nxc, cellCountY=nyc, cellCountZ=nzc, nu=nu_c, res=1.0,
+                    iSubgridStart=sub[0], jSubgridStart=sub[1], kSubgridStart=sub[2],
+                    iSubgridEnd=sub[3], jSubgridEnd=sub[4], kSubgridEnd=sub[5])
+    nxf, nyf, nzf = 2 * (sub[3] - sub[0]), 2 * (sub[4] - sub[1]), 2 * (sub[5] - sub[2])
+    ox, oy, oz = sub[0] - 0.25, sub[1] - 0.25, sub[2] - 0.25
+    fi = InfoStruct(gridID=1, cellCountX=nxf, cellCountY=nyf, cellCountZ=nzf, nu=2 * nu_c, res=0.5,
+                    ox=ox, oy=oy, oz=oz)
+    stl = STLStruct.from_triangles(icosphere(4, SPHERE_DIAMETER / 2, SPHERE_CENTER))
+    grids = []
+    for info, fine in ((ci, False), (fi, True)):
+        n = info.cellCountX * info.cellCountY * info.cellCountZ
+        mask = BoolArray(n)
+        applyMarkersInsideSTL(mask, stl, True, info)
+        solid = mask.get().astype(bool)
+        shape = (info.cellCountX, info.cellCountY, info.cellCountZ)
+        if fine:
+            k, j, i = np.meshgrid(np.arange(shape[2]), np.arange(shape[1]), np.arange(shape[0]), indexing="ij")
+            ghost = ((i == 0) | (i == shape[0] - 1) | (j == 0) | (j == shape[1] - 1) | (k == 0)
+                     | (k == shape[2] - 1)).reshape(-1)
+            flags = np.where(ghost, GHOST, np.where(solid, BOUNCEBACK, FLUID)).astype(np.uint8)
+        else:
+            flags = channel_flags(shape, solid)
+        g = GridStruct(info, storage)
+        g.set_flags(flags)
+        g.set_given_values(channel_given_values(u_inlet))
+        g.set_smagorinsky(cs)
+        if not fine:
+            g.set_periodic_copies(y=True)
+        fill_equilibrium(g)
+        grids.append(g)
+    # sphere box in fine cell indices (cell i centre at ox + (i + 1/2) res_f)
+    r = SPHERE_DIAMETER / 2
+    lo = [int((c - r - o) / 0.5) - 2 for c, o in zip(SPHERE_CENTER, (ox, oy, oz))]
+    hi = [int((c + r - o) / 0.5) + 3 for c, o in zip(SPHERE_CENTER, (ox, oy, oz))]
+    return grids, (lo, hi)
