@@ -127,6 +127,28 @@ def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
             "L": [d["L"] for d in doms], "re": [round(d["re"], 1) for d in doms]}
 
 
+def large_box_leg(P, cases, n: int, steps: int):
+    """The bulk kernel on a fully periodic n^3 box ((n+1)^2 n cells; >= 1 GB of
+    FP32 distributions at n = 256, beyond the 256 MB Infinity Cache), Taylor-Green
+    initial field, no Smagorinsky: MLUPS and the bulk kernel's HBM rate."""
+    out = {}
+    nx, ny, nz = cases.tgv_shape(n)
+    rho, ux, uy, uz = cases.tgv_fields(n)
+    for storage in ("f32", "f16"):
+        info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=0.005)
+        g = P.GridStruct(info, storage)
+        g.set_periodic_copies(x=True, y=True)
+        P.fill_equilibrium(g, rho, ux, uy, uz)
+        el = time_steps(P, g, steps, 10)
+        ms, launches = kernel_time(P, g, min(steps, 50))
+        gbs = BYTES_PER_UPDATE[storage] * g.n / (ms * 1e-3) / 1e9
+        out[storage] = {"cells": g.n, "f_bytes": g.n * 27 * (4 if storage == "f32" else 2),
+                        "mlups": g.n * steps / el / 1e6, "bulk_kernel_ms": ms, "achieved_GBs": gbs,
+                        "roofline_frac": gbs / HBM_PEAK_GBS, "tuning": g.tuning}
+        g.close()
+    return out
+
+
 def two_level_leg(P, cases, steps: int, storage: str = "f16"):
     """The sphere at Re = 1e5 on two levels (coarse 256x128x128 + fine 256x192x192
     around the sphere); MLUPS counts the updated cells of each level times its
@@ -217,6 +239,7 @@ def main():
                          "boundary) instead of autotuning")
     ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the 64-domain batch (0: skip)")
     ap.add_argument("--two-level-steps", type=int, default=200, help="coarse steps of the two-level sphere (0: skip)")
+    ap.add_argument("--large-n", type=int, default=256, help="edge of the large periodic box (0: skip)")
     ap.add_argument("--closure-n", type=int, default=512,
                     help="fine grid edge of the learned-closure leg (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -318,6 +341,11 @@ def main():
         bl = batch_leg(P, cases, args.batch_steps, 0 if (args.no_autotune or args.tuning is not None) else 3)
         log(f"  {bl['mlups']:.1f} MLUPS over {bl['cells']} cells")
         out["variants"]["batch"] = bl
+    if args.large_n:
+        log(f"[bench] large periodic box {args.large_n}^3")
+        lb = large_box_leg(P, cases, args.large_n, 100)
+        log("  " + ", ".join(f"{k}: {v['mlups']:.0f} MLUPS, {v['achieved_GBs']:.0f} GB/s" for k, v in lb.items()))
+        out["variants"]["large_box"] = lb
     if args.two_level_steps:
         log(f"[bench] two-level sphere, {args.two_level_steps} coarse steps")
         tl = two_level_leg(P, cases, args.two_level_steps)
