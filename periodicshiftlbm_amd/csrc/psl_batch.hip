@@ -218,7 +218,6 @@ BoundaryParams boundary_params_of(const psl_grid* g) {
   memset(&q, 0, sizeof(q));
   q.list = g->bc_list;
   q.n = g->n_bc;
-  q.blocks = (q.n + 255u) / 256u;
   q.nx = g->info.cellCountX;
   q.ny = g->info.cellCountY;
   q.nz = g->info.cellCountZ;

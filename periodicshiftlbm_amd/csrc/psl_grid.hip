@@ -477,7 +477,6 @@ static int launch_local_update(psl_grid* g) {
   BoundaryParams b;
   b.list = g->bc_list;
   b.n = g->flags ? g->n_bc : 0u;
-  b.blocks = blocks_for(b.n);
   b.nx = g->info.cellCountX;
   b.ny = g->info.cellCountY;
   b.nz = g->info.cellCountZ;

@@ -129,7 +129,6 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 struct BoundaryParams {
   const uint32_t* list;  // box-face cells in cell order
   uint32_t n;            // list length
-  uint32_t blocks;       // workgroups of the list ((n + 255) / 256)
   int nx, ny, nz;
   const float4* bc;      // 4 given-value slots (rho, ux, uy, uz)
 };
