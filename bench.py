@@ -44,17 +44,17 @@ def log(*a):
 TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "sphere_256x128x128_hbm_counters.md")
 
 
-def measured_traffic(kernel: str, storage: str, tuning: int):
-    """HBM bytes per launch of `kernel` at `storage` and tuning variant from the
-    committed rocprofv3 counter summary (FETCH_SIZE x 2 + WRITE_SIZE x 1, the
-    factors from a copy kernel of known size); None when not measured."""
+def measured_traffic(kernel: str, storage: str):
+    """HBM bytes per launch of `kernel` at `storage` from the committed
+    rocprofv3 counter summary (FETCH_SIZE x 2 + WRITE_SIZE x 1, the factors
+    from a copy kernel of known size); None when not measured."""
     import re
 
     try:
         text = open(TRAFFIC_SUMMARY).read()
     except OSError:
         return None
-    pat = re.compile(rf"^traffic {re.escape(kernel)} {re.escape(storage)} tuning={tuning} = (\d+)", re.M)
+    pat = re.compile(rf"^traffic {re.escape(kernel)} {re.escape(storage)} = (\d+)", re.M)
     m = pat.search(text)
     return int(m.group(1)) if m else None
 
@@ -85,14 +85,6 @@ def time_steps(P, g, steps: int, warmup: int):
     P.updateGrid([g], steps)
     P.synchronize()
     return time.perf_counter() - t0
-
-
-def kernel_time(P, g, steps: int):
-    g.profile(True)
-    P.updateGrid([g], steps)
-    total_ms, launches = g.profile_read()
-    g.profile(False)
-    return total_ms / max(1, launches), launches
 
 
 def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
@@ -140,7 +132,7 @@ def large_box_leg(P, cases, n: int, steps: int):
         g.set_periodic_copies(x=True, y=True)
         P.fill_equilibrium(g, rho, ux, uy, uz)
         el = time_steps(P, g, steps, 10)
-        ms, launches = kernel_time(P, g, min(steps, 50))
+        ms, _ = g.time_update_kernel(50)
         gbs = BYTES_PER_UPDATE[storage] * g.n / (ms * 1e-3) / 1e9
         out[storage] = {"cells": g.n, "f_bytes": g.n * 27 * (4 if storage == "f32" else 2),
                         "mlups": g.n * steps / el / 1e6, "bulk_kernel_ms": ms, "achieved_GBs": gbs,
@@ -282,24 +274,22 @@ def main():
         el = time_steps(P, g, args.steps, args.warmup)
         mlups = ncell * args.steps / el / 1e6
         bulk, nbc, skipped = g.census()
-        ksteps = min(args.steps, 200)
-        avg_ms, launches = kernel_time(P, g, ksteps)
-        # the timed launch: k_ps_fused updates the bulk and the boundary list,
-        # k_ps_bulk the bulk only (the list follows in its own launch)
-        fused = bool(g.tuning & P.TUNE_FUSED_BOUNDARY) and nbc > 0
-        kernel = "k_ps_fused" if fused else "k_ps_bulk"
-        cells_per_launch = bulk + (nbc if fused else 0)
-        bytes_per_launch = BYTES_PER_UPDATE[storage] * cells_per_launch
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         lo, hi = cases.sphere_box()
         fx = g.momentum_exchange(lo, hi)[0]
+        # the dominant launch, back to back between two events (after the run:
+        # it keeps colliding the final state): k_ps_fused updates the bulk and
+        # the boundary list, k_ps_bulk the bulk only
+        avg_ms, kernel = g.time_update_kernel(100)
+        cells_per_launch = bulk + (nbc if kernel == "k_ps_fused" else 0)
+        bytes_per_launch = BYTES_PER_UPDATE[storage] * cells_per_launch
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         results[storage] = {
             "mlups": mlups, "ms_per_step": el * 1e3 / args.steps, "bulk_kernel_ms": avg_ms,
             "kernel": kernel, "cells_per_launch": cells_per_launch,
             "bulk_cells": bulk, "boundary_cells": nbc, "solid_cells": nsolid,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
-                         "traffic": measured_traffic(kernel, storage, g.tuning)},
+                         "traffic": measured_traffic(kernel, storage)},
             "drag_force_x_after_run": fx,
             "tuning": g.tuning, "autotune_ms": tune_times,
         }

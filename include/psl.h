@@ -177,6 +177,12 @@ int psl_grid_get_tuning(const psl_grid* grid, int* tuning);
  * updates each after one warm-up; the state advances) and keeps the fastest;
  * best and times_ms[PSL_TUNE_VARIANTS] (mean ms per local update) are optional. */
 int psl_grid_autotune(psl_grid* grid, int steps, int* best, double* times_ms);
+/* Mean duration of the grid's dominant update kernel (kernel = 0: k_ps_bulk,
+ * 1: k_ps_fused, 2: the bulk kernel with the learned closure), launched
+ * `launches` times back to back between two events on the grid's stream.  The
+ * kernel updates the distributions each time (no streaming in between), so
+ * call it where the state no longer matters. */
+int psl_grid_time_update_kernel(psl_grid* grid, int launches, double* ms_per_launch, int* kernel);
 /* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
 int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
 

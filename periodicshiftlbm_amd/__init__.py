@@ -186,6 +186,7 @@ _SIGS = [
     ("psl_les_targets", _I, [_P, _I, ctypes.c_float, _P, _P]),
     ("psl_grid_set_closure", _I, [_P, _P]),
     ("psl_get_macroscopic_device", _I, [_P, _P]),
+    ("psl_grid_time_update_kernel", _I, [_P, _I, _P, _P]),
     ("psl_diad_create", _I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     ("psl_diad_destroy", _I, [_P]),
     ("psl_diad_set_flags", _I, [_P, _P]),
@@ -353,6 +354,13 @@ class GridStruct:
         out = DeviceBuffer(4 * self.n * 4)
         _check(lib().psl_get_macroscopic_device(self._h, out.ptr))
         return out
+
+    def time_update_kernel(self, launches: int = 50):
+        """(mean ms, kernel name) of the dominant update kernel launched back to back."""
+        ms = ctypes.c_double()
+        k = ctypes.c_int()
+        _check(lib().psl_grid_time_update_kernel(self._h, int(launches), ctypes.byref(ms), ctypes.byref(k)))
+        return ms.value, ("k_ps_bulk", "k_ps_fused", "k_ps_bulk_closure")[k.value]
 
     def set_closure(self, mlp):
         """Use a learned closure (MLP) in the bulk update; None restores Smagorinsky."""

@@ -455,7 +455,7 @@ static void launch_fused(bool les, uint32_t nb, hipStream_t stream, const StepPa
 }
 
 template <class St>
-static int launch_local_update(psl_grid* g) {
+static int launch_local_update(psl_grid* g, bool dominant_only = false) {
   typedef typename St::T T;
   StepParams p;
   p.n = ncells(g);
@@ -522,7 +522,7 @@ static int launch_local_update(psl_grid* g) {
     PSL_HIP_TRY(hipEventRecord(ev1, g->stream));
     g->prof_events.emplace_back(ev0, ev1);
   }
-  if (b.n && !fused) {
+  if (b.n && !fused && !dominant_only) {
     k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->stream>>>(p, b);
     PSL_LAUNCH_CHECK();
   }
@@ -541,6 +541,7 @@ static int dispatch_storage(int storage, A... args) {
 }
 
 template <class St> struct LocalUpdateFn { static int run(psl_grid* g) { return launch_local_update<St>(g); } };
+template <class St> struct DominantFn { static int run(psl_grid* g) { return launch_local_update<St>(g, true); } };
 
 template <class St> struct FillEqFn {
   static int run(psl_grid* g, const float* r, const float* a, const float* b, const float* c) {
@@ -1048,6 +1049,31 @@ int psl_grid_autotune(psl_grid* g, int steps, int* best, double* times_ms) {
 int psl_grid_set_closure(psl_grid* g, const psl_mlp* m) {
   PSL_CHECK_ARG(g, "null grid");
   g->closure = m;
+  return PSL_OK;
+}
+
+int psl_grid_time_update_kernel(psl_grid* g, int launches, double* ms_per_launch, int* kernel) {
+  PSL_CHECK_ARG(g && launches > 0 && ms_per_launch, "bad arguments");
+  const bool fused = g->flags && g->n_bc && (g->tuning & PSL_TUNE_FUSED_BOUNDARY) && !g->closure;
+  const bool was = g->profile;
+  g->profile = false;
+  hipEvent_t e0, e1;
+  PSL_HIP_TRY(hipEventCreate(&e0));
+  PSL_HIP_TRY(hipEventCreate(&e1));
+  int rc = dispatch_storage<DominantFn>(g->storage, g);  // warm
+  if (rc == PSL_OK && hipEventRecord(e0, g->stream) != hipSuccess) rc = PSL_E_HIP;
+  for (int k = 0; k < launches && rc == PSL_OK; ++k) rc = dispatch_storage<DominantFn>(g->storage, g);
+  if (rc == PSL_OK && hipEventRecord(e1, g->stream) != hipSuccess) rc = PSL_E_HIP;
+  float ms = 0.f;
+  if (rc == PSL_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = PSL_E_HIP;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g->profile = was;
+  if (rc == PSL_E_HIP) set_error("kernel timing failed");
+  if (rc) return rc;
+  *ms_per_launch = (double)ms / launches;
+  if (kernel) *kernel = g->closure ? 2 : fused ? 1 : 0;
   return PSL_OK;
 }
 

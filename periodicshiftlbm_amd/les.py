@@ -224,9 +224,9 @@ class Workflow:
             self.log(f"[les] coarse {model}: mean spectrum error {out[model]['mean_error']:.4f}")
         return out
 
-    def cost_ratio(self, n: int = 256, steps: int = 20, storage: str = "f32", cs: float = 0.17):
+    def cost_ratio(self, n: int = 256, launches: int = 50, storage: str = "f32", cs: float = 0.17):
         """Mean bulk-kernel time with the network / with Smagorinsky on a periodic
-        n^3 box (HIP events around the bulk launch)."""
+        n^3 box (launches back to back between two events)."""
         P = self.P
         u = [np.zeros((n, n, n), np.float32)] * 3
         times = {}
@@ -235,11 +235,7 @@ class Workflow:
             if model == "mlp":
                 g.set_closure(self.net)
             P.updateGrid([g], 2)
-            g.profile(True)
-            P.updateGrid([g], steps)
-            ms, launches = g.profile_read()
-            g.profile(False)
-            times[model] = ms / max(1, launches)
+            times[model], _ = g.time_update_kernel(launches)
             g.close()
         return times["mlp"] / times["smagorinsky"], times
 

@@ -132,7 +132,9 @@ def test_les_filter_and_targets_match_oracle(psl):
     S = C.strain_rate(got[1].astype(np.float64), got[2].astype(np.float64), got[3].astype(np.float64))
     nut, f_ref = C.les_targets(got[0].astype(np.float64), got[4:].astype(np.float64), S, nu)
     terr = np.max(np.abs(t_gpu - nut / C.NU_SCALE)) / max(np.max(nut / C.NU_SCALE), 1e-12)
-    ferr = np.max(np.abs(f_gpu - f_ref) / (np.abs(f_ref) + 1e-3 * np.max(np.abs(f_ref), axis=(0, 1, 2))))
+    # per feature, relative to its range (the direction components and the
+    # normalised invariant are O(1); magnitude and trace scale with the strain)
+    ferr = np.max(np.abs(f_gpu - f_ref) / np.max(np.abs(f_ref), axis=(0, 1, 2)))
     print(f"[les-targets] nu_t* range {nut.min():.2e}..{nut.max():.2e}; target err {terr:.2e}, "
           f"feature err {ferr:.2e}", flush=True)
     assert terr < 1e-4
@@ -189,7 +191,7 @@ def test_closure_collision_matches_oracle(psl, oracle_mod):
     got = g.get_f().T
     # expected: oracle collision with nu + nu_t, nu_t from the network on the oracle's features
     y = net.eval(raw.astype(np.float32))
-    exp = np.empty_like(f64)
+    exp = np.empty(f.shape, np.float64)
     for cidx in range(nc):
         nut = C.NU_SCALE * max(float(y[cidx]), 0.0)
         exp[cidx] = O.collide(f[cidx:cidx + 1].astype(np.float32), nu + nut, 0.0)[0]
@@ -217,7 +219,8 @@ def test_les_workflow_small(psl):
     """The learned-closure workflow at reduced size: 128^3 fine decaying
     turbulence, filtered to 32^3, trained, and coarse LES runs judged by their
     energy spectra against the filtered fine run.  Shows the gain of the
-    network over running without a model and reports it against Smagorinsky."""
+    network over Smagorinsky (the method it replaces) and reports both against
+    running without a model."""
     from periodicshiftlbm_amd import les
 
     P = psl
@@ -225,8 +228,14 @@ def test_les_workflow_small(psl):
                       log=lambda *a: print(*a, flush=True))
     wf.run_fine(steps=800, every=100, first_sample=100)
     wf.train(iters=400, batch=8192, lr=2e-3)
-    assert wf.loss_hist[-1] < 0.5 * wf.loss_hist[0]
+    # the pointwise optimal eddy viscosity is noisy; the network learns its
+    # conditional mean, so the loss settles near the target variance
+    var = wf.target_stats["std"] ** 2
+    print(f"[les-small] loss {wf.loss_hist[0]:.4f} -> {wf.loss_hist[-1]:.4f}, target variance {var:.4f}", flush=True)
+    assert wf.loss_hist[-1] < 0.9 * wf.loss_hist[0]
+    assert wf.loss_hist[-1] < 1.05 * var
     res = wf.compare(start=100)
-    print(f"[les-small] errors: none {res['none']['mean_error']:.4f}, smagorinsky "
+    print(f"[les-small] spectrum errors: none {res['none']['mean_error']:.4f}, smagorinsky "
           f"{res['smagorinsky']['mean_error']:.4f}, mlp {res['mlp']['mean_error']:.4f}", flush=True)
-    assert res["mlp"]["mean_error"] < res["none"]["mean_error"]
+    # the network replaces Smagorinsky (its fallback): it must do better than it
+    assert res["mlp"]["mean_error"] < res["smagorinsky"]["mean_error"]
