@@ -179,19 +179,18 @@ def closure_leg(P, n: int, log):
     every = 200 if n >= 256 else 100
     wf.run_fine(steps=steps, every=every, first_sample=every)
     wf.train(iters=2000, batch=16384, lr=2e-3)
-    res = wf.compare(start=every)
-    ratio32, t32 = wf.cost_ratio(n=256, storage="f32")
-    ratio16, t16 = wf.cost_ratio(n=256, storage="f16")
-    gain = 1.0 - res["mlp"]["mean_error"] / res["smagorinsky"]["mean_error"]
-    log(f"  closure: spectrum error none {res['none']['mean_error']:.4f}, smagorinsky "
-        f"{res['smagorinsky']['mean_error']:.4f}, mlp {res['mlp']['mean_error']:.4f} (gain {gain * 100:.1f} %); "
-        f"cost ratio f32 {ratio32:.2f} f16 {ratio16:.2f}")
+    res = wf.compare(start=every, every=2 if n >= 256 else 1)
+    wf.net = wf.scaled_net(res["mlp"]["param"])  # the calibrated network for the cost measurement
+    ratio32, t32 = wf.cost_ratio(n=256, storage="f32", cs=res["smagorinsky"]["param"])
+    ratio16, t16 = wf.cost_ratio(n=256, storage="f16", cs=res["smagorinsky"]["param"])
+    gain = res["gain_vs_smagorinsky"]
+    log(f"  closure: resolved-band spectrum error none {res['none']['evaluation_resolved']:.4f}, smagorinsky "
+        f"{res['smagorinsky']['evaluation_resolved']:.4f}, mlp {res['mlp']['evaluation_resolved']:.4f} "
+        f"(gain {gain * 100:.1f} %); cost ratio f32 {ratio32:.2f} f16 {ratio16:.2f}")
     return {"fine": f"{n}^3", "coarse": f"{n // 4}^3", "fine_steps": steps, "snapshot_every": every,
             "fine_mlups": wf.timings.get("fine_mlups"), "train_s": wf.timings.get("train_s"),
             "targets": wf.target_stats, "loss_first": float(wf.loss_hist[0]), "loss_last": float(wf.loss_hist[-1]),
-            "spectrum_error": {k: v["mean_error"] for k, v in res.items()},
-            "spectrum_error_per_snapshot": {k: v["errors"] for k, v in res.items()},
-            "gain_vs_smagorinsky": gain,
+            "spectrum_error": res, "gain_vs_smagorinsky": gain,
             "cost_ratio_vs_smagorinsky": {"f32": ratio32, "f16": ratio16},
             "bulk_kernel_ms_256cubed": {"f32": t32, "f16": t16}}
 

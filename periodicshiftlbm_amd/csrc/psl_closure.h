@@ -77,10 +77,10 @@ PSL_HD PSL_INLINE void closure_features(float rho, float k200, float k020, float
   x[8] = 7.348469228349534f * det * inv * inv * inv;  // 3 sqrt(6)
 }
 
-// Raw closure inputs of a cell from its populations (f, or f - w when W_OFF).
+// Raw closure inputs of a cell from the raw moments of its populations
+// (m = raw_moments of f, or of f - w when W_OFF).
 template <bool W_OFF>
-PSL_HD PSL_INLINE void cell_closure_features(const float (&v)[27], float (&x)[MLP_IN]) {
-  Moments m = raw_moments(v);
+PSL_HD PSL_INLINE void cell_closure_features(Moments m, float (&x)[MLP_IN]) {
   if (W_OFF) {
     m.m000 += 1.f;
     m.m200 += 1.f / 3.f;
@@ -97,9 +97,37 @@ PSL_HD PSL_INLINE void cell_closure_features(const float (&v)[27], float (&x)[ML
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-__device__ __forceinline__ float fast_tanh(float x) {
-  // 1 - 2 / (e^{2x} + 1): saturates to +-1 for large |x| (e^{2x} -> inf or 0)
-  return 1.f - __fdividef(2.f, __expf(2.f * x) + 1.f);
+// tanh by linear interpolation in a table held in LDS: TANH_N intervals on
+// [-TANH_X, TANH_X], entry i = (tanh(x_i), tanh(x_{i+1}) - tanh(x_i)); beyond
+// the range the value is tanh(+-TANH_X) = +-0.99975.  Interpolation error
+// <= h^2/8 max|tanh''| = 3e-5, clamp error 2.5e-4: both below the bf16
+// resolution (2^-8) the hidden activations are rounded to.  A table read and
+// two full-rate operations replace the two quarter-rate transcendentals of
+// 1 - 2 / (e^{2x} + 1).
+constexpr int TANH_N = 512;
+constexpr float TANH_X = 4.5f;
+
+struct TanhTable {
+  float2 e[TANH_N + 1];
+};
+
+// Fill by the whole workgroup (blockDim.x threads), then barrier: call before
+// any thread of the workgroup can leave.
+__device__ __forceinline__ void tanh_table_fill(TanhTable& t) {
+  const float h = 2.f * TANH_X / TANH_N;
+  for (int i = threadIdx.x; i <= TANH_N; i += blockDim.x) {
+    const float a = tanhf(-TANH_X + h * i);
+    const float b = tanhf(-TANH_X + h * (i + 1));
+    t.e[i] = make_float2(a, i < TANH_N ? b - a : 0.f);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ float table_tanh(const TanhTable& t, float x) {
+  const float u = (fminf(fmaxf(x, -TANH_X), TANH_X) + TANH_X) * (TANH_N / (2.f * TANH_X));
+  const float fl = floorf(u);
+  const float2 e = t.e[(int)fl];
+  return e.x + (u - fl) * e.y;
 }
 
 __device__ __forceinline__ bf16x8 load_a(const uint16_t (&a)[8]) {
@@ -111,7 +139,8 @@ __device__ __forceinline__ bf16x8 load_a(const uint16_t (&a)[8]) {
 
 // Network output y of this lane's cell.  x: raw features of the cell (any
 // value on lanes whose result is not used).  Must be called by all 64 lanes.
-__device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, const float (&raw)[MLP_IN]) {
+__device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, const TanhTable& tt,
+                                             const float (&raw)[MLP_IN]) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const int lane = threadIdx.x & 63;
   const int half = lane >> 5;
@@ -148,8 +177,8 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
     bf16x8 H0, H1;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      H0[j] = (__bf16)fast_tanh(acc[j]);
-      H1[j] = (__bf16)fast_tanh(acc[8 + j]);
+      H0[j] = (__bf16)table_tanh(tt, acc[j]);
+      H1[j] = (__bf16)table_tanh(tt, acc[8 + j]);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = nt->b2[lane][r];
@@ -157,7 +186,7 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A21, H1, acc, 0, 0, 0);
     float part = 0.f;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) part += fast_tanh(acc[r]) * nt->w3[lane][r];
+    for (int r = 0; r < 16; ++r) part += table_tanh(tt, acc[r]) * nt->w3[lane][r];
     const float y = part + __shfl_xor(part, 32) + net->b3;
     if (nb == half) y_own = y;  // block nb holds cells 32 nb .. 32 nb + 31
   }
@@ -173,7 +202,8 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
 // (they run it on rest-state inputs and discard the result).
 template <class St, bool FLAGS, int TUNE>
 __device__ __forceinline__ void bulk_update_closure(const StepParams& p, uint32_t block,
-                                                    const MlpNet* __restrict__ net, float nu) {
+                                                    const MlpNet* __restrict__ net, float nu, TanhTable& tt) {
+  tanh_table_fill(tt);  // before any lane may leave (barrier)
   const uint32_t block_first = block * 256u;
   const uint32_t cell = block_first + threadIdx.x;
   const uint32_t w_first = wave_first_cell(block_first);
@@ -190,16 +220,17 @@ __device__ __forceinline__ void bulk_update_closure(const StepParams& p, uint32_
 #pragma unroll
     for (int q = 0; q < 27; ++q) v[q] = St::W_OFF ? 0.f : weight(q);
   }
+  const Moments m = raw_moments(v);
   float raw[MLP_IN];
-  cell_closure_features<St::W_OFF>(v, raw);
-  const float y = closure_wave(net, raw);
+  cell_closure_features<St::W_OFF>(m, raw);
+  const float y = closure_wave(net, tt, raw);
   if (!active) return;
   if (kind == KIND_BOUNCEBACK) {
     bounceback(v);
   } else {
     const float tau = 3.f * (nu + MLP_NU_SCALE * fmaxf(y, 0.f)) + 0.5f;
     Relax rx{1.f / tau, tau, 0.f};
-    collide<St::W_OFF>(v, rx, 0.f);
+    collide_with_moments<St::W_OFF>(v, m, rx, 0.f);
   }
   store27<St, TUNE>(p, w_first, straddle, cell, v);
 }

@@ -218,12 +218,14 @@ __global__ __launch_bounds__(256) void k_feature_moments(const float* __restrict
 // step kernel; whole wavefronts run it, rows past n read row n - 1.
 __global__ __launch_bounds__(256) void k_mlp_eval(const MlpNet* __restrict__ net, const float* __restrict__ raw,
                                                   uint32_t n, float* __restrict__ y) {
+  __shared__ TanhTable tt;
+  tanh_table_fill(tt);
   const uint32_t c = blockIdx.x * 256u + threadIdx.x;
   const uint32_t r = c < n ? c : n - 1u;
   float f[MLP_IN];
 #pragma unroll
   for (int i = 0; i < MLP_IN; ++i) f[i] = raw[(size_t)r * MLP_IN + i];
-  const float out = closure_wave(net, f);
+  const float out = closure_wave(net, tt, f);
   if (c < n) y[c] = out;
 }
 

@@ -61,8 +61,10 @@ __global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
 // One workgroup per entry of the fused order (fused_order): boundary-list
 // ranges interleaved with the bulk blocks they border.
 template <class St, bool FLAGS, int TUNE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ps_bulk_closure(StepParams p, const MlpNet* __restrict__ net, float nu) {
-  bulk_update_closure<St, FLAGS, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, net, nu);
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ps_bulk_closure(
+    StepParams p, const MlpNet* __restrict__ net, float nu) {
+  __shared__ TanhTable tt;
+  bulk_update_closure<St, FLAGS, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, net, nu, tt);
 }
 
 template <class St, bool LES, int TUNE>

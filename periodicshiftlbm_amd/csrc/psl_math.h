@@ -230,11 +230,10 @@ PSL_HD PSL_INLINE float omega_les(float rho, float inv_rho, float k200, float k0
 }
 
 // The cumulant collision on values v, where v holds f (w_off = false) or
-// f - w (w_off = true).  On return v holds the post-collision values in the
-// same representation.
+// f - w (w_off = true), given m = raw_moments(v).  On return v holds the
+// post-collision values in the same representation.
 template <bool W_OFF>
-PSL_HD PSL_INLINE void collide(float (&v)[27], const Relax& rx, float cs) {
-  Moments m = raw_moments(v);
+PSL_HD PSL_INLINE void collide_with_moments(float (&v)[27], Moments m, const Relax& rx, float cs) {
   if (W_OFF) {
     m.m000 += 1.f;
     m.m200 += 1.f / 3.f;
@@ -274,6 +273,12 @@ PSL_HD PSL_INLINE void collide(float (&v)[27], const Relax& rx, float cs) {
 #pragma unroll
     for (int q = 0; q < 27; ++q) v[q] -= weight(q);
   }
+}
+
+// The collision of one cell (include/applyCollision.h:6-295).
+template <bool W_OFF>
+PSL_HD PSL_INLINE void collide(float (&v)[27], const Relax& rx, float cs) {
+  collide_with_moments<W_OFF>(v, raw_moments(v), rx, cs);
 }
 
 // Central-moment rescaling between grid levels (include/gridRefinementFunctions.h:6-281,

@@ -180,30 +180,36 @@ class Workflow:
         self.net, self.loss_hist = net, hist
         return net
 
-    def coarse_run(self, model: str, start: int, cs: float = 0.17):
-        """Coarse LES from the filtered snapshot at fine step `start`; spectra
-        at every later snapshot.  model: "smagorinsky", "mlp" or "none"."""
+    def coarse_run(self, model: str, start: int, param: float = 0.0, every: int = 1):
+        """Coarse LES from the filtered snapshot at fine step `start`, spectra at
+        every `every`-th later snapshot.  model "none", "smagorinsky" (param =
+        Cs as in include/cellFunctions.h:192-197) or "mlp" (param = scale of the
+        network output)."""
         P, m, r = self.P, self.m, self.r
         rho, ux, uy, uz = self.snapshots[start]
-        g = periodic_grid(P, m, self.nu_coarse, "f32", (ux, uy, uz), cs if model == "smagorinsky" else 0.0, rho)
+        g = periodic_grid(P, m, self.nu_coarse, "f32", (ux, uy, uz), param if model == "smagorinsky" else 0.0, rho)
+        net = None
         if model == "mlp":
-            g.set_closure(self.net)
+            net = self.scaled_net(param)
+            g.set_closure(net)
+        later = sorted(k for k in self.snapshots if k > start)[every - 1::every]
         errs = {}
         done = start
-        t_step, n_step = 0.0, 0
-        for s in sorted(k for k in self.snapshots if k > start):
-            nc = (s - done) // r
-            P.synchronize()
-            t1 = time.perf_counter()
-            P.updateGrid([g], nc)
-            P.synchronize()
-            t_step += time.perf_counter() - t1
-            n_step += nc
+        for s in later:
+            P.updateGrid([g], (s - done) // r)
             done = s
             e = energy_spectrum(*unique_velocity(g, m))
-            errs[s] = spectrum_error(e, self.reference_spectrum(s), m // 2)
+            e_ref = self.reference_spectrum(s)
+            errs[s] = (spectrum_error(e, e_ref, m // 4), spectrum_error(e, e_ref, m // 2))
         g.close()
-        return errs, t_step / max(n_step, 1)
+        return errs
+
+    def scaled_net(self, alpha: float):
+        """The trained network with its output scaled by alpha (last layer)."""
+        p, mean, std = self.net.get_params()
+        q = p.copy()
+        q[-33:] *= np.float32(alpha)  # W3 and b3
+        return self.P.MLP(q, mean, std)
 
     def reference_spectrum(self, s: int):
         """Energy spectrum of the filtered fine field at fine step s (cached)."""
@@ -214,14 +220,39 @@ class Workflow:
             self._eref[s] = energy_spectrum(fx.astype(np.float64), fy.astype(np.float64), fz.astype(np.float64))
         return self._eref[s]
 
-    def compare(self, start: int, cs: float = 0.17):
-        """Spectrum errors of the coarse LES runs (mean over the later snapshots)."""
-        out = {}
-        for model in ("none", "smagorinsky", "mlp"):
-            errs, sec = self.coarse_run(model, start, cs)
-            out[model] = {"mean_error": float(np.mean(list(errs.values()))),
-                          "errors": {int(k): round(v, 5) for k, v in errs.items()}, "ms_per_step": sec * 1e3}
-            self.log(f"[les] coarse {model}: mean spectrum error {out[model]['mean_error']:.4f}")
+    def compare(self, start: int, cs_values=(0.17, 0.03, 0.01, 0.003), alpha_values=(1.0, 0.5, 0.25, 0.1),
+                every: int = 1):
+        """Coarse LES runs judged by their energy spectra against the filtered
+        fine run.  Each model gets one scalar (the Smagorinsky constant, the
+        network's output scale) chosen on the first half of the later snapshots
+        (calibration); errors are reported on the second half (evaluation),
+        for shells k <= m/4 (resolved band) and k <= m/2."""
+        runs = {("none", 0.0): self.coarse_run("none", start, 0.0, every)}
+        for c in cs_values:
+            runs[("smagorinsky", c)] = self.coarse_run("smagorinsky", start, c, every)
+        for a in alpha_values:
+            runs[("mlp", a)] = self.coarse_run("mlp", start, a, every)
+        steps = sorted(next(iter(runs.values())).keys())
+        half = len(steps) // 2
+        cal, ev = steps[:half], steps[half:]
+
+        def score(errs, which, band):
+            return float(np.mean([errs[s][band] for s in which]))
+
+        out = {"calibration_steps": cal, "evaluation_steps": ev, "runs": {}}
+        for (model, prm), errs in runs.items():
+            out["runs"][f"{model}:{prm}"] = {"calibration": score(errs, cal, 0), "evaluation_resolved": score(errs, ev, 0),
+                                             "evaluation_all": score(errs, ev, 1)}
+        for model in ("smagorinsky", "mlp"):
+            cands = [(prm, e) for (mm, prm), e in runs.items() if mm == model]
+            prm, errs = min(cands, key=lambda c: score(c[1], cal, 0))
+            out[model] = {"param": prm, "evaluation_resolved": score(errs, ev, 0), "evaluation_all": score(errs, ev, 1)}
+        errs = runs[("none", 0.0)]
+        out["none"] = {"evaluation_resolved": score(errs, ev, 0), "evaluation_all": score(errs, ev, 1)}
+        out["gain_vs_smagorinsky"] = 1.0 - out["mlp"]["evaluation_resolved"] / out["smagorinsky"]["evaluation_resolved"]
+        for k in ("none", "smagorinsky", "mlp"):
+            self.log(f"[les] {k}: resolved-band spectrum error {out[k]['evaluation_resolved']:.4f}"
+                     + (f" (param {out[k]['param']})" if "param" in out[k] else ""))
         return out
 
     def cost_ratio(self, n: int = 256, launches: int = 50, storage: str = "f32", cs: float = 0.17):

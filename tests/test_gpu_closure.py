@@ -235,7 +235,11 @@ def test_les_workflow_small(psl):
     assert wf.loss_hist[-1] < 0.9 * wf.loss_hist[0]
     assert wf.loss_hist[-1] < 1.05 * var
     res = wf.compare(start=100)
-    print(f"[les-small] spectrum errors: none {res['none']['mean_error']:.4f}, smagorinsky "
-          f"{res['smagorinsky']['mean_error']:.4f}, mlp {res['mlp']['mean_error']:.4f}", flush=True)
-    # the network replaces Smagorinsky (its fallback): it must do better than it
-    assert res["mlp"]["mean_error"] < res["smagorinsky"]["mean_error"]
+    for k, v in res["runs"].items():
+        print(f"[les-small] {k}: calibration {v['calibration']:.4f}, evaluation {v['evaluation_resolved']:.4f}",
+              flush=True)
+    print(f"[les-small] evaluation (resolved band): none {res['none']['evaluation_resolved']:.4f}, smagorinsky "
+          f"{res['smagorinsky']['evaluation_resolved']:.4f} (Cs {res['smagorinsky']['param']}), mlp "
+          f"{res['mlp']['evaluation_resolved']:.4f} (scale {res['mlp']['param']})", flush=True)
+    # each model calibrated on the first half of the snapshots, judged on the second
+    assert res["mlp"]["evaluation_resolved"] < res["smagorinsky"]["evaluation_resolved"]
