@@ -88,16 +88,29 @@ def unique_velocity(g, n: int):
     return [x.reshape(sh)[:, :n, :n].astype(np.float64) for x in (a, b, c)]
 
 
-def energy_spectrum(ux, uy, uz):
-    """Shell-summed kinetic energy spectrum of a periodic n^3 field, k = 0..n/2."""
+def box_transfer2(m: int, r: int):
+    """|G(k)|^2 on an m^3 coarse grid of the r-cell box average of the fine
+    grid: per axis |sin(pi k / m) / (r sin(pi k / (r m)))|^2."""
+    k1 = np.abs(np.fft.fftfreq(m, 1.0 / m))
+    g = np.ones(m)
+    nz = k1 > 0
+    g[nz] = (np.sin(np.pi * k1[nz] / m) / (r * np.sin(np.pi * k1[nz] / (r * m)))) ** 2
+    return g[:, None, None] * g[None, :, None] * g[None, None, :]
+
+
+def energy_spectrum(ux, uy, uz, weight=None):
+    """Shell-summed kinetic energy spectrum of a periodic n^3 field, k = 0..n/2;
+    each mode's energy is multiplied by `weight` when given."""
     n = ux.shape[0]
     k1 = np.fft.fftfreq(n, 1.0 / n)
     kk = np.sqrt(k1[:, None, None] ** 2 + k1[None, :, None] ** 2 + k1[None, None, :] ** 2)
     shell = np.minimum(np.rint(kk).astype(int), n // 2).ravel()
     e = np.zeros(n // 2 + 1)
     for c in (ux, uy, uz):
-        F = np.fft.fftn(c) / c.size
-        np.add.at(e, shell, 0.5 * (np.abs(F) ** 2).ravel())
+        F = np.abs(np.fft.fftn(c) / c.size) ** 2
+        if weight is not None:
+            F = F * weight
+        np.add.at(e, shell, 0.5 * F.ravel())
     return e
 
 
@@ -199,8 +212,9 @@ class Workflow:
             P.updateGrid([g], (s - done) // r)
             done = s
             e = energy_spectrum(*unique_velocity(g, m))
-            e_ref = self.reference_spectrum(s)
-            errs[s] = (spectrum_error(e, e_ref, m // 4), spectrum_error(e, e_ref, m // 2))
+            e_box, e_dec = self.reference_spectrum(s)
+            errs[s] = (spectrum_error(e, e_dec, m // 4), spectrum_error(e, e_dec, m // 2),
+                       spectrum_error(e, e_box, m // 4))
         g.close()
         return errs
 
@@ -212,21 +226,27 @@ class Workflow:
         return self.P.MLP(q, mean, std)
 
     def reference_spectrum(self, s: int):
-        """Energy spectrum of the filtered fine field at fine step s (cached)."""
+        """Spectra of the filtered fine field at fine step s (cached): as
+        filtered, and with the box filter's transfer divided out (the fine
+        run's own spectrum at the coarse grid's scales, which a coarse LES
+        should reproduce)."""
         if not hasattr(self, "_eref"):
             self._eref = {}
         if s not in self._eref:
             _, fx, fy, fz = self.snapshots[s]
-            self._eref[s] = energy_spectrum(fx.astype(np.float64), fy.astype(np.float64), fz.astype(np.float64))
+            u = [a.astype(np.float64) for a in (fx, fy, fz)]
+            self._eref[s] = (energy_spectrum(*u), energy_spectrum(*u, weight=1.0 / box_transfer2(self.m, self.r)))
         return self._eref[s]
 
     def compare(self, start: int, cs_values=(0.17, 0.03, 0.01, 0.003), alpha_values=(1.0, 0.5, 0.25, 0.1),
                 every: int = 1):
-        """Coarse LES runs judged by their energy spectra against the filtered
-        fine run.  Each model gets one scalar (the Smagorinsky constant, the
-        network's output scale) chosen on the first half of the later snapshots
-        (calibration); errors are reported on the second half (evaluation),
-        for shells k <= m/4 (resolved band) and k <= m/2."""
+        """Coarse LES runs judged by their energy spectra against the fine run
+        (box transfer divided out of the filtered field's spectrum).  Each model
+        gets one scalar (the Smagorinsky constant, the network's output scale)
+        chosen on the first half of the later snapshots (calibration); errors
+        are reported on the second half (evaluation) for shells k <= m/4 (the
+        resolved band, the primary figure) and k <= m/2, and against the
+        filtered field's own spectrum (k <= m/4)."""
         runs = {("none", 0.0): self.coarse_run("none", start, 0.0, every)}
         for c in cs_values:
             runs[("smagorinsky", c)] = self.coarse_run("smagorinsky", start, c, every)
@@ -239,16 +259,18 @@ class Workflow:
         def score(errs, which, band):
             return float(np.mean([errs[s][band] for s in which]))
 
+        def scores(errs):
+            return {"evaluation_resolved": score(errs, ev, 0), "evaluation_all": score(errs, ev, 1),
+                    "evaluation_vs_filtered": score(errs, ev, 2)}
+
         out = {"calibration_steps": cal, "evaluation_steps": ev, "runs": {}}
         for (model, prm), errs in runs.items():
-            out["runs"][f"{model}:{prm}"] = {"calibration": score(errs, cal, 0), "evaluation_resolved": score(errs, ev, 0),
-                                             "evaluation_all": score(errs, ev, 1)}
+            out["runs"][f"{model}:{prm}"] = dict(scores(errs), calibration=score(errs, cal, 0))
         for model in ("smagorinsky", "mlp"):
             cands = [(prm, e) for (mm, prm), e in runs.items() if mm == model]
             prm, errs = min(cands, key=lambda c: score(c[1], cal, 0))
-            out[model] = {"param": prm, "evaluation_resolved": score(errs, ev, 0), "evaluation_all": score(errs, ev, 1)}
-        errs = runs[("none", 0.0)]
-        out["none"] = {"evaluation_resolved": score(errs, ev, 0), "evaluation_all": score(errs, ev, 1)}
+            out[model] = dict(scores(errs), param=prm)
+        out["none"] = scores(runs[("none", 0.0)])
         out["gain_vs_smagorinsky"] = 1.0 - out["mlp"]["evaluation_resolved"] / out["smagorinsky"]["evaluation_resolved"]
         for k in ("none", "smagorinsky", "mlp"):
             self.log(f"[les] {k}: resolved-band spectrum error {out[k]['evaluation_resolved']:.4f}"
