@@ -119,6 +119,23 @@ def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
             "L": [d["L"] for d in doms], "re": [round(d["re"], 1) for d in doms]}
 
 
+def copy_probe(P, mib: int = 2048, reps: int = 20):
+    """A plain device copy of `mib` MiB (beyond the 256 MB Infinity Cache): the
+    HBM rate a streaming kernel reaches on this device, read + write bytes."""
+    nbytes = mib << 20
+    a, b = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
+    P.device_copy(b, a, nbytes)
+    P.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        P.device_copy(b, a, nbytes)
+    P.synchronize()
+    el = time.perf_counter() - t0
+    a.close()
+    b.close()
+    return {"bytes_per_copy": 2 * nbytes, "GBs": 2 * nbytes * reps / el / 1e9, "copies": reps}
+
+
 def large_box_leg(P, cases, n: int, steps: int):
     """The bulk kernel on a fully periodic n^3 box ((n+1)^2 n cells; >= 1 GB of
     FP32 distributions at n = 256, beyond the 256 MB Infinity Cache), Taylor-Green
@@ -331,6 +348,9 @@ def main():
         log(f"  {bl['mlups']:.1f} MLUPS over {bl['cells']} cells")
         out["variants"]["batch"] = bl
     if args.large_n:
+        cp = copy_probe(P)
+        log(f"[bench] device copy of 2 GiB: {cp['GBs']:.0f} GB/s (read + write)")
+        out["variants"]["hbm_copy_probe"] = cp
         log(f"[bench] large periodic box {args.large_n}^3")
         lb = large_box_leg(P, cases, args.large_n, 100)
         log("  " + ", ".join(f"{k}: {v['mlups']:.0f} MLUPS, {v['achieved_GBs']:.0f} GB/s" for k, v in lb.items()))
