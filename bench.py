@@ -155,6 +155,8 @@ def large_box_leg(P, cases, n: int, steps: int):
                         "mlups": g.n * steps / el / 1e6, "bulk_kernel_ms": ms, "achieved_GBs": gbs,
                         "roofline_frac": gbs / HBM_PEAK_GBS, "tuning": g.tuning}
         g.close()
+        out[storage]["pattern_limit_GBs"] = P.probe_ps_pattern(nx, ny, nz, storage)
+        out[storage]["frac_of_pattern_limit"] = gbs / out[storage]["pattern_limit_GBs"]
     return out
 
 
@@ -299,9 +301,11 @@ def main():
         cells_per_launch = bulk + (nbc if kernel == "k_ps_fused" else 0)
         bytes_per_launch = BYTES_PER_UPDATE[storage] * cells_per_launch
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        pattern = P.probe_ps_pattern(*shape, storage)
         results[storage] = {
             "mlups": mlups, "ms_per_step": el * 1e3 / args.steps, "bulk_kernel_ms": avg_ms,
             "kernel": kernel, "cells_per_launch": cells_per_launch,
+            "pattern_limit_GBs": pattern, "frac_of_pattern_limit": achieved / pattern,
             "bulk_cells": bulk, "boundary_cells": nbc, "solid_cells": nsolid,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
@@ -310,7 +314,8 @@ def main():
             "tuning": g.tuning, "autotune_ms": tune_times,
         }
         log(f"  {mlups:.1f} MLUPS, {kernel} {avg_ms * 1e3:.1f} us ({achieved:.0f} GB/s, "
-            f"{achieved / HBM_PEAK_GBS * 100:.1f}% of HBM peak)")
+            f"{achieved / HBM_PEAK_GBS * 100:.1f}% of HBM peak, {achieved / pattern * 100:.0f}% of the access "
+            f"pattern's {pattern:.0f} GB/s)")
         g.close()
 
     head = results.get("f32") or next(iter(results.values()))
@@ -336,6 +341,7 @@ def main():
                          cells_per_launch=head["cells_per_launch"], avg_launch_ms=head["bulk_kernel_ms"]),
         "variants": {k: {"mlups": v["mlups"], "ms_per_step": v["ms_per_step"], "roofline_frac": v["roofline"]["frac"],
                          "achieved_GBs": v["roofline"]["achieved"], "kernel": v["kernel"],
+                         "pattern_limit_GBs": v["pattern_limit_GBs"], "frac_of_pattern_limit": v["frac_of_pattern_limit"],
                          "kernel_ms": v["bulk_kernel_ms"],
                          "tuning": v["tuning"], "autotune_ms": v["autotune_ms"]}
                      for k, v in results.items()},
@@ -353,7 +359,9 @@ def main():
         out["variants"]["hbm_copy_probe"] = cp
         log(f"[bench] large periodic box {args.large_n}^3")
         lb = large_box_leg(P, cases, args.large_n, 100)
-        log("  " + ", ".join(f"{k}: {v['mlups']:.0f} MLUPS, {v['achieved_GBs']:.0f} GB/s" for k, v in lb.items()))
+        log("  " + ", ".join(f"{k}: {v['mlups']:.0f} MLUPS, {v['achieved_GBs']:.0f} GB/s "
+                             f"({v['frac_of_pattern_limit'] * 100:.0f} % of the access pattern's "
+                             f"{v['pattern_limit_GBs']:.0f} GB/s)" for k, v in lb.items()))
         out["variants"]["large_box"] = lb
     if args.two_level_steps:
         log(f"[bench] two-level sphere, {args.two_level_steps} coarse steps")

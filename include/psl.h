@@ -154,6 +154,11 @@ int psl_grid_profile_enable(psl_grid* grid, int enable);
 int psl_grid_profile_read(psl_grid* grid, double* total_ms, int* launches);
 /* Plain device copy of `bytes` bytes (16-byte vector loads/stores): a kernel of
  * known traffic, used to calibrate hardware byte counters against the step kernels. */
+/* The HBM rate (read + write GB/s) of the periodic-shift access pattern alone
+ * on an nx x ny x nz grid: every cell reads its 27 shifted slots and writes
+ * them back, no collision, `launches` times.  The data-movement ceiling of the
+ * local update at that size and storage (no reference counterpart). */
+int psl_probe_ps_pattern(int nx, int ny, int nz, int storage, int launches, double* gbs);
 int psl_device_copy(void* dst, const void* src, size_t bytes);
 /* Device buffer helpers for callers without a device allocator. */
 int psl_device_alloc(size_t bytes, void** out);

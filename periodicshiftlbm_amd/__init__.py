@@ -187,6 +187,7 @@ _SIGS = [
     ("psl_grid_set_closure", _I, [_P, _P]),
     ("psl_get_macroscopic_device", _I, [_P, _P]),
     ("psl_grid_time_update_kernel", _I, [_P, _I, _P, _P]),
+    ("psl_probe_ps_pattern", _I, [_I, _I, _I, _I, _I, _P]),
     ("psl_diad_create", _I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     ("psl_diad_destroy", _I, [_P]),
     ("psl_diad_set_flags", _I, [_P, _P]),
@@ -691,6 +692,16 @@ def les_targets(coarse: DeviceBuffer, m: int, nu: float):
     tgt = DeviceBuffer(m ** 3 * 4)
     _check(lib().psl_les_targets(coarse.ptr, m, ctypes.c_float(nu), feat.ptr, tgt.ptr))
     return feat, tgt
+
+
+def probe_ps_pattern(nx: int, ny: int, nz: int, storage="f32", launches: int = 20) -> float:
+    """HBM rate (GB/s, read + write) of the periodic-shift access pattern alone on this grid size."""
+    if isinstance(storage, str):
+        storage = STORAGE_NAMES[storage]
+    require_device()
+    g = ctypes.c_double()
+    _check(lib().psl_probe_ps_pattern(nx, ny, nz, storage, launches, ctypes.byref(g)))
+    return g.value
 
 
 def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
