@@ -3,32 +3,64 @@
 // and the updateGrid level recursion.
 //
 // include/gridRefinementFunctions.h:283-536 and paper section 3 (eqs 3.21-3.26).
-// The six faces of each direction are processed by one launch each
-// (blockIdx.y = face): faces overlap only on edges/corners, where every
-// writer stores identical values, so the concurrent faces equal the
-// reference's sequential ones.  Fine->coarse runs before coarse->fine because
-// the interpolation stencil reads the coarse ring that the averaging writes.
+// Both transfers work on faces of coarse cells.  A workgroup owns a run of
+// coarse cells along the face's first axis with more than one cell and all
+// 2x2x2 fine children of each: lane = 2 * cell + (fine x parity), wave =
+// (fine y parity) + 2 * (fine z parity), so that a wave touches 64
+// consecutive fine cells of one row wherever the face extends in x.  The six
+// faces of a direction set are one launch (blockIdx.y = face): faces overlap
+// only on edges/corners, where every writer stores identical values, so the
+// concurrent faces equal the reference's sequential ones.  Fine->coarse runs
+// before coarse->fine because the interpolation stencil reads the coarse
+// ring that the averaging writes.
 #include "psl_internal.h"
 
 #include <algorithm>
 
 namespace psl {
 
-struct Box {
-  int s[3];
-  int e[3];
-};
-struct Boxes6 {
-  Box b[6];
-  int count;
-  int dirs[6][9];  // populations written to ghost fine cells, per face
-};
-
 struct GridView {
   Shift27 sh;
   uint32_t n;
   int nx, ny, nz;
 };
+
+// A face in coarse cells: cell (a, b, c) of the face is base + a*step_a +
+// b*step_b + c*step_c, a < na, b < nb; workgroup w covers cells
+// [(w % wa) * CELLS, +CELLS) along a at b = (w / wa) % nb, c = w / (wa * nb).
+struct Face {
+  int base[3];
+  int step_a[3], step_b[3], step_c[3];
+  int na, nb;
+  uint32_t wa;
+  uint32_t blocks;
+  uint32_t ghost_pops;  // bit q: population q is written to ghost fine cells
+};
+struct Faces {
+  Face f[6];
+  int count;
+};
+
+struct FaceCell {
+  int x, y, z;
+  bool valid;
+};
+
+__device__ __forceinline__ FaceCell face_cell(const Face& F, int cells_per_block, int cell) {
+  const uint32_t wa = blockIdx.x % F.wa, rest = blockIdx.x / F.wa;
+  const int b = (int)(rest % (uint32_t)F.nb), c = (int)(rest / (uint32_t)F.nb);
+  const int a = (int)wa * cells_per_block + cell;
+  FaceCell r;
+  r.x = F.base[0] + a * F.step_a[0] + b * F.step_b[0] + c * F.step_c[0];
+  r.y = F.base[1] + a * F.step_a[1] + b * F.step_b[1] + c * F.step_c[1];
+  r.z = F.base[2] + a * F.step_a[2] + b * F.step_b[2] + c * F.step_c[2];
+  r.valid = a < F.na;
+  return r;
+}
+
+__device__ __forceinline__ uint32_t cell_index(const GridView& g, int x, int y, int z) {
+  return ((uint32_t)z * (uint32_t)g.ny + (uint32_t)y) * (uint32_t)g.nx + (uint32_t)x;
+}
 
 __device__ __forceinline__ int64_t slot_of(const GridView& g, uint32_t cell, int q) {
   uint32_t s = cell + (uint32_t)g.sh.s[q];
@@ -45,99 +77,123 @@ __device__ __forceinline__ void load_cell(const typename St::T* f, const GridVie
   }
 }
 
-// writeToCoarseGridInterface (lines 283-336)
+// Value of the neighbouring lane of the pair (lane ^ 1), a DPP quad permutation.
+__device__ __forceinline__ float pair_swap(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+constexpr int TC_CELLS = 32;  // coarse cells per workgroup, fine->coarse
+constexpr int TF_CELLS = 64;  // coarse cells per workgroup, coarse->fine
+
+// writeToCoarseGridInterface (lines 283-336): each thread loads one fine
+// child, pairs add in registers, the four waves add through LDS, and one lane
+// per coarse cell rescales and stores.
 template <class St>
 __global__ __launch_bounds__(256) void k_to_coarse(typename St::T* __restrict__ fc, GridView gc, int iss, int jss,
                                                    int kss, const typename St::T* __restrict__ ff, GridView gf,
-                                                   Boxes6 boxes) {
-  const Box& b = boxes.b[blockIdx.y];
-  const int sx = b.e[0] - b.s[0], sy = b.e[1] - b.s[1], sz = b.e[2] - b.s[2];
-  if (sx <= 0 || sy <= 0 || sz <= 0) return;
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= (uint32_t)sx * sy * sz) return;
-  const int kr = t / (sx * sy);
-  const int rem = t - kr * sx * sy;
-  const int jr = rem / sx, ir = rem - jr * sx;
-  const int ic = b.s[0] + ir, jc = b.s[1] + jr, kc = b.s[2] + kr;
-  const int i0 = (ic - iss) * 2, j0 = (jc - jss) * 2, k0 = (kc - kss) * 2;
+                                                   Faces faces) {
+  __shared__ float part[4][27][TC_CELLS];
+  const Face& F = faces.f[blockIdx.y];
+  if (blockIdx.x >= F.blocks) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const FaceCell cc = face_cell(F, TC_CELLS, lane >> 1);
+  float v[27];
+  if (cc.valid) {
+    const int fx = 2 * (cc.x - iss) + (lane & 1), fy = 2 * (cc.y - jss) + (w & 1), fz = 2 * (cc.z - kss) + (w >> 1);
+    load_cell<St>(ff, gf, cell_index(gf, fx, fy, fz), v);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] = 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v[q] += pair_swap(v[q]);
+  if (!(lane & 1)) {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) part[w][q][lane >> 1] = v[q];
+  }
+  __syncthreads();
+  if (threadIdx.x >= TC_CELLS) return;
+  const int cell = threadIdx.x;
+  const FaceCell oc = face_cell(F, TC_CELLS, cell);
+  if (!oc.valid) return;
   float acc[27];
 #pragma unroll
-  for (int q = 0; q < 27; ++q) acc[q] = 0.f;
-  for (int ka = 0; ka < 2; ++ka)
-    for (int ja = 0; ja < 2; ++ja)
-      for (int ia = 0; ia < 2; ++ia) {
-        const uint32_t cf = (uint32_t)(k0 + ka) * gf.nx * gf.ny + (uint32_t)(j0 + ja) * gf.nx + (i0 + ia);
-        float v[27];
-        load_cell<St>(ff, gf, cf, v);
-#pragma unroll
-        for (int q = 0; q < 27; ++q) acc[q] += v[q];
-      }
-#pragma unroll
-  for (int q = 0; q < 27; ++q) acc[q] *= 0.125f;
+  for (int q = 0; q < 27; ++q)
+    acc[q] = ((part[0][q][cell] + part[1][q][cell]) + (part[2][q][cell] + part[3][q][cell])) * 0.125f;
   rescale(acc, false);
-  const uint32_t cc = (uint32_t)kc * gc.nx * gc.ny + (uint32_t)jc * gc.nx + ic;
+  const uint32_t c = cell_index(gc, oc.x, oc.y, oc.z);
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
     const float o = St::W_OFF ? acc[q] - weight(q) : acc[q];
-    fc[slot_of(gc, cc, q)] = St::store(o);
+    fc[slot_of(gc, c, q)] = St::store(o);
   }
 }
 
-// writeToFineGridInterface (lines 338-431)
+// writeToFineGridInterface (lines 338-431).  The interpolant of child
+// (sx, sy, sz) in {-1, 1}^3 of coarse cell c is
+//   f_c + sx hx (f_{i+1} - f_{i-1}) + sy hy (...) + sz hz (...),
+// h = 1/8 for the central difference and 1/4 for the one-sided difference at
+// the grid edge.  Phase 1 (one coarse cell per lane, populations split over
+// the four waves) puts f_c and the three scaled differences in LDS; phase 2
+// (one fine cell per thread, two halves of the coarse run) interpolates,
+// rescales and stores, only the inward populations in ghost cells.
 template <class St>
 __global__ __launch_bounds__(256) void k_to_fine(const typename St::T* __restrict__ fc, GridView gc, int iss, int jss,
                                                  int kss, typename St::T* __restrict__ ff, GridView gf,
-                                                 const uint8_t* __restrict__ fine_flags, Boxes6 boxes) {
-  const Box& b = boxes.b[blockIdx.y];
-  const int sx = b.e[0] - b.s[0], sy = b.e[1] - b.s[1], sz = b.e[2] - b.s[2];
-  if (sx <= 0 || sy <= 0 || sz <= 0) return;
-  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= (uint32_t)sx * sy * sz) return;
-  const int kr = t / (sx * sy);
-  const int rem = t - kr * sx * sy;
-  const int jr = rem / sx, ir = rem - jr * sx;
-  const int i_f = b.s[0] + ir, jf = b.s[1] + jr, kf = b.s[2] + kr;
-  const uint32_t cellf = (uint32_t)kf * gf.nx * gf.ny + (uint32_t)jf * gf.nx + i_f;
-  const int ic = i_f / 2 + iss, jc = jf / 2 + jss, kc = kf / 2 + kss;
-  const float dx = (i_f & 1) ? 0.25f : -0.25f;
-  const float dy = (jf & 1) ? 0.25f : -0.25f;
-  const float dz = (kf & 1) ? 0.25f : -0.25f;
-  float f[27];
-  load_cell<St>(fc, gc, (uint32_t)kc * gc.nx * gc.ny + (uint32_t)jc * gc.nx + ic, f);
-  int ip = ic + 1, im = ic - 1, jp = jc + 1, jm = jc - 1, kp = kc + 1, km = kc - 1;
-  float dix = 0.5f, diy = 0.5f, diz = 0.5f;
-  if (ip >= gc.nx) { ip = ic; dix = 1.f; }
-  if (im < 0) { im = 0; dix = 1.f; }
-  if (jp >= gc.ny) { jp = jc; diy = 1.f; }
-  if (jm < 0) { jm = 0; diy = 1.f; }
-  if (kp >= gc.nz) { kp = kc; diz = 1.f; }
-  if (km < 0) { km = 0; diz = 1.f; }
-  const int ni[6] = {ip, im, ic, ic, ic, ic};
-  const int nj[6] = {jc, jc, jp, jm, jc, jc};
-  const int nk[6] = {kc, kc, kc, kc, kp, km};
-  const float w6[6] = {dix * dx, -dix * dx, diy * dy, -diy * dy, diz * dz, -diz * dz};
-  for (int nb = 0; nb < 6; ++nb) {
-    const uint32_t cn = (uint32_t)nk[nb] * gc.nx * gc.ny + (uint32_t)nj[nb] * gc.nx + ni[nb];
-    float v[27];
-    load_cell<St>(fc, gc, cn, v);
-#pragma unroll
-    for (int q = 0; q < 27; ++q) f[q] += v[q] * w6[nb];
-  }
-  rescale(f, true);
-  const bool ghost = fine_flags && ((fine_flags[cellf] & 7) == KIND_GHOST);
-  if (ghost) {
-    for (int t9 = 0; t9 < 9; ++t9) {
-      const int q = boxes.dirs[blockIdx.y][t9];
-      float o = f[0];
-#pragma unroll
-      for (int r = 0; r < 27; ++r)
-        if (r == q) o = f[r];
-      if (St::W_OFF) o -= weight(q);
-      ff[slot_of(gf, cellf, q)] = St::store(o);
+                                                 const uint8_t* __restrict__ fine_flags, Faces faces) {
+  __shared__ float st[4][27][TF_CELLS];
+  const Face& F = faces.f[blockIdx.y];
+  if (blockIdx.x >= F.blocks) return;
+  {
+    const int cell = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const FaceCell cc = face_cell(F, TF_CELLS, cell);
+    if (cc.valid) {
+      int ip = cc.x + 1, im = cc.x - 1, jp = cc.y + 1, jm = cc.y - 1, kp = cc.z + 1, km = cc.z - 1;
+      float hx = 0.125f, hy = 0.125f, hz = 0.125f;
+      if (ip >= gc.nx) { ip = cc.x; hx = 0.25f; }
+      if (im < 0) { im = 0; hx = 0.25f; }
+      if (jp >= gc.ny) { jp = cc.y; hy = 0.25f; }
+      if (jm < 0) { jm = 0; hy = 0.25f; }
+      if (kp >= gc.nz) { kp = cc.z; hz = 0.25f; }
+      if (km < 0) { km = 0; hz = 0.25f; }
+      const uint32_t c0 = cell_index(gc, cc.x, cc.y, cc.z);
+      const uint32_t cxp = cell_index(gc, ip, cc.y, cc.z), cxm = cell_index(gc, im, cc.y, cc.z);
+      const uint32_t cyp = cell_index(gc, cc.x, jp, cc.z), cym = cell_index(gc, cc.x, jm, cc.z);
+      const uint32_t czp = cell_index(gc, cc.x, cc.y, kp), czm = cell_index(gc, cc.x, cc.y, km);
+      for (int q = w; q < 27; q += 4) {
+        const float wq = St::W_OFF ? weight(q) : 0.f;
+        const float c = St::load(fc[slot_of(gc, c0, q)]) + wq;
+        const float xp = St::load(fc[slot_of(gc, cxp, q)]), xm = St::load(fc[slot_of(gc, cxm, q)]);
+        const float yp = St::load(fc[slot_of(gc, cyp, q)]), ym = St::load(fc[slot_of(gc, cym, q)]);
+        const float zp = St::load(fc[slot_of(gc, czp, q)]), zm = St::load(fc[slot_of(gc, czm, q)]);
+        st[0][q][cell] = c;
+        st[1][q][cell] = hx * (xp - xm);
+        st[2][q][cell] = hy * (yp - ym);
+        st[3][q][cell] = hz * (zp - zm);
+      }
     }
-  } else {
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ia = lane & 1, ja = w & 1, ka = w >> 1;
+  const float sx = ia ? 1.f : -1.f, sy = ja ? 1.f : -1.f, sz = ka ? 1.f : -1.f;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int cell = h * (TF_CELLS / 2) + (lane >> 1);
+    const FaceCell cc = face_cell(F, TF_CELLS, cell);
+    if (!cc.valid) continue;
+    const uint32_t cellf = cell_index(gf, 2 * (cc.x - iss) + ia, 2 * (cc.y - jss) + ja, 2 * (cc.z - kss) + ka);
+    float f[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q)
+      f[q] = st[0][q][cell] + sx * st[1][q][cell] + sy * st[2][q][cell] + sz * st[3][q][cell];
+    rescale(f, true);
+    const bool ghost = fine_flags && ((fine_flags[cellf] & 7) == KIND_GHOST);
+    const uint32_t keep = ghost ? F.ghost_pops : 0x7ffffffu;
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
+      if (!((keep >> q) & 1u)) continue;
       const float o = St::W_OFF ? f[q] - weight(q) : f[q];
       ff[slot_of(gf, cellf, q)] = St::store(o);
     }
@@ -154,19 +210,39 @@ static GridView view_of(const psl_grid* g) {
   return v;
 }
 
-static uint32_t box_blocks(const Boxes6& B) {
-  uint64_t m = 0;
-  for (int i = 0; i < B.count; ++i) {
-    const Box& b = B.b[i];
-    const int64_t v = (int64_t)std::max(0, b.e[0] - b.s[0]) * std::max(0, b.e[1] - b.s[1]) * std::max(0, b.e[2] - b.s[2]);
-    m = std::max<uint64_t>(m, (uint64_t)v);
+// Appends the coarse box [s, e) as a face; the first axis with more than one
+// cell runs along the workgroup.
+static void add_face(Faces& F, const int s[3], const int e[3], int cells_per_block, uint32_t ghost_pops) {
+  int n[3];
+  for (int d = 0; d < 3; ++d) n[d] = std::max(0, e[d] - s[d]);
+  if (n[0] == 0 || n[1] == 0 || n[2] == 0) return;
+  const int a = n[0] > 1 ? 0 : n[1] > 1 ? 1 : n[2] > 1 ? 2 : 0;
+  const int b = a == 0 ? 1 : 0;
+  const int c = 3 - a - b;
+  Face& f = F.f[F.count++];
+  for (int d = 0; d < 3; ++d) {
+    f.base[d] = s[d];
+    f.step_a[d] = d == a;
+    f.step_b[d] = d == b;
+    f.step_c[d] = d == c;
   }
-  return (uint32_t)((m + 255) / 256);
+  f.na = n[a];
+  f.nb = n[b];
+  f.wa = (uint32_t)((n[a] + cells_per_block - 1) / cells_per_block);
+  f.blocks = f.wa * (uint32_t)n[b] * (uint32_t)n[c];
+  f.ghost_pops = ghost_pops;
 }
 
-static void set_box(Box& b, int s0, int s1, int s2, int e0, int e1, int e2) {
-  b.s[0] = s0; b.s[1] = s1; b.s[2] = s2;
-  b.e[0] = e0; b.e[1] = e1; b.e[2] = e2;
+static uint32_t face_blocks(const Faces& F) {
+  uint32_t m = 0;
+  for (int i = 0; i < F.count; ++i) m = std::max(m, F.f[i].blocks);
+  return m;
+}
+
+static uint32_t pop_mask(const int (&dirs)[9]) {
+  uint32_t m = 0;
+  for (int t = 0; t < 9; ++t) m |= 1u << dirs[t];
+  return m;
 }
 
 static const int POS_X[9] = {1, 7, 9, 12, 16, 20, 22, 23, 26};
@@ -177,27 +253,25 @@ static const int POS_Z[9] = {4, 8, 9, 14, 17, 20, 21, 24, 26};
 static const int NEG_Z[9] = {3, 7, 10, 13, 18, 19, 22, 23, 25};
 
 template <class St>
-static int to_coarse(psl_grid* c, psl_grid* f, const Boxes6& B) {
-  if (B.count == 0) return PSL_OK;
-  const uint32_t nb = box_blocks(B);
+static int to_coarse(psl_grid* c, psl_grid* f, const Faces& F) {
+  const uint32_t nb = face_blocks(F);
   if (nb == 0) return PSL_OK;
-  dim3 grid(nb, B.count);
+  dim3 grid(nb, F.count);
   k_to_coarse<St><<<grid, 256, 0, c->stream>>>((typename St::T*)c->f, view_of(c), c->info.iSubgridStart,
                                                c->info.jSubgridStart, c->info.kSubgridStart,
-                                               (const typename St::T*)f->f, view_of(f), B);
+                                               (const typename St::T*)f->f, view_of(f), F);
   PSL_LAUNCH_CHECK();
   return PSL_OK;
 }
 
 template <class St>
-static int to_fine(psl_grid* c, psl_grid* f, const Boxes6& B) {
-  if (B.count == 0) return PSL_OK;
-  const uint32_t nb = box_blocks(B);
+static int to_fine(psl_grid* c, psl_grid* f, const Faces& F) {
+  const uint32_t nb = face_blocks(F);
   if (nb == 0) return PSL_OK;
-  dim3 grid(nb, B.count);
+  dim3 grid(nb, F.count);
   k_to_fine<St><<<grid, 256, 0, c->stream>>>((const typename St::T*)c->f, view_of(c), c->info.iSubgridStart,
                                              c->info.jSubgridStart, c->info.kSubgridStart, (typename St::T*)f->f,
-                                             view_of(f), f->flags, B);
+                                             view_of(f), f->flags, F);
   PSL_LAUNCH_CHECK();
   return PSL_OK;
 }
@@ -205,50 +279,48 @@ static int to_fine(psl_grid* c, psl_grid* f, const Boxes6& B) {
 template <class St>
 static int communicate(psl_grid* c, psl_grid* f, bool full_fill) {
   const psl_info& C = c->info;
-  const psl_info& F = f->info;
-  Boxes6 B{};
+  const int is = C.iSubgridStart, js = C.jSubgridStart, ks = C.kSubgridStart;
+  const int ie = C.iSubgridEnd, je = C.jSubgridEnd, ke = C.kSubgridEnd;
+  Faces B{};
   if (full_fill) {
-    set_box(B.b[0], C.iSubgridStart + 1, C.jSubgridStart + 1, C.kSubgridStart + 1, C.iSubgridEnd - 1,
-            C.jSubgridEnd - 1, C.kSubgridEnd - 1);
-    B.count = 1;
+    const int s[3] = {is + 1, js + 1, ks + 1}, e[3] = {ie - 1, je - 1, ke - 1};
+    add_face(B, s, e, TC_CELLS, 0);
     return to_coarse<St>(c, f, B);
   }
   // fine -> coarse ring (lines 447-486)
-  B.count = 0;
   if (C.cellCountX > 1) {
-    set_box(B.b[B.count++], C.iSubgridStart + 1, C.jSubgridStart + 1, C.kSubgridStart + 1, C.iSubgridStart + 2,
-            C.jSubgridEnd - 1, C.kSubgridEnd - 1);
-    set_box(B.b[B.count++], C.iSubgridEnd - 2, C.jSubgridStart + 1, C.kSubgridStart + 1, C.iSubgridEnd - 1,
-            C.jSubgridEnd - 1, C.kSubgridEnd - 1);
+    const int s0[3] = {is + 1, js + 1, ks + 1}, e0[3] = {is + 2, je - 1, ke - 1};
+    const int s1[3] = {ie - 2, js + 1, ks + 1}, e1[3] = {ie - 1, je - 1, ke - 1};
+    add_face(B, s0, e0, TC_CELLS, 0);
+    add_face(B, s1, e1, TC_CELLS, 0);
   }
   if (C.cellCountY > 1) {
-    set_box(B.b[B.count++], C.iSubgridStart + 1, C.jSubgridStart + 1, C.kSubgridStart + 1, C.iSubgridEnd - 1,
-            C.jSubgridStart + 2, C.kSubgridEnd - 1);
-    set_box(B.b[B.count++], C.iSubgridStart + 1, C.jSubgridEnd - 2, C.kSubgridStart + 1, C.iSubgridEnd - 1,
-            C.jSubgridEnd - 1, C.kSubgridEnd - 1);
+    const int s0[3] = {is + 1, js + 1, ks + 1}, e0[3] = {ie - 1, js + 2, ke - 1};
+    const int s1[3] = {is + 1, je - 2, ks + 1}, e1[3] = {ie - 1, je - 1, ke - 1};
+    add_face(B, s0, e0, TC_CELLS, 0);
+    add_face(B, s1, e1, TC_CELLS, 0);
   }
   if (C.cellCountZ > 1) {
-    set_box(B.b[B.count++], C.iSubgridStart + 1, C.jSubgridStart + 1, C.kSubgridStart + 1, C.iSubgridEnd - 1,
-            C.jSubgridEnd - 1, C.kSubgridStart + 2);
-    set_box(B.b[B.count++], C.iSubgridStart + 1, C.jSubgridStart + 1, C.kSubgridEnd - 2, C.iSubgridEnd - 1,
-            C.jSubgridEnd - 1, C.kSubgridEnd - 1);
+    const int s0[3] = {is + 1, js + 1, ks + 1}, e0[3] = {ie - 1, je - 1, ks + 2};
+    const int s1[3] = {is + 1, js + 1, ke - 2}, e1[3] = {ie - 1, je - 1, ke - 1};
+    add_face(B, s0, e0, TC_CELLS, 0);
+    add_face(B, s1, e1, TC_CELLS, 0);
   }
   int rc = to_coarse<St>(c, f, B);
   if (rc) return rc;
-  // coarse -> fine layers (lines 488-525)
-  Boxes6 Fb{};
-  Fb.count = 0;
-  auto add = [&](int s0, int s1, int s2, int e0, int e1, int e2, const int* dirs) {
-    set_box(Fb.b[Fb.count], s0, s1, s2, e0, e1, e2);
-    for (int t = 0; t < 9; ++t) Fb.dirs[Fb.count][t] = dirs[t];
-    Fb.count++;
+  // coarse -> fine layers (lines 488-525): two fine cells deep, i.e. one
+  // coarse layer of the sub-grid, on each side that is not the grid edge
+  Faces Fb{};
+  auto add = [&](int s0, int s1, int s2, int e0, int e1, int e2, const int(&dirs)[9]) {
+    const int s[3] = {s0, s1, s2}, e[3] = {e0, e1, e2};
+    add_face(Fb, s, e, TF_CELLS, pop_mask(dirs));
   };
-  if (C.iSubgridStart > 0) add(0, 0, 0, 2, F.cellCountY, F.cellCountZ, POS_X);
-  if (C.iSubgridEnd < C.cellCountX - 1) add(F.cellCountX - 2, 0, 0, F.cellCountX, F.cellCountY, F.cellCountZ, NEG_X);
-  if (C.jSubgridStart > 0) add(0, 0, 0, F.cellCountX, 2, F.cellCountZ, POS_Y);
-  if (C.jSubgridEnd < C.cellCountY - 1) add(0, F.cellCountY - 2, 0, F.cellCountX, F.cellCountY, F.cellCountZ, NEG_Y);
-  if (C.kSubgridStart > 0) add(0, 0, 0, F.cellCountX, F.cellCountY, 2, POS_Z);
-  if (C.kSubgridEnd < C.cellCountZ - 1) add(0, 0, F.cellCountZ - 2, F.cellCountX, F.cellCountY, F.cellCountZ, NEG_Z);
+  if (is > 0) add(is, js, ks, is + 1, je, ke, POS_X);
+  if (ie < C.cellCountX - 1) add(ie - 1, js, ks, ie, je, ke, NEG_X);
+  if (js > 0) add(is, js, ks, ie, js + 1, ke, POS_Y);
+  if (je < C.cellCountY - 1) add(is, je - 1, ks, ie, je, ke, NEG_Y);
+  if (ks > 0) add(is, js, ks, ie, je, ks + 1, POS_Z);
+  if (ke < C.cellCountZ - 1) add(is, js, ke - 1, ie, je, ke, NEG_Z);
   return to_fine<St>(c, f, Fb);
 }
 

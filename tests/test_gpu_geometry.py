@@ -95,6 +95,84 @@ def test_two_level_recursion_matches_oracle(psl, oracle_mod):
     assert skipped == (12 - 4) * (10 - 4) * (9 - 4)
 
 
+def _physical(logical, shift):
+    n = logical.shape[1]
+    out = np.empty_like(logical)
+    idx = np.arange(n, dtype=np.int64)
+    for q in range(27):
+        out[q, (idx + int(shift[q])) % n] = logical[q]
+    return out
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_coarse_fine_communication_wide_faces(psl, oracle_mod, storage):
+    """Transfers on a sub-grid whose faces span several workgroups (80 x 30 x 30
+    coarse cells), touching the coarse grid's y = 0 edge (no fine face there,
+    one-sided differences in the interpolation), with wrapped population
+    shifts on both levels.  FP32 within 1e-5 relative; FP16 storage against
+    the oracle in FP64 on the stored (rounded) inputs, within 1e-4 absolute
+    (FP16 resolves f - w to 2^-11 relative, |f - w| < 0.05 here)."""
+    P, O = psl, oracle_mod
+    nxc, nyc, nzc = 100, 40, 36
+    sub = (2, 0, 3, 82, 30, 33)
+    nxf, nyf, nzf = 2 * (sub[3] - sub[0]), 2 * (sub[4] - sub[1]), 2 * (sub[5] - sub[2])
+    ci = P.InfoStruct(cellCountX=nxc, cellCountY=nyc, cellCountZ=nzc, nu=0.01, res=1.0,
+                      iSubgridStart=sub[0], jSubgridStart=sub[1], kSubgridStart=sub[2],
+                      iSubgridEnd=sub[3], jSubgridEnd=sub[4], kSubgridEnd=sub[5])
+    fi = P.InfoStruct(gridID=1, cellCountX=nxf, cellCountY=nyf, cellCountZ=nzf, nu=0.02, res=0.5,
+                      ox=sub[0] - 0.25, oy=sub[1] - 0.25, oz=sub[2] - 0.25)
+    gc = P.GridStruct(ci, storage)
+    gf = P.GridStruct(fi, storage)
+    k, j, i = np.meshgrid(np.arange(nzf), np.arange(nyf), np.arange(nxf), indexing="ij")
+    ghost = (i == 0) | (i == nxf - 1) | (j == 0) | (j == nyf - 1) | (k == 0) | (k == nzf - 1)
+    fflags = np.where(ghost, cases.GHOST, cases.FLUID).astype(np.uint8).reshape(-1)
+    gf.set_flags(fflags)
+    dt = np.float32 if storage == "f32" else np.float64
+    oc = O.Grid(nxc, nyc, nzc, dtype=dt, nu=0.01, sub=sub)
+    of = O.Grid(nxf, nyf, nzf, dtype=dt, nu=0.02, flags=fflags)
+    rng = np.random.default_rng(11)
+    for g, o, (nx, ny, nz), (x0, y0, z0, h) in ((gc, oc, (nxc, nyc, nzc), (0, 0, 0, 1.0)),
+                                                (gf, of, (nxf, nyf, nzf), (sub[0] - 0.25, sub[1] - 0.25,
+                                                                           sub[2] - 0.25, 0.5))):
+        kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        x = 2 * np.pi * (x0 + ii * h) / 37
+        y = 2 * np.pi * (y0 + jj * h) / 29
+        z = 2 * np.pi * (z0 + kk * h) / 31
+        rho = 1.0 + 0.01 * np.sin(x + 2 * y) * np.cos(z)
+        ux = 0.04 * np.sin(x) * np.cos(y) * np.cos(z)
+        uy = -0.03 * np.cos(x) * np.sin(y) * np.cos(2 * z)
+        uz = 0.02 * np.sin(x + y + z)
+        tmp = O.Grid(nx, ny, nz, dtype=np.float64)
+        tmp.fill_equilibrium(rho.reshape(-1), ux.reshape(-1), uy.reshape(-1), uz.reshape(-1))
+        shift = rng.integers(0, nx * ny * nz, 27).astype(np.int32)
+        g.shifter = shift
+        g.set_f(tmp.f.astype(np.float32))
+        stored = g.get_f()  # the inputs as the storage holds them
+        o.shift[:] = shift
+        o.f[:] = _physical(stored.astype(dt), shift)
+    print(f"[refine] wide faces, {storage} storage", flush=True)
+    P.applyCoarseFineGridCommunication(gc, gf)
+    O.coarse_fine_comm(oc, of)
+    if storage == "f32":
+        ec, ef = _rel_err(gc.get_f(), oc.logical_f()), _rel_err(gf.get_f(), of.logical_f())
+        print(f"  coarse {ec:.3e} fine {ef:.3e}")
+        assert ec < RTOL and ef < RTOL
+    else:
+        ec = float(np.max(np.abs(gc.get_f() - oc.logical_f())))
+        ef = float(np.max(np.abs(gf.get_f() - of.logical_f())))
+        print(f"  coarse {ec:.3e} fine {ef:.3e} (absolute)")
+        assert ec < 1e-4 and ef < 1e-4
+    # the fine faces were rewritten except the skipped y = 0 face (FP32 resolves
+    # the second-order interpolation error of this smooth field)
+    changed = np.any(gf.get_f() != stored, axis=0).reshape(nzf, nyf, nxf)
+    assert not changed[2:-2, 0, 2:-2].any()
+    if storage == "f32":
+        assert changed[:, 1:, 0].all() and changed[:, 1:, -1].all() and changed[0, 1:, :].all()
+        assert changed[:, -1, :].all()
+    gc.close()
+    gf.close()
+
+
 def test_voxelizer_matches_oracle_bit_exact(psl, oracle_mod):
     P, O = psl, oracle_mod
     tri = cases.icosphere(3, 7.3, (15.2, 13.7, 12.9))
