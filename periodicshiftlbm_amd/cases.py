@@ -234,16 +234,19 @@ TWO_LEVEL_SUB = (32, 16, 16, 160, 112, 112)
 
 
 def two_level_sphere(storage: str = "f16", reynolds: float = 1e5, u_inlet: float = 0.04, cs: float = 0.17,
-                     sub=TWO_LEVEL_SUB):
+                     sub=TWO_LEVEL_SUB, shape=None, center=None, diameter=None):
     """The sphere channel (256 x 128 x 128 coarse cells) with a 2x finer nested
     box around the sphere (flowAroundSphere/flowAroundSphereMainSingle.cu:188-213
     sets sub-grid bounds and the fine origin ox_c + iStart res_c - res_f / 2).
     The fine grid has ghost box faces; the sphere is voxelized on both levels
-    by the STL path.  Returns ([coarse, fine], fine sphere box (lo, hi))."""
+    by the STL path.  shape / center / diameter (coarse cells) default to the
+    256 x 128 x 128 channel.  Returns ([coarse, fine], fine sphere box (lo, hi))."""
     from . import BoolArray, GridStruct, InfoStruct, STLStruct, applyMarkersInsideSTL, fill_equilibrium
 
-    nxc, nyc, nzc = SPHERE_SHAPE
-    nu_c = u_inlet * SPHERE_DIAMETER / reynolds
+    center = center or SPHERE_CENTER
+    diameter = diameter or SPHERE_DIAMETER
+    nxc, nyc, nzc = shape or SPHERE_SHAPE
+    nu_c = u_inlet * diameter / reynolds
     ci = InfoStruct(cellCountX=nxc, cellCountY=nyc, cellCountZ=nzc, nu=nu_c, res=1.0,
                     iSubgridStart=sub[0], jSubgridStart=sub[1], kSubgridStart=sub[2],
                     iSubgridEnd=sub[3], jSubgridEnd=sub[4], kSubgridEnd=sub[5])
@@ -251,21 +254,21 @@ def two_level_sphere(storage: str = "f16", reynolds: float = 1e5, u_inlet: float
     ox, oy, oz = sub[0] - 0.25, sub[1] - 0.25, sub[2] - 0.25
     fi = InfoStruct(gridID=1, cellCountX=nxf, cellCountY=nyf, cellCountZ=nzf, nu=2 * nu_c, res=0.5,
                     ox=ox, oy=oy, oz=oz)
-    stl = STLStruct.from_triangles(icosphere(4, SPHERE_DIAMETER / 2, SPHERE_CENTER))
+    stl = STLStruct.from_triangles(icosphere(4, diameter / 2, center))
     grids = []
     for info, fine in ((ci, False), (fi, True)):
         n = info.cellCountX * info.cellCountY * info.cellCountZ
         mask = BoolArray(n)
         applyMarkersInsideSTL(mask, stl, True, info)
         solid = mask.get().astype(bool)
-        shape = (info.cellCountX, info.cellCountY, info.cellCountZ)
+        dims = (info.cellCountX, info.cellCountY, info.cellCountZ)
         if fine:
-            k, j, i = np.meshgrid(np.arange(shape[2]), np.arange(shape[1]), np.arange(shape[0]), indexing="ij")
-            ghost = ((i == 0) | (i == shape[0] - 1) | (j == 0) | (j == shape[1] - 1) | (k == 0)
-                     | (k == shape[2] - 1)).reshape(-1)
+            k, j, i = np.meshgrid(np.arange(dims[2]), np.arange(dims[1]), np.arange(dims[0]), indexing="ij")
+            ghost = ((i == 0) | (i == dims[0] - 1) | (j == 0) | (j == dims[1] - 1) | (k == 0)
+                     | (k == dims[2] - 1)).reshape(-1)
             flags = np.where(ghost, GHOST, np.where(solid, BOUNCEBACK, FLUID)).astype(np.uint8)
         else:
-            flags = channel_flags(shape, solid)
+            flags = channel_flags(dims, solid)
         g = GridStruct(info, storage)
         g.set_flags(flags)
         g.set_given_values(channel_given_values(u_inlet))
@@ -275,7 +278,7 @@ def two_level_sphere(storage: str = "f16", reynolds: float = 1e5, u_inlet: float
         fill_equilibrium(g)
         grids.append(g)
     # sphere box in fine cell indices (cell i centre at ox + (i + 1/2) res_f)
-    r = SPHERE_DIAMETER / 2
-    lo = [int((c - r - o) / 0.5) - 2 for c, o in zip(SPHERE_CENTER, (ox, oy, oz))]
-    hi = [int((c + r - o) / 0.5) + 3 for c, o in zip(SPHERE_CENTER, (ox, oy, oz))]
+    r = diameter / 2
+    lo = [int((c - r - o) / 0.5) - 2 for c, o in zip(center, (ox, oy, oz))]
+    hi = [int((c + r - o) / 0.5) + 3 for c, o in zip(center, (ox, oy, oz))]
     return grids, (lo, hi)

@@ -21,6 +21,10 @@
   the acoustic start-up transient in which the instantaneous drag crosses
   zero.  FP32 and FP16 must be within 1 % of the FP64 mean; BF16 (8-bit
   significand) is reported and bounded at 5 %.
+* Two-level sphere drag (the two-level configuration scaled by 1/4, Re =
+  1000): FP32 and FP16 storage through the whole recursion (coarse step, two
+  fine steps, coarse-fine communication) against the FP64 oracle's
+  recursion, mean fine-grid drag within 1 %.
 """
 import numpy as np
 import pytest
@@ -140,3 +144,64 @@ def test_sphere_drag_vs_fp64(psl, fp64_drag, storage):
     print(f"  mean Fx {mean:.6e} FP64 {mean64:.6e} rel {rel:.2e}; C_D {cases.drag_coefficient(mean, 0.04, 8.0):.4f} "
           f"(FP64 {cases.drag_coefficient(mean64, 0.04, 8.0):.4f})", flush=True)
     assert rel < BOUND[storage]
+
+
+# Two-level sphere, scaled by 1/4: coarse 64 x 32 x 32 channel, fine box
+# 2x finer over coarse cells [8, 40) x [4, 28) x [4, 28) around the sphere
+# (diameter 8 coarse = 16 fine cells), Re = 1000, Cs = 0.17.  The drag is the
+# fine grid's momentum exchange, averaged over coarse steps 1600-3200.
+TWO_LEVEL_SCALED = {"shape": (64, 32, 32), "center": (16.0, 16.0, 16.0), "diameter": 8.0,
+                    "sub": (8, 4, 4, 40, 28, 28)}
+
+
+def _scaled_two_level(P, storage):
+    c = TWO_LEVEL_SCALED
+    return cases.two_level_sphere(storage, reynolds=1000.0, sub=c["sub"], shape=c["shape"], center=c["center"],
+                                  diameter=c["diameter"])
+
+
+@pytest.fixture(scope="module")
+def fp64_two_level_drag(psl, oracle_mod):
+    """Mean fine-grid drag of the FP64 oracle's two-level recursion."""
+    P, O = psl, oracle_mod
+    (gc, gf), (lo, hi) = _scaled_two_level(P, "f32")
+    sub = TWO_LEVEL_SCALED["sub"]
+    oc = O.Grid(gc.Info.cellCountX, gc.Info.cellCountY, gc.Info.cellCountZ, dtype=np.float64, nu=gc.Info.nu,
+                cs=0.17, flags=gc.get_flags(), bcvals=cases.channel_given_values(0.04), sub=sub)
+    of = O.Grid(gf.Info.cellCountX, gf.Info.cellCountY, gf.Info.cellCountZ, dtype=np.float64, nu=gf.Info.nu,
+                cs=0.17, flags=gf.get_flags(), bcvals=cases.channel_given_values(0.04))
+    gc.close()
+    gf.close()
+    oc.fill_equilibrium()
+    of.fill_equilibrium()
+    first, last, every = WINDOW
+    samples = []
+    for s in range(1, last + 1):
+        oc.step(periodic_y=True)
+        of.step()
+        of.step()
+        O.coarse_fine_comm(oc, of)
+        if s >= first and (s - first) % every == 0:
+            samples.append(float(of.momentum_exchange(lo, hi)[0]))
+    return lo, hi, float(np.mean(samples))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_two_level_sphere_drag_vs_fp64(psl, fp64_two_level_drag, storage):
+    P = psl
+    lo, hi, mean64 = fp64_two_level_drag
+    (gc, gf), _ = _scaled_two_level(P, storage)
+    first, last, every = WINDOW
+    print(f"[two-level-drag-{storage}] coarse {TWO_LEVEL_SCALED['shape']} sub {TWO_LEVEL_SCALED['sub']} "
+          f"steps={last}", flush=True)
+    P.updateGrid([gc, gf], first)
+    samples = [float(gf.momentum_exchange(lo, hi)[0])]
+    while len(samples) < (last - first) // every + 1:
+        P.updateGrid([gc, gf], every)
+        samples.append(float(gf.momentum_exchange(lo, hi)[0]))
+    mean = float(np.mean(samples))
+    rel = abs(mean / mean64 - 1)
+    print(f"  mean Fx {mean:.6e} FP64 {mean64:.6e} rel {rel:.2e}", flush=True)
+    gc.close()
+    gf.close()
+    assert rel < 0.01
