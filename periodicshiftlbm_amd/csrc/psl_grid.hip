@@ -11,9 +11,12 @@
 //
 // Kernels of one step:
 //   k_ps_bulk      fluid + bounceback cells, one thread per cell, the hot path;
-//   k_ps_boundary  box-face boundary cells from a compact list (restore +
-//                  MBBC + collision); the cell sets are disjoint, so the two
-//                  launches together equal the reference's single update;
+//                  it also stages the box-face cells' loaded f (see Stage);
+//   k_ps_boundary_staged  box-face boundary cells from a compact list
+//                  (restore + MBBC + collision) on the staged f; the cell
+//                  sets are disjoint, so the two launches together equal the
+//                  reference's single update (k_ps_boundary: the same from
+//                  the slots, next to the learned-closure bulk kernel);
 //   k_ps_fused     both in one launch (a tuning variant);
 //   k_periodic     applyBCPeriodicX/Y surface copies (before the update).
 #include "psl_internal.h"
@@ -53,9 +56,17 @@ __global__ __launch_bounds__(256) void k_ps_boundary(StepParams p, BoundaryParam
   boundary_update<St>(p, b, blockIdx.x * 256u + threadIdx.x);
 }
 
-template <class St, bool FLAGS, bool LES, int TUNE>
-__global__ __launch_bounds__(256) void k_ps_bulk(StepParams p) {
-  bulk_update<St, FLAGS, LES, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x);
+// The listed cells from the records k_ps_bulk<..., STAGE = true> wrote.
+template <class St>
+__global__ __launch_bounds__(64) void k_ps_boundary_staged(StepParams p, BoundaryParams b, const float* stage) {
+  boundary_update_staged<St>(p, b, stage, blockIdx.x * 64u + threadIdx.x);
+}
+
+// (held at the plain kernel's 7 waves per SIMD: without the bound the staging
+// branch raises the register count to 5 waves)
+template <class St, bool FLAGS, bool LES, int TUNE, bool STAGE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_ps_bulk(StepParams p, Stage sg) {
+  bulk_update<St, FLAGS, LES, TUNE, STAGE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, sg);
 }
 
 // One workgroup per entry of the fused order (fused_order): boundary-list
@@ -400,9 +411,13 @@ int grid_rebuild_flags(psl_grid* g) {
     if (g->flags) PSL_HIP_TRY(hipFree(g->flags));
     if (g->bc_list) PSL_HIP_TRY(hipFree(g->bc_list));
     if (g->fused_dev) PSL_HIP_TRY(hipFree(g->fused_dev));
+    if (g->stage_buf) PSL_HIP_TRY(hipFree(g->stage_buf));
+    if (g->stage_base) PSL_HIP_TRY(hipFree(g->stage_base));
     g->flags = nullptr;
     g->bc_list = nullptr;
     g->fused_dev = nullptr;
+    g->stage_buf = nullptr;
+    g->stage_base = nullptr;
     g->fused_host.clear();
     g->n_bc = 0;
     return PSL_OK;
@@ -440,6 +455,23 @@ int grid_rebuild_flags(psl_grid* g) {
   }
   if (g->fused_dev) PSL_HIP_TRY(hipFree(g->fused_dev));
   g->fused_dev = nullptr;
+  if (g->stage_buf) PSL_HIP_TRY(hipFree(g->stage_buf));
+  if (g->stage_base) PSL_HIP_TRY(hipFree(g->stage_base));
+  g->stage_buf = nullptr;
+  g->stage_base = nullptr;
+  if (!list.empty()) {
+    // staging records and the per-wave count of listed cells below each wave
+    const uint32_t nw = (n + 63) / 64;
+    std::vector<uint32_t> base(nw);
+    size_t at = 0;
+    for (uint32_t w = 0; w < nw; ++w) {
+      while (at < list.size() && list[at] < 64u * w) ++at;
+      base[w] = (uint32_t)at;
+    }
+    PSL_HIP_TRY(hipMalloc(&g->stage_buf, list.size() * STAGE_STRIDE * sizeof(float)));
+    PSL_HIP_TRY(hipMalloc(&g->stage_base, nw * sizeof(uint32_t)));
+    PSL_HIP_TRY(hipMemcpy(g->stage_base, base.data(), nw * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
   g->fused_host = fused_order(list, n);
   if (!g->fused_host.empty()) {
     const size_t bytes = g->fused_host.size() * sizeof(FusedEntry);
@@ -458,13 +490,16 @@ static void drop_profile_events(psl_grid* g) {
 }
 
 template <class St, int TUNE>
-static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, const StepParams& p) {
-  if (flags) {
-    if (les) k_ps_bulk<St, true, true, TUNE><<<nb, 256, 0, stream>>>(p);
-    else k_ps_bulk<St, true, false, TUNE><<<nb, 256, 0, stream>>>(p);
+static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, const StepParams& p, const Stage& sg) {
+  if (sg.buf) {
+    if (les) k_ps_bulk<St, true, true, TUNE, true><<<nb, 256, 0, stream>>>(p, sg);
+    else k_ps_bulk<St, true, false, TUNE, true><<<nb, 256, 0, stream>>>(p, sg);
+  } else if (flags) {
+    if (les) k_ps_bulk<St, true, true, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
+    else k_ps_bulk<St, true, false, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
   } else {
-    if (les) k_ps_bulk<St, false, true, TUNE><<<nb, 256, 0, stream>>>(p);
-    else k_ps_bulk<St, false, false, TUNE><<<nb, 256, 0, stream>>>(p);
+    if (les) k_ps_bulk<St, false, true, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
+    else k_ps_bulk<St, false, false, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
   }
 }
 
@@ -538,11 +573,13 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
     }
   } else {
     const bool fl = g->flags != nullptr;
+    // the listed cells are staged by the bulk kernel whenever there are any
+    const Stage sg{b.n ? g->stage_buf : nullptr, g->stage_base};
     switch (nt) {
-      case 0: launch_bulk<St, 0>(fl, les, nb, g->stream, p); break;
-      case 1: launch_bulk<St, 1>(fl, les, nb, g->stream, p); break;
-      case 2: launch_bulk<St, 2>(fl, les, nb, g->stream, p); break;
-      default: launch_bulk<St, 3>(fl, les, nb, g->stream, p); break;
+      case 0: launch_bulk<St, 0>(fl, les, nb, g->stream, p, sg); break;
+      case 1: launch_bulk<St, 1>(fl, les, nb, g->stream, p, sg); break;
+      case 2: launch_bulk<St, 2>(fl, les, nb, g->stream, p, sg); break;
+      default: launch_bulk<St, 3>(fl, les, nb, g->stream, p, sg); break;
     }
   }
   PSL_LAUNCH_CHECK();
@@ -551,7 +588,8 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
     g->prof_events.emplace_back(ev0, ev1);
   }
   if (b.n && !fused && !dominant_only) {
-    k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->stream>>>(p, b);
+    if (net) k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->stream>>>(p, b);
+    else k_ps_boundary_staged<St><<<(b.n + 63) / 64, 64, 0, g->stream>>>(p, b, g->stage_buf);
     PSL_LAUNCH_CHECK();
   }
   return PSL_OK;
@@ -699,6 +737,8 @@ int psl_grid_destroy(psl_grid* g) {
   if (g->flags) (void)hipFree(g->flags);
   if (g->bc_list) (void)hipFree(g->bc_list);
   if (g->fused_dev) (void)hipFree(g->fused_dev);
+  if (g->stage_buf) (void)hipFree(g->stage_buf);
+  if (g->stage_base) (void)hipFree(g->stage_base);
   if (g->cs_cell) (void)hipFree(g->cs_cell);
   if (g->partials) (void)hipFree(g->partials);
   if (g->result) (void)hipFree(g->result);

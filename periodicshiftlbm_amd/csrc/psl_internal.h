@@ -107,6 +107,8 @@ struct psl_grid {
   uint32_t n_bc = 0;
   std::vector<psl::FusedEntry> fused_host;  // fused launch order (empty: no list)
   psl::FusedEntry* fused_dev = nullptr;     // device copy
+  float* stage_buf = nullptr;       // staging records of the listed cells (28 floats each)
+  uint32_t* stage_base = nullptr;   // listed cells below cell 64 w, per wave w
   psl::BcTable bcvals{};
   float4* bc_dev = nullptr;       // device copy of the given-value table (4 x float4)
   float cs = 0.f;

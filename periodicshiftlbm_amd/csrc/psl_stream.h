@@ -133,25 +133,34 @@ struct BoundaryParams {
   const float4* bc;      // 4 given-value slots (rho, ux, uy, uz)
 };
 
+// Staging of box-face cells.  The bulk kernel already reads the lines that
+// hold a box-face cell's 27 slots (its row neighbours share them), so with
+// staging it also loads the listed cells and writes their full f as one
+// contiguous record of STAGE_STRIDE floats at the cell's list index (the
+// loaded values as the bulk path holds them: f, or f - w for the reduced
+// storages, to which the boundary kernel adds w exactly as when it loads the
+// slots itself, so both paths compute identical results).  The
+// boundary kernel then reads the records instead of 27 strided slots, whose
+// lines it would otherwise fetch again for 2-4 useful bytes each.  base[w] is
+// the number of listed cells below cell 64 w.
+constexpr int STAGE_STRIDE = 28;
+struct Stage {
+  float* buf;
+  const uint32_t* base;
+};
+
+// Restore + MBBC + collision of listed cell `cell` (flags fl) whose full f is
+// in v, and the in-place store (include/applyLocalCellUpdate.h:46-71).
 template <class St>
-__device__ __forceinline__ void boundary_update(const StepParams& p, const BoundaryParams& b, uint32_t t) {
+__device__ __forceinline__ void boundary_finish(const StepParams& p, const BoundaryParams& b, uint32_t cell, int fl,
+                                                float (&v)[27]) {
   typedef typename St::T T;
-  if (t >= b.n) return;
-  const uint32_t cell = ld<uint32_t, false>(b.list + t);
-  const int fl = ld<uint8_t, false>(p.flags + cell);
   const int kind = fl & 7;
   const uint32_t nxy = (uint32_t)b.nx * (uint32_t)b.ny;
   const int k = (int)(cell / nxy);
   const uint32_t rem = cell - (uint32_t)k * nxy;
   const int j = (int)(rem / (uint32_t)b.nx);
   const int i = (int)(rem - (uint32_t)j * (uint32_t)b.nx);
-  float v[27];
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-    v[q] = St::load(ld<T, false>(base + cell));
-    if (St::W_OFF) v[q] += weight(q);
-  }
   const int code = outer_normal_code(i, j, k, b.nx, b.ny, b.nz, fl);
   // given (rho, u) of the cell's slot: one indexed load from the device table
   // (a select chain over the four slots was miscompiled for divergent slots,
@@ -170,8 +179,51 @@ __device__ __forceinline__ void boundary_update(const StepParams& p, const Bound
   }
 }
 
-template <class St, bool FLAGS, bool LES, int TUNE>
-__device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block) {
+// Listed cell t, loaded from its 27 slots.
+template <class St>
+__device__ __forceinline__ void boundary_update(const StepParams& p, const BoundaryParams& b, uint32_t t) {
+  typedef typename St::T T;
+  if (t >= b.n) return;
+  const uint32_t cell = ld<uint32_t, false>(b.list + t);
+  const int fl = ld<uint8_t, false>(p.flags + cell);
+  float v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+    v[q] = St::load(ld<T, false>(base + cell));
+    if (St::W_OFF) v[q] += weight(q);
+  }
+  boundary_finish<St>(p, b, cell, fl, v);
+}
+
+// Listed cell t, loaded from its staged record.
+template <class St>
+__device__ __forceinline__ void boundary_update_staged(const StepParams& p, const BoundaryParams& b,
+                                                       const float* stage, uint32_t t) {
+  if (t >= b.n) return;
+  const uint32_t cell = ld<uint32_t, false>(b.list + t);
+  const int fl = ld<uint8_t, false>(p.flags + cell);
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const PSL_GLOBAL f4* r = (const PSL_GLOBAL f4*)(stage + (size_t)t * STAGE_STRIDE);
+  float v[27];
+#pragma unroll
+  for (int i = 0; i < STAGE_STRIDE / 4; ++i) {
+    const f4 x = r[i];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (4 * i + c < 27) v[4 * i + c] = x[c];
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q)
+    if (St::W_OFF) v[q] += weight(q);
+  boundary_finish<St>(p, b, cell, fl, v);
+}
+
+// Fluid and bounceback cells of one block in cell order; with STAGE the
+// listed box-face cells of the block are loaded too and written to their
+// staging records (their update is the boundary kernel's).
+template <class St, bool FLAGS, bool LES, int TUNE, bool STAGE = false>
+__device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block, const Stage& sg = Stage{}) {
   const uint32_t block_first = block * 256u;
   const uint32_t cell = block_first + threadIdx.x;
   const uint32_t w_first = wave_first_cell(block_first);
@@ -181,9 +233,23 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block)
   if (cell >= p.n) return;
   int kind = KIND_FLUID;
   if (FLAGS) kind = ld<uint8_t, false>(p.flags + cell) & 7;
-  if (kind > KIND_BOUNCEBACK) return;
+  if (kind > (STAGE ? KIND_GIVEN_RHOU : KIND_BOUNCEBACK)) return;
   float v[27];
   load27<St, TUNE>(p, w_first, straddle, cell, v);
+  if (STAGE) {
+    const bool listed = kind >= KIND_GIVEN_RHO;
+    const uint64_t m = __ballot(listed);
+    if (m) {
+      if (listed) {
+        const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        const uint32_t t = ld<uint32_t, false>(sg.base + (w_first >> 6)) + rank;
+        PSL_GLOBAL float* r = (PSL_GLOBAL float*)(sg.buf + (size_t)t * STAGE_STRIDE);
+#pragma unroll
+        for (int q = 0; q < 27; ++q) r[q] = v[q];
+        return;
+      }
+    }
+  }
   if (kind == KIND_BOUNCEBACK) {
     // the swap commutes with "f - w" (opposite directions share a weight)
     bounceback(v);
