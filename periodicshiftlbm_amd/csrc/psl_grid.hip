@@ -69,8 +69,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
   bulk_update<St, FLAGS, LES, TUNE, STAGE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, sg);
 }
 
-// One workgroup per entry of the fused order (fused_order): boundary-list
-// ranges interleaved with the bulk blocks they border.
+// Bulk cells with the learned closure in place of the Smagorinsky rate.
 template <class St, bool FLAGS, int TUNE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ps_bulk_closure(
     StepParams p, const MlpNet* __restrict__ net, float nu) {
@@ -78,6 +77,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   bulk_update_closure<St, FLAGS, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, net, nu, tt);
 }
 
+// One workgroup per entry of the fused order (fused_order): boundary-list
+// ranges interleaved with the bulk blocks they border.
 template <class St, bool LES, int TUNE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_ps_fused(
     StepParams p, BoundaryParams b, const FusedEntry* __restrict__ order) {
