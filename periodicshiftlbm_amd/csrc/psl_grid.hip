@@ -103,14 +103,16 @@ __constant__ int c_neg_y[9] = {5, 11, 14, 16, 18, 20, 21, 23, 25};
 template <class T>
 __global__ __launch_bounds__(256) void k_periodic(T* __restrict__ f, Shift27 sh, uint32_t n, int nx, int ny, int nz,
                                                   int axis) {
-  // one thread per (surface cell, copied population): 18 independent copies
+  // one thread per (surface cell, copied population): 18 independent copies;
+  // the surface coordinate runs fastest, so for the y copies (rows along x) a
+  // wave moves one contiguous run of each population
   const int na = axis == 0 ? ny : nx;
   const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   if (t >= (uint64_t)na * (uint64_t)nz * 18u) return;
-  const int c = (int)(t % 18u);
-  const uint64_t rest = t / 18u;
-  const int a = (int)(rest % (uint64_t)na);
-  const int k = (int)(rest / (uint64_t)na);
+  const int a = (int)(t % (uint64_t)na);
+  const uint64_t rest = t / (uint64_t)na;
+  const int c = (int)(rest % 18u);
+  const int k = (int)(rest / 18u);
   const uint32_t plane = (uint32_t)k * (uint32_t)nx * (uint32_t)ny;
   uint32_t left, right;
   if (axis == 0) {
