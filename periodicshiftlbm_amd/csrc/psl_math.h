@@ -93,18 +93,9 @@ struct Moments {
   float m000, m100, m010, m001, m200, m020, m002, m110, m101, m011;
 };
 
-PSL_HD PSL_INLINE Moments raw_moments(const float (&v)[27]) {
-  // z pass: per (x,y) line: zeroth, first, second moment in z
-  float z0[3][3], z1[3][3], z2[3][3];
-#pragma unroll
-  for (int x = 0; x < 3; ++x)
-#pragma unroll
-    for (int y = 0; y < 3; ++y) {
-      const float a = v[LINE[x][y][0]], b = v[LINE[x][y][1]], c = v[LINE[x][y][2]];
-      z2[x][y] = a + c;
-      z0[x][y] = z2[x][y] + b;
-      z1[x][y] = c - a;
-    }
+// y and x passes of raw_moments, from the z pass's zeroth, first and second
+// moments of each (x, y) line.
+PSL_HD PSL_INLINE Moments moments_from_z(const float (&z0)[3][3], const float (&z1)[3][3], const float (&z2)[3][3]) {
   // y pass: (y-order, z-order) with total <= 2 per x plane
   float y00[3], y10[3], y20[3], y01[3], y11[3], y02[3];
 #pragma unroll
@@ -128,6 +119,21 @@ PSL_HD PSL_INLINE Moments raw_moments(const float (&v)[27]) {
   m.m002 = (y02[0] + y02[2]) + y02[1];
   m.m011 = (y11[0] + y11[2]) + y11[1];
   return m;
+}
+
+PSL_HD PSL_INLINE Moments raw_moments(const float (&v)[27]) {
+  // z pass: per (x,y) line: zeroth, first, second moment in z
+  float z0[3][3], z1[3][3], z2[3][3];
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = 0; y < 3; ++y) {
+      const float a = v[LINE[x][y][0]], b = v[LINE[x][y][1]], c = v[LINE[x][y][2]];
+      z2[x][y] = a + c;
+      z0[x][y] = z2[x][y] + b;
+      z1[x][y] = c - a;
+    }
+  return moments_from_z(z0, z1, z2);
 }
 
 // Backward chimera step along one axis for a line whose central moments

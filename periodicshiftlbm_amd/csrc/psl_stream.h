@@ -4,6 +4,8 @@
 
 #include "psl_internal.h"
 
+#include <type_traits>
+
 // FMA formation fixed by the source expressions (see psl_math.h).
 #pragma clang fp contract(on)
 
@@ -93,23 +95,131 @@ __device__ __forceinline__ void load27(const Tab& p, uint32_t w_first, bool stra
   }
 }
 
+// A storage word as loaded: 16-bit words zero-extended to 32 bits by the load
+// itself (so no masking is needed where only the low half is read).
+template <class St>
+using Word = typename std::conditional<sizeof(typename St::T) == 2, uint32_t, typename St::T>::type;
+
+// The 27 storage words of a cell, unconverted.
+template <class St, int TUNE, class Tab>
+__device__ __forceinline__ void load27_raw(const Tab& p, uint32_t w_first, bool straddle, uint32_t cell,
+                                           Word<St> (&r)[27]) {
+  typedef typename St::T T;
+  constexpr bool NT = (TUNE & TUNE_NT_LOAD) != 0;
+  if (!straddle) {
+    const uint32_t off = cell * (uint32_t)sizeof(T);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const char* base = (const char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
+      r[q] = (Word<St>)ld<T, NT>((const T*)(base + off));
+    }
+    // keeps the loads in this branch: merged with the per-lane branch they
+    // would all take 64-bit per-lane addresses
+    __asm__ volatile("" ::: "memory");
+    if constexpr (sizeof(T) == 2) {
+      // the words leave the branch as 32-bit values (the load's zero
+      // extension); otherwise the merge re-masks every 16-bit word
+#pragma unroll
+      for (int q = 0; q < 27; ++q) __asm__("" : "+v"(r[q]));
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+      r[q] = (Word<St>)ld<T, NT>(base + cell);
+    }
+    __asm__ volatile("" ::: "memory");
+    if constexpr (sizeof(T) == 2) {
+#pragma unroll
+      for (int q = 0; q < 27; ++q) __asm__("" : "+v"(r[q]));
+    }
+  }
+}
+
+template <class St>
+__device__ __forceinline__ void words_to_float(const Word<St> (&r)[27], float (&v)[27]) {
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v[q] = St::load((typename St::T)r[q]);
+}
+
+// v_fma_mix_f32 with f16 operands (the low 16 bits of their registers; the
+// high bits are not read) and a +-1 multiplier: one
+// rounding of the exact sum, the same value as adding the converted operands,
+// without the conversions.
+__device__ __forceinline__ float mix_add_hh(uint32_t a, uint32_t c) {  // a + c
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(c));
+  return r;
+}
+__device__ __forceinline__ float mix_sub_hh(uint32_t c, uint32_t a) {  // c - a
+  float r;
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(c));
+  return r;
+}
+__device__ __forceinline__ float mix_add_hf(uint32_t b, float z) {  // b + z
+  float r;
+  asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(b), "v"(z));
+  return r;
+}
+
+// raw_moments of the converted storage words; for FP16 storage the z pass
+// reads the f16 words directly (bit-identical to converting first).
+template <class St>
+__device__ __forceinline__ Moments raw_moments_words(const Word<St> (&r)[27]) {
+  if constexpr (std::is_same<St, StF16>::value) {
+    float z0[3][3], z1[3][3], z2[3][3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+      for (int y = 0; y < 3; ++y) {
+        const uint32_t a = r[LINE[x][y][0]], b = r[LINE[x][y][1]], c = r[LINE[x][y][2]];
+        z2[x][y] = mix_add_hh(a, c);
+        z0[x][y] = mix_add_hf(b, z2[x][y]);
+        z1[x][y] = mix_sub_hh(c, a);
+      }
+    return moments_from_z(z0, z1, z2);
+  } else {
+    float v[27];
+    words_to_float<St>(r, v);
+    return raw_moments(v);
+  }
+}
+
 template <class St, int TUNE, class Tab>
 __device__ __forceinline__ void store27(const Tab& p, uint32_t w_first, bool straddle, uint32_t cell,
                                         const float (&v)[27]) {
   typedef typename St::T T;
   constexpr bool NT = (TUNE & TUNE_NT_STORE) != 0;
+  // storage words; FP16 converts pairs with one v_cvt_pk_f16_f32 (round to
+  // nearest even, as the single conversion)
+  T w[27];
+  if constexpr (std::is_same<St, StF16>::value) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    typedef float f2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int q = 0; q + 1 < 27; q += 2) {
+      const f2 x = {v[q], v[q + 1]};
+      const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, h2));
+      w[q] = (T)u;
+      w[q + 1] = (T)(u >> 16);
+    }
+    w[26] = St::store(v[26]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) w[q] = St::store(v[q]);
+  }
   if (!straddle) {
     const uint32_t off = cell * (uint32_t)sizeof(T);
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       char* base = (char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      st<T, NT>((T*)(base + off), St::store(v[q]));
+      st<T, NT>((T*)(base + off), w[q]);
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      st<T, NT>(base + cell, St::store(v[q]));
+      st<T, NT>(base + cell, w[q]);
     }
   }
 }
@@ -234,13 +344,15 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
   int kind = KIND_FLUID;
   if (FLAGS) kind = ld<uint8_t, false>(p.flags + cell) & 7;
   if (kind > (STAGE ? KIND_GIVEN_RHOU : KIND_BOUNCEBACK)) return;
+  Word<St> r[27];
+  load27_raw<St, TUNE>(p, w_first, straddle, cell, r);
   float v[27];
-  load27<St, TUNE>(p, w_first, straddle, cell, v);
   if (STAGE) {
     const bool listed = kind >= KIND_GIVEN_RHO;
     const uint64_t m = __ballot(listed);
     if (m) {
       if (listed) {
+        words_to_float<St>(r, v);
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         const uint32_t t = ld<uint32_t, false>(sg.base + (w_first >> 6)) + rank;
         // seven 16-byte stores per record (one lane of the wave is active)
@@ -259,12 +371,14 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
   }
   if (kind == KIND_BOUNCEBACK) {
     // the swap commutes with "f - w" (opposite directions share a weight)
+    words_to_float<St>(r, v);
     bounceback(v);
   } else {
     Relax rx{p.omega, p.tau, p.cs};
     float cs = p.cs;
     if (LES && p.cs_cell) cs = ld<float, false>(p.cs_cell + cell);
-    collide<St::W_OFF>(v, rx, LES ? cs : 0.f);
+    // the collision reads only the moments; v receives the post-collision values
+    collide_with_moments<St::W_OFF>(v, raw_moments_words<St>(r), rx, LES ? cs : 0.f);
   }
   store27<St, TUNE>(p, w_first, straddle, cell, v);
 }
