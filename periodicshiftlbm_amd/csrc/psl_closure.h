@@ -125,9 +125,9 @@ __device__ __forceinline__ void tanh_table_fill(TanhTable& t) {
 
 __device__ __forceinline__ float table_tanh(const TanhTable& t, float x) {
   const float u = (fminf(fmaxf(x, -TANH_X), TANH_X) + TANH_X) * (TANH_N / (2.f * TANH_X));
-  const float fl = floorf(u);
-  const float2 e = t.e[(int)fl];
-  return e.x + (u - fl) * e.y;
+  // u >= 0: truncation is the floor, and fract(u) = u - floor(u)
+  const float2 e = t.e[(int)u];
+  return e.x + __builtin_amdgcn_fractf(u) * e.y;
 }
 
 __device__ __forceinline__ bf16x8 load_a(const uint16_t (&a)[8]) {
@@ -160,8 +160,10 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
   for (int nb = 0; nb < 2; ++nb) {
     // the per-lane bias / output-weight rows are re-read for each column block
     // (L1 hits) rather than held in 48 registers across both
-    const MlpNet* nt = net;
-    asm volatile("" : "+s"(nt));
+    const MlpNet* ntp = net;
+    asm volatile("" : "+s"(ntp));
+    // the laundered pointer is global memory (otherwise flat loads)
+    const PSL_GLOBAL MlpNet* nt = (const PSL_GLOBAL MlpNet*)ntp;
     bf16x8 B;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -213,19 +215,25 @@ __device__ __forceinline__ void bulk_update_closure(const StepParams& p, uint32_
   int kind = valid ? KIND_FLUID : KIND_GHOST;
   if (FLAGS && valid) kind = ld<uint8_t, false>(p.flags + cell) & 7;
   const bool active = kind <= KIND_BOUNCEBACK;
-  float v[27];
+  Word<St> r[27];
   if (active) {
-    load27<St, TUNE>(p, w_first, straddle, cell, v);
+    load27_raw<St, TUNE>(p, w_first, straddle, cell, r);
   } else {
+    // rest state: f = w, stored as 0 in the reduced storages
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = St::W_OFF ? 0.f : weight(q);
+    for (int q = 0; q < 27; ++q) {
+      if constexpr (St::W_OFF) r[q] = 0;
+      else r[q] = weight(q);
+    }
   }
-  const Moments m = raw_moments(v);
+  const Moments m = raw_moments_words<St>(r);
   float raw[MLP_IN];
   cell_closure_features<St::W_OFF>(m, raw);
   const float y = closure_wave(net, tt, raw);
   if (!active) return;
+  float v[27];
   if (kind == KIND_BOUNCEBACK) {
+    words_to_float<St>(r, v);
     bounceback(v);
   } else {
     const float tau = 3.f * (nu + MLP_NU_SCALE * fmaxf(y, 0.f)) + 0.5f;
