@@ -160,12 +160,18 @@ def large_box_leg(P, cases, n: int, steps: int):
     return out
 
 
-def two_level_leg(P, cases, steps: int, storage: str = "f16"):
+def two_level_leg(P, cases, steps: int, storage: str = "f16", autotune: bool = True):
     """The sphere at Re = 1e5 on two levels (coarse 256x128x128 + fine 256x192x192
     around the sphere); MLUPS counts the updated cells of each level times its
     steps per coarse step (sum_l N_l 2^l without the skipped covered coarse
     cells and fine ghost cells)."""
     (gc, gf), (lo, hi) = cases.two_level_sphere(storage)
+    tuning = {}
+    if autotune:
+        # each level's local-update variant, then both restart from rest
+        for name, g in (("coarse", gc), ("fine", gf)):
+            tuning[name] = g.autotune(10)[0]
+            P.fill_equilibrium(g)
     P.updateGrid([gc, gf], 5)
     P.synchronize()
     t0 = time.perf_counter()
@@ -178,7 +184,7 @@ def two_level_leg(P, cases, steps: int, storage: str = "f16"):
     force = gf.momentum_exchange(lo, hi)
     out = {"mlups": updates * steps / el / 1e6, "ms_per_coarse_step": el * 1e3 / steps, "coarse_steps": steps,
            "coarse_cells": gc.n, "covered_coarse_cells": int(skipped), "fine_cells": gf.n,
-           "fine_ghost_cells": int(sf), "storage": storage,
+           "fine_ghost_cells": int(sf), "storage": storage, "tuning": tuning,
            "fine_force_after_run": [float(x) for x in force]}
     gc.close()
     gf.close()
@@ -365,8 +371,8 @@ def main():
         out["variants"]["large_box"] = lb
     if args.two_level_steps:
         log(f"[bench] two-level sphere, {args.two_level_steps} coarse steps")
-        tl = two_level_leg(P, cases, args.two_level_steps)
-        log(f"  {tl['mlups']:.1f} MLUPS ({tl['ms_per_coarse_step']:.3f} ms per coarse step)")
+        tl = two_level_leg(P, cases, args.two_level_steps, autotune=not args.no_autotune)
+        log(f"  {tl['mlups']:.1f} MLUPS ({tl['ms_per_coarse_step']:.3f} ms per coarse step, variants {tl['tuning']})")
         out["variants"]["two_level"] = tl
     if args.closure_n:
         log(f"[bench] learned closure, {args.closure_n}^3 decaying turbulence")
