@@ -160,6 +160,36 @@ def large_box_leg(P, cases, n: int, steps: int):
     return out
 
 
+def esotwist_leg(P, n: int, steps: int):
+    """The esotwist update (DIAD grids, include/applyLocalCellUpdate.h:77-171) on a
+    dense, fully periodic n^3 box: every cell stored with its seven neighbour
+    indices (periodicity through the indices), uniform flow, no Smagorinsky.
+    Algorithmic bytes per cell: 27 reads + 27 writes of the storage and the
+    seven 4-byte neighbour indices."""
+    out = {}
+    ijk, nbr = P.DIADGridStruct.dense_box(n, n, n)
+    for storage in ("f32", "f16"):
+        info = P.InfoStruct(cellCountX=n, cellCountY=n, cellCountZ=n, nu=0.005)
+        g = P.DIADGridStruct(info, ijk, nbr, storage)
+        g.fill_equilibrium(1.0, 0.02, 0.0, 0.0)
+        for _ in range(5):
+            P.applyStreaming(g)
+            P.applyLocalCellUpdate(g)
+        P.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            P.applyStreaming(g)
+            P.applyLocalCellUpdate(g)
+        P.synchronize()
+        el = time.perf_counter() - t0
+        per_cell = 54 * (4 if storage == "f32" else 2) + 28
+        gbs = per_cell * g.n * steps / el / 1e9
+        out[storage] = {"cells": g.n, "mlups": g.n * steps / el / 1e6, "ms_per_step": el * 1e3 / steps,
+                        "bytes_per_cell": per_cell, "achieved_GBs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS}
+        g.close()
+    return out
+
+
 def two_level_leg(P, cases, steps: int, storage: str = "f16", autotune: bool = True):
     """The sphere at Re = 1e5 on two levels (coarse 256x128x128 + fine 256x192x192
     around the sphere); MLUPS counts the updated cells of each level times its
@@ -369,6 +399,10 @@ def main():
                              f"({v['frac_of_pattern_limit'] * 100:.0f} % of the access pattern's "
                              f"{v['pattern_limit_GBs']:.0f} GB/s)" for k, v in lb.items()))
         out["variants"]["large_box"] = lb
+        log(f"[bench] esotwist (DIAD) dense periodic box {args.large_n}^3")
+        eso = esotwist_leg(P, args.large_n, 50)
+        log("  " + ", ".join(f"{k}: {v['mlups']:.0f} MLUPS, {v['achieved_GBs']:.0f} GB/s" for k, v in eso.items()))
+        out["variants"]["esotwist"] = eso
     if args.two_level_steps:
         log(f"[bench] two-level sphere, {args.two_level_steps} coarse steps")
         tl = two_level_leg(P, cases, args.two_level_steps, autotune=not args.no_autotune)
