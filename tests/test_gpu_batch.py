@@ -107,16 +107,18 @@ def test_batch_autotune_keeps_results(psl):
     b.close()
 
 
-def test_single_grid_variants_identical(psl):
-    """Every local-update variant of the single-grid path gives the same bits."""
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_single_grid_variants_identical(psl, storage):
+    """Every local-update variant of the single-grid path (staged or fused box
+    faces, plain or non-temporal accesses, either block order) gives the same bits."""
     P = psl
     d = _domains()[1]
-    ref = cases.domain_grid(d, "f32")[0]
+    ref = cases.domain_grid(d, storage)[0]
     ref.tuning = 0
     P.updateGrid([ref], 9)
     f0 = ref.get_f()
     for v in range(1, P.TUNE_VARIANTS):
-        g = cases.domain_grid(d, "f32")[0]
+        g = cases.domain_grid(d, storage)[0]
         g.tuning = v
         P.updateGrid([g], 9)
         nbad = int(np.count_nonzero(g.get_f() != f0))
