@@ -357,13 +357,13 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
         const uint32_t t = ld<uint32_t, false>(sg.base + (w_first >> 6)) + rank;
         // seven 16-byte stores per record (one lane of the wave is active)
         typedef float f4 __attribute__((ext_vector_type(4)));
-        PSL_GLOBAL f4* r = (PSL_GLOBAL f4*)(sg.buf + (size_t)t * STAGE_STRIDE);
+        PSL_GLOBAL f4* rec = (PSL_GLOBAL f4*)(sg.buf + (size_t)t * STAGE_STRIDE);
 #pragma unroll
         for (int i = 0; i < STAGE_STRIDE / 4; ++i) {
           f4 x;
 #pragma unroll
           for (int c = 0; c < 4; ++c) x[c] = 4 * i + c < 27 ? v[4 * i + c] : 0.f;
-          r[i] = x;
+          rec[i] = x;
         }
         return;
       }
