@@ -74,27 +74,6 @@ __device__ __forceinline__ void st(T* p, T v) {
   else *g = v;
 }
 
-template <class St, int TUNE, class Tab>
-__device__ __forceinline__ void load27(const Tab& p, uint32_t w_first, bool straddle, uint32_t cell,
-                                       float (&v)[27]) {
-  typedef typename St::T T;
-  constexpr bool NT = (TUNE & TUNE_NT_LOAD) != 0;
-  if (!straddle) {
-    const uint32_t off = cell * (uint32_t)sizeof(T);
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const char* base = (const char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      v[q] = St::load(ld<T, NT>((const T*)(base + off)));
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-      const T* base = (const T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      v[q] = St::load(ld<T, NT>(base + cell));
-    }
-  }
-}
-
 // A storage word as loaded: 16-bit words zero-extended to 32 bits by the load
 // itself (so no masking is needed where only the low half is read).
 template <class St>
@@ -143,9 +122,8 @@ __device__ __forceinline__ void words_to_float(const Word<St> (&r)[27], float (&
 }
 
 // v_fma_mix_f32 with f16 operands (the low 16 bits of their registers; the
-// high bits are not read) and a +-1 multiplier: one
-// rounding of the exact sum, the same value as adding the converted operands,
-// without the conversions.
+// high bits are not read) and a +-1 multiplier: one rounding of the exact sum,
+// the same value as adding the converted operands, without the conversions.
 __device__ __forceinline__ float mix_add_hh(uint32_t a, uint32_t c) {  // a + c
   float r;
   asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(a), "v"(c));
