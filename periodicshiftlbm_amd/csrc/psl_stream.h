@@ -168,36 +168,37 @@ __device__ __forceinline__ void store27(const Tab& p, uint32_t w_first, bool str
                                         const float (&v)[27]) {
   typedef typename St::T T;
   constexpr bool NT = (TUNE & TUNE_NT_STORE) != 0;
-  // storage words; FP16 converts pairs with one v_cvt_pk_f16_f32 (round to
-  // nearest even, as the single conversion)
-  T w[27];
+  // Storage words in 32-bit registers: for FP16, pairs (q, q + 1) converted by
+  // one v_cvt_pk_f16_f32 (round to nearest even, as the single conversion);
+  // word(q) takes the pair's half inside each store loop, so that the shift
+  // of an odd q has one use and folds into global_store_short_d16_hi.
+  uint32_t pk[27];
   if constexpr (std::is_same<St, StF16>::value) {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
     typedef float f2 __attribute__((ext_vector_type(2)));
 #pragma unroll
     for (int q = 0; q + 1 < 27; q += 2) {
       const f2 x = {v[q], v[q + 1]};
-      const uint32_t u = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, h2));
-      w[q] = (T)u;
-      w[q + 1] = (T)(u >> 16);
+      pk[q] = __builtin_bit_cast(uint32_t, __builtin_convertvector(x, h2));
     }
-    w[26] = St::store(v[26]);
-  } else {
-#pragma unroll
-    for (int q = 0; q < 27; ++q) w[q] = St::store(v[q]);
+    pk[26] = St::store(v[26]);
   }
+  auto word = [&](int q) -> T {
+    if constexpr (std::is_same<St, StF16>::value) return (q & 1) ? (T)(pk[q - 1] >> 16) : (T)pk[q];
+    else return St::store(v[q]);
+  };
   if (!straddle) {
     const uint32_t off = cell * (uint32_t)sizeof(T);
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       char* base = (char*)(w_first >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      st<T, NT>((T*)(base + off), w[q]);
+      st<T, NT>((T*)(base + off), word(q));
     }
   } else {
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
       T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-      st<T, NT>(base + cell, w[q]);
+      st<T, NT>(base + cell, word(q));
     }
   }
 }
