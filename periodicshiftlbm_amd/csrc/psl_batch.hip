@@ -57,7 +57,7 @@ template <class St, int TUNE>
 __global__ __launch_bounds__(256) void k_batch_bulk(const StepParams* __restrict__ params,
                                                     const uint32_t* __restrict__ block_domain,
                                                     const uint32_t* __restrict__ block_first, uint32_t remap) {
-  const uint32_t b = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t b = remap ? xcd_block_chunked<64>(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint32_t d = block_domain[b];
   bulk_update<St, true, true, TUNE>(params[d], b - block_first[d]);
 }
@@ -85,7 +85,7 @@ template <class St, int TUNE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_fused(
     const StepParams* __restrict__ params, const BoundaryParams* __restrict__ bparams,
     const BatchEntry* __restrict__ order, uint32_t remap) {
-  const uint32_t w = remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t w = remap ? xcd_block_chunked<64>(blockIdx.x, gridDim.x) : blockIdx.x;
   const BatchEntry x = order[w];
   if (x.e.b != FUSED_BULK) {
     BoundaryParams b = bparams[x.d];

@@ -213,6 +213,20 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return xcd * per + min(xcd, rem) + idx;
 }
 
+// The same in chunks of K consecutive blocks dealt round-robin to the XCDs
+// (blocks past the last whole round of 8 K keep their order): neighbouring
+// blocks still share one L2, and every XCD gets a slice of every part of the
+// launch.  Used by the batch, whose grids differ in boundary work, so that
+// one contiguous eighth per XCD would load the XCDs unevenly.  A bijection
+// of [0, nb).
+template <uint32_t K>
+__device__ __forceinline__ uint32_t xcd_block_chunked(uint32_t b, uint32_t nb) {
+  const uint32_t full = nb / (8u * K) * (8u * K);
+  if (b >= full) return b;
+  const uint32_t x = b & 7u, i = b >> 3;
+  return (x + 8u * (i / K)) * K + (i % K);
+}
+
 // ------------------------------------------------------- per-cell updates
 // Box-face cells of one grid (include/applyLocalCellUpdate.h:46-67).
 struct BoundaryParams {
