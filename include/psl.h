@@ -282,6 +282,61 @@ int psl_diad_set_f(psl_diad* grid, const float* host);
 /* Cell tables of a dense nx*ny*nz box; periodic_mask bits 0/1/2 wrap x/y/z. */
 int psl_diad_dense_box(int nx, int ny, int nz, int periodic_mask, int* ijk3, int* nbr7);
 
+/* ------------------------------------------------ DIAD grid builder */
+/* buildDIADGrids (include/DIADFunctions.h:640-1046): automatic wall-refined
+ * levels on the device.  Level 0 starts as the whole base box; each coarser
+ * level keeps its fluid cells, their border and the coarse<->fine interface,
+ * and hands a region around every finest-resolution wall cell (widened by
+ * wall_refinement_span + levels - level - 1 layers and edited by the
+ * refinement rules) to the next level as 2x2x2 children.
+ *
+ * The drivers' getMarkers callbacks (evaluated by the reference on the device
+ * during the build) are given as an ordered rule list: a rule applies to the
+ * levels in level_mask (bit = gridID) and, where its shape contains the cell,
+ * sets the queried marker (`what`) to `value`; later rules win.  Positions are
+ * iCell * res + ox etc. in FP32.  Shapes (p: floats, idx: ints):
+ *   SPHERE_LE / SPHERE_LT: |x - p[0..2]|^2 <= / < p[3]^2
+ *   XCYLINDER: 0 < x - p[0] < p[4] and (y - p[1])^2 + (z - p[2])^2 < p[3]^2
+ *   BOX: p[0] < x < p[3], p[1] < y < p[4], p[2] < z < p[5]
+ *   INDEX_BOX: idx[0] <= i <= idx[1], idx[2] <= j <= idx[3], idx[4] <= k <= idx[5];
+ *              a negative bound counts from the level's box end (-1 = last). */
+enum { PSL_RULE_BOUNCEBACK = 0, PSL_RULE_REFINEMENT = 1, PSL_RULE_OUTLET = 2 };
+enum { PSL_SHAPE_SPHERE_LE = 0, PSL_SHAPE_SPHERE_LT = 1, PSL_SHAPE_XCYLINDER = 2, PSL_SHAPE_BOX = 3,
+       PSL_SHAPE_INDEX_BOX = 4 };
+typedef struct psl_marker_rule {
+  int what, value;
+  unsigned level_mask;
+  int shape;
+  float p[8];
+  int idx[6];
+} psl_marker_rule;
+/* Per-level arrays (counts in elements): */
+enum {
+  PSL_DIAD_IJK = 0,        /* int, 3 x n: i, j, k, cells sorted by (k, j, i) */
+  PSL_DIAD_BOUNCEBACK = 1, /* uint8, n: bouncebackMarkerArray */
+  PSL_DIAD_INTERFACE = 2,  /* uint8, n: interfaceMarkerArray (coarse levels) */
+  PSL_DIAD_NBR = 3,        /* int, 10 x n: EsotwistNbrArray +i +j +k +ij +ik +jk +ijk -i -j -k */
+  PSL_DIAD_C2F_READ = 4,   /* int: coarseToFineReadArray (coarse cells) */
+  PSL_DIAD_C2F_WRITE = 5,  /* int: coarseToFineWriteArray (fine cells) */
+  PSL_DIAD_F2C_WRITE = 6,  /* int: fineToCoarseWriteArray (coarse cells) */
+  PSL_DIAD_F2C_READ = 7,   /* int: fineToCoarseReadArray (fine cells) */
+  PSL_DIAD_OUTLET = 8,     /* int: outletCellArray */
+  PSL_DIAD_ARRAYS = 9
+};
+typedef struct psl_diad_levels psl_diad_levels;
+/* base: level-0 Info (res, ox, oy, oz, nu, dtPhys, cellCountX/Y/Z); finer levels
+ * get res/2, dtPhys/2, origin - res_fine/2, doubled counts and
+ * nu = dtPhys * nu_phys / (res / 1000)^2.  stls may be NULL when nstl = 0. */
+int psl_diad_build(const psl_info* base, int levels, int wall_refinement_span, float nu_phys,
+                   const psl_stl* const* stls, int nstl, const psl_marker_rule* rules, int nrules,
+                   psl_diad_levels** out);
+int psl_diad_levels_destroy(psl_diad_levels* levels);
+/* Level Info (cellCount = kept cells) and the element count of every array. */
+int psl_diad_levels_info(const psl_diad_levels* levels, int level, psl_info* info, int64_t counts[PSL_DIAD_ARRAYS]);
+int psl_diad_levels_get(const psl_diad_levels* levels, int level, int which, void* host);
+/* Build time of the last psl_diad_build (ms, device work and host sync included). */
+int psl_diad_levels_build_ms(const psl_diad_levels* levels, double* ms);
+
 /* ------------------------------------------------------------- geometry */
 /* readSTL (include/STLFunctions.h:3-89). */
 int psl_stl_read(const char* path, psl_stl** out);

@@ -84,6 +84,16 @@ def _declare(L):
                                             ctypes.c_int, ctypes.c_int, P]
     L.oracle_check_stl_edges.argtypes = [P, ctypes.c_int, P]
     L.oracle_esotwist_tables.argtypes = [P, P]
+    L.oracle_diad_build.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, P, P, P, ctypes.c_int, P,
+                                    ctypes.c_int]
+    L.oracle_diad_build.restype = P
+    L.oracle_diad_error.argtypes = [P]
+    L.oracle_diad_error.restype = ctypes.c_char_p
+    L.oracle_diad_count.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.oracle_diad_count.restype = ctypes.c_longlong
+    L.oracle_diad_get.argtypes = [P, ctypes.c_int, ctypes.c_int, P]
+    L.oracle_diad_info.argtypes = [P, ctypes.c_int, P, P]
+    L.oracle_diad_free.argtypes = [P]
 
 
 def _suf(a: np.ndarray) -> str:
@@ -283,3 +293,54 @@ def esotwist_tables():
     w = np.zeros(27, np.int32)
     lib().oracle_esotwist_tables(r.ctypes.data, w.ctypes.data)
     return r, w
+
+
+# ------------------------------------------------------------ DIAD builder
+# Marker rule record (same layout as the library's psl_marker_rule).
+RULE_DTYPE = np.dtype([("what", "<i4"), ("value", "<i4"), ("level_mask", "<u4"), ("shape", "<i4"),
+                       ("p", "<f4", (8,)), ("idx", "<i4", (6,))])
+DIAD_ARRAYS = ("ijk", "bounceback", "interface", "nbr", "c2f_read", "c2f_write", "f2c_write", "f2c_read", "outlet")
+
+
+def diad_build(base, levels, span, nu_phys, stls=(), rules=None):
+    """buildDIADGrids on the CPU.  base: dict(res, ox, oy, oz, nu, dt, nx, ny, nz);
+    stls: [(tri (n x 9), origin (3,))]; rules: RULE_DTYPE array.  Returns one
+    dict per level: the arrays named in DIAD_ARRAYS (ijk as 3 x n, nbr as 10 x n)
+    and the level's info."""
+    f6 = np.array([base["res"], base["ox"], base["oy"], base["oz"], base.get("nu", 0.0), base.get("dt", 1.0)],
+                  np.float32)
+    i3 = np.array([base["nx"], base["ny"], base["nz"]], np.int32)
+    tris = [np.ascontiguousarray(t, np.float32).reshape(-1, 9) for t, _ in stls]
+    alltri = np.ascontiguousarray(np.concatenate(tris) if tris else np.zeros((0, 9), np.float32))
+    counts = np.array([t.shape[0] for t in tris] or [0], np.int32)
+    origins = np.array([o for _, o in stls] or [(0, 0, 0)], np.float32).reshape(-1)
+    r = np.zeros(0, RULE_DTYPE) if rules is None else np.ascontiguousarray(rules, RULE_DTYPE)
+    L = lib()
+    h = L.oracle_diad_build(f6.ctypes.data, i3.ctypes.data, levels, span, nu_phys, alltri.ctypes.data,
+                            counts.ctypes.data, origins.ctypes.data, len(tris), r.ctypes.data if r.size else None,
+                            r.size)
+    try:
+        err = L.oracle_diad_error(h).decode()
+        if err:
+            raise RuntimeError(err)
+        out = []
+        for lev in range(levels):
+            fi = np.zeros(6, np.float32)
+            ii = np.zeros(4, np.int32)
+            L.oracle_diad_info(h, lev, fi.ctypes.data, ii.ctypes.data)
+            d = {"res": fi[0], "ox": fi[1], "oy": fi[2], "oz": fi[3], "nu": fi[4], "dt": fi[5],
+                 "nx": int(ii[0]), "ny": int(ii[1]), "nz": int(ii[2]), "n": int(ii[3])}
+            for w, name in enumerate(DIAD_ARRAYS):
+                cnt = L.oracle_diad_count(h, lev, w)
+                a = np.zeros(cnt, np.uint8 if name in ("bounceback", "interface") else np.int32)
+                if cnt:
+                    L.oracle_diad_get(h, lev, w, a.ctypes.data)
+                if name == "ijk":
+                    a = a.reshape(3, -1)
+                elif name == "nbr":
+                    a = a.reshape(10, -1)
+                d[name] = a
+            out.append(d)
+        return out
+    finally:
+        L.oracle_diad_free(h)

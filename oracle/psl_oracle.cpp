@@ -33,6 +33,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <string>
 #include <vector>
 
 namespace {
@@ -953,11 +954,19 @@ Footprint footprint(const float* ax, const float* ay, const float* bx, const flo
   return F;
 }
 
-// applyMarkersInsideSTL: include/STLFunctions.h:277-479.  tri: n x 9 floats
-// (a,b,c vertices) already in STL coordinates; (sox,soy,soz) STL origin,
-// (gox,goy,goz) grid origin.  Writes marker[cell] (0/1) for every cell.
-void markers_inside_stl(const float* tri, int ntri, float sox, float soy, float soz, float gox, float goy, float goz,
-                        float res, int nx, int ny, int nz, int inside_value, uint8_t* marker) {
+// Sorted hit layers per (i, j) column, padded to the largest count with
+// cellCountZ (include/STLFunctions.h:296-389: counter pass, hit-index pass,
+// per-column insertion sort).  Triangle vertices are moved into grid
+// coordinates as (vertex + STL origin) - grid origin, in that order.
+struct Columns {
+  int nx = 0, ny = 0, nz = 0;
+  size_t maxc = 1;
+  std::vector<int> layers;  // maxc x (nx * ny), layer-major
+  int at(size_t layer, int i, int j) const { return layers[layer * (size_t)nx * ny + (size_t)j * nx + i]; }
+};
+
+Columns stl_columns(const float* tri, int ntri, float sox, float soy, float soz, float gox, float goy, float goz,
+                    float res, int nx, int ny, int nz) {
   std::vector<std::vector<int>> cols((size_t)nx * ny);
   for (int t = 0; t < ntri; ++t) {
     const float* p = tri + 9 * (size_t)t;
@@ -977,19 +986,49 @@ void markers_inside_stl(const float* tri, int ntri, float sox, float soy, float 
         cols[(size_t)j * nx + i].push_back(k);
       }
   }
-  size_t maxc = 1;
-  for (auto& c : cols) maxc = std::max(maxc, c.size());
+  Columns C;
+  C.nx = nx; C.ny = ny; C.nz = nz;
+  for (auto& c : cols) C.maxc = std::max(C.maxc, c.size());
+  C.layers.assign(C.maxc * (size_t)nx * ny, nz);  // unused layers hold cellCountZ (line 369)
+  for (size_t col = 0; col < cols.size(); ++col) {
+    std::vector<int> v = cols[col];
+    v.resize(C.maxc, nz);
+    std::sort(v.begin(), v.end());
+    for (size_t l = 0; l < C.maxc; ++l) C.layers[l * (size_t)nx * ny + col] = v[l];
+  }
+  return C;
+}
+
+// Parity of cell (i, j, k) in its column (include/STLFunctions.h:371-389 for
+// whole grids, 449-477 for cell lists): the value of the interval holding k,
+// starting outside.
+bool column_value(const Columns& C, int i, int j, int k, int inside_value) {
+  bool val = !inside_value;
+  for (size_t iv = 0; iv <= C.maxc; ++iv) {
+    int s0, e0;
+    if (iv == 0) { s0 = 0; e0 = C.at(0, i, j); }
+    else if (iv == C.maxc) { s0 = C.at(C.maxc - 1, i, j); e0 = C.nz; }
+    else { s0 = C.at(iv - 1, i, j); e0 = C.at(iv, i, j); }
+    if (k >= s0 && k < e0) return val;
+    val = !val;
+  }
+  return false;  // not reached for 0 <= k < nz
+}
+
+// applyMarkersInsideSTL: include/STLFunctions.h:277-479.  tri: n x 9 floats
+// (a,b,c vertices) already in STL coordinates; (sox,soy,soz) STL origin,
+// (gox,goy,goz) grid origin.  Writes marker[cell] (0/1) for every cell.
+void markers_inside_stl(const float* tri, int ntri, float sox, float soy, float soz, float gox, float goy, float goz,
+                        float res, int nx, int ny, int nz, int inside_value, uint8_t* marker) {
+  const Columns C = stl_columns(tri, ntri, sox, soy, soz, gox, goy, goz, res, nx, ny, nz);
   for (int j = 0; j < ny; ++j)
     for (int i = 0; i < nx; ++i) {
-      std::vector<int> v = cols[(size_t)j * nx + i];
-      v.resize(maxc, nz);  // unused layers hold cellCountZ (line 369)
-      std::sort(v.begin(), v.end());
       bool val = !inside_value;
-      for (size_t iv = 0; iv <= maxc; ++iv) {
+      for (size_t iv = 0; iv <= C.maxc; ++iv) {
         int s0, e0;
-        if (iv == 0) { s0 = 0; e0 = v[0]; }
-        else if (iv == maxc) { s0 = v[maxc - 1]; e0 = nz; }
-        else { s0 = v[iv - 1]; e0 = v[iv]; }
+        if (iv == 0) { s0 = 0; e0 = C.at(0, i, j); }
+        else if (iv == C.maxc) { s0 = C.at(C.maxc - 1, i, j); e0 = nz; }
+        else { s0 = C.at(iv - 1, i, j); e0 = C.at(iv, i, j); }
         for (int k = s0; k < e0; ++k) marker[(size_t)k * nx * ny + (size_t)j * nx + i] = val ? 1 : 0;
         val = !val;
       }
@@ -1021,6 +1060,380 @@ int check_stl_edges(const float* tri, int ntri, int* counts) {
   for (int t = 0; t < ntri; ++t)
     if (counts[3 * t] != 2 || counts[3 * t + 1] != 2 || counts[3 * t + 2] != 2) ++bad;
   return bad;
+}
+
+// ------------------------------------------------------------ DIAD grid builder
+// buildDIADGrids: include/DIADFunctions.h:640-1046, with its helpers from the
+// same file (sortIJK 56-123, findMatchingIJKIndex 296-357,
+// findPeriodicNeighbourIJKIndex 360-429, markFinestFluidBounceback 500-555,
+// getDIADNeighbours 557-574, getDIADEsotwistNbrArray 576-612,
+// markDIADNeighbours 615-638).  The drivers' getMarkers callbacks are given as
+// an ordered rule list: every rule for the queried marker whose level mask
+// holds the level's gridID and whose shape contains the cell sets the marker
+// to the rule's value (a driver's sequence of `if (...) Marker.x = v;`).
+// Positions are iCell * res + ox etc. in FP32, as in the drivers.
+struct DRule { int what, value; unsigned level_mask; int shape; float p[8]; int idx[6]; };
+enum { R_BOUNCEBACK = 0, R_REFINEMENT = 1, R_OUTLET = 2 };
+enum { S_SPHERE_LE = 0, S_SPHERE_LT = 1, S_XCYLINDER = 2, S_BOX = 3, S_INDEX_BOX = 4 };
+
+struct DInfo { int gridID; float res, ox, oy, oz, nu, dt; int nx, ny, nz; };
+
+bool rule_contains(const DRule& r, const DInfo& I, int i, int j, int k) {
+  if (r.shape == S_INDEX_BOX) {
+    auto in = [](int v, int lo, int hi, int n) {
+      if (lo < 0) lo += n;
+      if (hi < 0) hi += n;
+      return v >= lo && v <= hi;
+    };
+    return in(i, r.idx[0], r.idx[1], I.nx) && in(j, r.idx[2], r.idx[3], I.ny) && in(k, r.idx[4], r.idx[5], I.nz);
+  }
+  const float x = (float)i * I.res + I.ox, y = (float)j * I.res + I.oy, z = (float)k * I.res + I.oz;
+  if (r.shape == S_BOX) return x > r.p[0] && x < r.p[3] && y > r.p[1] && y < r.p[4] && z > r.p[2] && z < r.p[5];
+  const float dx = x - r.p[0], dy = y - r.p[1], dz = z - r.p[2];
+  const float w2 = r.p[3] * r.p[3];
+  if (r.shape == S_XCYLINDER) {
+    const float a2 = dy * dy + dz * dz;
+    return dx > 0.f && dx < r.p[4] && a2 < w2;
+  }
+  const float r2 = dx * dx + dy * dy + dz * dz;
+  return r.shape == S_SPHERE_LE ? r2 <= w2 : r2 < w2;
+}
+
+struct DStl { const float* tri; int n; float ox, oy, oz; };
+
+struct DLevel {
+  DInfo info{};
+  std::vector<int> I, J, K;
+  std::vector<uint8_t> bb, iface, enfB, enfF, c2f, f2c;
+  std::vector<int> nbr10, c2f_read, c2f_write, f2c_write, f2c_read, outlet;
+  size_t n() const { return I.size(); }
+};
+
+struct DBuild {
+  int levels = 1, span = 0;
+  float nu_phys = 0.f;
+  std::vector<DStl> stls;
+  std::vector<DRule> rules;
+  std::vector<DLevel> lv;
+  std::string err;
+};
+
+bool ijk_less(int ia, int ja, int ka, int ib, int jb, int kb) {
+  if (ka != kb) return ka < kb;
+  if (ja != jb) return ja < jb;
+  return ia < ib;
+}
+
+// binary search in cells sorted by (k, j, i); -1 when absent
+int find_cell(const DLevel& L, int i, int j, int k) {
+  size_t lo = 0, hi = L.n();
+  while (lo < hi) {
+    const size_t mid = lo + (hi - lo) / 2;
+    if (ijk_less(L.I[mid], L.J[mid], L.K[mid], i, j, k)) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < L.n() && L.I[lo] == i && L.J[lo] == j && L.K[lo] == k) return (int)lo;
+  return -1;
+}
+
+template <class T>
+void permute(std::vector<T>& a, const std::vector<size_t>& order) {
+  if (a.empty()) return;
+  std::vector<T> b(order.size());
+  for (size_t x = 0; x < order.size(); ++x) b[x] = a[order[x]];
+  a.swap(b);
+}
+
+// sortIJK (with the enforce markers of finer levels)
+void sort_cells(DLevel& L) {
+  std::vector<size_t> order(L.n());
+  for (size_t x = 0; x < order.size(); ++x) order[x] = x;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return ijk_less(L.I[a], L.J[a], L.K[a], L.I[b], L.J[b], L.K[b]);
+  });
+  permute(L.I, order); permute(L.J, order); permute(L.K, order);
+  permute(L.enfB, order); permute(L.enfF, order);
+}
+
+// sortFinestGrid / sortCoarseGrid + resize: the cells are already sorted by
+// (k, j, i), so sorting by (keep desc, k, j, i) and truncating keeps the kept
+// cells in order.
+template <class T>
+void keep_only(std::vector<T>& a, const std::vector<uint8_t>& keep) {
+  if (a.empty()) return;
+  std::vector<T> b;
+  for (size_t x = 0; x < keep.size(); ++x)
+    if (keep[x]) b.push_back(a[x]);
+  a.swap(b);
+}
+
+// directions of getDIADNeighbours (include/DIADFunctions.h:561-563)
+const int DN_CX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, 0, 0, -1, 1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1};
+const int DN_CY[27] = {0, 0, 0, 0, 0, -1, 1, 0, 0, 0, 0, -1, 1, 1, -1, 1, -1, 1, -1, 1, -1, -1, 1, -1, 1, -1, 1};
+const int DN_CZ[27] = {0, 0, 0, -1, 1, 0, 0, -1, 1, 1, -1, 0, 0, -1, 1, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, -1, 1};
+
+std::vector<int> neighbours26(const DLevel& L) {
+  const size_t n = L.n();
+  std::vector<int> nb(26 * n);
+  for (int d = 1; d < 27; ++d)
+    for (size_t c = 0; c < n; ++c)
+      nb[(size_t)(d - 1) * n + c] = find_cell(L, L.I[c] + DN_CX[d], L.J[c] + DN_CY[d], L.K[c] + DN_CZ[d]);
+  return nb;
+}
+
+// markDIADNeighbours: target |= some existing neighbour is set in source
+void mark_neighbours(const std::vector<int>& nb, const std::vector<uint8_t>& src, std::vector<uint8_t>& tgt) {
+  const size_t n = src.size();
+  for (int d = 0; d < 26; ++d)
+    for (size_t c = 0; c < n; ++c) {
+      const int m = nb[(size_t)d * n + c];
+      if (m >= 0 && src[m]) tgt[c] = 1;
+    }
+}
+
+// ApplyMarkersInsideSTL on a cell list (include/STLFunctions.h:482-479+)
+std::vector<uint8_t> stl_inside(const DLevel& L, const DInfo& I, const DStl& S) {
+  const Columns C = stl_columns(S.tri, S.n, S.ox, S.oy, S.oz, I.ox, I.oy, I.oz, I.res, I.nx, I.ny, I.nz);
+  std::vector<uint8_t> m(L.n());
+  for (size_t c = 0; c < L.n(); ++c) m[c] = column_value(C, L.I[c], L.J[c], L.K[c], 1) ? 1 : 0;
+  return m;
+}
+
+// apply{Bounceback,Refinement,Outlet}MarkerFromFunction (lines 432-498)
+void apply_rules(const DBuild& B, int what, const DLevel& L, const DInfo& I, std::vector<uint8_t>& m) {
+  for (size_t c = 0; c < L.n(); ++c) {
+    int v = m[c];
+    for (const DRule& r : B.rules)
+      if (r.what == what && ((r.level_mask >> I.gridID) & 1u) && rule_contains(r, I, L.I[c], L.J[c], L.K[c]))
+        v = r.value;
+    m[c] = v ? 1 : 0;
+  }
+}
+
+std::vector<uint8_t> stl_union(const DBuild& B, const DLevel& L, const DInfo& I) {
+  std::vector<uint8_t> m(L.n(), 0);
+  for (const DStl& S : B.stls) {
+    const std::vector<uint8_t> s = stl_inside(L, I, S);
+    for (size_t c = 0; c < m.size(); ++c) m[c] |= s[c];
+  }
+  return m;
+}
+
+// getDIADEsotwistNbrArray: +i +j +k +ij +ik +jk +ijk -i -j -k, periodic walk
+// until an existing cell is met (the cell itself at the latest)
+void esotwist_neighbours(DLevel& L) {
+  static const int D[10][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1},
+                               {0, 1, 1}, {1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+  const size_t n = L.n();
+  const DInfo& I = L.info;
+  L.nbr10.assign(10 * n, -1);
+  for (int d = 0; d < 10; ++d)
+    for (size_t c = 0; c < n; ++c) {
+      int ti = L.I[c], tj = L.J[c], tk = L.K[c], r = -1;
+      while (r < 0) {
+        ti = ((ti + D[d][0]) % I.nx + I.nx) % I.nx;
+        tj = ((tj + D[d][1]) % I.ny + I.ny) % I.ny;
+        tk = ((tk + D[d][2]) % I.nz + I.nz) % I.nz;
+        r = find_cell(L, ti, tj, tk);
+      }
+      L.nbr10[(size_t)d * n + c] = r;
+    }
+}
+
+std::vector<uint8_t> b_not(const std::vector<uint8_t>& a) {
+  std::vector<uint8_t> r(a.size());
+  for (size_t c = 0; c < a.size(); ++c) r[c] = !a[c];
+  return r;
+}
+void b_or(std::vector<uint8_t>& a, const std::vector<uint8_t>& b) { for (size_t c = 0; c < a.size(); ++c) a[c] |= b[c]; }
+void b_and(std::vector<uint8_t>& a, const std::vector<uint8_t>& b) { for (size_t c = 0; c < a.size(); ++c) a[c] &= b[c]; }
+
+// markFinestFluidBounceback: include/DIADFunctions.h:500-555
+void finest_fluid_bounceback(const DBuild& B, const DLevel& L, std::vector<uint8_t>& fluid,
+                             std::vector<uint8_t>& finest_bb) {
+  const DInfo I0 = L.info;
+  fluid.assign(L.n(), 0);
+  finest_bb.assign(L.n(), 0);
+  for (int tl = I0.gridID; tl < B.levels; ++tl) {
+    const int shifts = 1 << (tl - I0.gridID);
+    const float target_res = I0.res / shifts;
+    const float start = -0.5f * I0.res + 0.5f * target_res;
+    for (int sx = 0; sx < shifts; ++sx)
+      for (int sy = 0; sy < shifts; ++sy)
+        for (int sz = 0; sz < shifts; ++sz) {
+          DInfo I = I0;
+          I.ox = I0.ox + start + sx * target_res;
+          I.oy = I0.oy + start + sy * target_res;
+          I.oz = I0.oz + start + sz * target_res;
+          std::vector<uint8_t> m = stl_union(B, L, I);
+          apply_rules(B, R_BOUNCEBACK, L, I, m);
+          b_or(finest_bb, m);
+          b_or(fluid, b_not(m));
+        }
+  }
+}
+
+void build_level(DBuild& B, int level) {
+  DLevel& L = B.lv[level];
+  const DInfo& I = L.info;
+  sort_cells(L);
+  const bool multi = B.levels > 1;
+  if (level == B.levels - 1) {
+    const std::vector<int> nb = neighbours26(L);
+    std::vector<uint8_t> bb = stl_union(B, L, I);
+    apply_rules(B, R_BOUNCEBACK, L, I, bb);
+    if (multi) b_or(bb, L.enfB);
+    std::vector<uint8_t> fluid = b_not(bb);
+    if (multi) b_or(fluid, L.enfF);
+    std::vector<uint8_t> keep = fluid;
+    mark_neighbours(nb, fluid, keep);
+    L.bb = b_not(fluid);
+    L.iface.assign(L.n(), 0);
+    L.enfB.clear(); L.enfF.clear();
+    keep_only(L.I, keep); keep_only(L.J, keep); keep_only(L.K, keep);
+    keep_only(L.bb, keep); keep_only(L.iface, keep);
+    esotwist_neighbours(L);
+    return;
+  }
+  const std::vector<int> nb = neighbours26(L);
+  std::vector<uint8_t> fluid, finest_bb;
+  finest_fluid_bounceback(B, L, fluid, finest_bb);
+  if (level != 0) b_or(fluid, L.enfF);
+  std::vector<uint8_t> border(L.n(), 0);
+  mark_neighbours(nb, fluid, border);
+  b_and(border, b_not(fluid));
+  std::vector<uint8_t> keep = fluid;
+  b_or(keep, border);
+  std::vector<uint8_t> ref = finest_bb;
+  b_or(ref, border);
+  std::vector<uint8_t> new_ref(L.n(), 0);
+  const int thickness = B.span + (B.levels - level - 1);
+  for (int layer = 0; layer < thickness; ++layer) {
+    mark_neighbours(nb, ref, new_ref);
+    b_or(new_ref, ref);
+    ref = new_ref;
+    b_and(ref, keep);
+  }
+  apply_rules(B, R_REFINEMENT, L, I, ref);
+  std::vector<uint8_t> f2c(L.n(), 0);
+  mark_neighbours(nb, ref, f2c);
+  b_and(f2c, keep);
+  b_and(f2c, b_not(ref));
+  std::vector<uint8_t> c2f(L.n(), 0);
+  mark_neighbours(nb, f2c, c2f);
+  b_and(c2f, keep);
+  b_and(c2f, b_not(f2c));
+  b_and(c2f, b_not(ref));
+  b_or(ref, f2c);
+  b_or(ref, c2f);
+  const std::vector<uint8_t> ref_inv = b_not(ref);
+  std::vector<uint8_t> coarse_bb(L.n(), 0);
+  apply_rules(B, R_BOUNCEBACK, L, I, coarse_bb);
+  b_or(coarse_bb, stl_union(B, L, I));
+  std::vector<uint8_t> under = c2f;
+  b_or(under, f2c);
+  std::vector<uint8_t> bb_under = under, fluid_under = under;
+  b_and(bb_under, coarse_bb);
+  b_and(fluid_under, b_not(coarse_bb));
+  keep = fluid;
+  b_or(keep, border);
+  b_and(keep, ref_inv);
+  b_or(keep, c2f);
+  b_or(keep, f2c);
+  if (level != 0) b_or(keep, L.enfF);
+
+  // children of the refined cells, in coarse order, (i, j, k) offsets nested i-j-k
+  DLevel& F = B.lv[level + 1];
+  F.I.clear(); F.J.clear(); F.K.clear(); F.enfB.clear(); F.enfF.clear();
+  for (size_t c = 0; c < L.n(); ++c) {
+    if (!ref[c]) continue;
+    for (int ip = 0; ip < 2; ++ip)
+      for (int jp = 0; jp < 2; ++jp)
+        for (int kp = 0; kp < 2; ++kp) {
+          F.I.push_back(2 * L.I[c] + ip);
+          F.J.push_back(2 * L.J[c] + jp);
+          F.K.push_back(2 * L.K[c] + kp);
+          F.enfB.push_back(bb_under[c]);
+          F.enfF.push_back(fluid_under[c]);
+        }
+  }
+  keep_only(L.I, keep); keep_only(L.J, keep); keep_only(L.K, keep);
+  keep_only(f2c, keep); keep_only(c2f, keep);
+  keep_only(L.enfB, keep); keep_only(L.enfF, keep);
+  L.iface = f2c;
+  b_or(L.iface, c2f);
+  std::vector<uint8_t> bb(L.n(), 0);
+  apply_rules(B, R_BOUNCEBACK, L, I, bb);
+  b_or(bb, stl_union(B, L, I));
+  if (level != 0) {
+    b_or(bb, L.enfB);
+    b_and(bb, b_not(L.enfF));
+  }
+  L.enfB.clear(); L.enfF.clear();
+  L.bb = bb;
+  b_and(c2f, b_not(bb));
+  b_and(f2c, b_not(bb));
+  L.c2f = c2f;
+  L.f2c = f2c;
+  esotwist_neighbours(L);
+  std::vector<uint8_t> out(L.n(), 0);
+  apply_rules(B, R_OUTLET, L, I, out);
+  L.outlet.clear();
+  for (size_t c = 0; c < L.n(); ++c)
+    if (out[c]) L.outlet.push_back((int)c);
+
+  DInfo& FI = F.info;
+  FI.gridID = I.gridID + 1;
+  FI.res = I.res * 0.5f;
+  FI.dt = I.dt * 0.5f;
+  FI.nu = (FI.dt * B.nu_phys) / ((FI.res / 1000.f) * (FI.res / 1000.f));
+  FI.ox = I.ox - FI.res * 0.5f;
+  FI.oy = I.oy - FI.res * 0.5f;
+  FI.oz = I.oz - FI.res * 0.5f;
+  FI.nx = I.nx * 2;
+  FI.ny = I.ny * 2;
+  FI.nz = I.nz * 2;
+}
+
+// mapInterfaceLambda (lines 964-1043): coarse cells of a marker, in order, and
+// the fine cell at (2i, 2j, 2k)
+bool map_interface(DBuild& B, int level, const std::vector<uint8_t>& m, std::vector<int>& coarse,
+                   std::vector<int>& fine) {
+  const DLevel& L = B.lv[level];
+  const DLevel& F = B.lv[level + 1];
+  coarse.clear(); fine.clear();
+  for (size_t c = 0; c < m.size(); ++c)
+    if (m[c]) {
+      coarse.push_back((int)c);
+      const int f = find_cell(F, 2 * L.I[c], 2 * L.J[c], 2 * L.K[c]);
+      if (f < 0) {
+        B.err = "invalid interface connection on level " + std::to_string(level);
+        return false;
+      }
+      fine.push_back(f);
+    }
+  return true;
+}
+
+DBuild* diad_build(const DInfo& base, int levels, int span, float nu_phys, const std::vector<DStl>& stls,
+                   const std::vector<DRule>& rules) {
+  DBuild* B = new DBuild;
+  B->levels = levels; B->span = span; B->nu_phys = nu_phys; B->stls = stls; B->rules = rules;
+  B->lv.resize(levels);
+  DLevel& L0 = B->lv[0];
+  L0.info = base;
+  for (int k = 0; k < base.nz; ++k)
+    for (int j = 0; j < base.ny; ++j)
+      for (int i = 0; i < base.nx; ++i) {
+        L0.I.push_back(i); L0.J.push_back(j); L0.K.push_back(k);
+      }
+  for (int l = 0; l < levels; ++l) build_level(*B, l);
+  for (int l = levels - 2; l >= 0; --l) {
+    DLevel& L = B->lv[l];
+    if (!map_interface(*B, l, L.c2f, L.c2f_read, L.c2f_write)) break;
+    if (!map_interface(*B, l, L.f2c, L.f2c_write, L.f2c_read)) break;
+  }
+  return B;
 }
 
 }  // namespace
@@ -1166,5 +1579,76 @@ int oracle_esotwist_tables(int* read_slot, int* write_slot) {
   }
   return 27;
 }
+
+
+// DIAD builder.  base6f = res, ox, oy, oz, nu, dtPhys; base3i = nx, ny, nz.
+// stl_tris: triangles of all meshes back to back, stl_counts[s] triangles each,
+// stl_origins[3 s]; rules: nrules records laid out as DRule (72 bytes).
+void* oracle_diad_build(const float* base6f, const int* base3i, int levels, int span, float nu_phys,
+                        const float* stl_tris, const int* stl_counts, const float* stl_origins, int nstl,
+                        const void* rules, int nrules) {
+  DInfo I{};
+  I.gridID = 0;
+  I.res = base6f[0]; I.ox = base6f[1]; I.oy = base6f[2]; I.oz = base6f[3]; I.nu = base6f[4]; I.dt = base6f[5];
+  I.nx = base3i[0]; I.ny = base3i[1]; I.nz = base3i[2];
+  std::vector<DStl> stls;
+  size_t off = 0;
+  for (int s = 0; s < nstl; ++s) {
+    stls.push_back({stl_tris + 9 * off, stl_counts[s], stl_origins[3 * s], stl_origins[3 * s + 1], stl_origins[3 * s + 2]});
+    off += (size_t)stl_counts[s];
+  }
+  std::vector<DRule> rl((const DRule*)rules, (const DRule*)rules + nrules);
+  return diad_build(I, levels, span, nu_phys, stls, rl);
+}
+
+const char* oracle_diad_error(void* h) { return ((DBuild*)h)->err.c_str(); }
+
+// which: 0 ijk (3n int), 1 bounceback (n u8), 2 interface (n u8), 3 neighbours
+// (10n int: +i +j +k +ij +ik +jk +ijk -i -j -k), 4 coarseToFineRead,
+// 5 coarseToFineWrite, 6 fineToCoarseWrite, 7 fineToCoarseRead, 8 outlet cells
+long long oracle_diad_count(void* h, int level, int which) {
+  const DLevel& L = ((DBuild*)h)->lv[level];
+  switch (which) {
+    case 0: return 3 * (long long)L.n();
+    case 1: case 2: return (long long)L.n();
+    case 3: return (long long)L.nbr10.size();
+    case 4: return (long long)L.c2f_read.size();
+    case 5: return (long long)L.c2f_write.size();
+    case 6: return (long long)L.f2c_write.size();
+    case 7: return (long long)L.f2c_read.size();
+    case 8: return (long long)L.outlet.size();
+  }
+  return -1;
+}
+
+void oracle_diad_get(void* h, int level, int which, void* out) {
+  const DLevel& L = ((DBuild*)h)->lv[level];
+  const size_t n = L.n();
+  auto put = [&](const std::vector<int>& v) { std::memcpy(out, v.data(), v.size() * sizeof(int)); };
+  switch (which) {
+    case 0:
+      std::memcpy(out, L.I.data(), n * sizeof(int));
+      std::memcpy((int*)out + n, L.J.data(), n * sizeof(int));
+      std::memcpy((int*)out + 2 * n, L.K.data(), n * sizeof(int));
+      break;
+    case 1: std::memcpy(out, L.bb.data(), n); break;
+    case 2: std::memcpy(out, L.iface.data(), n); break;
+    case 3: put(L.nbr10); break;
+    case 4: put(L.c2f_read); break;
+    case 5: put(L.c2f_write); break;
+    case 6: put(L.f2c_write); break;
+    case 7: put(L.f2c_read); break;
+    case 8: put(L.outlet); break;
+  }
+}
+
+// level info: f6 = res, ox, oy, oz, nu, dtPhys; i4 = nx, ny, nz, cellCount
+void oracle_diad_info(void* h, int level, float* f6, int* i4) {
+  const DLevel& L = ((DBuild*)h)->lv[level];
+  f6[0] = L.info.res; f6[1] = L.info.ox; f6[2] = L.info.oy; f6[3] = L.info.oz; f6[4] = L.info.nu; f6[5] = L.info.dt;
+  i4[0] = L.info.nx; i4[1] = L.info.ny; i4[2] = L.info.nz; i4[3] = (int)L.n();
+}
+
+void oracle_diad_free(void* h) { delete (DBuild*)h; }
 
 }  // extern "C"

@@ -201,6 +201,11 @@ _SIGS = [
     ("psl_diad_get_f", _I, [_P, _P]),
     ("psl_diad_set_f", _I, [_P, _P]),
     ("psl_diad_dense_box", _I, [_I, _I, _I, _I, _P, _P]),
+    ("psl_diad_build", _I, [_P, _I, _I, ctypes.c_float, _P, _I, _P, _I, _P]),
+    ("psl_diad_levels_destroy", _I, [_P]),
+    ("psl_diad_levels_info", _I, [_P, _I, _P, _P]),
+    ("psl_diad_levels_get", _I, [_P, _I, _I, _P]),
+    ("psl_diad_levels_build_ms", _I, [_P, _P]),
 ]
 
 
@@ -538,6 +543,98 @@ class DIADGridStruct:
     def set_f(self, f):
         a = np.ascontiguousarray(f, np.float32)
         _check(lib().psl_diad_set_f(self._h, a.ctypes.data))
+
+
+# ------------------------------------------------------ DIAD grid builder
+# psl_marker_rule (include/psl.h): the getMarkers callbacks of the drivers as
+# an ordered rule list evaluated on the device during buildDIADGrids.
+RULE_DTYPE = np.dtype([("what", "<i4"), ("value", "<i4"), ("level_mask", "<u4"), ("shape", "<i4"),
+                       ("p", "<f4", (8,)), ("idx", "<i4", (6,))])
+RULE_BOUNCEBACK, RULE_REFINEMENT, RULE_OUTLET = 0, 1, 2
+SHAPE_SPHERE_LE, SHAPE_SPHERE_LT, SHAPE_XCYLINDER, SHAPE_BOX, SHAPE_INDEX_BOX = range(5)
+DIAD_ARRAYS = ("ijk", "bounceback", "interface", "nbr", "c2f_read", "c2f_write", "f2c_write", "f2c_read", "outlet")
+ALL_LEVELS = 0xFFFFFFFF
+
+
+def marker_rule(what: int, shape: int, p=(), idx=(), value: int = 1, levels=ALL_LEVELS) -> np.ndarray:
+    """One rule; `levels` is a bit mask of gridIDs or an iterable of them."""
+    r = np.zeros(1, RULE_DTYPE)
+    mask = levels if isinstance(levels, int) else sum(1 << int(l) for l in levels)
+    r["what"], r["value"], r["level_mask"], r["shape"] = what, value, mask, shape
+    r["p"][0, :len(p)] = p
+    r["idx"][0, :len(idx)] = idx
+    return r
+
+
+def rules(*rs) -> np.ndarray:
+    return np.concatenate(rs) if rs else np.zeros(0, RULE_DTYPE)
+
+
+class DIADLevels:
+    """buildDIADGrids (include/DIADFunctions.h:640-1046): the per-level cell
+    tables produced on the device."""
+
+    def __init__(self, base: InfoStruct, levels: int, wall_refinement_span: int, nu_phys: float,
+                 stls: Sequence["STLStruct"] = (), marker_rules=None):
+        require_device()
+        r = np.zeros(0, RULE_DTYPE) if marker_rules is None else np.ascontiguousarray(marker_rules, RULE_DTYPE)
+        arr = (ctypes.c_void_p * max(1, len(stls)))(*[s.handle.value for s in stls])
+        ci = base.to_c()
+        self.levels = levels
+        self._h = ctypes.c_void_p()
+        _check(lib().psl_diad_build(ctypes.byref(ci), levels, wall_refinement_span, nu_phys, arr, len(stls),
+                                    r.ctypes.data if r.size else None, r.size, ctypes.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().psl_diad_levels_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def build_ms(self) -> float:
+        v = ctypes.c_double()
+        _check(lib().psl_diad_levels_build_ms(self._h, ctypes.byref(v)))
+        return v.value
+
+    def info(self, level: int):
+        ci = _Info()
+        counts = np.zeros(len(DIAD_ARRAYS), np.int64)
+        _check(lib().psl_diad_levels_info(self._h, level, ctypes.byref(ci), counts.ctypes.data))
+        return InfoStruct.from_c(ci), counts
+
+    def array(self, level: int, name: str) -> np.ndarray:
+        which = DIAD_ARRAYS.index(name)
+        _, counts = self.info(level)
+        a = np.zeros(int(counts[which]), np.uint8 if name in ("bounceback", "interface") else np.int32)
+        if a.size:
+            _check(lib().psl_diad_levels_get(self._h, level, which, a.ctypes.data))
+        if name == "ijk":
+            a = a.reshape(3, -1)
+        elif name == "nbr":
+            a = a.reshape(10, -1)
+        return a
+
+    def level(self, level: int) -> dict:
+        info, _ = self.info(level)
+        d = {"info": info, "n": info.cellCount}
+        for name in DIAD_ARRAYS:
+            d[name] = self.array(level, name)
+        return d
+
+
+def buildDIADGrids(base: InfoStruct, levels: int, wall_refinement_span: int, nu_phys: float,
+                   stls: Sequence["STLStruct"] = (), marker_rules=None) -> DIADLevels:
+    return DIADLevels(base, levels, wall_refinement_span, nu_phys, stls, marker_rules)
 
 
 # ------------------------------------------------------------ operators

@@ -26,8 +26,11 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 EXTRA = {
     # voxelizer masks must be bit-identical to the CPU restatement
     "psl_stl.hip": ["-ffp-contract=off"],
+    # DIAD builder: marker-rule positions and shifted origins round as written
+    "psl_diad_build.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["psl_grid.hip", "psl_refine.hip", "psl_stl.hip", "psl_esotwist.hip", "psl_batch.hip", "psl_closure.hip"]
+SOURCES = ["psl_grid.hip", "psl_refine.hip", "psl_stl.hip", "psl_esotwist.hip", "psl_batch.hip", "psl_closure.hip",
+           "psl_diad_build.hip"]
 
 
 def hipcc() -> str:

@@ -142,6 +142,17 @@ struct psl_stl {
 };
 
 namespace psl {
+// Packed cell coordinates of DIAD cell lists: (k << 42) | (j << 21) | i, so
+// that the integer order is the (k, j, i) order of the reference's sortIJK.
+constexpr int KEY_BITS = 21;
+constexpr uint64_t KEY_MASK = (1ull << KEY_BITS) - 1;
+__host__ __device__ inline uint64_t pack_key(int i, int j, int k) {
+  return ((uint64_t)k << (2 * KEY_BITS)) | ((uint64_t)j << KEY_BITS) | (uint64_t)i;
+}
+// implemented in psl_stl.hip: ORs the inside marker of an STL (grid origin go,
+// spacing res, box nx x ny x nz) into out[c] for n cells given as packed keys
+int stl_mark_keys(const psl_stl* s, float gox, float goy, float goz, float res, int nx, int ny, int nz,
+                  const uint64_t* keys, int64_t n, uint8_t* out);
 // implemented in psl_grid.hip
 int grid_rebuild_flags(psl_grid* g);
 int ensure_partials(psl_grid* g, size_t n);

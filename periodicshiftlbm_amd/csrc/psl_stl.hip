@@ -71,7 +71,8 @@ __host__ __device__ inline float ray_hit_z(float ax, float ay, float az, float b
 }
 
 struct VoxParams {
-  float sox, soy, soz;  // STL origin minus grid origin
+  float sox, soy, soz;  // STL origin
+  float gox, goy, goz;  // grid origin
   float res;
   int nx, ny, nz;
 };
@@ -84,9 +85,10 @@ struct TriLattice {
 
 __device__ inline TriLattice tri_lattice(const float* t, const VoxParams& p) {
   TriLattice L;
-  L.ax = t[0] + p.sox; L.ay = t[1] + p.soy; L.az = t[2] + p.soz;
-  L.bx = t[3] + p.sox; L.by = t[4] + p.soy; L.bz = t[5] + p.soz;
-  L.cx = t[6] + p.sox; L.cy = t[7] + p.soy; L.cz = t[8] + p.soz;
+  // (vertex + STL origin) - grid origin, in that order (lines 300-305)
+  L.ax = t[0] + p.sox - p.gox; L.ay = t[1] + p.soy - p.goy; L.az = t[2] + p.soz - p.goz;
+  L.bx = t[3] + p.sox - p.gox; L.by = t[4] + p.soy - p.goy; L.bz = t[5] + p.soz - p.goz;
+  L.cx = t[6] + p.sox - p.gox; L.cy = t[7] + p.soy - p.goy; L.cz = t[8] + p.soz - p.goz;
   const float scale = 50.0f / p.res;
   L.ak = (long long)roundf(L.ax * scale) * 2 + 1;
   L.al = (long long)roundf(L.ay * scale) * 2 + 1;
@@ -149,9 +151,8 @@ __global__ void k_hit_index(const float* __restrict__ tri, int ntri, VoxParams p
     }
 }
 
-// K17: per-column sort and parity fill
-__global__ void k_fill_columns(int* __restrict__ layers, int maxc, VoxParams p, int inside_value,
-                               uint8_t* __restrict__ mask) {
+// K17: per-column insertion sort (the result does not depend on the atomic order)
+__global__ void k_sort_columns(int* __restrict__ layers, int maxc, VoxParams p) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= p.nx || j >= p.ny) return;
@@ -166,6 +167,16 @@ __global__ void k_fill_columns(int* __restrict__ layers, int maxc, VoxParams p, 
     }
     layers[(size_t)(s + 1) * ncol + col] = key;
   }
+}
+
+// parity fill of whole columns (lines 371-389)
+__global__ void k_fill_columns(const int* __restrict__ layers, int maxc, VoxParams p, int inside_value,
+                               uint8_t* __restrict__ mask) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= p.nx || j >= p.ny) return;
+  const size_t ncol = (size_t)p.nx * p.ny;
+  const size_t col = (size_t)j * p.nx + i;
   bool val = !inside_value;
   for (int iv = 0; iv <= maxc; ++iv) {
     int s0, e0;
@@ -174,6 +185,30 @@ __global__ void k_fill_columns(int* __restrict__ layers, int maxc, VoxParams p, 
     else { s0 = layers[(size_t)(iv - 1) * ncol + col]; e0 = layers[(size_t)iv * ncol + col]; }
     for (int k = s0; k < e0; ++k) mask[(size_t)k * ncol + col] = val ? 1 : 0;
     val = !val;
+  }
+}
+
+// cell-list form (lines 449-477): cells given as packed (k, j, i) keys inside
+// the nx x ny x nz box; sets out[c] = 1 where the cell lies inside
+__global__ void k_mark_keys(const int* __restrict__ layers, int maxc, VoxParams p, const uint64_t* __restrict__ keys,
+                            int64_t n, uint8_t* __restrict__ out) {
+  const size_t ncol = (size_t)p.nx * p.ny;
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = keys[c];
+    const int i = (int)(key & KEY_MASK), j = (int)((key >> KEY_BITS) & KEY_MASK), k = (int)(key >> (2 * KEY_BITS));
+    const size_t col = (size_t)j * p.nx + i;
+    bool val = false;  // starting outside
+    for (int iv = 0; iv <= maxc; ++iv) {
+      int s0, e0;
+      if (iv == 0) { s0 = 0; e0 = layers[col]; }
+      else if (iv == maxc) { s0 = layers[(size_t)(maxc - 1) * ncol + col]; e0 = p.nz; }
+      else { s0 = layers[(size_t)(iv - 1) * ncol + col]; e0 = layers[(size_t)iv * ncol + col]; }
+      if (k >= s0 && k < e0) {
+        if (val) out[c] = 1;
+        break;
+      }
+      val = !val;
+    }
   }
 }
 
@@ -249,6 +284,75 @@ static void stl_bounds(psl_stl* s) {
 }
 
 static unsigned grid1d(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 65535); }
+
+VoxParams vox_params(const psl_stl* s, float gox, float goy, float goz, float res, int nx, int ny, int nz) {
+  VoxParams p;
+  p.sox = s->ox; p.soy = s->oy; p.soz = s->oz;
+  p.gox = gox; p.goy = goy; p.goz = goz;
+  p.res = res;
+  p.nx = nx; p.ny = ny; p.nz = nz;
+  return p;
+}
+
+// Sorted hit layers per column (maxc x nx*ny ints, unused layers = nz), the
+// counter, hit-index and sort passes of include/STLFunctions.h:296-369.
+int stl_layers(const psl_stl* s, const VoxParams& p, int** layers_out, int* maxc_out) {
+  const size_t ncol = (size_t)p.nx * p.ny;
+  int* counter = nullptr;
+  int* dmax = nullptr;
+  int* layers = nullptr;
+  int rc = PSL_OK;
+  int maxc = 1;
+  auto fail = [&](const char* what) {
+    set_error(std::string("voxelizer: ") + what);
+    rc = PSL_E_HIP;
+  };
+  if (hipMalloc(&counter, ncol * sizeof(int)) != hipSuccess || hipMalloc(&dmax, sizeof(int)) != hipSuccess) fail("alloc");
+  if (rc == PSL_OK) {
+    (void)hipMemset(counter, 0, ncol * sizeof(int));
+    (void)hipMemset(dmax, 0, sizeof(int));
+    if (s->count > 0) k_count_hits<<<(s->count + 255) / 256, 256>>>(s->d_tri, s->count, p, counter);
+    k_max_count<<<grid1d(ncol), 256>>>(counter, ncol, dmax);
+    if (hipGetLastError() != hipSuccess || hipMemcpy(&maxc, dmax, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      fail("count");
+    if (maxc < 1) maxc = 1;
+  }
+  if (rc == PSL_OK && hipMalloc(&layers, (size_t)maxc * ncol * sizeof(int)) != hipSuccess) fail("alloc layers");
+  if (rc == PSL_OK) {
+    k_fill_int<<<grid1d((size_t)maxc * ncol), 256>>>(layers, (size_t)maxc * ncol, p.nz);
+    (void)hipMemset(counter, 0, ncol * sizeof(int));
+    if (s->count > 0) k_hit_index<<<(s->count + 255) / 256, 256>>>(s->d_tri, s->count, p, counter, layers, maxc);
+    dim3 g2((p.nx + 255) / 256, p.ny);
+    k_sort_columns<<<g2, 256>>>(layers, maxc, p);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) fail("hits");
+  }
+  if (counter) (void)hipFree(counter);
+  if (dmax) (void)hipFree(dmax);
+  if (rc != PSL_OK && layers) {
+    (void)hipFree(layers);
+    layers = nullptr;
+  }
+  *layers_out = layers;
+  *maxc_out = maxc;
+  return rc;
+}
+
+int stl_mark_keys(const psl_stl* s, float gox, float goy, float goz, float res, int nx, int ny, int nz,
+                  const uint64_t* keys, int64_t n, uint8_t* out) {
+  const VoxParams p = vox_params(s, gox, goy, goz, res, nx, ny, nz);
+  int* layers = nullptr;
+  int maxc = 1;
+  int rc = stl_layers(s, p, &layers, &maxc);
+  if (rc == PSL_OK && n > 0) {
+    k_mark_keys<<<grid1d((size_t)n), 256>>>(layers, maxc, p, keys, n, out);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      set_error("voxelizer: cell list");
+      rc = PSL_E_HIP;
+    }
+  }
+  if (layers) (void)hipFree(layers);
+  return rc;
+}
 
 }  // namespace psl
 
@@ -415,45 +519,18 @@ int psl_markers_inside_stl(psl_mask* mask, const psl_stl* s, int inside_value, c
   const int nx = info->cellCountX, ny = info->cellCountY, nz = info->cellCountZ;
   PSL_CHECK_ARG(nx > 0 && ny > 0 && nz > 0, "bad grid counts");
   PSL_CHECK_ARG(mask->n == (int64_t)nx * ny * nz, "mask size does not match the grid");
-  VoxParams p;
-  p.sox = s->ox - info->ox;
-  p.soy = s->oy - info->oy;
-  p.soz = s->oz - info->oz;
-  p.res = info->res;
-  p.nx = nx; p.ny = ny; p.nz = nz;
-  const size_t ncol = (size_t)nx * ny;
-  int* counter = nullptr;
-  int* dmax = nullptr;
+  const VoxParams p = vox_params(s, info->ox, info->oy, info->oz, info->res, nx, ny, nz);
   int* layers = nullptr;
-  int rc = PSL_OK;
   int maxc = 1;
-  auto fail = [&](const char* what) {
-    set_error(std::string("voxelizer: ") + what);
-    rc = PSL_E_HIP;
-  };
-  if (hipMalloc(&counter, ncol * sizeof(int)) != hipSuccess || hipMalloc(&dmax, sizeof(int)) != hipSuccess) fail("alloc");
+  int rc = stl_layers(s, p, &layers, &maxc);
   if (rc == PSL_OK) {
-    (void)hipMemset(counter, 0, ncol * sizeof(int));
-    (void)hipMemset(dmax, 0, sizeof(int));
-    if (s->count > 0) k_count_hits<<<(s->count + 255) / 256, 256>>>(s->d_tri, s->count, p, counter);
-    k_max_count<<<grid1d(ncol), 256>>>(counter, ncol, dmax);
-    if (hipGetLastError() != hipSuccess || hipMemcpy(&maxc, dmax, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
-      fail("count");
-    if (maxc < 1) maxc = 1;
-  }
-  if (rc == PSL_OK) {
-    if (hipMalloc(&layers, (size_t)maxc * ncol * sizeof(int)) != hipSuccess) fail("alloc layers");
-  }
-  if (rc == PSL_OK) {
-    k_fill_int<<<grid1d((size_t)maxc * ncol), 256>>>(layers, (size_t)maxc * ncol, nz);
-    (void)hipMemset(counter, 0, ncol * sizeof(int));
-    if (s->count > 0) k_hit_index<<<(s->count + 255) / 256, 256>>>(s->d_tri, s->count, p, counter, layers, maxc);
     dim3 g2((nx + 255) / 256, ny);
     k_fill_columns<<<g2, 256>>>(layers, maxc, p, inside_value, mask->d);
-    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) fail("fill");
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+      set_error("voxelizer: fill");
+      rc = PSL_E_HIP;
+    }
   }
-  if (counter) (void)hipFree(counter);
-  if (dmax) (void)hipFree(dmax);
   if (layers) (void)hipFree(layers);
   return rc;
 }
