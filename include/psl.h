@@ -336,6 +336,31 @@ int psl_diad_levels_info(const psl_diad_levels* levels, int level, psl_info* inf
 int psl_diad_levels_get(const psl_diad_levels* levels, int level, int which, void* host);
 /* Build time of the last psl_diad_build (ms, device work and host sync included). */
 int psl_diad_levels_build_ms(const psl_diad_levels* levels, double* ms);
+/* An esotwist grid of one built level, made on the device: cell tables, the
+ * first seven neighbour columns, flags BOUNCEBACK / FLUID from the level's
+ * bounceback marker (psl_diad_set_flags overrides them) and, below the finest
+ * level, the interface tables.  Equivalent to the reference's DIADGridStruct
+ * after buildDIADGrids. */
+int psl_diad_create_level(const psl_diad_levels* levels, int level, int storage, psl_diad** out);
+
+/* ---------------------------------------- DIAD levels: transfers and steps */
+/* coarseToFine{Read,Write}Array / fineToCoarse{Write,Read}Array of a coarse
+ * DIADGridStruct (include/types.h:168-169) from host arrays. */
+int psl_diad_set_interface(psl_diad* coarse, const psl_diad* fine, const int* c2f_read, const int* c2f_write,
+                           int64_t n_c2f, const int* f2c_write, const int* f2c_read, int64_t n_f2c);
+/* applyCoarseFineGridCommunication(DIADGridStruct&, DIADGridStruct&)
+ * (include/gridRefinementFunctions.h:540-708): fine -> coarse means, then
+ * coarse -> fine injection, both rescaled (rescaleF).  The two grids must share
+ * the storage type and the stream. */
+int psl_diad_coarse_fine_communication(psl_diad* coarse, psl_diad* fine);
+/* updateGrid over DIAD levels (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:200-209):
+ * streaming + local update of level l, two steps of level l+1, then the
+ * communication of l and l+1; nsteps coarse steps. */
+int psl_diad_update_grid(psl_diad* const* levels, int nlevels, int nsteps);
+/* getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
+ * all three components in FP64: sum over BOUNCEBACK-flagged cells of
+ * rho (u - u after bounceback), populations through the current write maps. */
+int psl_diad_reduce_momentum_exchange(psl_diad* grid, double out[3]);
 
 /* ------------------------------------------------------------- geometry */
 /* readSTL (include/STLFunctions.h:3-89). */

@@ -84,6 +84,12 @@ def _declare(L):
                                             ctypes.c_int, ctypes.c_int, P]
     L.oracle_check_stl_edges.argtypes = [P, ctypes.c_int, P]
     L.oracle_esotwist_tables.argtypes = [P, P]
+    for suf in ("f32", "f64"):
+        for name in ("to_coarse", "to_fine"):
+            getattr(L, f"oracle_eso_{name}_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
+                                                                ctypes.c_longlong, P, ctypes.c_int, P, P,
+                                                                ctypes.c_longlong]
+        getattr(L, f"oracle_eso_momentum_exchange_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P, P]
     L.oracle_diad_build.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, P, P, P, ctypes.c_int, P,
                                     ctypes.c_int]
     L.oracle_diad_build.restype = P
@@ -293,6 +299,32 @@ def esotwist_tables():
     w = np.zeros(27, np.int32)
     lib().oracle_esotwist_tables(r.ctypes.data, w.ctypes.data)
     return r, w
+
+
+def eso_to_coarse(fc, nbc7, flip_c, ff, nbf7, flip_f, write, read):
+    """writeToCoarseGridInterface(DIAD) on raw esotwist arrays f[q, cell] (in place on fc)."""
+    w = np.ascontiguousarray(write, np.int32)
+    r = np.ascontiguousarray(read, np.int32)
+    getattr(lib(), f"oracle_eso_to_coarse_{_suf(fc)}")(fc.ctypes.data, fc.shape[1], nbc7.ctypes.data, int(flip_c),
+                                                       ff.ctypes.data, ff.shape[1], nbf7.ctypes.data, int(flip_f),
+                                                       w.ctypes.data, r.ctypes.data, w.size)
+
+
+def eso_to_fine(fc, nbc7, flip_c, ff, nbf7, flip_f, write, read):
+    """writeToFineGridInterface(DIAD) on raw esotwist arrays (in place on ff)."""
+    w = np.ascontiguousarray(write, np.int32)
+    r = np.ascontiguousarray(read, np.int32)
+    getattr(lib(), f"oracle_eso_to_fine_{_suf(fc)}")(fc.ctypes.data, fc.shape[1], nbc7.ctypes.data, int(flip_c),
+                                                     ff.ctypes.data, ff.shape[1], nbf7.ctypes.data, int(flip_f),
+                                                     w.ctypes.data, r.ctypes.data, w.size)
+
+
+def eso_momentum_exchange(f, nbr7, flipper, flags):
+    out = np.zeros(3, np.float64)
+    fl = np.ascontiguousarray(flags, np.uint8)
+    getattr(lib(), f"oracle_eso_momentum_exchange_{_suf(f)}")(f.ctypes.data, f.shape[1], nbr7.ctypes.data,
+                                                              int(flipper), fl.ctypes.data, out.ctypes.data)
+    return out
 
 
 # ------------------------------------------------------------ DIAD builder

@@ -877,6 +877,86 @@ void esotwist_update(R* fa, long long n, const int* const* nbr, const int* ijk_i
   }
 }
 
+// Populations of a DIAD cell as the last update left them, through the write
+// maps of the current parity (getEsotwistWriteIndex, include/applyStreaming.h:138-192).
+template <typename R>
+void eso_get(const R* fa, long long n, const int* const* nbr, long long cell, bool flipper, R (&f)[27]) {
+  long long slot[8];
+  slot[0] = cell;
+  for (int t = 0; t < 7; ++t) slot[t + 1] = nbr[t][cell];
+  for (int q = 0; q < 27; ++q) f[q] = fa[(long long)(flipper ? opp(q) : q) * n + slot[ESO_WRITE[q]]];
+}
+template <typename R>
+void eso_put(R* fa, long long n, const int* const* nbr, long long cell, bool flipper, const R (&f)[27]) {
+  long long slot[8];
+  slot[0] = cell;
+  for (int t = 0; t < 7; ++t) slot[t + 1] = nbr[t][cell];
+  for (int q = 0; q < 27; ++q) fa[(long long)(flipper ? opp(q) : q) * n + slot[ESO_WRITE[q]]] = f[q];
+}
+
+// writeToCoarseGridInterface(DIAD): include/gridRefinementFunctions.h:540-622.
+// Mean of the read cell and its +i +j +k +ij +ik +jk +ijk fine neighbours
+// (summed in that order), rescaled fine -> coarse, into the coarse write cell.
+template <typename R>
+void eso_to_coarse(R* fc, long long nc, const int* const* nbc, bool flip_c, const R* ff, long long nf,
+                   const int* const* nbf, bool flip_f, const int* write, const int* read, long long m) {
+  for (long long t = 0; t < m; ++t) {
+    const long long rc = read[t];
+    long long fine[8];
+    fine[0] = rc;
+    for (int a = 0; a < 7; ++a) fine[a + 1] = nbf[a][rc];
+    R f[27] = {0};
+    for (int w = 0; w < 8; ++w) {
+      R g[27];
+      eso_get(ff, nf, nbf, fine[w], flip_f, g);
+      for (int q = 0; q < 27; ++q) f[q] += g[q];
+    }
+    for (int q = 0; q < 27; ++q) f[q] = f[q] * (R)0.125;
+    rescale_f(f, false);
+    eso_put(fc, nc, nbc, write[t], flip_c, f);
+  }
+}
+
+// writeToFineGridInterface(DIAD): include/gridRefinementFunctions.h:624-702.
+// The coarse read cell, rescaled coarse -> fine, into the fine write cell and
+// its +i +j +k +ij +ik +jk +ijk neighbours.
+template <typename R>
+void eso_to_fine(const R* fc, long long nc, const int* const* nbc, bool flip_c, R* ff, long long nf,
+                 const int* const* nbf, bool flip_f, const int* write, const int* read, long long m) {
+  for (long long t = 0; t < m; ++t) {
+    R f[27];
+    eso_get(fc, nc, nbc, read[t], flip_c, f);
+    rescale_f(f, true);
+    const long long wc = write[t];
+    long long fine[8];
+    fine[0] = wc;
+    for (int a = 0; a < 7; ++a) fine[a + 1] = nbf[a][wc];
+    for (int w = 0; w < 8; ++w) eso_put(ff, nf, nbf, fine[w], flip_f, f);
+  }
+}
+
+// getSphereDrag(DIADGridStruct&): flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198,
+// all three components: sum over bounceback cells of rho (u - u after bounceback).
+template <typename R>
+void eso_momentum_exchange(const R* fa, long long n, const int* const* nbr, bool flipper, const uint8_t* flags,
+                           double* out3) {
+  double acc[3] = {0, 0, 0};
+  for (long long c = 0; c < n; ++c) {
+    if ((flags[c] & 7) != K_BOUNCEBACK) continue;
+    R f[27];
+    eso_get(fa, n, nbr, c, flipper, f);
+    R r, a, b, d;
+    get_rho_u(f, r, a, b, d);
+    bounceback(f);
+    R r2, a2, b2, d2;
+    get_rho_u(f, r2, a2, b2, d2);
+    acc[0] += (double)(r * (a - a2));
+    acc[1] += (double)(r * (b - b2));
+    acc[2] += (double)(r * (d - d2));
+  }
+  for (int a = 0; a < 3; ++a) out3[a] = acc[a];
+}
+
 // ------------------------------------------------------------------- STL voxelizer
 // getRayHitYesNo: include/STLFunctions.h:184-245 (exact integer predicate)
 bool ray_hit(long long ax, long long ay, long long bx, long long by, long long cx, long long cy) {
@@ -1553,6 +1633,31 @@ void oracle_velocity_set(int* cx, int* cy, int* cz) {
 
 ORACLE_PER_CELL(f32, float)
 ORACLE_PER_CELL(f64, double)
+
+#define ORACLE_ESO_LEVELS(SUF, R)                                                                              \
+  void oracle_eso_to_coarse_##SUF(R* fc, long long nc, const int* nbc7, int flip_c, const R* ff, long long nf,  \
+                                  const int* nbf7, int flip_f, const int* write, const int* read, long long m) { \
+    const int* a[7];                                                                                           \
+    const int* b[7];                                                                                           \
+    for (int t = 0; t < 7; ++t) { a[t] = nbc7 + (size_t)t * nc; b[t] = nbf7 + (size_t)t * nf; }              \
+    eso_to_coarse(fc, nc, a, flip_c != 0, ff, nf, b, flip_f != 0, write, read, m);                            \
+  }                                                                                                            \
+  void oracle_eso_to_fine_##SUF(const R* fc, long long nc, const int* nbc7, int flip_c, R* ff, long long nf,    \
+                                const int* nbf7, int flip_f, const int* write, const int* read, long long m) {   \
+    const int* a[7];                                                                                           \
+    const int* b[7];                                                                                           \
+    for (int t = 0; t < 7; ++t) { a[t] = nbc7 + (size_t)t * nc; b[t] = nbf7 + (size_t)t * nf; }              \
+    eso_to_fine(fc, nc, a, flip_c != 0, ff, nf, b, flip_f != 0, write, read, m);                              \
+  }                                                                                                            \
+  void oracle_eso_momentum_exchange_##SUF(const R* fa, long long n, const int* nbr7, int flipper,              \
+                                          const uint8_t* flags, double* out3) {                                \
+    const int* a[7];                                                                                           \
+    for (int t = 0; t < 7; ++t) a[t] = nbr7 + (size_t)t * n;                                                   \
+    eso_momentum_exchange(fa, n, a, flipper != 0, flags, out3);                                                \
+  }
+
+ORACLE_ESO_LEVELS(f32, float)
+ORACLE_ESO_LEVELS(f64, double)
 
 void oracle_stream_shift(int* shift, int nx, int ny, int nz) { stream_shift(shift, nx, ny, nz); }
 

@@ -206,6 +206,11 @@ _SIGS = [
     ("psl_diad_levels_info", _I, [_P, _I, _P, _P]),
     ("psl_diad_levels_get", _I, [_P, _I, _I, _P]),
     ("psl_diad_levels_build_ms", _I, [_P, _P]),
+    ("psl_diad_create_level", _I, [_P, _I, _I, _P]),
+    ("psl_diad_set_interface", _I, [_P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64]),
+    ("psl_diad_coarse_fine_communication", _I, [_P, _P]),
+    ("psl_diad_update_grid", _I, [_P, _I, _I]),
+    ("psl_diad_reduce_momentum_exchange", _I, [_P, _P]),
 ]
 
 
@@ -473,10 +478,17 @@ class Batch:
 class DIADGridStruct:
     """include/types.h:166-171 -- esotwist grid over an explicit cell list."""
 
-    def __init__(self, info: InfoStruct, ijk3: np.ndarray, nbr7: np.ndarray, storage: int | str = F32):
+    def __init__(self, info: InfoStruct, ijk3: np.ndarray, nbr7: np.ndarray, storage: int | str = F32,
+                 _handle: ctypes.c_void_p | None = None):
         if isinstance(storage, str):
             storage = STORAGE_NAMES[storage]
         require_device()
+        if _handle is not None:
+            self._h = _handle
+            self.Info = info
+            self.n = info.cellCount
+            self.storage = storage
+            return
         ijk3 = np.ascontiguousarray(ijk3, np.int32)
         nbr7 = np.ascontiguousarray(nbr7, np.int32)
         self.n = ijk3.shape[1]
@@ -510,6 +522,26 @@ class DIADGridStruct:
     @property
     def handle(self):
         return self._h
+
+    @staticmethod
+    def from_levels(levels: "DIADLevels", level: int, storage: int | str = F32) -> "DIADGridStruct":
+        """The grid of one built level (tables stay on the device)."""
+        st = STORAGE_NAMES[storage] if isinstance(storage, str) else storage
+        info, _ = levels.info(level)
+        h = ctypes.c_void_p()
+        _check(lib().psl_diad_create_level(levels.handle, level, st, ctypes.byref(h)))
+        return DIADGridStruct(info, None, None, st, _handle=h)
+
+    def set_interface(self, fine: "DIADGridStruct", c2f_read, c2f_write, f2c_write, f2c_read):
+        a = [np.ascontiguousarray(x, np.int32) for x in (c2f_read, c2f_write, f2c_write, f2c_read)]
+        _check(lib().psl_diad_set_interface(self._h, fine.handle, a[0].ctypes.data, a[1].ctypes.data, a[0].size,
+                                            a[2].ctypes.data, a[3].ctypes.data, a[2].size))
+
+    def momentum_exchange(self) -> np.ndarray:
+        """getSphereDrag(DIADGridStruct&), all three components in FP64."""
+        out = np.zeros(3, np.float64)
+        _check(lib().psl_diad_reduce_momentum_exchange(self._h, out.ctypes.data))
+        return out
 
     def set_flags(self, flags):
         a = None if flags is None else np.ascontiguousarray(flags, np.uint8)
@@ -623,6 +655,9 @@ class DIADLevels:
         elif name == "nbr":
             a = a.reshape(10, -1)
         return a
+
+    def grids(self, storage: int | str = F32) -> list:
+        return [DIADGridStruct.from_levels(self, l, storage) for l in range(self.levels)]
 
     def level(self, level: int) -> dict:
         info, _ = self.info(level)
@@ -829,17 +864,25 @@ def applyBCPeriodicY(grid: GridStruct):
     _check(lib().psl_apply_bc_periodic_y(grid.handle))
 
 
-def applyCoarseFineGridCommunication(coarse: GridStruct, fine: GridStruct):
-    _check(lib().psl_coarse_fine_communication(coarse.handle, fine.handle))
+def applyCoarseFineGridCommunication(coarse, fine):
+    """GridStruct pair (include/gridRefinementFunctions.h:433-526) or DIADGridStruct pair (540-708)."""
+    if isinstance(coarse, DIADGridStruct):
+        _check(lib().psl_diad_coarse_fine_communication(coarse.handle, fine.handle))
+    else:
+        _check(lib().psl_coarse_fine_communication(coarse.handle, fine.handle))
 
 
 def fillCoarseGridFromFine(coarse: GridStruct, fine: GridStruct):
     _check(lib().psl_fill_coarse_from_fine(coarse.handle, fine.handle))
 
 
-def updateGrid(grids: Sequence[GridStruct], steps: int = 1):
+def updateGrid(grids, steps: int = 1):
+    """The level recursion over GridStruct or DIADGridStruct levels, `steps` coarse steps."""
     arr = (ctypes.c_void_p * len(grids))(*[g.handle.value for g in grids])
-    _check(lib().psl_update_grid(arr, len(grids), steps))
+    if grids and isinstance(grids[0], DIADGridStruct):
+        _check(lib().psl_diad_update_grid(arr, len(grids), steps))
+    else:
+        _check(lib().psl_update_grid(arr, len(grids), steps))
 
 
 # ------------------------------------------------------------ geometry

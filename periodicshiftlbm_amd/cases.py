@@ -282,3 +282,61 @@ def two_level_sphere(storage: str = "f16", reynolds: float = 1e5, u_inlet: float
     lo = [int((c - r - o) / 0.5) - 2 for c, o in zip(center, (ox, oy, oz))]
     hi = [int((c + r - o) / 0.5) + 3 for c, o in zip(center, (ox, oy, oz))]
     return grids, (lo, hi)
+
+
+# ------------------------------------------------------------ DIAD sphere
+def diad_sphere_flags(ijk: np.ndarray, bounceback: np.ndarray, shape, level: int) -> np.ndarray:
+    """Flag bytes of one level of the wall-refined sphere channel
+    (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:41-96): on level 0
+    given u on the j, k and i = 0 faces, given rho at i = Nx-1; on finer levels
+    the box faces are ghost cells.  Bounceback cells come from the builder."""
+    i, j, k = (np.asarray(a) for a in ijk)
+    nx, ny, nz = shape
+    flags = np.full(i.size, FLUID, np.uint8)
+    if level == 0:
+        flags[i == nx - 1] = GIVEN_RHO | (1 << 6)
+        flags[i == 0] = GIVEN_U
+        flags[(k == 0) | (k == nz - 1)] = GIVEN_U
+        flags[(j == 0) | (j == ny - 1)] = GIVEN_U
+    else:
+        edge = (i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)
+        flags[edge] = GHOST
+    flags[np.asarray(bounceback, bool)] = BOUNCEBACK
+    return flags
+
+
+def diad_sphere_rules(P, radius: float, levels: int, refine=((1.5, 4.0), (1.0, 2.5), (0.7, 1.0), (0.6, 0.6))):
+    """Marker rules of the DIAD sphere driver: the sphere wall on every level and,
+    on the coarser levels, a refinement region made of a wake cylinder of
+    half-width w D and length l D downstream plus a sphere of radius w D
+    (flowAroundSphereMainSingleDIAD.cu:47-80)."""
+    d = 2.0 * radius
+    rs = [P.marker_rule(P.RULE_BOUNCEBACK, P.SHAPE_SPHERE_LE, p=(0.0, 0.0, 0.0, radius))]
+    for lev, (w, l) in enumerate(refine[:max(0, levels - 1)]):
+        rs.append(P.marker_rule(P.RULE_REFINEMENT, P.SHAPE_XCYLINDER, p=(0.0, 0.0, 0.0, w * d, l * d), levels=[lev]))
+        rs.append(P.marker_rule(P.RULE_REFINEMENT, P.SHAPE_SPHERE_LT, p=(0.0, 0.0, 0.0, w * d), levels=[lev]))
+    return P.rules(*rs)
+
+
+def diad_sphere(P, storage="f32", n0: int = 24, levels: int = 3, span: int = 1, radius: float = 2.0,
+                res: float = 1.0, nu0: float = 0.02, u_inlet: float = 0.04, cs: float = 0.1):
+    """Wall-refined sphere channel on DIAD levels: a base box of 2 n0 x n0 x n0
+    cells of size res with the sphere (centre at the origin) at a quarter of the
+    length; returns (levels, grids).  Physical viscosity and time step are chosen
+    so that level 0 has lattice viscosity nu0 (finer levels: 2^l nu0)."""
+    dt = 1e-3
+    nu_phys = nu0 * (res / 1000.0) ** 2 / dt
+    info = P.InfoStruct(res=res, cellCountX=2 * n0, cellCountY=n0, cellCountZ=n0, ox=-0.5 * n0 * res + 0.5 * res,
+                        oy=-0.5 * n0 * res + 0.5 * res, oz=-0.5 * n0 * res + 0.5 * res, nu=nu0, dtPhys=dt)
+    lv = P.buildDIADGrids(info, levels, span, nu_phys, (), diad_sphere_rules(P, radius, levels))
+    grids = []
+    for l in range(levels):
+        g = P.DIADGridStruct.from_levels(lv, l, storage)
+        li, _ = lv.info(l)
+        g.set_flags(diad_sphere_flags(lv.array(l, "ijk"), lv.array(l, "bounceback"),
+                                      (li.cellCountX, li.cellCountY, li.cellCountZ), l))
+        g.set_given_values(channel_given_values(u_inlet))
+        g.set_smagorinsky(cs)
+        g.fill_equilibrium()
+        grids.append(g)
+    return lv, grids

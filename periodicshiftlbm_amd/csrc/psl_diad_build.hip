@@ -300,6 +300,11 @@ __global__ void k_map_children(const uint64_t* __restrict__ ckey, const int* __r
   }
 }
 
+__global__ void k_flags_from_bb(const uint8_t* __restrict__ bb, int64_t n, uint8_t* __restrict__ flags) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+    flags[c] = bb[c] ? (uint8_t)KIND_BOUNCEBACK : (uint8_t)KIND_FLUID;
+}
+
 __global__ void k_unpack_ijk(const uint64_t* __restrict__ key, int64_t n, int* __restrict__ ijk) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
     int i, j, k;
@@ -874,6 +879,57 @@ int psl_diad_levels_get(const psl_diad_levels* levels, int level, int which, voi
       return PSL_OK;
     }
   }
+}
+
+int psl_diad_create_level(const psl_diad_levels* levels, int level, int storage, psl_diad** out) {
+  PSL_CHECK_ARG(levels && out && level >= 0 && level < levels->levels, "bad level");
+  PSL_CHECK_ARG(storage == PSL_F32 || storage == PSL_F16DEV || storage == PSL_BF16DEV, "unknown storage");
+  const DiadLevel& L = levels->lv[level];
+  const int64_t n = L.n;
+  PSL_CHECK_ARG(n > 0, "empty level");
+  psl_diad* d = new psl_diad();
+  d->info = L.info;
+  d->info.gridID = (float)level;
+  d->info.cellCount = (int)n;
+  d->storage = storage;
+  d->n = n;
+  auto fail = [&](const char* what) {
+    set_error(std::string("DIAD level grid: ") + what);
+    psl_diad_destroy(d);
+    return PSL_E_HIP;
+  };
+  const size_t fb = 27ull * (size_t)n * storage_bytes(storage);
+  if (hipMalloc(&d->f, fb) != hipSuccess || hipMalloc(&d->bc_dev, 4 * sizeof(float4)) != hipSuccess ||
+      hipMemset(d->bc_dev, 0, 4 * sizeof(float4)) != hipSuccess ||
+      hipMalloc(&d->ijk, 3ull * (size_t)n * sizeof(int)) != hipSuccess ||
+      hipMalloc(&d->nbr, 7ull * (size_t)n * sizeof(int)) != hipSuccess || hipMalloc(&d->flags, (size_t)n) != hipSuccess)
+    return fail("allocation");
+  k_unpack_ijk<<<blocks_for(n), TPB>>>(L.key.p, n, d->ijk);
+  k_flags_from_bb<<<blocks_for(n), TPB>>>(L.bb.p, n, d->flags);
+  if (hipGetLastError() != hipSuccess) return fail("tables");
+  // the first seven neighbour columns (+i +j +k +ij +ik +jk +ijk)
+  if (hipMemcpy(d->nbr, L.nbr.p, 7ull * (size_t)n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess)
+    return fail("neighbours");
+  const int64_t nc = L.cnt[A_C2F_READ], nf = L.cnt[A_F2C_WRITE];
+  if (nc) {
+    if (hipMalloc(&d->c2f_read, (size_t)nc * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d->c2f_write, (size_t)nc * sizeof(int)) != hipSuccess ||
+        hipMemcpy(d->c2f_read, L.arr[A_C2F_READ].p, (size_t)nc * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(d->c2f_write, L.arr[A_C2F_WRITE].p, (size_t)nc * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess)
+      return fail("interface");
+  }
+  if (nf) {
+    if (hipMalloc(&d->f2c_write, (size_t)nf * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d->f2c_read, (size_t)nf * sizeof(int)) != hipSuccess ||
+        hipMemcpy(d->f2c_write, L.arr[A_F2C_WRITE].p, (size_t)nf * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(d->f2c_read, L.arr[A_F2C_READ].p, (size_t)nf * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess)
+      return fail("interface");
+  }
+  d->n_c2f = nc;
+  d->n_f2c = nf;
+  if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
+  *out = d;
+  return PSL_OK;
 }
 
 }  // extern "C"

@@ -123,21 +123,118 @@ __global__ __launch_bounds__(256) void k_eso_io(typename St::T* __restrict__ f, 
   }
 }
 
-}  // namespace psl
+// Populations of a cell as its last update left them: the write maps of the
+// current parity (getEsotwistWriteIndex, include/applyStreaming.h:138-192),
+// full values (the lattice weights added back for 16-bit storage).
+template <class St, bool FLIP>
+__device__ __forceinline__ void eso_get(const typename St::T* __restrict__ f, const EsoParams& p, uint32_t cell,
+                                        float (&v)[27]) {
+  uint32_t slot[8];
+  eso_slots<St>(p, cell, slot);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int fi = FLIP ? opp(q) : q;
+    float x = St::load(f[(size_t)fi * p.n + slot[ESO_WRITE[q]]]);
+    if (St::W_OFF) x += weight(q);
+    v[q] = x;
+  }
+}
 
-struct psl_diad {
-  psl_info info;
-  int storage = PSL_F32;
-  int64_t n = 0;
-  void* f = nullptr;
-  int* ijk = nullptr;
-  int* nbr = nullptr;
-  uint8_t* flags = nullptr;
-  float4* bc_dev = nullptr;
-  float cs = 0.f;
-  bool flipper = false;
-  hipStream_t stream = nullptr;
-};
+template <class St, bool FLIP>
+__device__ __forceinline__ void eso_put(typename St::T* __restrict__ f, const EsoParams& p, uint32_t cell,
+                                        const float (&v)[27]) {
+  uint32_t slot[8];
+  eso_slots<St>(p, cell, slot);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int fi = FLIP ? opp(q) : q;
+    float x = v[q];
+    if (St::W_OFF) x -= weight(q);
+    f[(size_t)fi * p.n + slot[ESO_WRITE[q]]] = St::store(x);
+  }
+}
+
+// writeToCoarseGridInterface(DIAD): include/gridRefinementFunctions.h:540-622.
+// One thread per interface cell: the mean of the fine read cell and its +i
+// +j +k +ij +ik +jk +ijk neighbours (summed in that order), rescaled
+// fine -> coarse, written into the coarse cell.
+template <class St, bool FC, bool FF>
+__global__ __launch_bounds__(256) void k_eso_to_coarse(typename St::T* __restrict__ fc, EsoParams pc,
+                                                       const typename St::T* __restrict__ ff, EsoParams pf,
+                                                       const int* __restrict__ write, const int* __restrict__ read,
+                                                       uint32_t m) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= m) return;
+  const uint32_t rc = (uint32_t)read[t];
+  uint32_t fine[8];
+  fine[0] = rc;
+#pragma unroll
+  for (int a = 0; a < 7; ++a) fine[a + 1] = (uint32_t)pf.nbr[(size_t)a * pf.n + rc];
+  float acc[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) acc[q] = 0.f;
+  for (int w = 0; w < 8; ++w) {
+    float g[27];
+    eso_get<St, FF>(ff, pf, fine[w], g);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) acc[q] += g[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) acc[q] = acc[q] * 0.125f;
+  rescale(acc, false);
+  eso_put<St, FC>(fc, pc, (uint32_t)write[t], acc);
+}
+
+// writeToFineGridInterface(DIAD): include/gridRefinementFunctions.h:624-702.
+// The coarse read cell, rescaled coarse -> fine, into the fine write cell and
+// its +i +j +k +ij +ik +jk +ijk neighbours.
+template <class St, bool FC, bool FF>
+__global__ __launch_bounds__(256) void k_eso_to_fine(const typename St::T* __restrict__ fc, EsoParams pc,
+                                                     typename St::T* __restrict__ ff, EsoParams pf,
+                                                     const int* __restrict__ write, const int* __restrict__ read,
+                                                     uint32_t m) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= m) return;
+  float v[27];
+  eso_get<St, FC>(fc, pc, (uint32_t)read[t], v);
+  rescale(v, true);
+  const uint32_t wc = (uint32_t)write[t];
+  eso_put<St, FF>(ff, pf, wc, v);
+  for (int a = 0; a < 7; ++a) eso_put<St, FF>(ff, pf, (uint32_t)pf.nbr[(size_t)a * pf.n + wc], v);
+}
+
+// getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
+// all three components: per bounceback cell rho (u - u after bounceback), in
+// FP64 per-block partial sums.
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_momentum(const typename St::T* __restrict__ f, EsoParams p,
+                                                      double* __restrict__ partials) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (cell < p.n && p.flags && (p.flags[cell] & 7) == KIND_BOUNCEBACK) {
+    float v[27];
+    eso_get<St, FLIP>(f, p, cell, v);
+    float r, a, b, c;
+    rho_u(v, r, a, b, c);
+    bounceback(v);
+    float r2, a2, b2, c2;
+    rho_u(v, r2, a2, b2, c2);
+    acc[0] = (double)(r * (a - a2));
+    acc[1] = (double)(r * (b - b2));
+    acc[2] = (double)(r * (c - c2));
+  }
+  __shared__ double sh[3][256];
+  for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] = acc[a];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < (unsigned)s)
+      for (int a = 0; a < 3; ++a) sh[a][threadIdx.x] += sh[a][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) partials[3 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+}  // namespace psl
 
 using namespace psl;
 
@@ -188,6 +285,38 @@ static int eso_io(psl_diad* d, float* buf) {
   return PSL_OK;
 }
 
+template <class St, bool FC, bool FF>
+static int eso_comm_t(psl_diad* c, psl_diad* f) {
+  typedef typename St::T T;
+  const EsoParams pc = eso_params(c), pf = eso_params(f);
+  if (c->n_f2c) {
+    k_eso_to_coarse<St, FC, FF><<<(unsigned)((c->n_f2c + 255) / 256), 256, 0, c->stream>>>(
+        (T*)c->f, pc, (const T*)f->f, pf, c->f2c_write, c->f2c_read, (uint32_t)c->n_f2c);
+    PSL_LAUNCH_CHECK();
+  }
+  if (c->n_c2f) {
+    k_eso_to_fine<St, FC, FF><<<(unsigned)((c->n_c2f + 255) / 256), 256, 0, c->stream>>>(
+        (const T*)c->f, pc, (T*)f->f, pf, c->c2f_write, c->c2f_read, (uint32_t)c->n_c2f);
+    PSL_LAUNCH_CHECK();
+  }
+  return PSL_OK;
+}
+
+template <class St>
+static int eso_comm(psl_diad* c, psl_diad* f) {
+  if (c->flipper) return f->flipper ? eso_comm_t<St, true, true>(c, f) : eso_comm_t<St, true, false>(c, f);
+  return f->flipper ? eso_comm_t<St, false, true>(c, f) : eso_comm_t<St, false, false>(c, f);
+}
+
+template <class St>
+static int eso_momentum(psl_diad* d, uint32_t nb) {
+  const EsoParams p = eso_params(d);
+  if (d->flipper) k_eso_momentum<St, true><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, d->partials);
+  else k_eso_momentum<St, false><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, d->partials);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
 #define PSL_ESO_DISPATCH(fn, ...)                                         \
   (d->storage == PSL_F32 ? fn<StF32>(__VA_ARGS__)                         \
                          : d->storage == PSL_F16DEV ? fn<StF16>(__VA_ARGS__) : fn<StBF16>(__VA_ARGS__))
@@ -230,6 +359,9 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->nbr) (void)hipFree(d->nbr);
   if (d->flags) (void)hipFree(d->flags);
   if (d->bc_dev) (void)hipFree(d->bc_dev);
+  for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read})
+    if (a) (void)hipFree(a);
+  if (d->partials) (void)hipFree(d->partials);
   delete d;
   return PSL_OK;
 }
@@ -334,6 +466,90 @@ int psl_diad_set_f(psl_diad* d, const float* host) {
   (void)hipFree(buf);
   if (rc == PSL_E_HIP) set_error("DIAD set_f failed");
   return rc;
+}
+
+static int upload_ints(int** dst, const int* src, int64_t n) {
+  if (*dst) PSL_HIP_TRY(hipFree(*dst));
+  *dst = nullptr;
+  if (n == 0) return PSL_OK;
+  PSL_HIP_TRY(hipMalloc(dst, (size_t)n * sizeof(int)));
+  PSL_HIP_TRY(hipMemcpy(*dst, src, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+  return PSL_OK;
+}
+
+int psl_diad_set_interface(psl_diad* coarse, const psl_diad* fine, const int* c2f_read, const int* c2f_write,
+                           int64_t n_c2f, const int* f2c_write, const int* f2c_read, int64_t n_f2c) {
+  PSL_CHECK_ARG(coarse && fine && n_c2f >= 0 && n_f2c >= 0, "bad interface arguments");
+  PSL_CHECK_ARG((n_c2f == 0 || (c2f_read && c2f_write)) && (n_f2c == 0 || (f2c_write && f2c_read)), "null table");
+  for (int64_t t = 0; t < n_c2f; ++t)
+    PSL_CHECK_ARG(c2f_read[t] >= 0 && c2f_read[t] < coarse->n && c2f_write[t] >= 0 && c2f_write[t] < fine->n,
+                  "coarse-to-fine index out of range");
+  for (int64_t t = 0; t < n_f2c; ++t)
+    PSL_CHECK_ARG(f2c_write[t] >= 0 && f2c_write[t] < coarse->n && f2c_read[t] >= 0 && f2c_read[t] < fine->n,
+                  "fine-to-coarse index out of range");
+  int rc = upload_ints(&coarse->c2f_read, c2f_read, n_c2f);
+  if (rc == PSL_OK) rc = upload_ints(&coarse->c2f_write, c2f_write, n_c2f);
+  if (rc == PSL_OK) rc = upload_ints(&coarse->f2c_write, f2c_write, n_f2c);
+  if (rc == PSL_OK) rc = upload_ints(&coarse->f2c_read, f2c_read, n_f2c);
+  coarse->n_c2f = rc == PSL_OK ? n_c2f : 0;
+  coarse->n_f2c = rc == PSL_OK ? n_f2c : 0;
+  return rc;
+}
+
+int psl_diad_coarse_fine_communication(psl_diad* coarse, psl_diad* fine) {
+  PSL_CHECK_ARG(coarse && fine, "null grid");
+  PSL_CHECK_ARG(coarse->storage == fine->storage, "levels must share the storage type");
+  PSL_CHECK_ARG(coarse->stream == fine->stream, "levels must share the stream");
+  psl_diad* d = coarse;
+  return PSL_ESO_DISPATCH(eso_comm, coarse, fine);
+}
+
+// updateGrid(std::vector<DIADGridStruct>&, level)
+// (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:200-209)
+static int diad_recurse(psl_diad* const* lv, int nl, int l) {
+  psl_diad* d = lv[l];
+  d->flipper = !d->flipper;
+  int rc = PSL_ESO_DISPATCH(eso_update, d);
+  if (rc != PSL_OK || l == nl - 1) return rc;
+  for (int s = 0; s < 2 && rc == PSL_OK; ++s) rc = diad_recurse(lv, nl, l + 1);
+  if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_comm, d, lv[l + 1]);
+  return rc;
+}
+
+int psl_diad_update_grid(psl_diad* const* levels, int nlevels, int nsteps) {
+  PSL_CHECK_ARG(levels && nlevels >= 1 && nsteps >= 0, "bad level list");
+  for (int l = 0; l < nlevels; ++l) {
+    PSL_CHECK_ARG(levels[l], "null level");
+    PSL_CHECK_ARG(levels[l]->storage == levels[0]->storage, "levels must share the storage type");
+    PSL_CHECK_ARG(levels[l]->stream == levels[0]->stream, "levels must share the stream");
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    const int rc = diad_recurse(levels, nlevels, 0);
+    if (rc != PSL_OK) return rc;
+  }
+  return PSL_OK;
+}
+
+int psl_diad_reduce_momentum_exchange(psl_diad* d, double out[3]) {
+  PSL_CHECK_ARG(d && out, "null argument");
+  const uint32_t nb = (uint32_t)((d->n + 255) / 256);
+  if (d->n_partials < 3 * (size_t)nb) {
+    if (d->partials) PSL_HIP_TRY(hipFree(d->partials));
+    d->partials = nullptr;
+    d->n_partials = 0;
+    PSL_HIP_TRY(hipMalloc(&d->partials, 3 * (size_t)nb * sizeof(double)));
+    d->n_partials = 3 * (size_t)nb;
+  }
+  const int rc = PSL_ESO_DISPATCH(eso_momentum, d, nb);
+  if (rc != PSL_OK) return rc;
+  std::vector<double> h(3 * (size_t)nb);
+  PSL_HIP_TRY(hipMemcpyAsync(h.data(), d->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(d->stream));
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (uint32_t b = 0; b < nb; ++b)
+    for (int a = 0; a < 3; ++a) acc[a] += h[3 * (size_t)b + a];
+  for (int a = 0; a < 3; ++a) out[a] = acc[a];
+  return PSL_OK;
 }
 
 // Neighbour maps of a dense nx*ny*nz box (cell = (k*ny + j)*nx + i); axes with

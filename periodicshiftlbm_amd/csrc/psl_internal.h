@@ -133,6 +133,32 @@ struct psl_mask {
   int64_t n = 0;
 };
 
+// esotwist (DIAD) grid: fArray(q, cell) at f[q * n + cell], ijk[3][n],
+// nbr[7][n] (+i +j +k +ij +ik +jk +ijk), and, on a coarser level of a
+// hierarchy, the coarse<->fine interface tables of DIADGridStruct
+// (include/types.h:166-171).
+struct psl_diad {
+  psl_info info;
+  int storage = PSL_F32;
+  int64_t n = 0;
+  void* f = nullptr;
+  int* ijk = nullptr;
+  int* nbr = nullptr;
+  uint8_t* flags = nullptr;
+  float4* bc_dev = nullptr;
+  float cs = 0.f;
+  bool flipper = false;
+  hipStream_t stream = nullptr;
+  int* c2f_read = nullptr;   // coarse cells
+  int* c2f_write = nullptr;  // fine cells
+  int64_t n_c2f = 0;
+  int* f2c_write = nullptr;  // coarse cells
+  int* f2c_read = nullptr;   // fine cells
+  int64_t n_f2c = 0;
+  double* partials = nullptr;  // reduction scratch
+  size_t n_partials = 0;
+};
+
 struct psl_stl {
   std::vector<float> tri;  // 9 floats per triangle (host copy)
   float* d_tri = nullptr;  // device copy

@@ -1,0 +1,92 @@
+"""DIAD level transfers in the oracle (include/gridRefinementFunctions.h:540-708): CPU pins.
+
+On a two-level wall-refined sphere channel built by the oracle's builder:
+a uniform rest equilibrium passes both transfers unchanged; every coarse
+fine-to-coarse cell receives rescaleF(mean of its eight children, fine->coarse)
+and every child of a coarse-to-fine cell receives rescaleF(coarse, coarse->fine),
+where the children are looked up by their (2i+a, 2j+b, 2k+c) coordinates in the
+fine cell table, not through the esotwist neighbour maps the transfers use.
+FP64 instantiation, so the pins are tight.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import eso_levels as E
+
+import periodicshiftlbm_amd as P
+from periodicshiftlbm_amd import cases
+
+
+@pytest.fixture(scope="module")
+def two_levels():
+    base = dict(res=1.0, ox=-5.5, oy=-5.5, oz=-5.5, nu=0.02, dt=1e-3, nx=24, ny=12, nz=12)
+    lv = O.diad_build(base, 2, 1, 2e-5, (), cases.diad_sphere_rules(P, 1.6, 2))
+    bc = cases.channel_given_values(0.04)
+    levels = []
+    for l, d in enumerate(lv):
+        flags = cases.diad_sphere_flags(d["ijk"], d["bounceback"], (d["nx"], d["ny"], d["nz"]), l)
+        levels.append(E.OracleLevel(d, flags, bc, d["nu"], 0.1, dtype=np.float64))
+    assert lv[0]["f2c_write"].size > 0 and lv[0]["c2f_read"].size > 0
+    return lv, levels
+
+
+def _children(d_c, d_f, cells):
+    i, j, k = (a.astype(np.int64) for a in d_c["ijk"])
+    fi, fj, fk = (a.astype(np.int64) for a in d_f["ijk"])
+    key = (fk * d_f["ny"] + fj) * d_f["nx"] + fi
+    order = np.argsort(key)
+    out = []
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                want = ((2 * k[cells] + c) * d_f["ny"] + 2 * j[cells] + b) * d_f["nx"] + 2 * i[cells] + a
+                pos = np.searchsorted(key[order], want)
+                assert np.all(key[order][pos] == want)
+                out.append(order[pos])
+    return np.array(out)  # 8 x m
+
+
+def test_uniform_equilibrium_passes_unchanged(two_levels):
+    lv, (c, f) = two_levels
+    # a rest state (the polynomial equilibrium at u != 0 is not a fixed point of
+    # rescaleF: its higher central moments differ from the product form)
+    e = O.feq(np.array([1.02, 0.0, 0.0, 0.0]))
+    for L, flip in ((c, False), (f, True)):
+        L.flip = flip
+        L.set(np.repeat(e[:, None], L.n, axis=1))
+    E.communicate(c, f)
+    for L in (c, f):
+        assert np.max(np.abs(L.get() - e[:, None])) < 1e-14
+
+
+def test_transfers_route_to_children(two_levels):
+    lv, (c, f) = two_levels
+    rng = np.random.default_rng(5)
+    for L in (c, f):
+        L.flip = bool(rng.integers(2))
+        ru = np.column_stack([1 + 0.01 * rng.standard_normal(L.n), 0.03 * rng.standard_normal((L.n, 3))])
+        L.set((O.feq(ru) + 1e-3 * rng.standard_normal((L.n, 27))).T)
+    fc0, ff0 = c.get(), f.get()
+    E.communicate(c, f)
+    fc1, ff1 = c.get(), f.get()
+    # fine -> coarse
+    w, r = lv[0]["f2c_write"], lv[0]["f2c_read"]
+    ch = _children(lv[0], lv[1], w)
+    assert np.array_equal(ch[0], r)
+    mean = ff0[:, ch].mean(axis=1).T  # m x 27
+    want = O.rescale(np.ascontiguousarray(mean), False)
+    assert np.max(np.abs(fc1[:, w].T - want)) < 1e-14
+    # coarse -> fine
+    w2, r2 = lv[0]["c2f_write"], lv[0]["c2f_read"]
+    ch2 = _children(lv[0], lv[1], r2)
+    assert np.array_equal(ch2[0], w2)
+    want2 = O.rescale(np.ascontiguousarray(fc0[:, r2].T), True)
+    for a in range(8):
+        assert np.max(np.abs(ff1[:, ch2[a]].T - want2)) < 1e-14
+    # everything else untouched
+    others = np.setdiff1d(np.arange(c.n), w)
+    assert np.array_equal(fc1[:, others], fc0[:, others])
+    touched = np.unique(ch2)
+    others_f = np.setdiff1d(np.arange(f.n), touched)
+    assert np.array_equal(ff1[:, others_f], ff0[:, others_f])

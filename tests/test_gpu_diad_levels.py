@@ -1,0 +1,119 @@
+"""Wall-refined DIAD levels on the device: transfers, level recursion, drag.
+
+A sphere channel is built on the device (psl_diad_build; its tables equal the
+oracle's bit for bit, tests/test_gpu_diad_builder.py), each level becomes an
+esotwist grid (psl_diad_create_level) with the DIAD sphere driver's flags, and
+the run is compared with the oracle's recursion over the same tables
+(tests/eso_levels.py): populations of every level after each block of coarse
+steps within 1e-5 relative (FP32 storage) and the FP64 momentum exchange on
+the finest level.  The coarse<->fine communication alone is compared on a
+random state, and FP16 storage is held to the FP32 oracle with a bound on the
+absolute error.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import eso_levels as E
+
+from periodicshiftlbm_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _rel_err(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)) / np.maximum(np.abs(b), 1e-3)))
+
+
+def _oracle_levels(P, lv, dtype=np.float32, cs=0.1):
+    out = []
+    for l in range(lv.levels):
+        d = lv.level(l)
+        info = d["info"]
+        dd = dict(d, nx=info.cellCountX, ny=info.cellCountY, nz=info.cellCountZ)
+        flags = cases.diad_sphere_flags(d["ijk"], d["bounceback"], (dd["nx"], dd["ny"], dd["nz"]), l)
+        out.append(E.OracleLevel(dd, flags, cases.channel_given_values(0.04), info.nu, cs, dtype=dtype))
+    return out
+
+
+def _setup(P, storage, n0=16, levels=3, ux0=0.0):
+    lv, grids = cases.diad_sphere(P, storage, n0=n0, levels=levels, span=1, radius=2.0)
+    ol = _oracle_levels(P, lv)
+    for g, o in zip(grids, ol):
+        if ux0:
+            g.fill_equilibrium(ux=ux0)
+        o.set(g.get_f())
+    counts = [g.n for g in grids]
+    print(f"[diad-levels] storage={storage} cells per level {counts}", flush=True)
+    return lv, grids, ol
+
+
+def test_diad_levels_match_oracle(psl):
+    P = psl
+    # started from uniform flow so that the sphere carries a load from the first step
+    lv, grids, ol = _setup(P, "f32", ux0=0.04)
+    for block in range(3):
+        P.updateGrid(grids, 2)
+        for _ in range(2):
+            E.update_grid(ol)
+        for l, (g, o) in enumerate(zip(grids, ol)):
+            assert g.esotwistFlipper == o.flip
+            err = _rel_err(g.get_f(), o.get())
+            print(f"  after {2 * (block + 1)} coarse steps, level {l}: max rel err {err:.3e}", flush=True)
+            assert err < RTOL
+    fx = grids[-1].momentum_exchange()
+    fo = O.eso_momentum_exchange(ol[-1].f, ol[-1].nbr7, ol[-1].flip, ol[-1].flags)
+    print(f"  momentum exchange GPU {fx} oracle {fo}", flush=True)
+    assert abs(fx[0] - fo[0]) <= 1e-4 * abs(fo[0])
+    assert np.all(np.abs(fx - fo) <= 1e-4 * abs(fo[0]))
+    # FP64 from the GPU's own populations: bounceback reverses the momentum, so
+    # rho (u - u_bounced) = 2 sum_q c_q f_q per bounceback cell
+    f = grids[-1].get_f().astype(np.float64)
+    bb = ol[-1].flags == cases.BOUNCEBACK
+    cx, cy, cz = (np.asarray(c, np.float64) for c in O.velocity_set())
+    m = np.stack([2.0 * (c[:, None] * f[:, bb]).sum(axis=0) for c in (cx, cy, cz)])
+    ref = m.sum(axis=1)
+    # FP32 rounding bound of the per-cell sums: 27 terms at 2^-24 relative each
+    scale = np.array([2.0 * (np.abs(c)[:, None] * np.abs(f[:, bb])).sum() for c in (cx, cy, cz)])
+    print(f"  FP64 from the populations {ref}", flush=True)
+    assert np.all(np.abs(fx - ref) <= 4e-6 * scale)
+
+
+def test_diad_communication_random_state(psl):
+    P = psl
+    lv, grids, ol = _setup(P, "f32", n0=12, levels=2)
+    rng = np.random.default_rng(3)
+    for g, o in zip(grids, ol):
+        ru = np.column_stack([1 + 0.01 * rng.standard_normal(g.n), 0.03 * rng.standard_normal((g.n, 3))])
+        f = (O.feq(ru.astype(np.float32)) + np.float32(1e-3) * rng.standard_normal((g.n, 27)).astype(np.float32)).T
+        f = np.ascontiguousarray(f, np.float32)
+        g.set_f(f)
+        o.set(f)
+    P.applyCoarseFineGridCommunication(grids[0], grids[1])
+    E.communicate(ol[0], ol[1])
+    for l in range(2):
+        err = _rel_err(grids[l].get_f(), ol[l].get())
+        print(f"  communication level {l}: max rel err {err:.3e}", flush=True)
+        assert err < RTOL
+
+
+def test_diad_levels_f16_storage(psl):
+    P = psl
+    lv, grids, ol = _setup(P, "f16")
+    P.updateGrid(grids, 4)
+    for _ in range(4):
+        E.update_grid(ol)
+    for l, (g, o) in enumerate(zip(grids, ol)):
+        err = float(np.max(np.abs(g.get_f().astype(np.float64) - o.get())))
+        print(f"  f16 level {l}: max abs err {err:.3e}", flush=True)
+        assert err < 2e-3
+
+
+def test_diad_level_grid_rejects_mixed_levels(psl):
+    P = psl
+    lv, grids = cases.diad_sphere(P, "f32", n0=12, levels=2, span=1, radius=2.0)
+    other = P.DIADGridStruct.from_levels(lv, 1, "f16")
+    with pytest.raises(P.PslError):
+        P.applyCoarseFineGridCommunication(grids[0], other)
