@@ -221,6 +221,39 @@ def two_level_leg(P, cases, steps: int, storage: str = "f16", autotune: bool = T
     return out
 
 
+def diad_leg(P, cases, levels: int, steps: int, storage: str = "f16"):
+    """The wall-refined DIAD sphere (D = 2000 mm on a 220^3 base grid of 100 mm,
+    `levels` levels, wall refinement span 3, Re = 1e6, Cs = 0.1): the device
+    builder's time and cell counts, then `steps` coarse steps of the level
+    recursion.  MLUPS counts sum_l N_l 2^l cell updates per coarse step; the
+    bytes per update are the storage's 54 values plus the seven 4-byte
+    neighbour indices and the flag byte."""
+    t0 = time.perf_counter()
+    lv, grids = cases.diad_sphere_reference(P, storage, levels=levels)
+    setup_s = time.perf_counter() - t0
+    counts = [g.n for g in grids]
+    updates = sum(c << l for l, c in enumerate(counts))
+    P.updateGrid(grids, 2)
+    P.synchronize()
+    t0 = time.perf_counter()
+    P.updateGrid(grids, steps)
+    P.synchronize()
+    el = time.perf_counter() - t0
+    per_update = 54 * (4 if storage == "f32" else 2) + 28 + 1
+    mlups = updates * steps / el / 1e6
+    force = grids[-1].momentum_exchange()
+    out = {"mlups": mlups, "ms_per_coarse_step": el * 1e3 / steps, "coarse_steps": steps, "storage": storage,
+           "levels": levels, "cells_per_level": counts, "updates_per_coarse_step": updates,
+           "build_ms": lv.build_ms, "setup_s": setup_s, "bytes_per_update": per_update,
+           "achieved_GBs": mlups * 1e6 * per_update / 1e9,
+           "roofline_frac": mlups * 1e6 * per_update / 1e9 / HBM_PEAK_GBS,
+           "finest_force_after_run": [float(x) for x in force]}
+    for g in grids:
+        g.close()
+    lv.close()
+    return out
+
+
 def closure_leg(P, n: int, log):
     """Learned closure at the decaying-turbulence configuration: n^3 fine run
     (FP16 storage, Cs 0.1), filtered to (n/4)^3 every 200 steps, network trained
@@ -286,6 +319,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the 64-domain batch (0: skip)")
     ap.add_argument("--two-level-steps", type=int, default=200, help="coarse steps of the two-level sphere (0: skip)")
     ap.add_argument("--large-n", type=int, default=256, help="edge of the large periodic box (0: skip)")
+    ap.add_argument("--diad-steps", type=int, default=20,
+                    help="coarse steps of the wall-refined DIAD sphere (0: skip)")
+    ap.add_argument("--diad-levels", type=int, default=6, help="levels of the DIAD sphere")
     ap.add_argument("--closure-n", type=int, default=512,
                     help="fine grid edge of the learned-closure leg (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -408,6 +444,14 @@ def main():
         tl = two_level_leg(P, cases, args.two_level_steps, autotune=not args.no_autotune)
         log(f"  {tl['mlups']:.1f} MLUPS ({tl['ms_per_coarse_step']:.3f} ms per coarse step, variants {tl['tuning']})")
         out["variants"]["two_level"] = tl
+    if args.diad_steps:
+        log(f"[bench] wall-refined DIAD sphere, {args.diad_levels} levels, {args.diad_steps} coarse steps")
+        dl = {}
+        for st in ("f16", "f32"):
+            dl[st] = diad_leg(P, cases, args.diad_levels, args.diad_steps, st)
+            log(f"  {st}: {dl[st]['mlups']:.0f} MLUPS ({dl[st]['ms_per_coarse_step']:.2f} ms per coarse step), "
+                f"cells per level {dl[st]['cells_per_level']}, build {dl[st]['build_ms']:.0f} ms")
+        out["variants"]["diad_sphere"] = dl
     if args.closure_n:
         log(f"[bench] learned closure, {args.closure_n}^3 decaying turbulence")
         out["variants"]["closure"] = closure_leg(P, args.closure_n, log)

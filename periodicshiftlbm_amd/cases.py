@@ -340,3 +340,30 @@ def diad_sphere(P, storage="f32", n0: int = 24, levels: int = 3, span: int = 1, 
         g.fill_equilibrium()
         grids.append(g)
     return lv, grids
+
+
+def diad_sphere_reference(P, storage="f16", levels: int = 6, span: int = 3, u_inlet: float = 0.07,
+                          reynolds: float = 1e6, cs: float = 0.1):
+    """The wall-refined sphere of the DIAD sphere driver
+    (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:1-29, 222-247):
+    D = 2000 mm, base resolution 100 mm, an 11 D cube (220^3 base cells) with the
+    sphere 2 D behind the inlet, `levels` levels with wall refinement span 3,
+    u = 0.07, Re = 1e6, Smagorinsky 0.1, started at rest.  Returns (levels, grids)."""
+    d, res = 2000.0, 100.0
+    nu_phys = u_inlet * (d / 1000.0) / reynolds
+    dt = (res / 1000.0)  # u_lattice = u_phys
+    n = int(math.ceil(11.0 * d / res))
+    info = P.InfoStruct(res=res, cellCountX=n, cellCountY=n, cellCountZ=n, ox=-2.0 * d, oy=-5.5 * d, oz=-5.5 * d,
+                        nu=dt * nu_phys / (res / 1000.0) ** 2, dtPhys=dt)
+    lv = P.buildDIADGrids(info, levels, span, nu_phys, (), diad_sphere_rules(P, d / 2, levels))
+    grids = []
+    for l in range(levels):
+        g = P.DIADGridStruct.from_levels(lv, l, storage)
+        li, _ = lv.info(l)
+        g.set_flags(diad_sphere_flags(lv.array(l, "ijk"), lv.array(l, "bounceback"),
+                                      (li.cellCountX, li.cellCountY, li.cellCountZ), l))
+        g.set_given_values(channel_given_values(u_inlet))
+        g.set_smagorinsky(cs)
+        g.fill_equilibrium()
+        grids.append(g)
+    return lv, grids

@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <numeric>
 
 namespace psl {
@@ -193,13 +194,35 @@ __global__ void k_rules(const uint64_t* __restrict__ key, int64_t n, RuleInfo I,
   }
 }
 
-// markFinestFluidBounceback without meshes: all shifted origins in one pass
+// markFinestFluidBounceback without meshes: all shifted origins in one pass.
+// boxes[6 b]: bounds of the value-1 bounceback rules of the level; a cell
+// whose extent (all shifted positions lie within half a cell of its centre)
+// misses every box cannot be marked by any shift, so its result (fluid, no
+// wall) is written without the loop.  nbox < 0 disables the test.
 __global__ void k_finest_rules(const uint64_t* __restrict__ key, int64_t n, RuleInfo I0,
                                const float* __restrict__ origins, int nshift, const psl_marker_rule* __restrict__ rules,
-                               int nrules, uint8_t* __restrict__ fluid, uint8_t* __restrict__ fbb) {
+                               int nrules, const float* __restrict__ boxes, int nbox, uint8_t* __restrict__ fluid,
+                               uint8_t* __restrict__ fbb) {
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
     int i, j, k;
     unpack(key[c], i, j, k);
+    if (nbox >= 0) {
+      const float xc[3] = {(float)i * I0.res + I0.ox, (float)j * I0.res + I0.oy, (float)k * I0.res + I0.oz};
+      bool near = false;
+      for (int b = 0; b < nbox && !near; ++b) {
+        bool hit = true;
+        for (int a = 0; a < 3; ++a) {
+          const float h = 0.5f * I0.res + 1e-4f * (fabsf(xc[a]) + I0.res);
+          hit = hit && xc[a] + h >= boxes[6 * b + a] && xc[a] - h <= boxes[6 * b + 3 + a];
+        }
+        near = hit;
+      }
+      if (!near) {
+        fluid[c] = nshift > 0 ? 1 : 0;
+        fbb[c] = 0;
+        continue;
+      }
+    }
     uint8_t f = 0, b = 0;
     RuleInfo I = I0;
     for (int s = 0; s < nshift; ++s) {
@@ -248,16 +271,36 @@ struct EsoBounds {
   int64_t steps[10];
 };
 
+// occupancy of 16^3 bricks of the level's box, one bit each
+constexpr int BRICK_SHIFT = 4;
+__global__ void k_brick_mark(const uint64_t* __restrict__ key, int64_t n, int bx, int by,
+                             uint32_t* __restrict__ words) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    // one atomic per run of a row inside a brick: the first cell of the run
+    const uint64_t kc = key[c];
+    if (c > 0 && (key[c - 1] >> BRICK_SHIFT) == (kc >> BRICK_SHIFT)) continue;
+    int i, j, k;
+    unpack(kc, i, j, k);
+    const int64_t b = ((int64_t)(k >> BRICK_SHIFT) * by + (j >> BRICK_SHIFT)) * bx + (i >> BRICK_SHIFT);
+    const uint32_t bit = 1u << (b & 31);
+    if (!(words[b >> 5] & bit)) atomicOr(&words[b >> 5], bit);
+  }
+}
+
 // getDIADEsotwistNbrArray / findPeriodicNeighbourIJKIndex: the first existing
 // cell along the periodic ray +i +j +k +ij +ik +jk +ijk -i -j -k (the cell
 // itself at the latest).  Along +-i the cells of a (j, k) row are contiguous
 // in key order, so the answer is the adjacent entry or the row's wrap-around
-// end; the other directions walk with one binary search per step, bounded by
-// the period of the ray.
+// end.  The other directions walk the ray, bounded by its period: a point in
+// an empty brick is skipped together with every following point of the ray
+// inside that brick (up to the brick face or the box edge, whichever comes
+// first); a point in an occupied brick costs one binary search.
 __global__ void k_eso_nbrs(const uint64_t* __restrict__ key, int64_t n, int nx, int ny, int nz, EsoBounds bounds,
-                           int* __restrict__ out, int* __restrict__ fail) {
+                           const uint32_t* __restrict__ bricks, int bx, int by, int* __restrict__ out,
+                           int* __restrict__ fail) {
   const int D[10][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1},
                         {0, 1, 1}, {1, 1, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+  constexpr int BM = (1 << BRICK_SHIFT) - 1;
   for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t kc = key[c];
     const uint64_t row = kc >> KEY_BITS;
@@ -272,13 +315,32 @@ __global__ void k_eso_nbrs(const uint64_t* __restrict__ key, int64_t n, int nx, 
         r = (c > 0 && (key[c - 1] >> KEY_BITS) == row) ? (int)(c - 1)
                                                        : (int)lower_bound_key(key, n, (row + 1) << KEY_BITS) - 1;
       } else {
-        int a = i, b = j, e = k;
-        for (int64_t s = 0; s < bounds.steps[d] && r < 0; ++s) {
-          a += D[d][0]; b += D[d][1]; e += D[d][2];
-          if (a >= nx) a = 0; else if (a < 0) a = nx - 1;
-          if (b >= ny) b = 0; else if (b < 0) b = ny - 1;
-          if (e >= nz) e = 0; else if (e < 0) e = nz - 1;
+        const int dx = D[d][0], dy = D[d][1], dz = D[d][2];
+        auto wrap = [](int v, int m) { return v >= m ? v - m : (v < 0 ? v + m : v); };
+        int a = wrap(i + dx, nx), b = wrap(j + dy, ny), e = wrap(k + dz, nz);
+        int64_t s = 1;  // points of the ray visited so far, this one included
+        while (s <= bounds.steps[d]) {
+          const int64_t br = ((int64_t)(e >> BRICK_SHIFT) * by + (b >> BRICK_SHIFT)) * bx + (a >> BRICK_SHIFT);
+          if (!((bricks[br >> 5] >> (br & 31)) & 1u)) {
+            int jump = 1 << 30;
+            if (dx > 0) jump = min(jump, min(BM + 1 - (a & BM), nx - a));
+            if (dx < 0) jump = min(jump, (a & BM) + 1);
+            if (dy > 0) jump = min(jump, min(BM + 1 - (b & BM), ny - b));
+            if (dy < 0) jump = min(jump, (b & BM) + 1);
+            if (dz > 0) jump = min(jump, min(BM + 1 - (e & BM), nz - e));
+            if (dz < 0) jump = min(jump, (e & BM) + 1);
+            a = wrap(a + dx * jump, nx);
+            b = wrap(b + dy * jump, ny);
+            e = wrap(e + dz * jump, nz);
+            s += jump;
+            continue;
+          }
           r = find_key(key, n, pack_key(a, b, e));
+          if (r >= 0) break;
+          a = wrap(a + dx, nx);
+          b = wrap(b + dy, ny);
+          e = wrap(e + dz, nz);
+          ++s;
         }
         if (r < 0) atomicAdd(fail, 1);
       }
@@ -349,6 +411,7 @@ struct Builder {
   float nu_phys = 0.f;
   std::vector<const psl_stl*> stls;
   DBuf<psl_marker_rule> rules;
+  std::vector<psl_marker_rule> host_rules;
   int nrules = 0;
   DBuf<uint8_t> tmp;
   size_t tmp_bytes = 0;
@@ -446,8 +509,14 @@ int esotwist_nbrs(Builder& B, DiadLevel& L) {
   }
   PSL_HIP_TRY(L.nbr.alloc(10 * (size_t)std::max<int64_t>(L.n, 1)));
   PSL_HIP_TRY(hipMemset(B.dnum.p, 0, sizeof(int)));
+  const int bx = (nx + 15) >> 4, by = (ny + 15) >> 4, bz = (nz + 15) >> 4;
+  const size_t nwords = ((size_t)bx * by * bz + 31) / 32;
+  DBuf<uint32_t> bricks;
+  PSL_HIP_TRY(bricks.alloc(nwords));
+  PSL_HIP_TRY(hipMemset(bricks.p, 0, nwords * sizeof(uint32_t)));
   if (L.n) {
-    k_eso_nbrs<<<blocks_for(L.n), TPB>>>(L.key.p, L.n, nx, ny, nz, eb, L.nbr.p, B.dnum.p);
+    k_brick_mark<<<blocks_for(L.n), TPB>>>(L.key.p, L.n, bx, by, bricks.p);
+    k_eso_nbrs<<<blocks_for(L.n), TPB>>>(L.key.p, L.n, nx, ny, nz, eb, bricks.p, bx, by, L.nbr.p, B.dnum.p);
     DB_TRY(launch_check("esotwist neighbours"));
   }
   int fail = 0;
@@ -484,7 +553,36 @@ int finest_fluid_bb(Builder& B, const DiadLevel& L, int level, DBuf<uint8_t>& fl
     DBuf<float> dor;
     PSL_HIP_TRY(dor.alloc(origins.size()));
     PSL_HIP_TRY(hipMemcpy(dor.p, origins.data(), origins.size() * sizeof(float), hipMemcpyHostToDevice));
-    k_finest_rules<<<blocks_for(n), TPB>>>(L.key.p, n, I0, dor.p, nshift, B.rules.p, B.nrules, fluid.p, fbb.p);
+    // bounds of the rules that can mark a wall on this level
+    std::vector<float> boxes;
+    int nbox = 0;
+    for (const psl_marker_rule& r : B.host_rules) {
+      if (r.what != PSL_RULE_BOUNCEBACK || !r.value || !((r.level_mask >> level) & 1u)) continue;
+      if (r.shape == PSL_SHAPE_INDEX_BOX) {
+        nbox = -1;
+        break;
+      }
+      float lo[3], hi[3];
+      if (r.shape == PSL_SHAPE_BOX) {
+        for (int a = 0; a < 3; ++a) { lo[a] = r.p[a]; hi[a] = r.p[3 + a]; }
+      } else if (r.shape == PSL_SHAPE_XCYLINDER) {
+        const float w = std::fabs(r.p[3]);
+        lo[0] = r.p[0]; hi[0] = r.p[0] + std::fabs(r.p[4]);
+        for (int a = 1; a < 3; ++a) { lo[a] = r.p[a] - w; hi[a] = r.p[a] + w; }
+      } else {
+        const float w = std::fabs(r.p[3]);
+        for (int a = 0; a < 3; ++a) { lo[a] = r.p[a] - w; hi[a] = r.p[a] + w; }
+      }
+      for (int a = 0; a < 3; ++a) boxes.push_back(lo[a]);
+      for (int a = 0; a < 3; ++a) boxes.push_back(hi[a]);
+      ++nbox;
+    }
+    DBuf<float> dbox;
+    PSL_HIP_TRY(dbox.alloc(std::max<size_t>(boxes.size(), 1)));
+    if (!boxes.empty())
+      PSL_HIP_TRY(hipMemcpy(dbox.p, boxes.data(), boxes.size() * sizeof(float), hipMemcpyHostToDevice));
+    k_finest_rules<<<blocks_for(n), TPB>>>(L.key.p, n, I0, dor.p, nshift, B.rules.p, B.nrules, dbox.p,
+                                           nbox < 0 ? -1 : nbox, fluid.p, fbb.p);
     DB_TRY(launch_check("finest fluid"));
     PSL_HIP_TRY(hipDeviceSynchronize());
     return PSL_OK;
@@ -795,6 +893,7 @@ int psl_diad_build(const psl_info* base, int levels, int wall_refinement_span, f
   B.nu_phys = nu_phys;
   B.stls.assign(stls, stls + nstl);
   B.nrules = nrules;
+  B.host_rules.assign(rules, rules + nrules);
   B.out = res;
   int rc = PSL_OK;
   if (B.rules.alloc((size_t)std::max(nrules, 1)) != hipSuccess || B.dnum.alloc(1) != hipSuccess) {

@@ -155,52 +155,60 @@ __device__ __forceinline__ void eso_put(typename St::T* __restrict__ f, const Es
 }
 
 // writeToCoarseGridInterface(DIAD): include/gridRefinementFunctions.h:540-622.
-// One thread per interface cell: the mean of the fine read cell and its +i
-// +j +k +ij +ik +jk +ijk neighbours (summed in that order), rescaled
-// fine -> coarse, written into the coarse cell.
+// Eight lanes per interface cell, one per fine child (the read cell and its
+// +i +j +k +ij +ik +jk +ijk neighbours): each lane gathers its child, the
+// group sums the eight in child order through lane shuffles (the reference's
+// sequential order, so every lane holds the same sum), and lane 0 rescales
+// fine -> coarse and writes the coarse cell.  Eight independent gathers per
+// cell keep enough loads in flight on the thin interface lists.
 template <class St, bool FC, bool FF>
 __global__ __launch_bounds__(256) void k_eso_to_coarse(typename St::T* __restrict__ fc, EsoParams pc,
                                                        const typename St::T* __restrict__ ff, EsoParams pf,
                                                        const int* __restrict__ write, const int* __restrict__ read,
                                                        uint32_t m) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= m) return;
-  const uint32_t rc = (uint32_t)read[t];
-  uint32_t fine[8];
-  fine[0] = rc;
+  const uint32_t g = t >> 3, w = t & 7;
+  const bool valid = g < m;  // the eight lanes of a group agree
+  float v[27];
+  if (valid) {
+    const uint32_t rc = (uint32_t)read[g];
+    const uint32_t child = w == 0 ? rc : (uint32_t)pf.nbr[(size_t)(w - 1) * pf.n + rc];
+    eso_get<St, FF>(ff, pf, child, v);
+  } else {
 #pragma unroll
-  for (int a = 0; a < 7; ++a) fine[a + 1] = (uint32_t)pf.nbr[(size_t)a * pf.n + rc];
+    for (int q = 0; q < 27; ++q) v[q] = 0.f;
+  }
   float acc[27];
 #pragma unroll
-  for (int q = 0; q < 27; ++q) acc[q] = 0.f;
-  for (int w = 0; w < 8; ++w) {
-    float g[27];
-    eso_get<St, FF>(ff, pf, fine[w], g);
+  for (int q = 0; q < 27; ++q) {
+    float a = 0.f;
 #pragma unroll
-    for (int q = 0; q < 27; ++q) acc[q] += g[q];
+    for (int src = 0; src < 8; ++src) a += __shfl(v[q], src, 8);
+    acc[q] = a * 0.125f;
   }
-#pragma unroll
-  for (int q = 0; q < 27; ++q) acc[q] = acc[q] * 0.125f;
+  if (!valid || w != 0) return;
   rescale(acc, false);
-  eso_put<St, FC>(fc, pc, (uint32_t)write[t], acc);
+  eso_put<St, FC>(fc, pc, (uint32_t)write[g], acc);
 }
 
 // writeToFineGridInterface(DIAD): include/gridRefinementFunctions.h:624-702.
-// The coarse read cell, rescaled coarse -> fine, into the fine write cell and
-// its +i +j +k +ij +ik +jk +ijk neighbours.
+// Eight lanes per interface cell: each reads the coarse cell (the same lines),
+// rescales coarse -> fine (identical in every lane) and writes one child: the
+// write cell or its +i +j +k +ij +ik +jk +ijk neighbour.
 template <class St, bool FC, bool FF>
 __global__ __launch_bounds__(256) void k_eso_to_fine(const typename St::T* __restrict__ fc, EsoParams pc,
                                                      typename St::T* __restrict__ ff, EsoParams pf,
                                                      const int* __restrict__ write, const int* __restrict__ read,
                                                      uint32_t m) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= m) return;
+  const uint32_t g = t >> 3, w = t & 7;
+  if (g >= m) return;
   float v[27];
-  eso_get<St, FC>(fc, pc, (uint32_t)read[t], v);
+  eso_get<St, FC>(fc, pc, (uint32_t)read[g], v);
   rescale(v, true);
-  const uint32_t wc = (uint32_t)write[t];
-  eso_put<St, FF>(ff, pf, wc, v);
-  for (int a = 0; a < 7; ++a) eso_put<St, FF>(ff, pf, (uint32_t)pf.nbr[(size_t)a * pf.n + wc], v);
+  const uint32_t wc = (uint32_t)write[g];
+  const uint32_t child = w == 0 ? wc : (uint32_t)pf.nbr[(size_t)(w - 1) * pf.n + wc];
+  eso_put<St, FF>(ff, pf, child, v);
 }
 
 // getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
@@ -290,12 +298,12 @@ static int eso_comm_t(psl_diad* c, psl_diad* f) {
   typedef typename St::T T;
   const EsoParams pc = eso_params(c), pf = eso_params(f);
   if (c->n_f2c) {
-    k_eso_to_coarse<St, FC, FF><<<(unsigned)((c->n_f2c + 255) / 256), 256, 0, c->stream>>>(
+    k_eso_to_coarse<St, FC, FF><<<(unsigned)((8 * c->n_f2c + 255) / 256), 256, 0, c->stream>>>(
         (T*)c->f, pc, (const T*)f->f, pf, c->f2c_write, c->f2c_read, (uint32_t)c->n_f2c);
     PSL_LAUNCH_CHECK();
   }
   if (c->n_c2f) {
-    k_eso_to_fine<St, FC, FF><<<(unsigned)((c->n_c2f + 255) / 256), 256, 0, c->stream>>>(
+    k_eso_to_fine<St, FC, FF><<<(unsigned)((8 * c->n_c2f + 255) / 256), 256, 0, c->stream>>>(
         (const T*)c->f, pc, (T*)f->f, pf, c->c2f_write, c->c2f_read, (uint32_t)c->n_c2f);
     PSL_LAUNCH_CHECK();
   }
