@@ -322,6 +322,7 @@ def main():
     ap.add_argument("--diad-steps", type=int, default=20,
                     help="coarse steps of the wall-refined DIAD sphere (0: skip)")
     ap.add_argument("--diad-levels", type=int, default=6, help="levels of the DIAD sphere")
+    ap.add_argument("--diad-storages", default="f16,f32", help="storages of the DIAD sphere leg")
     ap.add_argument("--closure-n", type=int, default=512,
                     help="fine grid edge of the learned-closure leg (0: skip)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -447,7 +448,7 @@ def main():
     if args.diad_steps:
         log(f"[bench] wall-refined DIAD sphere, {args.diad_levels} levels, {args.diad_steps} coarse steps")
         dl = {}
-        for st in ("f16", "f32"):
+        for st in args.diad_storages.split(","):
             dl[st] = diad_leg(P, cases, args.diad_levels, args.diad_steps, st)
             log(f"  {st}: {dl[st]['mlups']:.0f} MLUPS ({dl[st]['ms_per_coarse_step']:.2f} ms per coarse step), "
                 f"cells per level {dl[st]['cells_per_level']}, build {dl[st]['build_ms']:.0f} ms")

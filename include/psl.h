@@ -357,6 +357,16 @@ int psl_diad_coarse_fine_communication(psl_diad* coarse, psl_diad* fine);
  * streaming + local update of level l, two steps of level l+1, then the
  * communication of l and l+1; nsteps coarse steps. */
 int psl_diad_update_grid(psl_diad* const* levels, int nlevels, int nsteps);
+/* outletCellArray (include/types.h:171) and kMinNbrArray (include/types.h:161, n
+ * entries) from host arrays; psl_diad_create_level sets them for the levels
+ * whose outlet rules marked cells. */
+int psl_diad_set_outlet(psl_diad* grid, const int* outlet_cells, int64_t n_outlet, const int* kmin);
+/* applyNonReflectiveOutletZ (include/boundaryConditions/applyNonReflectiveOutlet.h:9-73):
+ * for the nine c_z < 0 populations of every outlet cell,
+ * f = invSqrt3 f(-k neighbour) + (1 - invSqrt3) f(cell) from the last update's
+ * write maps, stored at the read maps of the next step; called before
+ * psl_diad_apply_streaming. */
+int psl_diad_apply_nonreflective_outlet_z(psl_diad* grid);
 /* getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
  * all three components in FP64: sum over BOUNCEBACK-flagged cells of
  * rho (u - u after bounceback), populations through the current write maps. */

@@ -90,6 +90,8 @@ def _declare(L):
                                                                 ctypes.c_longlong, P, ctypes.c_int, P, P,
                                                                 ctypes.c_longlong]
         getattr(L, f"oracle_eso_momentum_exchange_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P, P]
+        getattr(L, f"oracle_eso_outlet_z_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
+                                                             ctypes.c_longlong, P]
     L.oracle_diad_build.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, P, P, P, ctypes.c_int, P,
                                     ctypes.c_int]
     L.oracle_diad_build.restype = P
@@ -317,6 +319,14 @@ def eso_to_fine(fc, nbc7, flip_c, ff, nbf7, flip_f, write, read):
     getattr(lib(), f"oracle_eso_to_fine_{_suf(fc)}")(fc.ctypes.data, fc.shape[1], nbc7.ctypes.data, int(flip_c),
                                                      ff.ctypes.data, ff.shape[1], nbf7.ctypes.data, int(flip_f),
                                                      w.ctypes.data, r.ctypes.data, w.size)
+
+
+def eso_outlet_z(f, nbr7, flipper, outlet, kmin):
+    """applyNonReflectiveOutletZ on a raw esotwist array (in place)."""
+    o = np.ascontiguousarray(outlet, np.int32)
+    k = np.ascontiguousarray(kmin, np.int32)
+    getattr(lib(), f"oracle_eso_outlet_z_{_suf(f)}")(f.ctypes.data, f.shape[1], nbr7.ctypes.data, int(flipper),
+                                                     o.ctypes.data, o.size, k.ctypes.data)
 
 
 def eso_momentum_exchange(f, nbr7, flipper, flags):

@@ -935,6 +935,31 @@ void eso_to_fine(const R* fc, long long nc, const int* const* nbc, bool flip_c, 
   }
 }
 
+// applyNonReflectiveOutletZ: include/boundaryConditions/applyNonReflectiveOutlet.h:9-73.
+// Outlet cells in list order; kmin: the -k neighbour of every cell.
+template <typename R>
+void eso_outlet_z(R* fa, long long n, const int* const* nbr, bool flipper, const int* outlet, long long m,
+                  const int* kmin) {
+  static const int QZ[9] = {3, 7, 10, 13, 18, 19, 22, 23, 25};
+  const float inv_sqrt3 = 0.577350269f;
+  for (long long t = 0; t < m; ++t) {
+    const long long cell = outlet[t];
+    R fc[27], fk[27];
+    eso_get(fa, n, nbr, cell, flipper, fc);
+    eso_get(fa, n, nbr, (long long)kmin[cell], flipper, fk);
+    long long slot[8];
+    slot[0] = cell;
+    for (int a = 0; a < 7; ++a) slot[a + 1] = nbr[a][cell];
+    for (int a = 0; a < 9; ++a) {
+      const int q = QZ[a];
+      const R v = (R)inv_sqrt3 * fk[q] + (R)(1.f - inv_sqrt3) * fc[q];
+      // read index of the inverted parity
+      const int fi = !flipper ? q : opp(q);
+      fa[(long long)fi * n + slot[ESO_READ[q]]] = v;
+    }
+  }
+}
+
 // getSphereDrag(DIADGridStruct&): flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198,
 // all three components: sum over bounceback cells of rho (u - u after bounceback).
 template <typename R>
@@ -1648,6 +1673,12 @@ ORACLE_PER_CELL(f64, double)
     const int* b[7];                                                                                           \
     for (int t = 0; t < 7; ++t) { a[t] = nbc7 + (size_t)t * nc; b[t] = nbf7 + (size_t)t * nf; }              \
     eso_to_fine(fc, nc, a, flip_c != 0, ff, nf, b, flip_f != 0, write, read, m);                              \
+  }                                                                                                            \
+  void oracle_eso_outlet_z_##SUF(R* fa, long long n, const int* nbr7, int flipper, const int* outlet,        \
+                                 long long m, const int* kmin) {                                               \
+    const int* a[7];                                                                                           \
+    for (int t = 0; t < 7; ++t) a[t] = nbr7 + (size_t)t * n;                                                   \
+    eso_outlet_z(fa, n, a, flipper != 0, outlet, m, kmin);                                                     \
   }                                                                                                            \
   void oracle_eso_momentum_exchange_##SUF(const R* fa, long long n, const int* nbr7, int flipper,              \
                                           const uint8_t* flags, double* out3) {                                \

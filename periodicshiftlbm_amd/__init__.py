@@ -211,6 +211,8 @@ _SIGS = [
     ("psl_diad_coarse_fine_communication", _I, [_P, _P]),
     ("psl_diad_update_grid", _I, [_P, _I, _I]),
     ("psl_diad_reduce_momentum_exchange", _I, [_P, _P]),
+    ("psl_diad_set_outlet", _I, [_P, _P, ctypes.c_int64, _P]),
+    ("psl_diad_apply_nonreflective_outlet_z", _I, [_P]),
 ]
 
 
@@ -537,6 +539,12 @@ class DIADGridStruct:
         _check(lib().psl_diad_set_interface(self._h, fine.handle, a[0].ctypes.data, a[1].ctypes.data, a[0].size,
                                             a[2].ctypes.data, a[3].ctypes.data, a[2].size))
 
+    def set_outlet(self, outlet_cells, kmin):
+        """outletCellArray and kMinNbrArray (n entries)."""
+        o = np.ascontiguousarray(outlet_cells, np.int32)
+        k = np.ascontiguousarray(kmin, np.int32)
+        _check(lib().psl_diad_set_outlet(self._h, o.ctypes.data, o.size, k.ctypes.data))
+
     def momentum_exchange(self) -> np.ndarray:
         """getSphereDrag(DIADGridStruct&), all three components in FP64."""
         out = np.zeros(3, np.float64)
@@ -854,6 +862,11 @@ def applyLocalCellUpdate(grid):
         _check(lib().psl_diad_apply_local_cell_update(grid.handle))
     else:
         _check(lib().psl_apply_local_cell_update(grid.handle))
+
+
+def applyNonReflectiveOutletZ(grid: DIADGridStruct):
+    """include/boundaryConditions/applyNonReflectiveOutlet.h:9-73 (before applyStreaming)."""
+    _check(lib().psl_diad_apply_nonreflective_outlet_z(grid.handle))
 
 
 def applyBCPeriodicX(grid: GridStruct):

@@ -1026,6 +1026,15 @@ int psl_diad_create_level(const psl_diad_levels* levels, int level, int storage,
   }
   d->n_c2f = nc;
   d->n_f2c = nf;
+  const int64_t no = L.cnt[A_OUTLET];
+  if (no) {
+    if (hipMalloc(&d->outlet, (size_t)no * sizeof(int)) != hipSuccess ||
+        hipMalloc(&d->kmin, (size_t)n * sizeof(int)) != hipSuccess ||
+        hipMemcpy(d->outlet, L.arr[A_OUTLET].p, (size_t)no * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(d->kmin, L.nbr.p + 9 * (size_t)n, (size_t)n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess)
+      return fail("outlet");
+  }
+  d->n_outlet = no;
   if (hipDeviceSynchronize() != hipSuccess) return fail("sync");
   *out = d;
   return PSL_OK;

@@ -211,6 +211,47 @@ __global__ __launch_bounds__(256) void k_eso_to_fine(const typename St::T* __res
   eso_put<St, FF>(ff, pf, child, v);
 }
 
+// applyNonReflectiveOutletZ (include/boundaryConditions/applyNonReflectiveOutlet.h:9-73):
+// for the nine populations with c_z < 0 of every listed outlet cell,
+// f = invSqrt3 f(-k neighbour) + (1 - invSqrt3) f(cell), both as the last update
+// wrote them, stored where the next step reads them (the read maps of the
+// opposite parity).
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_outlet_z(typename St::T* __restrict__ f, EsoParams p,
+                                                      const int* __restrict__ outlet, const int* __restrict__ kmin,
+                                                      uint32_t m) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= m) return;
+  constexpr int QZ[9] = {3, 7, 10, 13, 18, 19, 22, 23, 25};
+  constexpr float INV_SQRT3 = 0.577350269f;  // the drivers' invSqrt3
+  const uint32_t cell = (uint32_t)outlet[t];
+  const uint32_t km = (uint32_t)kmin[cell];
+  uint32_t sc[8], sk[8];
+  eso_slots<St>(p, cell, sc);
+  eso_slots<St>(p, km, sk);
+  float v[9];
+#pragma unroll
+  for (int a = 0; a < 9; ++a) {
+    const int q = QZ[a];
+    const int fi = FLIP ? opp(q) : q;
+    float x = St::load(f[(size_t)fi * p.n + sk[ESO_WRITE[q]]]);
+    float y = St::load(f[(size_t)fi * p.n + sc[ESO_WRITE[q]]]);
+    if (St::W_OFF) {
+      x += weight(q);
+      y += weight(q);
+    }
+    v[a] = INV_SQRT3 * x + (1.f - INV_SQRT3) * y;
+  }
+#pragma unroll
+  for (int a = 0; a < 9; ++a) {
+    const int q = QZ[a];
+    const int fi = FLIP ? opp(q) : q;  // the read maps of parity !FLIP
+    float x = v[a];
+    if (St::W_OFF) x -= weight(q);
+    f[(size_t)fi * p.n + sc[ESO_READ[q]]] = St::store(x);
+  }
+}
+
 // getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
 // all three components: per bounceback cell rho (u - u after bounceback), in
 // FP64 per-block partial sums.
@@ -325,6 +366,18 @@ static int eso_momentum(psl_diad* d, uint32_t nb) {
   return PSL_OK;
 }
 
+template <class St>
+static int eso_outlet(psl_diad* d) {
+  if (!d->n_outlet) return PSL_OK;
+  const EsoParams p = eso_params(d);
+  const unsigned nb = (unsigned)((d->n_outlet + 255) / 256);
+  typedef typename St::T T;
+  if (d->flipper) k_eso_outlet_z<St, true><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->outlet, d->kmin, (uint32_t)d->n_outlet);
+  else k_eso_outlet_z<St, false><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->outlet, d->kmin, (uint32_t)d->n_outlet);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
 #define PSL_ESO_DISPATCH(fn, ...)                                         \
   (d->storage == PSL_F32 ? fn<StF32>(__VA_ARGS__)                         \
                          : d->storage == PSL_F16DEV ? fn<StF16>(__VA_ARGS__) : fn<StBF16>(__VA_ARGS__))
@@ -367,7 +420,7 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->nbr) (void)hipFree(d->nbr);
   if (d->flags) (void)hipFree(d->flags);
   if (d->bc_dev) (void)hipFree(d->bc_dev);
-  for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read})
+  for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read, d->outlet, d->kmin})
     if (a) (void)hipFree(a);
   if (d->partials) (void)hipFree(d->partials);
   delete d;
@@ -502,6 +555,22 @@ int psl_diad_set_interface(psl_diad* coarse, const psl_diad* fine, const int* c2
   coarse->n_c2f = rc == PSL_OK ? n_c2f : 0;
   coarse->n_f2c = rc == PSL_OK ? n_f2c : 0;
   return rc;
+}
+
+int psl_diad_set_outlet(psl_diad* d, const int* outlet_cells, int64_t n_outlet, const int* kmin) {
+  PSL_CHECK_ARG(d && n_outlet >= 0 && (n_outlet == 0 || (outlet_cells && kmin)), "bad outlet arguments");
+  for (int64_t t = 0; t < n_outlet; ++t) PSL_CHECK_ARG(outlet_cells[t] >= 0 && outlet_cells[t] < d->n, "outlet cell out of range");
+  if (n_outlet)
+    for (int64_t c = 0; c < d->n; ++c) PSL_CHECK_ARG(kmin[c] >= 0 && kmin[c] < d->n, "-k neighbour out of range");
+  int rc = upload_ints(&d->outlet, outlet_cells, n_outlet);
+  if (rc == PSL_OK) rc = upload_ints(&d->kmin, kmin, n_outlet ? d->n : 0);
+  d->n_outlet = rc == PSL_OK ? n_outlet : 0;
+  return rc;
+}
+
+int psl_diad_apply_nonreflective_outlet_z(psl_diad* d) {
+  PSL_CHECK_ARG(d, "null grid");
+  return PSL_ESO_DISPATCH(eso_outlet, d);
 }
 
 int psl_diad_coarse_fine_communication(psl_diad* coarse, psl_diad* fine) {

@@ -155,6 +155,9 @@ struct psl_diad {
   int* f2c_write = nullptr;  // coarse cells
   int* f2c_read = nullptr;   // fine cells
   int64_t n_f2c = 0;
+  int* outlet = nullptr;     // outletCellArray
+  int64_t n_outlet = 0;
+  int* kmin = nullptr;       // kMinNbrArray (n entries), with the outlet
   double* partials = nullptr;  // reduction scratch
   size_t n_partials = 0;
 };

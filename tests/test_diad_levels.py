@@ -90,3 +90,61 @@ def test_transfers_route_to_children(two_levels):
     touched = np.unique(ch2)
     others_f = np.setdiff1d(np.arange(f.n), touched)
     assert np.array_equal(ff1[:, others_f], ff0[:, others_f])
+
+
+# ------------------------------------------------ non-reflective z outlet
+QZ = [3, 7, 10, 13, 18, 19, 22, 23, 25]
+INV_SQRT3 = np.float64(np.float32(0.577350269))
+ONE_MINUS = np.float64(np.float32(1.0) - np.float32(0.577350269))
+
+
+def outlet_level():
+    """Level 0 of a two-level sphere box with an outlet rule on its last k plane;
+    the refined region stays clear of the two last planes, so every outlet
+    cell's -k neighbour is the cell below it (the operator's reads and writes
+    then touch disjoint slots, as in the drivers' geometries)."""
+    base = dict(res=1.0, ox=-5.5, oy=-5.5, oz=-11.5, nu=0.02, dt=1e-3, nx=16, ny=12, nz=24)
+    rules = P.rules(cases.diad_sphere_rules(P, 1.6, 2),
+                    P.marker_rule(P.RULE_OUTLET, P.SHAPE_INDEX_BOX, idx=(0, -1, 0, -1, -1, -1), levels=[0]))
+    return O.diad_build(base, 2, 1, 2e-5, (), rules)[0]
+
+
+def _read_slots(d, flip):
+    """(fi, slot) of the read maps of parity `flip` for every (q, cell)."""
+    R, _ = O.esotwist_tables()
+    s = np.vstack([np.arange(d["n"])[None], d["nbr"][:7].astype(np.int64)])
+    return [((q if flip else E.OPP[q]), s[R[q]]) for q in range(27)]
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_outlet_extrapolates_uniform_and_linear_states(flip):
+    d = outlet_level()
+    nbr7 = np.ascontiguousarray(d["nbr"][:7], np.int32)
+    kmin = d["nbr"][9]
+    out = d["outlet"]
+    i, j, k = d["ijk"]
+    assert out.size == d["nx"] * d["ny"] and np.all(k[out] == d["nz"] - 1) and np.all(k[kmin[out]] == d["nz"] - 2)
+    rng = np.random.default_rng(1)
+    # uniform state: the extrapolation returns the state; nothing else changes
+    e = rng.random(27)
+    raw = np.zeros((27, d["n"]))
+    E.eso_set_all(raw, nbr7, flip, np.repeat(e[:, None], d["n"], axis=1))
+    raw0 = raw.copy()
+    O.eso_outlet_z(raw, nbr7, flip, out, kmin)
+    rs = _read_slots(d, not flip)
+    touched = np.zeros_like(raw, bool)
+    for q in QZ:
+        fi, sl = rs[q]
+        assert np.max(np.abs(raw[fi, sl[out]] - e[q])) < 1e-15
+        touched[fi, sl[out]] = True
+    assert np.array_equal(raw[~touched], raw0[~touched])
+    # linear in k: invSqrt3 f(-k neighbour) + (1 - invSqrt3) f(cell)
+    a, b = rng.random(27), rng.random(27)
+    f = a[:, None] + b[:, None] * k[None, :].astype(np.float64)
+    raw = np.zeros((27, d["n"]))
+    E.eso_set_all(raw, nbr7, flip, f)
+    O.eso_outlet_z(raw, nbr7, flip, out, kmin)
+    for q in QZ:
+        fi, sl = rs[q]
+        want = INV_SQRT3 * (a[q] + b[q] * k[kmin[out]]) + ONE_MINUS * (a[q] + b[q] * k[out])
+        assert np.max(np.abs(raw[fi, sl[out]] - want)) < 1e-12

@@ -117,3 +117,47 @@ def test_diad_level_grid_rejects_mixed_levels(psl):
     other = P.DIADGridStruct.from_levels(lv, 1, "f16")
     with pytest.raises(P.PslError):
         P.applyCoarseFineGridCommunication(grids[0], other)
+
+
+@pytest.mark.parametrize("flip_first", [False, True])
+def test_nonreflective_outlet_matches_oracle(psl, flip_first):
+    """applyNonReflectiveOutletZ on a level built with an outlet rule (outlet
+    list and -k neighbours come from the builder), on a random state, then one
+    more streaming + local update."""
+    P = psl
+    from tests.test_gpu_diad_builder import _gpu_build
+    from tests.test_diad_levels import outlet_level
+    base = dict(res=1.0, ox=-5.5, oy=-5.5, oz=-11.5, nu=0.02, dt=1e-3, nx=16, ny=12, nz=24)
+    rules = P.rules(cases.diad_sphere_rules(P, 1.6, 2),
+                    P.marker_rule(P.RULE_OUTLET, P.SHAPE_INDEX_BOX, idx=(0, -1, 0, -1, -1, -1), levels=[0]))
+    lv = _gpu_build(P, base, 2, 1, [], rules, nu_phys=2e-5)
+    d = outlet_level()
+    assert np.array_equal(lv.array(0, "outlet"), d["outlet"])
+    g = P.DIADGridStruct.from_levels(lv, 0, "f32")
+    if flip_first:
+        P.applyStreaming(g)
+    nbr7 = np.ascontiguousarray(d["nbr"][:7], np.int32)
+    rng = np.random.default_rng(7)
+    ru = np.column_stack([1 + 0.01 * rng.standard_normal(g.n), 0.03 * rng.standard_normal((g.n, 3))])
+    f = np.ascontiguousarray((O.feq(ru.astype(np.float32)) + np.float32(1e-3) *
+                              rng.standard_normal((g.n, 27)).astype(np.float32)).T, np.float32)
+    g.set_f(f)
+    raw = np.zeros((27, g.n), np.float32)
+    E.eso_set_all(raw, nbr7, flip_first, f)
+    P.applyNonReflectiveOutletZ(g)
+    O.eso_outlet_z(raw, nbr7, flip_first, d["outlet"], d["nbr"][9])
+    err = _rel_err(g.get_f(), E.eso_get_all(raw, nbr7, flip_first))
+    print(f"  outlet (flipper {flip_first}): max rel err {err:.3e} over {d['outlet'].size} outlet cells", flush=True)
+    assert err < RTOL
+    # the next step reads what the operator wrote
+    flags = cases.diad_sphere_flags(d["ijk"], d["bounceback"], (d["nx"], d["ny"], d["nz"]), 0)
+    g.set_flags(flags)
+    g.set_given_values(cases.channel_given_values(0.04))
+    P.applyStreaming(g)
+    P.applyLocalCellUpdate(g)
+    info = O.make_info(d["nx"], d["ny"], d["nz"], nu=0.02, cs=0.0)
+    O.esotwist_update(raw, nbr7, np.ascontiguousarray(d["ijk"], np.int32), info, flags,
+                      cases.channel_given_values(0.04).reshape(-1), not flip_first)
+    err = _rel_err(g.get_f(), E.eso_get_all(raw, nbr7, not flip_first))
+    print(f"  after one step: max rel err {err:.3e}", flush=True)
+    assert err < RTOL
