@@ -197,4 +197,86 @@ inline void sumBoolArrays(BoolArrayType& a, BoolArrayType& b, BoolArrayType& r) 
 }
 inline void invertBoolArray(BoolArrayType& a) { check(psl_mask_not(a.handle())); }
 
+// ------------------------------------------------------- DIAD (esotwist) levels
+// buildDIADGrids (include/DIADFunctions.h:640-1046) on the device.  The drivers'
+// getMarkers decisions for bounceback, refinement and outlet cells are given as
+// an ordered psl_marker_rule list (see psl.h).
+class DIADLevels {
+ public:
+  DIADLevels(const InfoStruct& base, int levels, int wallRefinementSpan, float nuPhys,
+             const std::vector<const STLStruct*>& stls, const std::vector<psl_marker_rule>& rules) {
+    std::vector<const psl_stl*> h;
+    for (auto* s : stls) h.push_back(s->handle());
+    check(psl_diad_build(&base, levels, wallRefinementSpan, nuPhys, h.data(), (int)h.size(), rules.data(),
+                         (int)rules.size(), &h_));
+    levels_ = levels;
+  }
+  DIADLevels(const DIADLevels&) = delete;
+  DIADLevels& operator=(const DIADLevels&) = delete;
+  ~DIADLevels() { psl_diad_levels_destroy(h_); }
+  psl_diad_levels* handle() const { return h_; }
+  int levels() const { return levels_; }
+  InfoStruct info(int level) const {
+    InfoStruct i{};
+    check(psl_diad_levels_info(h_, level, &i, nullptr));
+    return i;
+  }
+  // one of the PSL_DIAD_* arrays of a level (int or uint8 elements, host copy)
+  template <class T>
+  std::vector<T> array(int level, int which) const {
+    int64_t counts[PSL_DIAD_ARRAYS];
+    check(psl_diad_levels_info(h_, level, nullptr, counts));
+    std::vector<T> v((size_t)counts[which]);
+    if (!v.empty()) check(psl_diad_levels_get(h_, level, which, v.data()));
+    return v;
+  }
+
+ private:
+  psl_diad_levels* h_ = nullptr;
+  int levels_ = 0;
+};
+
+// DIADGridStruct (include/types.h:166-171) of one built level
+class DIADGridStruct {
+ public:
+  DIADGridStruct(const DIADLevels& lv, int level, int storage = PSL_F32) : info_(lv.info(level)) {
+    check(psl_diad_create_level(lv.handle(), level, storage, &h_));
+    n_ = info_.cellCount;
+  }
+  DIADGridStruct(const DIADGridStruct&) = delete;
+  DIADGridStruct& operator=(const DIADGridStruct&) = delete;
+  ~DIADGridStruct() { psl_diad_destroy(h_); }
+  psl_diad* handle() const { return h_; }
+  const InfoStruct& info() const { return info_; }
+  int64_t cellCount() const { return n_; }
+  // flag bytes of the level's cells: getMarkers resolved on the host from the
+  // cell table (ijk) and the builder's bounceback marker
+  void setFlags(const std::vector<uint8_t>& flags) { check(psl_diad_set_flags(h_, flags.data())); }
+  void setGivenValues(const std::vector<float>& values16) { check(psl_diad_set_bc_values(h_, values16.data())); }
+  void setSmagorinsky(float cs) { check(psl_diad_set_smagorinsky(h_, cs)); }
+  void fillEquilibrium() { check(psl_diad_fill_equilibrium(h_, nullptr, nullptr, nullptr, nullptr)); }
+
+ private:
+  psl_diad* h_ = nullptr;
+  InfoStruct info_{};
+  int64_t n_ = 0;
+};
+
+inline void applyStreaming(DIADGridStruct& g) { check(psl_diad_apply_streaming(g.handle())); }
+inline void applyLocalCellUpdate(DIADGridStruct& g) { check(psl_diad_apply_local_cell_update(g.handle())); }
+inline void applyCoarseFineGridCommunication(DIADGridStruct& c, DIADGridStruct& f) {
+  check(psl_diad_coarse_fine_communication(c.handle(), f.handle()));
+}
+inline void applyNonReflectiveOutletZ(DIADGridStruct& g) { check(psl_diad_apply_nonreflective_outlet_z(g.handle())); }
+// updateGrid over DIAD levels (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:200-209)
+inline void updateGrid(std::vector<DIADGridStruct*>& grids, int steps = 1) {
+  std::vector<psl_diad*> h;
+  for (auto* g : grids) h.push_back(g->handle());
+  check(psl_diad_update_grid(h.data(), (int)h.size(), steps));
+}
+// getSphereDrag(DIADGridStruct&), all components, FP64
+inline void momentumExchange(DIADGridStruct& g, double (&out)[3]) {
+  check(psl_diad_reduce_momentum_exchange(g.handle(), out));
+}
+
 }  // namespace psl_facade
