@@ -367,6 +367,13 @@ int psl_diad_set_outlet(psl_diad* grid, const int* outlet_cells, int64_t n_outle
  * write maps, stored at the read maps of the next step; called before
  * psl_diad_apply_streaming. */
 int psl_diad_apply_nonreflective_outlet_z(psl_diad* grid);
+/* Moving walls (Marker.movingBounceback with the wall velocity of
+ * getGivenRhoUxUyUz; M35Full/applyLocalCellUpdateCustomized.h:69-77, applyMovingBounceback
+ * include/boundaryConditions/applyMovingBounceback.h:8-62): n distinct cells and
+ * their velocities u3[3 n].  The cells must be flagged PSL_BOUNCEBACK; each
+ * local update (and psl_diad_update_grid) then applies the moving-wall
+ * bounceback to them.  n = 0 clears the list. */
+int psl_diad_set_moving_bounceback(psl_diad* grid, const int* cells, int64_t n, const float* u3);
 /* getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
  * all three components in FP64: sum over BOUNCEBACK-flagged cells of
  * rho (u - u after bounceback), populations through the current write maps. */

@@ -90,6 +90,8 @@ def _declare(L):
                                                                 ctypes.c_longlong, P, ctypes.c_int, P, P,
                                                                 ctypes.c_longlong]
         getattr(L, f"oracle_eso_momentum_exchange_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P, P]
+        getattr(L, f"oracle_eso_moving_bounceback_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
+                                                                      ctypes.c_longlong, P]
         getattr(L, f"oracle_eso_outlet_z_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
                                                              ctypes.c_longlong, P]
     L.oracle_diad_build.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, P, P, P, ctypes.c_int, P,
@@ -327,6 +329,14 @@ def eso_outlet_z(f, nbr7, flipper, outlet, kmin):
     k = np.ascontiguousarray(kmin, np.int32)
     getattr(lib(), f"oracle_eso_outlet_z_{_suf(f)}")(f.ctypes.data, f.shape[1], nbr7.ctypes.data, int(flipper),
                                                      o.ctypes.data, o.size, k.ctypes.data)
+
+
+def eso_moving_bounceback(f, nbr7, flipper, cells, u3):
+    """applyMovingBounceback on listed cells of a raw esotwist array (in place)."""
+    c = np.ascontiguousarray(cells, np.int32)
+    u = np.ascontiguousarray(u3, np.float32).reshape(-1)
+    getattr(lib(), f"oracle_eso_moving_bounceback_{_suf(f)}")(f.ctypes.data, f.shape[1], nbr7.ctypes.data,
+                                                              int(flipper), c.ctypes.data, c.size, u.ctypes.data)
 
 
 def eso_momentum_exchange(f, nbr7, flipper, flags):

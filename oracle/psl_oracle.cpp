@@ -960,6 +960,36 @@ void eso_outlet_z(R* fa, long long n, const int* const* nbr, bool flipper, const
   }
 }
 
+// applyMovingBounceback: include/boundaryConditions/applyMovingBounceback.h:8-62, on
+// listed cells read through the read maps and written through the write maps
+// (M35Full/applyLocalCellUpdateCustomized.h:69-77).  u3: 3 per listed cell.
+template <typename R>
+void eso_moving_bounceback(R* fa, long long n, const int* const* nbr, bool flipper, const int* cells, long long m,
+                           const float* u3) {
+  for (long long t = 0; t < m; ++t) {
+    const long long cell = cells[t];
+    long long slot[8];
+    slot[0] = cell;
+    for (int a = 0; a < 7; ++a) slot[a + 1] = nbr[a][cell];
+    R f[27];
+    for (int q = 0; q < 27; ++q) f[q] = fa[(long long)(flipper ? q : opp(q)) * n + slot[ESO_READ[q]]];
+    const float ux = u3[3 * t], uy = u3[3 * t + 1], uz = u3[3 * t + 2];
+    const float w1 = 4.0f / 9.0f, w2 = 1.0f / 9.0f, w3 = 1.0f / 36.0f;
+    // (odd member of the pair, c . u as written in the reference, weight)
+    const struct { int q; float cu, w; } P[13] = {
+        {1, ux, w1}, {3, -uz, w1}, {5, -uy, w1},
+        {7, ux - uz, w2}, {9, ux + uz, w2}, {11, -ux - uy, w2}, {13, uy - uz, w2}, {15, -ux + uy, w2},
+        {17, uy + uz, w2},
+        {19, -ux + uy - uz, w3}, {21, -ux - uy + uz, w3}, {23, ux - uy - uz, w3}, {25, -ux - uy - uz, w3}};
+    for (const auto& p : P) {
+      const R tmp = f[p.q];
+      f[p.q] = f[p.q + 1] + (R)(p.w * p.cu);
+      f[p.q + 1] = tmp - (R)(p.w * p.cu);
+    }
+    for (int q = 0; q < 27; ++q) fa[(long long)(flipper ? opp(q) : q) * n + slot[ESO_WRITE[q]]] = f[q];
+  }
+}
+
 // getSphereDrag(DIADGridStruct&): flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198,
 // all three components: sum over bounceback cells of rho (u - u after bounceback).
 template <typename R>
@@ -1679,6 +1709,12 @@ ORACLE_PER_CELL(f64, double)
     const int* a[7];                                                                                           \
     for (int t = 0; t < 7; ++t) a[t] = nbr7 + (size_t)t * n;                                                   \
     eso_outlet_z(fa, n, a, flipper != 0, outlet, m, kmin);                                                     \
+  }                                                                                                            \
+  void oracle_eso_moving_bounceback_##SUF(R* fa, long long n, const int* nbr7, int flipper, const int* cells, \
+                                         long long m, const float* u3) {                                       \
+    const int* a[7];                                                                                           \
+    for (int t = 0; t < 7; ++t) a[t] = nbr7 + (size_t)t * n;                                                   \
+    eso_moving_bounceback(fa, n, a, flipper != 0, cells, m, u3);                                              \
   }                                                                                                            \
   void oracle_eso_momentum_exchange_##SUF(const R* fa, long long n, const int* nbr7, int flipper,              \
                                           const uint8_t* flags, double* out3) {                                \

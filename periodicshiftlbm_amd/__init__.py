@@ -213,6 +213,7 @@ _SIGS = [
     ("psl_diad_reduce_momentum_exchange", _I, [_P, _P]),
     ("psl_diad_set_outlet", _I, [_P, _P, ctypes.c_int64, _P]),
     ("psl_diad_apply_nonreflective_outlet_z", _I, [_P]),
+    ("psl_diad_set_moving_bounceback", _I, [_P, _P, ctypes.c_int64, _P]),
 ]
 
 
@@ -538,6 +539,12 @@ class DIADGridStruct:
         a = [np.ascontiguousarray(x, np.int32) for x in (c2f_read, c2f_write, f2c_write, f2c_read)]
         _check(lib().psl_diad_set_interface(self._h, fine.handle, a[0].ctypes.data, a[1].ctypes.data, a[0].size,
                                             a[2].ctypes.data, a[3].ctypes.data, a[2].size))
+
+    def set_moving_bounceback(self, cells, u3):
+        """Moving-wall cells (flagged BOUNCEBACK) and their velocities (n x 3)."""
+        c = np.ascontiguousarray(cells, np.int32)
+        u = np.ascontiguousarray(u3, np.float32).reshape(-1)
+        _check(lib().psl_diad_set_moving_bounceback(self._h, c.ctypes.data, c.size, u.ctypes.data))
 
     def set_outlet(self, outlet_cells, kmin):
         """outletCellArray and kMinNbrArray (n entries)."""

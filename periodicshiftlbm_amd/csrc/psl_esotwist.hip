@@ -252,6 +252,40 @@ __global__ __launch_bounds__(256) void k_eso_outlet_z(typename St::T* __restrict
   }
 }
 
+// applyMovingBounceback (include/boundaryConditions/applyMovingBounceback.h:8-62)
+// on the listed cells, which the local update skips as bounceback cells
+// (M35Full/applyLocalCellUpdateCustomized.h:69-77): populations read through
+// the read maps, each opposite pair swapped with f_q = f_opp(q) + 6 w_q (c_q . u) and
+// f_opp(q) = f_q - 6 w_q (c_q . u) for the odd member q of the pair,
+// written through the write maps.  w_q = w_opp(q), so the swap acts on the
+// stored values (f or f - w) directly, and a wall at rest leaves them exactly
+// as they were, which is the implicit bounceback of a skipped cell.
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_moving_bb(typename St::T* __restrict__ f, EsoParams p,
+                                                       const int* __restrict__ cells, const float* __restrict__ u3,
+                                                       uint32_t m) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= m) return;
+  const uint32_t cell = (uint32_t)cells[t];
+  const float ux = u3[3 * (size_t)t], uy = u3[3 * (size_t)t + 1], uz = u3[3 * (size_t)t + 2];
+  uint32_t slot[8];
+  eso_slots<St>(p, cell, slot);
+  float v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) v[q] = St::load(f[(size_t)(FLIP ? q : opp(q)) * p.n + slot[ESO_READ[q]]]);
+#pragma unroll
+  for (int q = 1; q < 27; q += 2) {
+    const float cu = (float)CX[q] * ux + (float)CY[q] * uy + (float)CZ[q] * uz;
+    // 2 w_q / c_s^2 as the reference writes it: 4/9 faces, 1/9 edges, 1/36 corners
+    const float a = (q < 7 ? 4.0f / 9.0f : (q < 19 ? 1.0f / 9.0f : 1.0f / 36.0f)) * cu;
+    const float lo = v[q];
+    v[q] = v[q + 1] + a;
+    v[q + 1] = lo - a;
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) f[(size_t)(FLIP ? opp(q) : q) * p.n + slot[ESO_WRITE[q]]] = St::store(v[q]);
+}
+
 // getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
 // all three components: per bounceback cell rho (u - u after bounceback), in
 // FP64 per-block partial sums.
@@ -378,6 +412,20 @@ static int eso_outlet(psl_diad* d) {
   return PSL_OK;
 }
 
+template <class St>
+static int eso_moving(psl_diad* d) {
+  if (!d->n_moving) return PSL_OK;
+  const EsoParams p = eso_params(d);
+  const unsigned nb = (unsigned)((d->n_moving + 255) / 256);
+  typedef typename St::T T;
+  if (d->flipper)
+    k_eso_moving_bb<St, true><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->moving, d->moving_u, (uint32_t)d->n_moving);
+  else
+    k_eso_moving_bb<St, false><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->moving, d->moving_u, (uint32_t)d->n_moving);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
 #define PSL_ESO_DISPATCH(fn, ...)                                         \
   (d->storage == PSL_F32 ? fn<StF32>(__VA_ARGS__)                         \
                          : d->storage == PSL_F16DEV ? fn<StF16>(__VA_ARGS__) : fn<StBF16>(__VA_ARGS__))
@@ -420,8 +468,9 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->nbr) (void)hipFree(d->nbr);
   if (d->flags) (void)hipFree(d->flags);
   if (d->bc_dev) (void)hipFree(d->bc_dev);
-  for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read, d->outlet, d->kmin})
+  for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read, d->outlet, d->kmin, d->moving})
     if (a) (void)hipFree(a);
+  if (d->moving_u) (void)hipFree(d->moving_u);
   if (d->partials) (void)hipFree(d->partials);
   delete d;
   return PSL_OK;
@@ -483,7 +532,9 @@ int psl_diad_apply_streaming(psl_diad* d) {
 
 int psl_diad_apply_local_cell_update(psl_diad* d) {
   PSL_CHECK_ARG(d, "null grid");
-  return PSL_ESO_DISPATCH(eso_update, d);
+  const int rc = PSL_ESO_DISPATCH(eso_update, d);
+  if (rc != PSL_OK) return rc;
+  return PSL_ESO_DISPATCH(eso_moving, d);
 }
 
 int psl_diad_get_flipper(const psl_diad* d, int* out) {
@@ -568,6 +619,33 @@ int psl_diad_set_outlet(psl_diad* d, const int* outlet_cells, int64_t n_outlet, 
   return rc;
 }
 
+int psl_diad_set_moving_bounceback(psl_diad* d, const int* cells, int64_t n, const float* u3) {
+  PSL_CHECK_ARG(d && n >= 0 && (n == 0 || (cells && u3)), "bad moving-bounceback arguments");
+  std::vector<uint8_t> seen;
+  for (int64_t t = 0; t < n; ++t) PSL_CHECK_ARG(cells[t] >= 0 && cells[t] < d->n, "moving-bounceback cell out of range");
+  if (n) {
+    seen.assign((size_t)d->n, 0);
+    for (int64_t t = 0; t < n; ++t) {
+      PSL_CHECK_ARG(!seen[cells[t]], "moving-bounceback cell listed twice");
+      seen[cells[t]] = 1;
+    }
+  }
+  int rc = upload_ints(&d->moving, cells, n);
+  if (rc == PSL_OK) {
+    if (d->moving_u) (void)hipFree(d->moving_u);
+    d->moving_u = nullptr;
+    if (n) {
+      if (hipMalloc(&d->moving_u, 3 * (size_t)n * sizeof(float)) != hipSuccess ||
+          hipMemcpy(d->moving_u, u3, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("moving-bounceback upload failed");
+        rc = PSL_E_HIP;
+      }
+    }
+  }
+  d->n_moving = rc == PSL_OK ? n : 0;
+  return rc;
+}
+
 int psl_diad_apply_nonreflective_outlet_z(psl_diad* d) {
   PSL_CHECK_ARG(d, "null grid");
   return PSL_ESO_DISPATCH(eso_outlet, d);
@@ -587,6 +665,7 @@ static int diad_recurse(psl_diad* const* lv, int nl, int l) {
   psl_diad* d = lv[l];
   d->flipper = !d->flipper;
   int rc = PSL_ESO_DISPATCH(eso_update, d);
+  if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_moving, d);
   if (rc != PSL_OK || l == nl - 1) return rc;
   for (int s = 0; s < 2 && rc == PSL_OK; ++s) rc = diad_recurse(lv, nl, l + 1);
   if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_comm, d, lv[l + 1]);

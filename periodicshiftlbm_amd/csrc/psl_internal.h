@@ -158,6 +158,9 @@ struct psl_diad {
   int* outlet = nullptr;     // outletCellArray
   int64_t n_outlet = 0;
   int* kmin = nullptr;       // kMinNbrArray (n entries), with the outlet
+  int* moving = nullptr;     // moving-bounceback cells
+  float* moving_u = nullptr; // their wall velocities (3 per cell)
+  int64_t n_moving = 0;
   double* partials = nullptr;  // reduction scratch
   size_t n_partials = 0;
 };

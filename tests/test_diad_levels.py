@@ -148,3 +148,29 @@ def test_outlet_extrapolates_uniform_and_linear_states(flip):
         fi, sl = rs[q]
         want = INV_SQRT3 * (a[q] + b[q] * k[kmin[out]]) + ONE_MINUS * (a[q] + b[q] * k[out])
         assert np.max(np.abs(raw[fi, sl[out]] - want)) < 1e-12
+
+
+# ------------------------------------------------------------ moving walls
+@pytest.mark.parametrize("flip", [False, True])
+def test_moving_bounceback_pins(flip):
+    """A wall at rest leaves every stored value as it was (the implicit
+    bounceback of a skipped cell); a moving wall conserves the cell's mass and
+    turns its momentum m into -m + 2 u (sum_q 6 w_q c_q c_q = 2 I)."""
+    d = outlet_level()
+    nbr7 = np.ascontiguousarray(d["nbr"][:7], np.int32)
+    rng = np.random.default_rng(4 + flip)
+    raw = rng.random((27, d["n"]))
+    cells = np.sort(rng.choice(d["n"], 200, replace=False)).astype(np.int32)
+    before = raw.copy()
+    O.eso_moving_bounceback(raw, nbr7, flip, cells, np.zeros((cells.size, 3), np.float32))
+    assert np.array_equal(raw, before)
+    u = (0.05 * rng.standard_normal((cells.size, 3))).astype(np.float32)
+    R, _ = O.esotwist_tables()
+    s = np.vstack([np.arange(d["n"])[None], d["nbr"][:7].astype(np.int64)])
+    f_in = np.stack([raw[q if flip else E.OPP[q], s[R[q]][cells]] for q in range(27)])  # read maps
+    O.eso_moving_bounceback(raw, nbr7, flip, cells, u)
+    f_out = E.eso_get_all(raw, nbr7, flip)[:, cells]                                    # write maps
+    c = np.array(O.velocity_set(), np.float64)                                          # 3 x 27
+    assert np.max(np.abs(f_out.sum(0) - f_in.sum(0))) < 1e-12
+    m_in, m_out = c @ f_in, c @ f_out
+    assert np.max(np.abs(m_out - (-m_in + 2.0 * u.T))) < 1e-7
