@@ -379,6 +379,30 @@ int psl_diad_set_moving_bounceback(psl_diad* grid, const int* cells, int64_t n, 
  * rho (u - u after bounceback), populations through the current write maps. */
 int psl_diad_reduce_momentum_exchange(psl_diad* grid, double out[3]);
 
+/* ----------------------------------------------------------- output */
+/* exportSectionCutPlotXY/ZY/ZX(GridStruct&) (include/plotter/exportSectionCutPlot.h:2-104):
+ * plane 0 = XY at k = cut, 1 = ZY at i = cut, 2 = ZX at j = cut; rows x cols
+ * pixels (dims[0] = rows, dims[1] = cols; host_out = NULL only reports them)
+ * of 5 floats: p = (rho - 1) rho_nominal soundspeed^2, the horizontal,
+ * vertical and normal velocity in m/s (u res / 1000 / dtPhys), and 1 for a
+ * bounceback cell. */
+int psl_section_cut(psl_grid* grid, int plane, int cut, float rho_nominal, float soundspeed, float* host_out,
+                    int* dims);
+/* The same over DIAD levels (exportSectionCutPlotGeneral(std::vector<DIADGridStruct>&), lines 377-589):
+ * the plane at the finest level whose plane has fewer than max_pixels pixels
+ * (dims3 = rows, cols, levels used; the reference uses 20 000 000), 6 floats
+ * per pixel (the five above and the level); cut is an index on the finest
+ * level.  Levels paint coarse to fine; where several finer cells fall on one
+ * pixel the one with the largest cell index wins (the reference leaves that
+ * to the order of its threads). */
+int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cut, float rho_nominal,
+                         float soundspeed, int64_t max_pixels, float* host_out, int* dims3);
+/* /dev/shm/sim_data.bin format of the reference's plotters (exportSectionCutPlot.h:64-86):
+ * int {plot_number, rows, cols, components}, then the floats. */
+int psl_write_sim_data(const char* path, int plot_number, int rows, int cols, int components, const float* data);
+/* historyData.bin (exportSectionCutPlot.h:994-1001): int count, then count floats. */
+int psl_write_history(const char* path, const float* values, int count);
+
 /* ------------------------------------------------------------- geometry */
 /* readSTL (include/STLFunctions.h:3-89). */
 int psl_stl_read(const char* path, psl_stl** out);

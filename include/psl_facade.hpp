@@ -278,5 +278,40 @@ inline void updateGrid(std::vector<DIADGridStruct*>& grids, int steps = 1) {
 inline void momentumExchange(DIADGridStruct& g, double (&out)[3]) {
   check(psl_diad_reduce_momentum_exchange(g.handle(), out));
 }
+// movingBounceback cells with their wall velocities (cells flagged PSL_BOUNCEBACK)
+inline void setMovingBounceback(DIADGridStruct& g, const std::vector<int>& cells, const std::vector<float>& u3) {
+  check(psl_diad_set_moving_bounceback(g.handle(), cells.data(), (int64_t)cells.size(), u3.data()));
+}
+
+// ------------------------------------------------------------------ output
+// exportSectionCutPlotXY/ZY/ZX (include/plotter/exportSectionCutPlot.h): the cut
+// written to path in the plotters' sim_data.bin format
+enum PlaneEnum { XY = 0, ZY = 1, ZX = 2 };
+inline void exportSectionCutPlot(GridStruct& g, PlaneEnum plane, int cut, int plotNumber, float rhoNominalPhys,
+                                 float soundspeedPhys, const char* path = "/dev/shm/sim_data.bin") {
+  int dims[2];
+  check(psl_section_cut(g.handle(), plane, cut, rhoNominalPhys, soundspeedPhys, nullptr, dims));
+  std::vector<float> data((size_t)dims[0] * dims[1] * 5);
+  check(psl_section_cut(g.handle(), plane, cut, rhoNominalPhys, soundspeedPhys, data.data(), dims));
+  check(psl_write_sim_data(path, plotNumber, dims[0], dims[1], 5, data.data()));
+}
+inline void exportSectionCutPlot(std::vector<DIADGridStruct*>& grids, PlaneEnum plane, int cut, int plotNumber,
+                                 float rhoNominalPhys, float soundspeedPhys,
+                                 const char* path = "/dev/shm/sim_data.bin") {
+  std::vector<psl_diad*> h;
+  for (auto* g : grids) h.push_back(g->handle());
+  int dims[3];
+  check(psl_diad_section_cut(h.data(), (int)h.size(), plane, cut, rhoNominalPhys, soundspeedPhys, 20000000, nullptr,
+                             dims));
+  std::vector<float> data((size_t)dims[0] * dims[1] * 6);
+  check(psl_diad_section_cut(h.data(), (int)h.size(), plane, cut, rhoNominalPhys, soundspeedPhys, 20000000,
+                             data.data(), dims));
+  check(psl_write_sim_data(path, plotNumber, dims[0], dims[1], 6, data.data()));
+}
+// exportHistoryData (exportSectionCutPlot.h:994-1001)
+inline void exportHistoryData(const std::vector<float>& history, int currentIteration,
+                              const char* path = "/dev/shm/historyData.bin") {
+  check(psl_write_history(path, history.data(), currentIteration + 1));
+}
 
 }  // namespace psl_facade

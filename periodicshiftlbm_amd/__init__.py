@@ -214,6 +214,10 @@ _SIGS = [
     ("psl_diad_set_outlet", _I, [_P, _P, ctypes.c_int64, _P]),
     ("psl_diad_apply_nonreflective_outlet_z", _I, [_P]),
     ("psl_diad_set_moving_bounceback", _I, [_P, _P, ctypes.c_int64, _P]),
+    ("psl_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
+    ("psl_diad_section_cut", _I, [_P, _I, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
+    ("psl_write_sim_data", _I, [ctypes.c_char_p, _I, _I, _I, _I, _P]),
+    ("psl_write_history", _I, [ctypes.c_char_p, _P, _I]),
 ]
 
 
@@ -685,6 +689,49 @@ class DIADLevels:
 def buildDIADGrids(base: InfoStruct, levels: int, wall_refinement_span: int, nu_phys: float,
                    stls: Sequence["STLStruct"] = (), marker_rules=None) -> DIADLevels:
     return DIADLevels(base, levels, wall_refinement_span, nu_phys, stls, marker_rules)
+
+
+# ------------------------------------------------------------ output
+PLANES = {"XY": 0, "ZY": 1, "ZX": 2}
+
+
+def section_cut(grid, plane: str, cut: int, rho_nominal: float = 1.225, soundspeed: float = 1.0) -> np.ndarray:
+    """exportSectionCutPlotGeneral on a GridStruct (rows x cols x 5) or a list of
+    DIADGridStruct levels (rows x cols x 6, the last component the level)."""
+    pl = PLANES[plane]
+    if isinstance(grid, (list, tuple)):
+        arr = (ctypes.c_void_p * len(grid))(*[g.handle.value for g in grid])
+        dims = np.zeros(3, np.int32)
+        _check(lib().psl_diad_section_cut(arr, len(grid), pl, cut, rho_nominal, soundspeed, 20000000, None,
+                                          dims.ctypes.data))
+        out = np.zeros((dims[0], dims[1], 6), np.float32)
+        _check(lib().psl_diad_section_cut(arr, len(grid), pl, cut, rho_nominal, soundspeed, 20000000,
+                                          out.ctypes.data, dims.ctypes.data))
+        return out
+    dims = np.zeros(2, np.int32)
+    _check(lib().psl_section_cut(grid.handle, pl, cut, rho_nominal, soundspeed, None, dims.ctypes.data))
+    out = np.zeros((dims[0], dims[1], 5), np.float32)
+    _check(lib().psl_section_cut(grid.handle, pl, cut, rho_nominal, soundspeed, out.ctypes.data, dims.ctypes.data))
+    return out
+
+
+def exportSectionCutPlot(grid, plane: str, cut: int, plot_number: int, path: str = "/dev/shm/sim_data.bin",
+                         rho_nominal: float = 1.225, soundspeed: float = 1.0) -> np.ndarray:
+    """exportSectionCutPlotXY/ZY/ZX: the cut written in the plotters' sim_data.bin format."""
+    data = section_cut(grid, plane, cut, rho_nominal, soundspeed)
+    write_sim_data(path, plot_number, data)
+    return data
+
+
+def write_sim_data(path: str, plot_number: int, data: np.ndarray):
+    d = np.ascontiguousarray(data, np.float32)
+    _check(lib().psl_write_sim_data(path.encode(), plot_number, d.shape[0], d.shape[1], d.shape[2], d.ctypes.data))
+
+
+def exportHistoryData(values, current_iteration: int, path: str = "/dev/shm/historyData.bin"):
+    """exportHistoryData (exportSectionCutPlot.h:994-1001): the first current_iteration + 1 values."""
+    v = np.ascontiguousarray(values, np.float32)[: current_iteration + 1]
+    _check(lib().psl_write_history(path.encode(), v.ctypes.data, v.size))
 
 
 # ------------------------------------------------------------ operators

@@ -286,6 +286,23 @@ __global__ __launch_bounds__(256) void k_eso_moving_bb(typename St::T* __restric
   for (int q = 0; q < 27; ++q) f[(size_t)(FLIP ? opp(q) : q) * p.n + slot[ESO_WRITE[q]]] = St::store(v[q]);
 }
 
+// rho, ux, uy, uz of every cell (getRhoUxUyUz, include/cellFunctions.h:135-155)
+// from the populations the last update wrote: out[a * n + cell]
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_macro(const typename St::T* __restrict__ f, EsoParams p,
+                                                   float* __restrict__ out) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= p.n) return;
+  float v[27];
+  eso_get<St, FLIP>(f, p, cell, v);
+  float r, a, b, c;
+  rho_u(v, r, a, b, c);
+  out[cell] = r;
+  out[(size_t)p.n + cell] = a;
+  out[2 * (size_t)p.n + cell] = b;
+  out[3 * (size_t)p.n + cell] = c;
+}
+
 // getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
 // all three components: per bounceback cell rho (u - u after bounceback), in
 // FP64 per-block partial sums.
@@ -422,6 +439,16 @@ static int eso_moving(psl_diad* d) {
     k_eso_moving_bb<St, true><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->moving, d->moving_u, (uint32_t)d->n_moving);
   else
     k_eso_moving_bb<St, false><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->moving, d->moving_u, (uint32_t)d->n_moving);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+template <class St>
+static int eso_macro(psl_diad* d, float* out) {
+  const EsoParams p = eso_params(d);
+  const unsigned nb = (unsigned)((d->n + 255) / 256);
+  if (d->flipper) k_eso_macro<St, true><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, out);
+  else k_eso_macro<St, false><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, out);
   PSL_LAUNCH_CHECK();
   return PSL_OK;
 }
@@ -736,3 +763,7 @@ int psl_diad_dense_box(int nx, int ny, int nz, int periodic_mask, int* ijk3, int
 }
 
 }  // extern "C"
+
+namespace psl {
+int diad_macro_device(psl_diad* d, float* out4n) { return PSL_ESO_DISPATCH(eso_macro, d, out4n); }
+}  // namespace psl

@@ -185,6 +185,9 @@ __host__ __device__ inline uint64_t pack_key(int i, int j, int k) {
 // spacing res, box nx x ny x nz) into out[c] for n cells given as packed keys
 int stl_mark_keys(const psl_stl* s, float gox, float goy, float goz, float res, int nx, int ny, int nz,
                   const uint64_t* keys, int64_t n, uint8_t* out);
+// implemented in psl_esotwist.hip: rho, ux, uy, uz per cell (4 x n floats,
+// device) on the grid's stream
+int diad_macro_device(psl_diad* d, float* out4n);
 // implemented in psl_grid.hip
 int grid_rebuild_flags(psl_grid* g);
 int ensure_partials(psl_grid* g, size_t n);

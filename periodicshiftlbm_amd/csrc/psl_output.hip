@@ -1,0 +1,270 @@
+// Section-cut and history output in the reference's plotter formats.
+//
+// include/plotter/exportSectionCutPlot.h: a plane of a grid (XY at k = cut,
+// ZY at i = cut, ZX at j = cut) as rows of (p, u_horizontal, u_vertical,
+// u_normal, bounceback marker[, gridID]) with the pressure
+// p = (rho - 1) rho_nominal c_s^2 and velocities in m/s (convertToPhysical*,
+// include/cellFunctions.h:200-219), and the binary files the reference's
+// Python plotters read: sim_data.bin = int header {plotNumber, rows, cols,
+// components} + float rows, historyData.bin = int count + count floats.
+//
+// The fields are evaluated on the device: the macroscopic fields of the grid,
+// then one thread per pixel of the plane.  For DIAD levels every level paints
+// its cells on the plane in level order, coarse to fine, at the resolution of
+// the finest level whose plane has fewer than max_pixels pixels (coarse cells
+// fill their whole footprint; where several finer cells fall on one pixel,
+// the one with the largest cell index wins).
+#include "psl_internal.h"
+
+#include <cstdio>
+
+namespace psl {
+namespace {
+
+struct CutParams {
+  int plane;  // 0 XY, 1 ZY, 2 ZX
+  int cut;
+  int h, v;   // pixels horizontally / vertically
+  float vel_scale;  // res / 1000 / dtPhys
+  float p_scale;    // rho_nominal * c_s^2
+};
+
+__device__ __forceinline__ void plane_components(int plane, float ux, float uy, float uz, float& uh, float& uv,
+                                                 float& un) {
+  if (plane == 0) { uh = ux; uv = uy; un = uz; }
+  else if (plane == 1) { uh = uz; uv = uy; un = ux; }
+  else { uh = uz; uv = ux; un = uy; }
+}
+
+// exportSectionCutPlotGeneral(GridStruct&) (lines 2-89), one thread per pixel
+__global__ void k_cut_grid(const float* __restrict__ macro, const uint8_t* __restrict__ flags, int nx, int ny,
+                           size_t n, CutParams c, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y;
+  if (x >= c.h || y >= c.v) return;
+  int i, j, k;
+  if (c.plane == 0) { i = x; j = y; k = c.cut; }
+  else if (c.plane == 1) { i = c.cut; j = y; k = x; }
+  else { i = y; j = c.cut; k = x; }
+  const size_t cell = ((size_t)k * ny + j) * nx + i;
+  const float rho = macro[cell], ux = macro[n + cell] * c.vel_scale, uy = macro[2 * n + cell] * c.vel_scale,
+              uz = macro[3 * n + cell] * c.vel_scale;
+  float uh, uv, un;
+  plane_components(c.plane, ux, uy, uz, uh, uv, un);
+  float* o = out + 5 * ((size_t)y * c.h + x);
+  o[0] = (rho - 1.f) * c.p_scale;
+  o[1] = uh;
+  o[2] = uv;
+  o[3] = un;
+  o[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK) ? 1.f : 0.f;
+}
+
+// exportSectionCutPlotGeneral(std::vector<DIADGridStruct>&) (lines 377-573):
+// the cells of one level on the plane claim their pixels (largest cell index wins)
+__global__ void k_cut_claim(const int* __restrict__ ijk, int64_t n, int cell_scale, int target_scale, CutParams c,
+                            int* __restrict__ winner) {
+  for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < n;
+       cell += (int64_t)gridDim.x * blockDim.x) {
+    const int i = ijk[cell] * cell_scale, j = ijk[n + cell] * cell_scale, k = ijk[2 * n + cell] * cell_scale;
+    int ih, iv, ic;
+    if (c.plane == 0) { ic = k; ih = i; iv = j; }
+    else if (c.plane == 1) { ic = i; iv = j; ih = k; }
+    else { ic = j; iv = i; ih = k; }
+    if (c.cut < ic || c.cut >= ic + cell_scale) continue;
+    const int y0 = iv / target_scale, x0 = ih / target_scale;
+    const int span = max(1, cell_scale / target_scale);
+    for (int dy = 0; dy < span; ++dy) {
+      const int y = y0 + dy;
+      if (y >= c.v) break;
+      for (int dx = 0; dx < span; ++dx) {
+        const int x = x0 + dx;
+        if (x >= c.h) break;
+        atomicMax(&winner[(size_t)y * c.h + x], (int)cell);
+      }
+    }
+  }
+}
+
+// the claimed pixels take the winning cell's fields (converted with its level)
+__global__ void k_cut_paint(const int* __restrict__ winner, const float* __restrict__ macro,
+                            const uint8_t* __restrict__ flags, int64_t n, CutParams c, int level,
+                            float* __restrict__ out) {
+  const size_t pix = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (pix >= (size_t)c.h * c.v) return;
+  const int cell = winner[pix];
+  if (cell < 0) return;
+  const float rho = macro[cell], ux = macro[n + cell] * c.vel_scale, uy = macro[2 * n + cell] * c.vel_scale,
+              uz = macro[3 * n + cell] * c.vel_scale;
+  float uh, uv, un;
+  plane_components(c.plane, ux, uy, uz, uh, uv, un);
+  float* o = out + 6 * pix;
+  o[0] = (rho - 1.f) * c.p_scale;
+  o[1] = uh;
+  o[2] = uv;
+  o[3] = un;
+  o[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK) ? 1.f : 0.f;
+  o[5] = (float)level;
+}
+
+__global__ void k_cut_fill(float* __restrict__ out, size_t pixels) {
+  const size_t pix = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (pix >= pixels) return;
+  float* o = out + 6 * pix;  // rho = 1, u = 0, marker 1, gridID 0 (lines 414-419)
+  o[0] = 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = 0.f; o[4] = 1.f; o[5] = 0.f;
+}
+
+void plane_dims(int plane, int nx, int ny, int nz, int& h, int& v) {
+  if (plane == 0) { h = nx; v = ny; }
+  else if (plane == 1) { h = nz; v = ny; }
+  else { h = nz; v = nx; }
+}
+
+}  // namespace
+}  // namespace psl
+
+using namespace psl;
+
+extern "C" {
+
+int psl_section_cut(psl_grid* g, int plane, int cut, float rho_nominal, float soundspeed, float* host_out,
+                    int* dims) {
+  PSL_CHECK_ARG(g && dims && plane >= 0 && plane <= 2, "bad section-cut arguments");
+  const psl_info& I = g->info;
+  int h, v;
+  plane_dims(plane, I.cellCountX, I.cellCountY, I.cellCountZ, h, v);
+  const int depth = plane == 0 ? I.cellCountZ : (plane == 1 ? I.cellCountX : I.cellCountY);
+  PSL_CHECK_ARG(cut >= 0 && cut < depth, "cut index outside the grid");
+  PSL_CHECK_ARG(I.dtPhys > 0.f, "dtPhys must be positive for physical velocities");
+  dims[0] = v;
+  dims[1] = h;
+  if (!host_out) return PSL_OK;
+  const size_t n = (size_t)I.cellCount;
+  float* macro = nullptr;
+  float* out = nullptr;
+  int rc = PSL_OK;
+  if (hipMalloc(&macro, 4 * n * sizeof(float)) != hipSuccess ||
+      hipMalloc(&out, 5 * (size_t)h * v * sizeof(float)) != hipSuccess) {
+    set_error("section cut: allocation failed");
+    rc = PSL_E_NOMEM;
+  }
+  if (rc == PSL_OK) rc = psl_get_macroscopic_device(g, macro);
+  if (rc == PSL_OK) {
+    CutParams c{plane, cut, h, v, (I.res / 1000.f) / I.dtPhys, rho_nominal * soundspeed * soundspeed};
+    dim3 grid((h + 255) / 256, v);
+    k_cut_grid<<<grid, 256, 0, g->stream>>>(macro, g->flags, I.cellCountX, I.cellCountY, n, c, out);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(host_out, out, 5 * (size_t)h * v * sizeof(float), hipMemcpyDeviceToHost, g->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess) {
+      set_error("section cut: kernel or copy failed");
+      rc = PSL_E_HIP;
+    }
+  }
+  if (macro) (void)hipFree(macro);
+  if (out) (void)hipFree(out);
+  return rc;
+}
+
+int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cut, float rho_nominal,
+                         float soundspeed, int64_t max_pixels, float* host_out, int* dims3) {
+  PSL_CHECK_ARG(levels && nlevels >= 1 && dims3 && plane >= 0 && plane <= 2, "bad section-cut arguments");
+  for (int l = 0; l < nlevels; ++l) PSL_CHECK_ARG(levels[l] && levels[l]->info.dtPhys > 0.f, "bad level");
+  // the finest level whose plane fits (lines 380-399)
+  int target = nlevels, h = 0, v = 0;
+  while (target > 0) {
+    const psl_info& I = levels[target - 1]->info;
+    plane_dims(plane, I.cellCountX, I.cellCountY, I.cellCountZ, h, v);
+    if ((int64_t)h * v < max_pixels || target == 1) break;
+    --target;
+  }
+  const int target_scale = 1 << (nlevels - target);
+  dims3[0] = v;
+  dims3[1] = h;
+  dims3[2] = target;
+  if (!host_out) return PSL_OK;
+  const size_t pixels = (size_t)h * v;
+  hipStream_t st = levels[0]->stream;
+  float* out = nullptr;
+  int* winner = nullptr;
+  int rc = PSL_OK;
+  if (hipMalloc(&out, 6 * pixels * sizeof(float)) != hipSuccess || hipMalloc(&winner, pixels * sizeof(int)) != hipSuccess) {
+    set_error("section cut: allocation failed");
+    rc = PSL_E_NOMEM;
+  }
+  if (rc == PSL_OK) {
+    k_cut_fill<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(out, pixels);
+    if (hipGetLastError() != hipSuccess) rc = PSL_E_HIP;
+  }
+  for (int l = 0; l < nlevels && rc == PSL_OK; ++l) {
+    psl_diad* d = levels[l];
+    const psl_info& I = d->info;
+    float* macro = nullptr;
+    if (hipMalloc(&macro, 4 * (size_t)d->n * sizeof(float)) != hipSuccess) {
+      set_error("section cut: allocation failed");
+      rc = PSL_E_NOMEM;
+      break;
+    }
+    if (hipStreamSynchronize(d->stream) != hipSuccess) rc = PSL_E_HIP;
+    if (rc == PSL_OK) rc = diad_macro_device(d, macro);
+    if (rc == PSL_OK && hipStreamSynchronize(d->stream) != hipSuccess) rc = PSL_E_HIP;
+    if (rc == PSL_OK) {
+      CutParams c{plane, cut, h, v, (I.res / 1000.f) / I.dtPhys, rho_nominal * soundspeed * soundspeed};
+      const int cell_scale = 1 << (nlevels - l - 1);
+      if (hipMemsetAsync(winner, 0xff, pixels * sizeof(int), st) != hipSuccess) rc = PSL_E_HIP;
+      const unsigned nb = (unsigned)std::min<int64_t>((d->n + 255) / 256, 1 << 20);
+      k_cut_claim<<<nb, 256, 0, st>>>(d->ijk, d->n, cell_scale, target_scale, c, winner);
+      k_cut_paint<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(winner, macro, d->flags, d->n, c, l, out);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = PSL_E_HIP;
+    }
+    (void)hipFree(macro);
+    if (rc == PSL_E_HIP) set_error("section cut: kernel failed");
+  }
+  if (rc == PSL_OK &&
+      (hipMemcpy(host_out, out, 6 * pixels * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)) {
+    set_error("section cut: copy failed");
+    rc = PSL_E_HIP;
+  }
+  if (out) (void)hipFree(out);
+  if (winner) (void)hipFree(winner);
+  return rc;
+}
+
+// sim_data.bin (exportSectionCutPlot.h:64-86, 543-569): int header {plotNumber,
+// rows, cols, components} and rows x cols x components floats.
+int psl_write_sim_data(const char* path, int plot_number, int rows, int cols, int components, const float* data) {
+  PSL_CHECK_ARG(path && data && rows > 0 && cols > 0 && components > 0, "bad sim_data arguments");
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) {
+    set_error(std::string("cannot open ") + path);
+    return PSL_E_IO;
+  }
+  const int header[4] = {plot_number, rows, cols, components};
+  const size_t count = (size_t)rows * cols * components;
+  const bool ok = std::fwrite(header, sizeof(int), 4, fp) == 4 && std::fwrite(data, sizeof(float), count, fp) == count;
+  std::fclose(fp);
+  if (!ok) {
+    set_error(std::string("short write to ") + path);
+    return PSL_E_IO;
+  }
+  return PSL_OK;
+}
+
+// historyData.bin (exportSectionCutPlot.h:994-1001): int count and count floats.
+int psl_write_history(const char* path, const float* values, int count) {
+  PSL_CHECK_ARG(path && count >= 0 && (count == 0 || values), "bad history arguments");
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) {
+    set_error(std::string("cannot open ") + path);
+    return PSL_E_IO;
+  }
+  const bool ok = std::fwrite(&count, sizeof(int), 1, fp) == 1 &&
+                  std::fwrite(values, sizeof(float), (size_t)count, fp) == (size_t)count;
+  std::fclose(fp);
+  if (!ok) {
+    set_error(std::string("short write to ") + path);
+    return PSL_E_IO;
+  }
+  return PSL_OK;
+}
+
+}  // extern "C"
