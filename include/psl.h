@@ -397,6 +397,12 @@ int psl_section_cut(psl_grid* grid, int plane, int cut, float rho_nominal, float
  * to the order of its threads). */
 int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cut, float rho_nominal,
                          float soundspeed, int64_t max_pixels, float* host_out, int* dims3);
+/* getFlowReportXY/ZY/ZX (include/flowReportFunctions.h:2-263): the DIAD plane cut
+ * clipped to bounds6 = (xmin, ymin, zmin, xmax, ymax, zmax) (physical units),
+ * then over its pixels outside walls the mean ux, uy, uz and rho in lattice
+ * units and the area in mm^2: out5 = ux, uy, uz, rho, area (FP64 sums). */
+int psl_diad_flow_report(psl_diad* const* levels, int nlevels, int plane, int cut, const float* bounds6,
+                         double* out5);
 /* /dev/shm/sim_data.bin format of the reference's plotters (exportSectionCutPlot.h:64-86):
  * int {plot_number, rows, cols, components}, then the floats. */
 int psl_write_sim_data(const char* path, int plot_number, int rows, int cols, int components, const float* data);

@@ -216,6 +216,7 @@ _SIGS = [
     ("psl_diad_set_moving_bounceback", _I, [_P, _P, ctypes.c_int64, _P]),
     ("psl_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
     ("psl_diad_section_cut", _I, [_P, _I, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
+    ("psl_diad_flow_report", _I, [_P, _I, _I, _I, _P, _P]),
     ("psl_write_sim_data", _I, [ctypes.c_char_p, _I, _I, _I, _I, _P]),
     ("psl_write_history", _I, [ctypes.c_char_p, _P, _I]),
 ]
@@ -713,6 +714,17 @@ def section_cut(grid, plane: str, cut: int, rho_nominal: float = 1.225, soundspe
     out = np.zeros((dims[0], dims[1], 5), np.float32)
     _check(lib().psl_section_cut(grid.handle, pl, cut, rho_nominal, soundspeed, out.ctypes.data, dims.ctypes.data))
     return out
+
+
+def getFlowReport(levels, plane: str, cut: int, bounds) -> dict:
+    """getFlowReportXY/ZY/ZX over DIAD levels: mean ux, uy, uz, rho (lattice units)
+    and the open area (mm^2) of the plane clipped to bounds (xmin, ymin, zmin,
+    xmax, ymax, zmax)."""
+    arr = (ctypes.c_void_p * len(levels))(*[g.handle.value for g in levels])
+    b = np.ascontiguousarray(bounds, np.float32)
+    out = np.zeros(5, np.float64)
+    _check(lib().psl_diad_flow_report(arr, len(levels), PLANES[plane], cut, b.ctypes.data, out.ctypes.data))
+    return {"ux": out[0], "uy": out[1], "uz": out[2], "rho": out[3], "areamm2": out[4]}
 
 
 def exportSectionCutPlot(grid, plane: str, cut: int, plot_number: int, path: str = "/dev/shm/sim_data.bin",

@@ -16,7 +16,10 @@
 // the one with the largest cell index wins).
 #include "psl_internal.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <vector>
 
 namespace psl {
 namespace {
@@ -27,6 +30,8 @@ struct CutParams {
   int h, v;   // pixels horizontally / vertically
   float vel_scale;  // res / 1000 / dtPhys
   float p_scale;    // rho_nominal * c_s^2
+  int h0 = 0, v0 = 0;  // first pixel of a cropped window
+  int raw = 0;         // 1: rho, ux, uy, uz in lattice units (flow reports)
 };
 
 __device__ __forceinline__ void plane_components(int plane, float ux, float uy, float uz, float& uh, float& uv,
@@ -71,13 +76,15 @@ __global__ void k_cut_claim(const int* __restrict__ ijk, int64_t n, int cell_sca
     else if (c.plane == 1) { ic = i; iv = j; ih = k; }
     else { ic = j; iv = i; ih = k; }
     if (c.cut < ic || c.cut >= ic + cell_scale) continue;
-    const int y0 = iv / target_scale, x0 = ih / target_scale;
+    const int y0 = iv / target_scale - c.v0, x0 = ih / target_scale - c.h0;
     const int span = max(1, cell_scale / target_scale);
     for (int dy = 0; dy < span; ++dy) {
       const int y = y0 + dy;
+      if (y < 0) continue;
       if (y >= c.v) break;
       for (int dx = 0; dx < span; ++dx) {
         const int x = x0 + dx;
+        if (x < 0) continue;
         if (x >= c.h) break;
         atomicMax(&winner[(size_t)y * c.h + x], (int)cell);
       }
@@ -95,22 +102,29 @@ __global__ void k_cut_paint(const int* __restrict__ winner, const float* __restr
   if (cell < 0) return;
   const float rho = macro[cell], ux = macro[n + cell] * c.vel_scale, uy = macro[2 * n + cell] * c.vel_scale,
               uz = macro[3 * n + cell] * c.vel_scale;
-  float uh, uv, un;
-  plane_components(c.plane, ux, uy, uz, uh, uv, un);
   float* o = out + 6 * pix;
-  o[0] = (rho - 1.f) * c.p_scale;
-  o[1] = uh;
-  o[2] = uv;
-  o[3] = un;
+  if (c.raw) {
+    o[0] = rho;
+    o[1] = ux;
+    o[2] = uy;
+    o[3] = uz;
+  } else {
+    float uh, uv, un;
+    plane_components(c.plane, ux, uy, uz, uh, uv, un);
+    o[0] = (rho - 1.f) * c.p_scale;
+    o[1] = uh;
+    o[2] = uv;
+    o[3] = un;
+  }
   o[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK) ? 1.f : 0.f;
   o[5] = (float)level;
 }
 
-__global__ void k_cut_fill(float* __restrict__ out, size_t pixels) {
+__global__ void k_cut_fill(float* __restrict__ out, size_t pixels, int raw) {
   const size_t pix = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (pix >= pixels) return;
   float* o = out + 6 * pix;  // rho = 1, u = 0, marker 1, gridID 0 (lines 414-419)
-  o[0] = 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = 0.f; o[4] = 1.f; o[5] = 0.f;
+  o[0] = raw ? 1.f : 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = 0.f; o[4] = 1.f; o[5] = 0.f;
 }
 
 void plane_dims(int plane, int nx, int ny, int nz, int& h, int& v) {
@@ -165,37 +179,28 @@ int psl_section_cut(psl_grid* g, int plane, int cut, float rho_nominal, float so
   return rc;
 }
 
-int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cut, float rho_nominal,
-                         float soundspeed, int64_t max_pixels, float* host_out, int* dims3) {
-  PSL_CHECK_ARG(levels && nlevels >= 1 && dims3 && plane >= 0 && plane <= 2, "bad section-cut arguments");
-  for (int l = 0; l < nlevels; ++l) PSL_CHECK_ARG(levels[l] && levels[l]->info.dtPhys > 0.f, "bad level");
-  // the finest level whose plane fits (lines 380-399)
-  int target = nlevels, h = 0, v = 0;
-  while (target > 0) {
-    const psl_info& I = levels[target - 1]->info;
-    plane_dims(plane, I.cellCountX, I.cellCountY, I.cellCountZ, h, v);
-    if ((int64_t)h * v < max_pixels || target == 1) break;
-    --target;
-  }
-  const int target_scale = 1 << (nlevels - target);
-  dims3[0] = v;
-  dims3[1] = h;
-  dims3[2] = target;
-  if (!host_out) return PSL_OK;
-  const size_t pixels = (size_t)h * v;
+}  // extern "C"
+
+namespace psl {
+namespace {
+// Paints the levels onto a window of h x v pixels at 1 / target_scale of the
+// finest resolution, starting at pixel (h0, v0); out: 6 floats per pixel.
+int paint_levels(psl_diad* const* levels, int nlevels, const CutParams& base, int target_scale, float* host_out) {
+  const size_t pixels = (size_t)base.h * base.v;
   hipStream_t st = levels[0]->stream;
   float* out = nullptr;
   int* winner = nullptr;
   int rc = PSL_OK;
-  if (hipMalloc(&out, 6 * pixels * sizeof(float)) != hipSuccess || hipMalloc(&winner, pixels * sizeof(int)) != hipSuccess) {
+  if (hipMalloc(&out, 6 * std::max<size_t>(pixels, 1) * sizeof(float)) != hipSuccess ||
+      hipMalloc(&winner, std::max<size_t>(pixels, 1) * sizeof(int)) != hipSuccess) {
     set_error("section cut: allocation failed");
     rc = PSL_E_NOMEM;
   }
-  if (rc == PSL_OK) {
-    k_cut_fill<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(out, pixels);
+  if (rc == PSL_OK && pixels) {
+    k_cut_fill<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(out, pixels, base.raw);
     if (hipGetLastError() != hipSuccess) rc = PSL_E_HIP;
   }
-  for (int l = 0; l < nlevels && rc == PSL_OK; ++l) {
+  for (int l = 0; l < nlevels && rc == PSL_OK && pixels; ++l) {
     psl_diad* d = levels[l];
     const psl_info& I = d->info;
     float* macro = nullptr;
@@ -208,7 +213,8 @@ int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cu
     if (rc == PSL_OK) rc = diad_macro_device(d, macro);
     if (rc == PSL_OK && hipStreamSynchronize(d->stream) != hipSuccess) rc = PSL_E_HIP;
     if (rc == PSL_OK) {
-      CutParams c{plane, cut, h, v, (I.res / 1000.f) / I.dtPhys, rho_nominal * soundspeed * soundspeed};
+      CutParams c = base;
+      c.vel_scale = base.raw ? 1.f : (I.res / 1000.f) / I.dtPhys;
       const int cell_scale = 1 << (nlevels - l - 1);
       if (hipMemsetAsync(winner, 0xff, pixels * sizeof(int), st) != hipSuccess) rc = PSL_E_HIP;
       const unsigned nb = (unsigned)std::min<int64_t>((d->n + 255) / 256, 1 << 20);
@@ -219,7 +225,7 @@ int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cu
     (void)hipFree(macro);
     if (rc == PSL_E_HIP) set_error("section cut: kernel failed");
   }
-  if (rc == PSL_OK &&
+  if (rc == PSL_OK && pixels &&
       (hipMemcpy(host_out, out, 6 * pixels * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)) {
     set_error("section cut: copy failed");
     rc = PSL_E_HIP;
@@ -227,6 +233,93 @@ int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cu
   if (out) (void)hipFree(out);
   if (winner) (void)hipFree(winner);
   return rc;
+}
+}  // namespace
+}  // namespace psl
+
+extern "C" {
+
+int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cut, float rho_nominal,
+                         float soundspeed, int64_t max_pixels, float* host_out, int* dims3) {
+  PSL_CHECK_ARG(levels && nlevels >= 1 && dims3 && plane >= 0 && plane <= 2, "bad section-cut arguments");
+  for (int l = 0; l < nlevels; ++l) PSL_CHECK_ARG(levels[l] && levels[l]->info.dtPhys > 0.f, "bad level");
+  // the finest level whose plane fits (lines 380-399)
+  int target = nlevels, h = 0, v = 0;
+  while (target > 0) {
+    const psl_info& I = levels[target - 1]->info;
+    plane_dims(plane, I.cellCountX, I.cellCountY, I.cellCountZ, h, v);
+    if ((int64_t)h * v < max_pixels || target == 1) break;
+    --target;
+  }
+  dims3[0] = v;
+  dims3[1] = h;
+  dims3[2] = target;
+  if (!host_out) return PSL_OK;
+  CutParams c{plane, cut, h, v, 1.f, rho_nominal * soundspeed * soundspeed};
+  return paint_levels(levels, nlevels, c, 1 << (nlevels - target), host_out);
+}
+
+// getFlowReportXY/ZY/ZX (include/flowReportFunctions.h:2-263): the plane cut
+// clipped to bounds6 = (xmin, ymin, zmin, xmax, ymax, zmax) in physical units,
+// then over its non-solid pixels the mean ux, uy, uz, rho (lattice units) and
+// the area in mm^2 (pixels x res^2 of the level the window was painted at):
+// out5 = ux, uy, uz, rho, area.  Sums in FP64.
+int psl_diad_flow_report(psl_diad* const* levels, int nlevels, int plane, int cut, const float* bounds6,
+                         double* out5) {
+  PSL_CHECK_ARG(levels && nlevels >= 1 && bounds6 && out5 && plane >= 0 && plane <= 2, "bad flow-report arguments");
+  for (int l = 0; l < nlevels; ++l) PSL_CHECK_ARG(levels[l], "bad level");
+  const psl_info& F = levels[nlevels - 1]->info;
+  const int iMin = std::max(0, (int)std::floor((bounds6[0] - F.ox) / F.res));
+  const int iMax = std::min(F.cellCountX, (int)std::ceil((bounds6[3] - F.ox) / F.res));
+  const int jMin = std::max(0, (int)std::floor((bounds6[1] - F.oy) / F.res));
+  const int jMax = std::min(F.cellCountY, (int)std::ceil((bounds6[4] - F.oy) / F.res));
+  const int kMin = std::max(0, (int)std::floor((bounds6[2] - F.oz) / F.res));
+  const int kMax = std::min(F.cellCountZ, (int)std::ceil((bounds6[5] - F.oz) / F.res));
+  int h0f, h1f, v0f, v1f;
+  if (plane == 0) { h0f = iMin; h1f = iMax; v0f = jMin; v1f = jMax; }
+  else if (plane == 1) { h0f = kMin; h1f = kMax; v0f = jMin; v1f = jMax; }
+  else { h0f = kMin; h1f = kMax; v0f = iMin; v1f = iMax; }
+  int target = nlevels, scale = 1, h0 = 0, v0 = 0, w = 0, ht = 0;
+  while (target > 0) {  // lines 36-58
+    scale = 1 << (nlevels - target);
+    h0 = h0f / scale;
+    v0 = v0f / scale;
+    w = std::max(0, (h1f + scale - 1) / scale - h0);
+    ht = std::max(0, (v1f + scale - 1) / scale - v0);
+    if ((int64_t)w * ht < 20000000 || target == 1) break;
+    --target;
+  }
+  std::vector<float> px(6 * (size_t)w * ht);
+  CutParams c{plane, cut, w, ht, 1.f, 1.f};
+  c.h0 = h0;
+  c.v0 = v0;
+  c.raw = 1;
+  const int rc = paint_levels(levels, nlevels, c, scale, px.data());
+  if (rc != PSL_OK) return rc;
+  int64_t cells = 0;
+  double sum[4] = {0, 0, 0, 0};
+  for (size_t p = 0; p < (size_t)w * ht; ++p) {
+    const float* o = &px[6 * p];
+    if (o[4] != 0.f) continue;
+    ++cells;
+    sum[0] += o[1];
+    sum[1] += o[2];
+    sum[2] += o[3];
+    sum[3] += (double)o[0] - 1.0;
+  }
+  if (cells > 0) {
+    const double res = levels[target - 1]->info.res;
+    out5[0] = sum[0] / (double)cells;
+    out5[1] = sum[1] / (double)cells;
+    out5[2] = sum[2] / (double)cells;
+    out5[3] = sum[3] / (double)cells + 1.0;
+    out5[4] = (double)cells * res * res;
+  } else {
+    out5[0] = out5[1] = out5[2] = 0.0;
+    out5[3] = 1.0;
+    out5[4] = 0.0;
+  }
+  return PSL_OK;
 }
 
 // sim_data.bin (exportSectionCutPlot.h:64-86, 543-569): int header {plotNumber,

@@ -113,3 +113,43 @@ def test_section_cut_diad_levels(psl, plane, cut):
     assert np.array_equal(got[..., 4], want[..., 4].astype(np.float32))
     assert np.max(np.abs(got[..., 0] - want[..., 0])) < 1e-5 * RHO0 * CS * CS
     assert np.max(np.abs(got[..., 1:4] - want[..., 1:4])) < 1e-5 * 1.0
+
+
+def test_flow_report_diad_levels(psl):
+    """getFlowReportXY over DIAD levels: the plane clipped to physical bounds,
+    then means over the pixels outside walls, against the host painter."""
+    P = psl
+    lv, grids = cases.diad_sphere(P, "f32", n0=16, levels=3, span=1, radius=2.0)
+    for g in grids:
+        g.fill_equilibrium(ux=0.04)
+    P.updateGrid(grids, 3)
+    bounds = np.array([-6.0, -4.0, -100.0, 8.0, 5.0, 100.0], np.float32)
+    cut = 32
+    rep = P.getFlowReport(grids, "XY", cut, bounds)
+    fi, _ = lv.info(2)
+    f32 = np.float32
+    i0 = max(0, int(np.floor((bounds[0] - f32(fi.ox)) / f32(fi.res))))
+    i1 = min(fi.cellCountX, int(np.ceil((bounds[3] - f32(fi.ox)) / f32(fi.res))))
+    j0 = max(0, int(np.floor((bounds[1] - f32(fi.oy)) / f32(fi.res))))
+    j1 = min(fi.cellCountY, int(np.ceil((bounds[4] - f32(fi.oy)) / f32(fi.res))))
+    levels = []
+    for l, g in enumerate(grids):
+        li, _ = lv.info(l)
+        flags = cases.diad_sphere_flags(lv.array(l, "ijk"), lv.array(l, "bounceback"),
+                                        (li.cellCountX, li.cellCountY, li.cellCountZ), l)
+        # lattice units: a level info with res / 1000 / dtPhys = 1
+        raw = P.InfoStruct(res=1000.0, dtPhys=1.0)
+        levels.append((g, raw, lv.array(l, "ijk"), flags))
+    full = _paint(levels, "XY", cut, 1, 3, fi.cellCountX, fi.cellCountY)
+    win = full[j0:j1, i0:i1]
+    open_px = win[..., 4] == 0
+    rho = win[..., 0][open_px] / (RHO0 * CS * CS) + 1.0
+    # _paint stores the plane components (XY: ux, uy, uz) already
+    want = {"ux": win[..., 1][open_px].mean(), "uy": win[..., 2][open_px].mean(),
+            "uz": win[..., 3][open_px].mean(), "rho": (rho - 1.0).mean() + 1.0,
+            "areamm2": open_px.sum() * float(fi.res) ** 2}
+    print(f"[flow report] {rep}\n  host {want}", flush=True)
+    assert open_px.sum() > 0 and (~open_px).sum() > 0
+    for key in ("ux", "uy", "uz", "rho"):
+        assert abs(rep[key] - want[key]) < 1e-6, key
+    assert rep["areamm2"] == pytest.approx(want["areamm2"], rel=1e-12)
