@@ -379,6 +379,24 @@ int psl_diad_set_moving_bounceback(psl_diad* grid, const int* cells, int64_t n, 
  * rho (u - u after bounceback), populations through the current write maps. */
 int psl_diad_reduce_momentum_exchange(psl_diad* grid, double out[3]);
 
+/* ------------------------------------------------- scalar transport */
+/* ScalarTransportStruct (include/types.h:121) on a periodic-shift grid with FP32
+ * distributions: 27 scalar populations per cell streamed by the grid's
+ * shifter.  enable allocates them at zero (TArray.setValue(0)) with relaxation
+ * time tauT = 3 D + 1/2. */
+int psl_scalar_enable(psl_grid* grid, float tau_t);
+/* getScalarTransportMarkers / getGivenT: a prescribed value per cell (N
+ * floats, NaN where none), or NULL for none. */
+int psl_scalar_set_given(psl_grid* grid, const float* given);
+/* Scalar populations in logical order: host[q * N + cell]. */
+int psl_scalar_get(psl_grid* grid, float* host);
+int psl_scalar_set(psl_grid* grid, const float* host);
+/* applyLocalCellUpdate(GridStruct&, ScalarTransportStruct&) (include/scalarTransportFunctions.h:10-103)
+ * with applyScalarTransportCollision (lines 106-170): the fluid update of
+ * psl_apply_local_cell_update plus the scalar, in one launch.  Streaming is
+ * psl_apply_streaming. */
+int psl_scalar_apply_local_cell_update(psl_grid* grid);
+
 /* ----------------------------------------------------------- output */
 /* exportSectionCutPlotXY/ZY/ZX(GridStruct&) (include/plotter/exportSectionCutPlot.h:2-104):
  * plane 0 = XY at k = cut, 1 = ZY at i = cut, 2 = ZX at j = cut; rows x cols

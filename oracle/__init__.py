@@ -92,6 +92,8 @@ def _declare(L):
         getattr(L, f"oracle_eso_momentum_exchange_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P, P]
         getattr(L, f"oracle_eso_moving_bounceback_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
                                                                       ctypes.c_longlong, P]
+        getattr(L, f"oracle_scalar_update_{suf}").argtypes = [P, P, P, ctypes.POINTER(OracleInfo), P, P, P, R]
+        getattr(L, f"oracle_scalar_collide_{suf}").argtypes = [P, P, ctypes.c_longlong, R]
         getattr(L, f"oracle_eso_outlet_z_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
                                                              ctypes.c_longlong, P]
     L.oracle_diad_build.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, P, P, P, ctypes.c_int, P,
@@ -205,6 +207,14 @@ class Grid:
     def periodic(self, axis):
         getattr(lib(), f"oracle_periodic_{self.suf}")(self.f.ctypes.data, self.shift.ctypes.data,
                                                      ctypes.byref(self.info), axis)
+
+    def scalar_update(self, T, tau_t, given_t=None):
+        """applyLocalCellUpdate(Grid, ScalarTransport) on raw arrays (in place on
+        self.f and T, the scalar populations in the same shifted layout)."""
+        g = None if given_t is None else np.ascontiguousarray(given_t, self.f.dtype)
+        getattr(lib(), f"oracle_scalar_update_{self.suf}")(self.f.ctypes.data, T.ctypes.data,
+                                                          self.shift.ctypes.data, ctypes.byref(self.info),
+                                                          _p(self.flags), self.bcvals.ctypes.data, _p(g), tau_t)
 
     def local_update(self):
         getattr(lib(), f"oracle_local_update_{self.suf}")(self.f.ctypes.data, self.shift.ctypes.data,
@@ -321,6 +331,14 @@ def eso_to_fine(fc, nbc7, flip_c, ff, nbf7, flip_f, write, read):
     getattr(lib(), f"oracle_eso_to_fine_{_suf(fc)}")(fc.ctypes.data, fc.shape[1], nbc7.ctypes.data, int(flip_c),
                                                      ff.ctypes.data, ff.shape[1], nbf7.ctypes.data, int(flip_f),
                                                      w.ctypes.data, r.ctypes.data, w.size)
+
+
+def scalar_collide(f, T, tau_t):
+    """applyScalarTransportCollision per cell: f, T as (..., 27); returns the new T."""
+    f = np.ascontiguousarray(f)
+    T = np.array(T, dtype=f.dtype, copy=True, order="C")
+    getattr(lib(), f"oracle_scalar_collide_{_suf(f)}")(f.ctypes.data, T.ctypes.data, f.size // 27, tau_t)
+    return T
 
 
 def eso_outlet_z(f, nbr7, flipper, outlet, kmin):

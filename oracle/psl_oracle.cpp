@@ -570,10 +570,17 @@ inline void outer_normal(int i, int j, int k, const OracleInfo& I, int& nx, int&
   else if (k == I.nz - 1) nz = 1;
 }
 
-// applyLocalCellUpdate(GridStruct&): include/applyLocalCellUpdate.h:1-74
+template <typename R>
+void scalar_collide(const R (&f)[27], R (&T)[27], R tauT);
+
+// applyLocalCellUpdate(GridStruct&) (include/applyLocalCellUpdate.h:1-74) and,
+// with Ta, applyLocalCellUpdate(GridStruct&,
+// ScalarTransportStruct&) (include/scalarTransportFunctions.h:10-103): the
+// scalar populations stream with f, bounce back with f, take a given value
+// where givenT is not NaN, and collide after f (with the post-collision f).
 template <typename R>
 void local_update(R* fa, const int* shift, const OracleInfo& I, const uint8_t* flags, const R* bcvals,
-                  const R* cs_cell) {
+                  const R* cs_cell, R* Ta = nullptr, const R* givenT = nullptr, R tauT = 1) {
   const long long N = cell_count(I);
   const int nxy = I.nx * I.ny;
 #pragma omp parallel for schedule(static)
@@ -588,14 +595,16 @@ void local_update(R* fa, const int* shift, const OracleInfo& I, const uint8_t* f
     if ((i > I.i_sub_start + 1 && i < I.i_sub_end - 2) && (j > I.j_sub_start + 1 && j < I.j_sub_end - 2) &&
         (k > I.k_sub_start + 1 && k < I.k_sub_end - 2))
       continue;
-    R f[27];
+    R f[27], T[27];
     long long sl[27];
     for (int q = 0; q < 27; ++q) {
       sl[q] = shifted(cell, shift[q], N);
       f[q] = fa[(long long)q * N + sl[q]];
+      if (Ta) T[q] = Ta[(long long)q * N + sl[q]];
     }
     if (kind == K_BOUNCEBACK) {
       bounceback(f);
+      if (Ta) bounceback(T);
     } else {
       if (kind != K_FLUID) {
         int nx, ny, nz;
@@ -611,10 +620,48 @@ void local_update(R* fa, const int* shift, const OracleInfo& I, const uint8_t* f
         else if (kind == K_GIVEN_NONE) bc_restore_rhou(code, f, rho, ux, uy, uz);
         bc_mbbc(code, f, rho, ux, uy, uz);
       }
+      if (Ta && givenT && givenT[cell] == givenT[cell])  // applyScalarTransportGivenT, lines 172-175
+        for (int q = 0; q < 27; ++q) T[q] = givenT[cell];
       const R cs = cs_cell ? cs_cell[cell] : (R)I.cs;
       collide(f, (R)I.nu, cs);
+      if (Ta) scalar_collide(f, T, tauT);
     }
     for (int q = 0; q < 27; ++q) fa[(long long)q * N + sl[q]] = f[q];
+    if (Ta)
+      for (int q = 0; q < 27; ++q) Ta[(long long)q * N + sl[q]] = T[q];
+  }
+}
+
+// applyScalarTransportCollision: include/scalarTransportFunctions.h:106-170
+// (Zhang, Fan, Chen 2011, eqs 2.5-2.7).  The momentum-like sums run over the
+// populations with a non-zero velocity component in increasing q, as written.
+template <typename R>
+void scalar_collide(const R (&f)[27], R (&T)[27], R tauT) {
+  R rho, ux, uy, uz;
+  get_rho_u(f, rho, ux, uy, uz);
+  R Tsum = 0;
+  for (int q = 0; q < 27; ++q) Tsum += f[q] * T[q];
+  Tsum = Tsum / rho;
+  R d[27];
+  for (int q = 0; q < 27; ++q) d[q] = f[q] * (T[q] - Tsum);
+  R S[3];
+  const int* C[3] = {CX, CY, CZ};
+  for (int a = 0; a < 3; ++a) {
+    bool first = true;
+    R acc = 0;
+    for (int q = 0; q < 27; ++q) {
+      if (!C[a][q]) continue;
+      const R term = C[a][q] > 0 ? d[q] : -d[q];
+      acc = first ? term : acc + term;
+      first = false;
+    }
+    S[a] = acc;
+  }
+  const R rhodiv = (R)3 / rho;
+  const R bracket = (R)1 - (R)1 / tauT;
+  for (int q = 0; q < 27; ++q) {
+    const R Fi = rhodiv * (((R)CX[q] - ux) * S[0] + ((R)CY[q] - uy) * S[1] + ((R)CZ[q] - uz) * S[2]);
+    T[q] = Tsum + bracket * Fi;
   }
 }
 
@@ -1703,6 +1750,18 @@ ORACLE_PER_CELL(f64, double)
     const int* b[7];                                                                                           \
     for (int t = 0; t < 7; ++t) { a[t] = nbc7 + (size_t)t * nc; b[t] = nbf7 + (size_t)t * nf; }              \
     eso_to_fine(fc, nc, a, flip_c != 0, ff, nf, b, flip_f != 0, write, read, m);                              \
+  }                                                                                                            \
+  void oracle_scalar_update_##SUF(R* fa, R* Ta, const int* shift, const OracleInfo* I, const uint8_t* flags,   \
+                                  const R* bcvals, const R* givenT, R tauT) {                                  \
+    local_update(fa, shift, *I, flags, bcvals, (const R*)nullptr, Ta, givenT, tauT);                         \
+  }                                                                                                            \
+  void oracle_scalar_collide_##SUF(const R* f, R* T, long long n, R tauT) {                                   \
+    for (long long c = 0; c < n; ++c) {                                                                        \
+      R a[27], b[27];                                                                                          \
+      for (int q = 0; q < 27; ++q) { a[q] = f[27 * c + q]; b[q] = T[27 * c + q]; }                           \
+      scalar_collide(a, b, tauT);                                                                              \
+      for (int q = 0; q < 27; ++q) T[27 * c + q] = b[q];                                                       \
+    }                                                                                                          \
   }                                                                                                            \
   void oracle_eso_outlet_z_##SUF(R* fa, long long n, const int* nbr7, int flipper, const int* outlet,        \
                                  long long m, const int* kmin) {                                               \

@@ -216,6 +216,11 @@ _SIGS = [
     ("psl_diad_set_moving_bounceback", _I, [_P, _P, ctypes.c_int64, _P]),
     ("psl_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
     ("psl_diad_section_cut", _I, [_P, _I, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
+    ("psl_scalar_enable", _I, [_P, ctypes.c_float]),
+    ("psl_scalar_set_given", _I, [_P, _P]),
+    ("psl_scalar_get", _I, [_P, _P]),
+    ("psl_scalar_set", _I, [_P, _P]),
+    ("psl_scalar_apply_local_cell_update", _I, [_P]),
     ("psl_diad_flow_report", _I, [_P, _I, _I, _I, _P, _P]),
     ("psl_write_sim_data", _I, [ctypes.c_char_p, _I, _I, _I, _I, _P]),
     ("psl_write_history", _I, [ctypes.c_char_p, _P, _I]),
@@ -690,6 +695,34 @@ class DIADLevels:
 def buildDIADGrids(base: InfoStruct, levels: int, wall_refinement_span: int, nu_phys: float,
                    stls: Sequence["STLStruct"] = (), marker_rules=None) -> DIADLevels:
     return DIADLevels(base, levels, wall_refinement_span, nu_phys, stls, marker_rules)
+
+
+# ------------------------------------------------------- scalar transport
+class ScalarTransportStruct:
+    """include/types.h:121 on a periodic-shift GridStruct (FP32 storage)."""
+
+    def __init__(self, grid: "GridStruct", tauT: float):
+        self.grid = grid
+        self.tauT = float(tauT)
+        _check(lib().psl_scalar_enable(grid.handle, self.tauT))
+
+    def set_given(self, given):
+        a = None if given is None else np.ascontiguousarray(given, np.float32)
+        _check(lib().psl_scalar_set_given(self.grid.handle, _ptr(a)))
+
+    def get_T(self) -> np.ndarray:
+        out = np.empty((27, self.grid.n), np.float32)
+        _check(lib().psl_scalar_get(self.grid.handle, out.ctypes.data))
+        return out
+
+    def set_T(self, T):
+        a = np.ascontiguousarray(T, np.float32)
+        _check(lib().psl_scalar_set(self.grid.handle, a.ctypes.data))
+
+
+def applyScalarTransportLocalCellUpdate(grid: "GridStruct", scalar: ScalarTransportStruct):
+    """applyLocalCellUpdate(GridStruct&, ScalarTransportStruct&)."""
+    _check(lib().psl_scalar_apply_local_cell_update(grid.handle))
 
 
 # ------------------------------------------------------------ output

@@ -30,7 +30,8 @@ EXTRA = {
     "psl_diad_build.hip": ["-ffp-contract=off"],
 }
 SOURCES = ["psl_grid.hip", "psl_refine.hip", "psl_stl.hip", "psl_esotwist.hip", "psl_batch.hip", "psl_closure.hip",
-           "psl_diad_build.hip", "psl_output.hip"]
+           "psl_diad_build.hip", "psl_output.hip",
+           "psl_scalar.hip"]
 
 
 def hipcc() -> str:

@@ -746,6 +746,8 @@ int psl_grid_destroy(psl_grid* g) {
   if (g->partials) (void)hipFree(g->partials);
   if (g->result) (void)hipFree(g->result);
   if (g->bc_dev) (void)hipFree(g->bc_dev);
+  if (g->scalar_T) (void)hipFree(g->scalar_T);
+  if (g->scalar_given) (void)hipFree(g->scalar_given);
   delete g;
   return PSL_OK;
 }

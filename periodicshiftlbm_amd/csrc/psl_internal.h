@@ -123,6 +123,9 @@ struct psl_grid {
   int64_t census[3] = {0, 0, 0};  // bulk, boundary-list, skipped cells
   int tuning = PSL_TUNE_XCD_REMAP;  // bulk-kernel variant (psl_grid_set_tuning / autotune)
   const psl_mlp* closure = nullptr;  // learned closure (null: Smagorinsky)
+  float* scalar_T = nullptr;      // scalar-transport populations (27 * N FP32, same layout as f)
+  float* scalar_given = nullptr;  // prescribed scalar per cell (NaN: free), or null
+  float tau_t = 1.f;              // scalar relaxation time
 };
 
 // device image of the learned closure of a psl_mlp (psl_closure.hip)
