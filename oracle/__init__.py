@@ -92,8 +92,9 @@ def _declare(L):
         getattr(L, f"oracle_eso_momentum_exchange_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P, P]
         getattr(L, f"oracle_eso_moving_bounceback_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
                                                                       ctypes.c_longlong, P]
-        getattr(L, f"oracle_scalar_update_{suf}").argtypes = [P, P, P, ctypes.POINTER(OracleInfo), P, P, P, R]
-        getattr(L, f"oracle_scalar_collide_{suf}").argtypes = [P, P, ctypes.c_longlong, R]
+        RT = ctypes.c_float if suf == "f32" else ctypes.c_double
+        getattr(L, f"oracle_scalar_update_{suf}").argtypes = [P, P, P, ctypes.POINTER(OracleInfo), P, P, P, RT]
+        getattr(L, f"oracle_scalar_collide_{suf}").argtypes = [P, P, ctypes.c_longlong, RT]
         getattr(L, f"oracle_eso_outlet_z_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
                                                              ctypes.c_longlong, P]
     L.oracle_diad_build.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_float, P, P, P, ctypes.c_int, P,

@@ -66,3 +66,12 @@ def test_sinusoidal_scalar_decays_at_the_diffusivity():
     d = np.log(amps[0] / amps[1]) / (steps // 2) / (2 * np.pi / nx) ** 2
     print(f"diffusivity {d:.5f} vs (tau - 1/2) / 3 = {(tau - 0.5) / 3:.5f}")
     assert abs(d / ((tau - 0.5) / 3) - 1) < 0.01
+
+
+def test_fp32_instantiation_agrees_with_fp64():
+    rng = np.random.default_rng(3)
+    f = _feq(100, rng) + 1e-3 * rng.standard_normal((100, 27))
+    T = rng.random((100, 27))
+    a = O.scalar_collide(f, T, 0.7)
+    b = O.scalar_collide(f.astype(np.float32), T.astype(np.float32), 0.7)
+    assert np.max(np.abs(a - b)) < 1e-5
