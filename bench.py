@@ -254,6 +254,32 @@ def diad_leg(P, cases, levels: int, steps: int, storage: str = "f16"):
     return out
 
 
+def scalar_leg(P, cases, steps: int):
+    """Scalar transport on the sphere channel (256x128x128, FP32): f and the 27
+    scalar populations updated in one launch per step after the streaming; the
+    algorithmic bytes are 2 x 216 B per cell update."""
+    g, _, _, _ = build_sphere_grid(P, cases, "f32")
+    st = P.ScalarTransportStruct(g, 0.6)
+    given = np.full(g.n, np.nan, np.float32)
+    given[np.arange(g.n) % 256 == 0] = 1.0  # scalar prescribed at the inlet
+    st.set_given(given)
+    for _ in range(5):
+        P.applyStreaming(g)
+        P.applyScalarTransportLocalCellUpdate(g, st)
+    P.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P.applyStreaming(g)
+        P.applyScalarTransportLocalCellUpdate(g, st)
+    P.synchronize()
+    el = time.perf_counter() - t0
+    mlups = g.n * steps / el / 1e6
+    out = {"mlups": mlups, "ms_per_step": el * 1e3 / steps, "bytes_per_update": 432,
+           "achieved_GBs": mlups * 1e6 * 432 / 1e9, "roofline_frac": mlups * 1e6 * 432 / 1e9 / HBM_PEAK_GBS}
+    g.close()
+    return out
+
+
 def closure_leg(P, n: int, log):
     """Learned closure at the decaying-turbulence configuration: n^3 fine run
     (FP16 storage, Cs 0.1), filtered to (n/4)^3 every 200 steps, network trained
@@ -319,6 +345,8 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the 64-domain batch (0: skip)")
     ap.add_argument("--two-level-steps", type=int, default=200, help="coarse steps of the two-level sphere (0: skip)")
     ap.add_argument("--large-n", type=int, default=256, help="edge of the large periodic box (0: skip)")
+    ap.add_argument("--scalar-steps", type=int, default=100,
+                    help="steps of the scalar-transport sphere channel (0: skip)")
     ap.add_argument("--diad-steps", type=int, default=20,
                     help="coarse steps of the wall-refined DIAD sphere (0: skip)")
     ap.add_argument("--diad-levels", type=int, default=6, help="levels of the DIAD sphere")
@@ -445,6 +473,11 @@ def main():
         tl = two_level_leg(P, cases, args.two_level_steps, autotune=not args.no_autotune)
         log(f"  {tl['mlups']:.1f} MLUPS ({tl['ms_per_coarse_step']:.3f} ms per coarse step, variants {tl['tuning']})")
         out["variants"]["two_level"] = tl
+    if args.scalar_steps:
+        log(f"[bench] scalar transport, sphere channel, {args.scalar_steps} steps")
+        sl = scalar_leg(P, cases, args.scalar_steps)
+        log(f"  {sl['mlups']:.0f} MLUPS, {sl['achieved_GBs']:.0f} GB/s (f and scalar, 432 B per cell)")
+        out["variants"]["scalar_transport"] = sl
     if args.diad_steps:
         log(f"[bench] wall-refined DIAD sphere, {args.diad_levels} levels, {args.diad_steps} coarse steps")
         dl = {}
