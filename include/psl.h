@@ -376,7 +376,9 @@ int psl_diad_apply_nonreflective_outlet_z(psl_diad* grid);
 int psl_diad_set_moving_bounceback(psl_diad* grid, const int* cells, int64_t n, const float* u3);
 /* getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
  * all three components in FP64: sum over BOUNCEBACK-flagged cells of
- * rho (u - u after bounceback), populations through the current write maps. */
+ * rho (u - u after bounceback), populations through the current write maps.
+ * Moving-wall cells are left out, as the reference's drag leaves out its
+ * movingBounceback cells. */
 int psl_diad_reduce_momentum_exchange(psl_diad* grid, double out[3]);
 
 /* ------------------------------------------------- scalar transport */

@@ -306,12 +306,18 @@ __global__ __launch_bounds__(256) void k_eso_macro(const typename St::T* __restr
 // getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
 // all three components: per bounceback cell rho (u - u after bounceback), in
 // FP64 per-block partial sums.
+// With `list`, the cells list[0, m) instead (the moving-wall cells, whose
+// share is taken out of the sum: the reference does not count them as
+// bounceback cells).
 template <class St, bool FLIP>
 __global__ __launch_bounds__(256) void k_eso_momentum(const typename St::T* __restrict__ f, EsoParams p,
-                                                      double* __restrict__ partials) {
-  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+                                                      double* __restrict__ partials, const int* __restrict__ list,
+                                                      uint32_t m) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const bool in = list ? t < m : (t < p.n && p.flags && (p.flags[t] & 7) == KIND_BOUNCEBACK);
+  const uint32_t cell = list ? (in ? (uint32_t)list[t] : 0u) : t;
   double acc[3] = {0.0, 0.0, 0.0};
-  if (cell < p.n && p.flags && (p.flags[cell] & 7) == KIND_BOUNCEBACK) {
+  if (in) {
     float v[27];
     eso_get<St, FLIP>(f, p, cell, v);
     float r, a, b, c;
@@ -409,10 +415,12 @@ static int eso_comm(psl_diad* c, psl_diad* f) {
 }
 
 template <class St>
-static int eso_momentum(psl_diad* d, uint32_t nb) {
+static int eso_momentum(psl_diad* d, uint32_t nb, const int* list, uint32_t m, double* partials) {
   const EsoParams p = eso_params(d);
-  if (d->flipper) k_eso_momentum<St, true><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, d->partials);
-  else k_eso_momentum<St, false><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, d->partials);
+  if (d->flipper)
+    k_eso_momentum<St, true><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, partials, list, m);
+  else
+    k_eso_momentum<St, false><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, partials, list, m);
   PSL_LAUNCH_CHECK();
   return PSL_OK;
 }
@@ -716,22 +724,28 @@ int psl_diad_update_grid(psl_diad* const* levels, int nlevels, int nsteps) {
 int psl_diad_reduce_momentum_exchange(psl_diad* d, double out[3]) {
   PSL_CHECK_ARG(d && out, "null argument");
   const uint32_t nb = (uint32_t)((d->n + 255) / 256);
-  if (d->n_partials < 3 * (size_t)nb) {
+  const uint32_t nbm = (uint32_t)((d->n_moving + 255) / 256);
+  const size_t need = 3 * ((size_t)nb + nbm);
+  if (d->n_partials < need) {
     if (d->partials) PSL_HIP_TRY(hipFree(d->partials));
     d->partials = nullptr;
     d->n_partials = 0;
-    PSL_HIP_TRY(hipMalloc(&d->partials, 3 * (size_t)nb * sizeof(double)));
-    d->n_partials = 3 * (size_t)nb;
+    PSL_HIP_TRY(hipMalloc(&d->partials, need * sizeof(double)));
+    d->n_partials = need;
   }
-  const int rc = PSL_ESO_DISPATCH(eso_momentum, d, nb);
+  int rc = PSL_ESO_DISPATCH(eso_momentum, d, nb, nullptr, 0u, d->partials);
+  if (rc == PSL_OK && nbm)
+    rc = PSL_ESO_DISPATCH(eso_momentum, d, nbm, d->moving, (uint32_t)d->n_moving, d->partials + 3 * (size_t)nb);
   if (rc != PSL_OK) return rc;
-  std::vector<double> h(3 * (size_t)nb);
-  PSL_HIP_TRY(hipMemcpyAsync(h.data(), d->partials, h.size() * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  std::vector<double> h(need);
+  PSL_HIP_TRY(hipMemcpyAsync(h.data(), d->partials, need * sizeof(double), hipMemcpyDeviceToHost, d->stream));
   PSL_HIP_TRY(hipStreamSynchronize(d->stream));
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[3] = {0.0, 0.0, 0.0}, mov[3] = {0.0, 0.0, 0.0};
   for (uint32_t b = 0; b < nb; ++b)
     for (int a = 0; a < 3; ++a) acc[a] += h[3 * (size_t)b + a];
-  for (int a = 0; a < 3; ++a) out[a] = acc[a];
+  for (uint32_t b = 0; b < nbm; ++b)
+    for (int a = 0; a < 3; ++a) mov[a] += h[3 * ((size_t)nb + b) + a];
+  for (int a = 0; a < 3; ++a) out[a] = acc[a] - mov[a];
   return PSL_OK;
 }
 

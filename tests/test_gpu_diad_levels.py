@@ -181,7 +181,10 @@ def test_moving_bounceback_matches_oracle(psl, storage):
     rng = np.random.default_rng(11)
     flags = cases.diad_sphere_flags(d["ijk"], d["bounceback"], (d["nx"], d["ny"], d["nz"]), 0)
     interior = np.flatnonzero(flags == cases.FLUID)
-    cells = np.sort(rng.choice(interior, 150, replace=False)).astype(np.int32)
+    picked = rng.choice(interior, 250, replace=False)
+    cells = np.sort(picked[:150]).astype(np.int32)
+    flags[picked[150:]] = cases.BOUNCEBACK  # static walls beside the moving ones
+    walls_only = flags.copy()  # the reference's bounceback marker leaves moving walls out
     flags[cells] = cases.BOUNCEBACK
     u = (0.02 * rng.standard_normal((cells.size, 3))).astype(np.float32)
     g.set_flags(flags)
@@ -209,3 +212,10 @@ def test_moving_bounceback_matches_oracle(psl, storage):
             err = float(np.max(np.abs(got.astype(np.float64) - ref)))
             print(f"  step {step + 1}: max abs err {err:.3e} (f16 storage vs FP32 oracle)", flush=True)
             assert err < 2e-3
+    # the drag leaves the moving-wall cells out
+    fx = g.momentum_exchange()
+    fo = O.eso_momentum_exchange(raw, nbr7, flip, walls_only)
+    print(f"  momentum exchange without moving walls: GPU {fx} oracle {fo}", flush=True)
+    assert np.abs(fo).max() > 0
+    tol = (1e-4 if storage == "f32" else 2e-2) * np.abs(fo).max()
+    assert np.all(np.abs(fx - fo) <= tol)
