@@ -374,12 +374,30 @@ int psl_diad_apply_nonreflective_outlet_z(psl_diad* grid);
  * local update (and psl_diad_update_grid) then applies the moving-wall
  * bounceback to them.  n = 0 clears the list. */
 int psl_diad_set_moving_bounceback(psl_diad* grid, const int* cells, int64_t n, const float* u3);
+/* Forced-velocity cells (Marker.forcedVelocity with getForcing;
+ * M35Full/applyLocalCellUpdateCustomized.h:80-86,114-121, M35Full/m35Main.cu:100-118, forcing
+ * collision M35Full/applyCollisionCustomized.h:5-17): n distinct cells and their
+ * target velocities ut3[3 n] (lattice units).  The cells must be flagged
+ * PSL_BOUNCEBACK, so that the bulk update skips them; each local update (and
+ * psl_diad_update_grid) then updates them with the force
+ * g = rho (u_target - u) and the forcing collision.  A cell may not be both a
+ * moving-wall and a forced cell.  n = 0 clears the list. */
+int psl_diad_set_forced_velocity(psl_diad* grid, const int* cells, int64_t n, const float* ut3);
 /* getSphereDrag(DIADGridStruct&) (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:132-198),
  * all three components in FP64: sum over BOUNCEBACK-flagged cells of
  * rho (u - u after bounceback), populations through the current write maps.
- * Moving-wall cells are left out, as the reference's drag leaves out its
- * movingBounceback cells. */
+ * Moving-wall and forced cells are left out, as the reference's drag counts
+ * neither its movingBounceback nor its forcedVelocity cells. */
 int psl_diad_reduce_momentum_exchange(psl_diad* grid, double out[3]);
+/* getTorque (M35Full/m35Main.cu:176-260) in FP64 over the forced cells, six
+ * components: out[0..2] = sum of the forces g = rho (u_target - u), from the
+ * populations the next update reads (as getTorque reads them), in newtons
+ * (convertToPhysicalForce, include/cellFunctions.h:221-226, with the nominal
+ * density rho_nominal_phys in kg/m3), out[3..5] = sum of r x g about the origin
+ * in newton metres, r = (i res + ox, j res + oy, k res + oz) in millimetres.
+ * out[5] is the reference's torque; its k < 5 and interface-cell exclusions
+ * are the caller's choice of the forced-cell list. */
+int psl_diad_reduce_forcing(psl_diad* grid, float rho_nominal_phys, double out[6]);
 
 /* ------------------------------------------------- scalar transport */
 /* ScalarTransportStruct (include/types.h:121) on a periodic-shift grid with FP32
@@ -439,6 +457,14 @@ int psl_stl_destroy(psl_stl* stl);
 int psl_stl_info(const psl_stl* stl, int* triangle_count, float* bounds6);
 int psl_stl_set_origin(psl_stl* stl, float ox, float oy, float oz);
 int psl_stl_get_triangles(const psl_stl* stl, float* tri9);
+/* rotateSTLAlongX / Y / Z (include/STLFunctions.h:698-809): rotates every
+ * vertex by `radians` about axis 0 (x), 1 (y) or 2 (z) through the STL's own
+ * origin (right-handed), in FP32. */
+int psl_stl_rotate(psl_stl* stl, int axis, float radians);
+/* ApplyMarkersInsideSTL (include/STLFunctions.h:482-696) on a DIAD grid's cells:
+ * out[c] = 1 for a cell inside the STL, 0 outside (n host bytes).  The
+ * M35 drivers re-mark their forced cells this way after each rotation. */
+int psl_diad_markers_inside_stl(const psl_diad* grid, const psl_stl* stl, uint8_t* out);
 /* checkSTLEdges (include/STLFunctions.h:92-182): triangles whose edges are not shared
  * exactly twice; per-edge counts (3 per triangle) when counts != NULL. */
 int psl_stl_check_edges(const psl_stl* stl, int* bad_triangles, int* counts);

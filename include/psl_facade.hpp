@@ -282,6 +282,16 @@ inline void momentumExchange(DIADGridStruct& g, double (&out)[3]) {
 inline void setMovingBounceback(DIADGridStruct& g, const std::vector<int>& cells, const std::vector<float>& u3) {
   check(psl_diad_set_moving_bounceback(g.handle(), cells.data(), (int64_t)cells.size(), u3.data()));
 }
+// forcedVelocity cells with their target velocities (cells flagged PSL_BOUNCEBACK)
+inline void setForcedVelocity(DIADGridStruct& g, const std::vector<int>& cells, const std::vector<float>& ut3) {
+  check(psl_diad_set_forced_velocity(g.handle(), cells.data(), (int64_t)cells.size(), ut3.data()));
+}
+// getTorque (M35Full/m35Main.cu:176-260): torque about z in N m of the forced cells
+inline double getTorque(DIADGridStruct& g, float rhoNominalPhys) {
+  double out[6];
+  check(psl_diad_reduce_forcing(g.handle(), rhoNominalPhys, out));
+  return out[5];
+}
 
 // ------------------------------------------------------------------ output
 // exportSectionCutPlotXY/ZY/ZX (include/plotter/exportSectionCutPlot.h): the cut

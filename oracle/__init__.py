@@ -93,6 +93,11 @@ def _declare(L):
         getattr(L, f"oracle_eso_moving_bounceback_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
                                                                       ctypes.c_longlong, P]
         RT = ctypes.c_float if suf == "f32" else ctypes.c_double
+        getattr(L, f"oracle_eso_forced_velocity_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
+                                                                     ctypes.c_longlong, P, RT, RT]
+        getattr(L, f"oracle_eso_forcing_sum_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
+                                                                 ctypes.c_longlong, P, P] + [ctypes.c_float] * 6 + [P]
+        getattr(L, f"oracle_collide_forced_{suf}").argtypes = [P, RT, RT, P]
         getattr(L, f"oracle_scalar_update_{suf}").argtypes = [P, P, P, ctypes.POINTER(OracleInfo), P, P, P, RT]
         getattr(L, f"oracle_scalar_collide_{suf}").argtypes = [P, P, ctypes.c_longlong, RT]
         getattr(L, f"oracle_eso_outlet_z_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
@@ -356,6 +361,38 @@ def eso_moving_bounceback(f, nbr7, flipper, cells, u3):
     u = np.ascontiguousarray(u3, np.float32).reshape(-1)
     getattr(lib(), f"oracle_eso_moving_bounceback_{_suf(f)}")(f.ctypes.data, f.shape[1], nbr7.ctypes.data,
                                                               int(flipper), c.ctypes.data, c.size, u.ctypes.data)
+
+
+def collide_forced(f, nu, cs, g):
+    """The forcing collision of one cell (27 values) with force g (3 values)."""
+    f = np.array(f, copy=True, order="C")
+    g = np.ascontiguousarray(g, f.dtype)
+    getattr(lib(), f"oracle_collide_forced_{_suf(f)}")(f.ctypes.data, nu, cs, g.ctypes.data)
+    return f
+
+
+def eso_forced_velocity(f, nbr7, flipper, cells, ut3, nu, cs):
+    """Forced-velocity cells (read, force towards the targets ut3, forcing
+    collision, write) on a raw esotwist array, in place."""
+    c = np.ascontiguousarray(cells, np.int32)
+    u = np.ascontiguousarray(ut3, np.float32).reshape(-1)
+    getattr(lib(), f"oracle_eso_forced_velocity_{_suf(f)}")(f.ctypes.data, f.shape[1], nbr7.ctypes.data,
+                                                            int(flipper), c.ctypes.data, c.size, u.ctypes.data,
+                                                            nu, cs)
+
+
+def eso_forcing_sum(f, nbr7, flipper, cells, ut3, ijk, info, rho_phys):
+    """(Fx, Fy, Fz) in N and the torque about the origin (Tx, Ty, Tz) in N m
+    of the forced-velocity cells.  info: dict(res, ox, oy, oz, dt)."""
+    c = np.ascontiguousarray(cells, np.int32)
+    u = np.ascontiguousarray(ut3, np.float32).reshape(-1)
+    ijk = np.ascontiguousarray(ijk, np.int32)
+    out = np.zeros(6, np.float64)
+    getattr(lib(), f"oracle_eso_forcing_sum_{_suf(f)}")(f.ctypes.data, f.shape[1], nbr7.ctypes.data, int(flipper),
+                                                        c.ctypes.data, c.size, u.ctypes.data, ijk.ctypes.data,
+                                                        info["res"], info["ox"], info["oy"], info["oz"],
+                                                        info["dt"], rho_phys, out.ctypes.data)
+    return out
 
 
 def eso_momentum_exchange(f, nbr7, flipper, flags):

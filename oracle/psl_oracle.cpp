@@ -303,12 +303,22 @@ void central_to_f(R rho, R ux, R uy, R uz, R ks_000, R ks_100, R ks_010, R ks_00
   f[26] = (ks_cc0 * (uz * uz + uz) + ks_cc1 * (R(2) * uz + R(1)) + ks_cc2) * h;
 }
 
-// applyCollision: include/applyCollision.h:6-295 (cumulant "AllOne", no forcing)
+// applyCollision: include/applyCollision.h:6-295 (cumulant "AllOne"); with a
+// force g (three values, or null for none) the forcing version of
+// M35Full/applyCollisionCustomized.h:5-17, which moves the velocity by half the
+// force before the central moments.  The first-order central moments are then
+// -g/2 and the backward step restores them as +g/2, so the cell's momentum
+// gains g.
 template <typename R>
-void collide(R (&f)[27], R nu, R cs) {
+void collide_g(R (&f)[27], R nu, R cs, const R* g) {
   typedef typename Wide<R>::type W;
   R rho, ux, uy, uz;
   get_rho_u(f, rho, ux, uy, uz);
+  if (g) {
+    ux = ((ux * rho) + g[0] / R(2)) / rho;
+    uy = ((uy * rho) + g[1] / R(2)) / rho;
+    uz = ((uz * rho) + g[2] / R(2)) / rho;
+  }
   const Central<R> k = central_moments(f, ux, uy, uz);
   const R C_110 = k.k110, C_101 = k.k101, C_011 = k.k011;
   const R C_200 = k.k200, C_020 = k.k020, C_002 = k.k002;
@@ -338,6 +348,10 @@ void collide(R (&f)[27], R nu, R cs) {
   const R Cs_002 = (R)(W(R(1)) / W(3.0) * W(Eq61 - R(2) * Eq64 + Eq65));
 
   central_to_f(rho, ux, uy, uz, k.k000, -k.k100, -k.k010, -k.k001, Cs_200, Cs_020, Cs_002, Cs_110, Cs_101, Cs_011, f);
+}
+template <typename R>
+void collide(R (&f)[27], R nu, R cs) {
+  collide_g(f, nu, cs, (const R*)nullptr);
 }
 
 // rescaleF: include/gridRefinementFunctions.h:6-281 (paper eqs 3.17-3.20)
@@ -1057,6 +1071,68 @@ void eso_momentum_exchange(const R* fa, long long n, const int* const* nbr, bool
     acc[2] += (double)(r * (d - d2));
   }
   for (int a = 0; a < 3; ++a) out3[a] = acc[a];
+}
+
+// Forced-velocity cells: M35Full/applyLocalCellUpdateCustomized.h:80-86,114-121
+// with getForcing (M35Full/m35Main.cu:100-118): on each listed cell, read
+// through the read maps, force g = rho (u_target - u) from the cell's own
+// rho and u, the forcing collision, written through the write maps.  No
+// box-face treatment.  ut3: 3 target velocities per listed cell.
+template <typename R>
+void eso_forced_velocity(R* fa, long long n, const int* const* nbr, bool flipper, const int* cells, long long m,
+                         const float* ut3, R nu, R cs) {
+  for (long long t = 0; t < m; ++t) {
+    const long long cell = cells[t];
+    long long slot[8];
+    slot[0] = cell;
+    for (int a = 0; a < 7; ++a) slot[a + 1] = nbr[a][cell];
+    R f[27];
+    for (int q = 0; q < 27; ++q) f[q] = fa[(long long)(flipper ? q : opp(q)) * n + slot[ESO_READ[q]]];
+    R rho, ux, uy, uz;
+    get_rho_u(f, rho, ux, uy, uz);
+    const R g[3] = {rho * ((R)ut3[3 * t] - ux), rho * ((R)ut3[3 * t + 1] - uy), rho * ((R)ut3[3 * t + 2] - uz)};
+    collide_g(f, nu, cs, g);
+    for (int q = 0; q < 27; ++q) fa[(long long)(flipper ? opp(q) : q) * n + slot[ESO_WRITE[q]]] = f[q];
+  }
+}
+
+// getTorque (M35Full/m35Main.cu:176-260) over the listed cells, all six
+// components: the force g = rho (u_target - u) of each cell from the
+// populations the next update will read (read maps of the next parity, as
+// getTorque reads them, m35Main.cu:240-245), in newtons through convertToPhysicalForce
+// (include/cellFunctions.h:221-226), its sum, and the sum of r x g about the
+// origin with r = (i res + ox, j res + oy, k res + oz) in millimetres, divided
+// by 1000 (newton metres).  out6 = (Fx, Fy, Fz, Tx, Ty, Tz); Tz is the
+// reference's torque.
+template <typename R>
+void eso_forcing_sum(const R* fa, long long n, const int* const* nbr, bool flipper, const int* cells, long long m,
+                     const float* ut3, const int* ijk3, float res, float ox, float oy, float oz, float dt,
+                     float rho_phys, double* out6) {
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const float h = res / 1000.f;
+  const float scale = rho_phys * h * h * h * h / (dt * dt);
+  for (long long t = 0; t < m; ++t) {
+    const long long cell = cells[t];
+    long long slot[8];
+    slot[0] = cell;
+    for (int a = 0; a < 7; ++a) slot[a + 1] = nbr[a][cell];
+    R f[27];
+    for (int q = 0; q < 27; ++q) f[q] = fa[(long long)(flipper ? opp(q) : q) * n + slot[ESO_READ[q]]];
+    R rho, ux, uy, uz;
+    get_rho_u(f, rho, ux, uy, uz);
+    const R gx = rho * ((R)ut3[3 * t] - ux) * (R)scale;
+    const R gy = rho * ((R)ut3[3 * t + 1] - uy) * (R)scale;
+    const R gz = rho * ((R)ut3[3 * t + 2] - uz) * (R)scale;
+    const R x = (R)(ijk3[cell] * res + ox), y = (R)(ijk3[n + cell] * res + oy), z = (R)(ijk3[2 * n + cell] * res + oz);
+    acc[0] += (double)gx;
+    acc[1] += (double)gy;
+    acc[2] += (double)gz;
+    acc[3] += (double)(y * gz - z * gy);
+    acc[4] += (double)(z * gx - x * gz);
+    acc[5] += (double)(-gx * y + gy * x);
+  }
+  for (int a = 0; a < 3; ++a) out6[a] = acc[a];
+  for (int a = 3; a < 6; ++a) out6[a] = acc[a] / 1000.0;
 }
 
 // ------------------------------------------------------------------- STL voxelizer
@@ -1780,6 +1856,25 @@ ORACLE_PER_CELL(f64, double)
     const int* a[7];                                                                                           \
     for (int t = 0; t < 7; ++t) a[t] = nbr7 + (size_t)t * n;                                                   \
     eso_momentum_exchange(fa, n, a, flipper != 0, flags, out3);                                                \
+  }                                                                                                            \
+  void oracle_eso_forced_velocity_##SUF(R* fa, long long n, const int* nbr7, int flipper, const int* cells,     \
+                                        long long m, const float* ut3, R nu, R cs) {                           \
+    const int* a[7];                                                                                           \
+    for (int t = 0; t < 7; ++t) a[t] = nbr7 + (size_t)t * n;                                                   \
+    eso_forced_velocity(fa, n, a, flipper != 0, cells, m, ut3, nu, cs);                                        \
+  }                                                                                                            \
+  void oracle_eso_forcing_sum_##SUF(const R* fa, long long n, const int* nbr7, int flipper, const int* cells,   \
+                                    long long m, const float* ut3, const int* ijk3, float res, float ox,       \
+                                    float oy, float oz, float dt, float rho_phys, double* out6) {              \
+    const int* a[7];                                                                                           \
+    for (int t = 0; t < 7; ++t) a[t] = nbr7 + (size_t)t * n;                                                   \
+    eso_forcing_sum(fa, n, a, flipper != 0, cells, m, ut3, ijk3, res, ox, oy, oz, dt, rho_phys, out6);         \
+  }                                                                                                            \
+  void oracle_collide_forced_##SUF(R* f27, R nu, R cs, const R* g3) {                                          \
+    R f[27];                                                                                                   \
+    for (int q = 0; q < 27; ++q) f[q] = f27[q];                                                                \
+    collide_g(f, nu, cs, g3);                                                                                  \
+    for (int q = 0; q < 27; ++q) f27[q] = f[q];                                                                \
   }
 
 ORACLE_ESO_LEVELS(f32, float)

@@ -146,6 +146,8 @@ _SIGS = [
     ("psl_stl_info", _I, [_P, _P, _P]),
     ("psl_stl_set_origin", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     ("psl_stl_get_triangles", _I, [_P, _P]),
+    ("psl_stl_rotate", _I, [_P, _I, ctypes.c_float]),
+    ("psl_diad_markers_inside_stl", _I, [_P, _P, _P]),
     ("psl_stl_check_edges", _I, [_P, _P, _P]),
     ("psl_mask_create", _I, [ctypes.c_int64, _I, _P]),
     ("psl_mask_destroy", _I, [_P]),
@@ -214,6 +216,8 @@ _SIGS = [
     ("psl_diad_set_outlet", _I, [_P, _P, ctypes.c_int64, _P]),
     ("psl_diad_apply_nonreflective_outlet_z", _I, [_P]),
     ("psl_diad_set_moving_bounceback", _I, [_P, _P, ctypes.c_int64, _P]),
+    ("psl_diad_set_forced_velocity", _I, [_P, _P, ctypes.c_int64, _P]),
+    ("psl_diad_reduce_forcing", _I, [_P, ctypes.c_float, _P]),
     ("psl_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
     ("psl_diad_section_cut", _I, [_P, _I, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
     ("psl_scalar_enable", _I, [_P, ctypes.c_float]),
@@ -555,6 +559,26 @@ class DIADGridStruct:
         c = np.ascontiguousarray(cells, np.int32)
         u = np.ascontiguousarray(u3, np.float32).reshape(-1)
         _check(lib().psl_diad_set_moving_bounceback(self._h, c.ctypes.data, c.size, u.ctypes.data))
+
+    def set_forced_velocity(self, cells, ut3):
+        """Forced-velocity cells (flagged BOUNCEBACK) and their target
+        velocities (n x 3, lattice units)."""
+        c = np.ascontiguousarray(cells, np.int32)
+        u = np.ascontiguousarray(ut3, np.float32).reshape(-1)
+        _check(lib().psl_diad_set_forced_velocity(self._h, c.ctypes.data, c.size, u.ctypes.data))
+
+    def forcing(self, rho_nominal_phys: float) -> np.ndarray:
+        """Force (N) and torque about the origin (N m) of the forced cells:
+        (Fx, Fy, Fz, Tx, Ty, Tz) in FP64; Tz is getTorque's value."""
+        out = np.zeros(6, np.float64)
+        _check(lib().psl_diad_reduce_forcing(self._h, rho_nominal_phys, out.ctypes.data))
+        return out
+
+    def markers_inside_stl(self, stl: "STLStruct") -> np.ndarray:
+        """ApplyMarkersInsideSTL on this grid's cells: 1 inside, 0 outside."""
+        out = np.zeros(self.n, np.uint8)
+        _check(lib().psl_diad_markers_inside_stl(self._h, stl.handle, out.ctypes.data))
+        return out
 
     def set_outlet(self, outlet_cells, kmin):
         """outletCellArray and kMinNbrArray (n entries)."""
@@ -968,6 +992,24 @@ def applyNonReflectiveOutletZ(grid: DIADGridStruct):
     _check(lib().psl_diad_apply_nonreflective_outlet_z(grid.handle))
 
 
+def rotateSTLAlongX(stl: "STLStruct", radians: float):
+    stl.rotate(0, radians)
+
+
+def rotateSTLAlongY(stl: "STLStruct", radians: float):
+    stl.rotate(1, radians)
+
+
+def rotateSTLAlongZ(stl: "STLStruct", radians: float):
+    stl.rotate(2, radians)
+
+
+def getTorque(grid: DIADGridStruct, rho_nominal_phys: float) -> float:
+    """getTorque (M35Full/m35Main.cu:176-260): torque about the z axis (N m) of
+    the grid's forced-velocity cells."""
+    return float(grid.forcing(rho_nominal_phys)[5])
+
+
 def applyBCPeriodicX(grid: GridStruct):
     _check(lib().psl_apply_bc_periodic_x(grid.handle))
 
@@ -1038,6 +1080,10 @@ class STLStruct:
         out = np.zeros((self.triangleCount, 9), np.float32)
         _check(lib().psl_stl_get_triangles(self._h, out.ctypes.data))
         return out
+
+    def rotate(self, axis: int, radians: float):
+        """rotateSTLAlongX/Y/Z (axis 0, 1, 2) about the STL's own origin."""
+        _check(lib().psl_stl_rotate(self._h, int(axis), radians))
 
     @staticmethod
     def from_triangles(tri) -> "STLStruct":

@@ -286,6 +286,89 @@ __global__ __launch_bounds__(256) void k_eso_moving_bb(typename St::T* __restric
   for (int q = 0; q < 27; ++q) f[(size_t)(FLIP ? opp(q) : q) * p.n + slot[ESO_WRITE[q]]] = St::store(v[q]);
 }
 
+// Forced-velocity cells (M35Full/applyLocalCellUpdateCustomized.h:80-86,114-121,
+// getForcing in M35Full/m35Main.cu:100-118) on the listed cells, which the
+// local update skips as bounceback cells: populations read through the read
+// maps, force g = rho (u_target - u) from the cell's own rho and u, the
+// forcing collision, written through the write maps.  No box-face treatment.
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_forced(typename St::T* __restrict__ f, EsoParams p,
+                                                    const int* __restrict__ cells, const float* __restrict__ ut3,
+                                                    uint32_t m) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= m) return;
+  const uint32_t cell = (uint32_t)cells[t];
+  uint32_t slot[8];
+  eso_slots<St>(p, cell, slot);
+  float v[27];
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    float x = St::load(f[(size_t)(FLIP ? q : opp(q)) * p.n + slot[ESO_READ[q]]]);
+    if (St::W_OFF) x += weight(q);
+    v[q] = x;
+  }
+  float r, a, b, c;
+  rho_u(v, r, a, b, c);
+  const float gx = r * (ut3[3 * (size_t)t] - a), gy = r * (ut3[3 * (size_t)t + 1] - b),
+              gz = r * (ut3[3 * (size_t)t + 2] - c);
+  const Relax rx{p.omega, p.tau, p.cs};
+  collide_forced(v, rx, p.cs, gx, gy, gz);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    float x = v[q];
+    if (St::W_OFF) x -= weight(q);
+    f[(size_t)(FLIP ? opp(q) : q) * p.n + slot[ESO_WRITE[q]]] = St::store(x);
+  }
+}
+
+// getTorque (M35Full/m35Main.cu:176-260) over the listed cells, all six
+// components in FP64 per-block partial sums: per cell the force
+// g = rho (u_target - u) from the populations the next update will read
+// (read maps of the parity after FLIP, as getTorque reads them), times
+// `scale` (convertToPhysicalForce, include/cellFunctions.h:221-226: newtons),
+// and r x g with r = (i res + ox, j res + oy, k res + oz) in millimetres.
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_forcing_sum(const typename St::T* __restrict__ f, EsoParams p,
+                                                         double* __restrict__ partials, const int* __restrict__ cells,
+                                                         const float* __restrict__ ut3, uint32_t m, float scale,
+                                                         float res, float ox, float oy, float oz) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (t < m) {
+    const uint32_t cell = (uint32_t)cells[t];
+    uint32_t slot[8];
+    eso_slots<St>(p, cell, slot);
+    float v[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      float x = St::load(f[(size_t)(FLIP ? opp(q) : q) * p.n + slot[ESO_READ[q]]]);
+      if (St::W_OFF) x += weight(q);
+      v[q] = x;
+    }
+    float r, a, b, c;
+    rho_u(v, r, a, b, c);
+    const float gx = r * (ut3[3 * (size_t)t] - a) * scale, gy = r * (ut3[3 * (size_t)t + 1] - b) * scale,
+                gz = r * (ut3[3 * (size_t)t + 2] - c) * scale;
+    const float x = (float)p.ijk[cell] * res + ox, y = (float)p.ijk[(size_t)p.n + cell] * res + oy,
+                z = (float)p.ijk[2 * (size_t)p.n + cell] * res + oz;
+    acc[0] = (double)gx;
+    acc[1] = (double)gy;
+    acc[2] = (double)gz;
+    acc[3] = (double)(y * gz - z * gy);
+    acc[4] = (double)(z * gx - x * gz);
+    acc[5] = (double)(-gx * y + gy * x);
+  }
+  __shared__ double sh[6][256];
+  for (int a = 0; a < 6; ++a) sh[a][threadIdx.x] = acc[a];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < (unsigned)s)
+      for (int a = 0; a < 6; ++a) sh[a][threadIdx.x] += sh[a][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) partials[6 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 // rho, ux, uy, uz of every cell (getRhoUxUyUz, include/cellFunctions.h:135-155)
 // from the populations the last update wrote: out[a * n + cell]
 template <class St, bool FLIP>
@@ -452,6 +535,35 @@ static int eso_moving(psl_diad* d) {
 }
 
 template <class St>
+static int eso_forced(psl_diad* d) {
+  if (!d->n_forced) return PSL_OK;
+  const EsoParams p = eso_params(d);
+  const unsigned nb = (unsigned)((d->n_forced + 255) / 256);
+  typedef typename St::T T;
+  if (d->flipper)
+    k_eso_forced<St, true><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->forced, d->forced_u, (uint32_t)d->n_forced);
+  else
+    k_eso_forced<St, false><<<nb, 256, 0, d->stream>>>((T*)d->f, p, d->forced, d->forced_u, (uint32_t)d->n_forced);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+template <class St>
+static int eso_forcing_sum(psl_diad* d, uint32_t nb, double* partials, float scale) {
+  const EsoParams p = eso_params(d);
+  const psl_info& I = d->info;
+  typedef typename St::T T;
+  if (d->flipper)
+    k_eso_forcing_sum<St, true><<<nb, 256, 0, d->stream>>>((const T*)d->f, p, partials, d->forced, d->forced_u,
+                                                           (uint32_t)d->n_forced, scale, I.res, I.ox, I.oy, I.oz);
+  else
+    k_eso_forcing_sum<St, false><<<nb, 256, 0, d->stream>>>((const T*)d->f, p, partials, d->forced, d->forced_u,
+                                                            (uint32_t)d->n_forced, scale, I.res, I.ox, I.oy, I.oz);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+template <class St>
 static int eso_macro(psl_diad* d, float* out) {
   const EsoParams p = eso_params(d);
   const unsigned nb = (unsigned)((d->n + 255) / 256);
@@ -503,9 +615,10 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->nbr) (void)hipFree(d->nbr);
   if (d->flags) (void)hipFree(d->flags);
   if (d->bc_dev) (void)hipFree(d->bc_dev);
-  for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read, d->outlet, d->kmin, d->moving})
+  for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read, d->outlet, d->kmin, d->moving, d->forced})
     if (a) (void)hipFree(a);
   if (d->moving_u) (void)hipFree(d->moving_u);
+  if (d->forced_u) (void)hipFree(d->forced_u);
   if (d->partials) (void)hipFree(d->partials);
   delete d;
   return PSL_OK;
@@ -567,9 +680,10 @@ int psl_diad_apply_streaming(psl_diad* d) {
 
 int psl_diad_apply_local_cell_update(psl_diad* d) {
   PSL_CHECK_ARG(d, "null grid");
-  const int rc = PSL_ESO_DISPATCH(eso_update, d);
+  int rc = PSL_ESO_DISPATCH(eso_update, d);
+  if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_moving, d);
   if (rc != PSL_OK) return rc;
-  return PSL_ESO_DISPATCH(eso_moving, d);
+  return PSL_ESO_DISPATCH(eso_forced, d);
 }
 
 int psl_diad_get_flipper(const psl_diad* d, int* out) {
@@ -654,31 +768,52 @@ int psl_diad_set_outlet(psl_diad* d, const int* outlet_cells, int64_t n_outlet, 
   return rc;
 }
 
-int psl_diad_set_moving_bounceback(psl_diad* d, const int* cells, int64_t n, const float* u3) {
-  PSL_CHECK_ARG(d && n >= 0 && (n == 0 || (cells && u3)), "bad moving-bounceback arguments");
-  std::vector<uint8_t> seen;
-  for (int64_t t = 0; t < n; ++t) PSL_CHECK_ARG(cells[t] >= 0 && cells[t] < d->n, "moving-bounceback cell out of range");
+// Upload of a cell list with three floats per cell (moving walls, forced
+// cells).  Cells must be in range, listed once, and not in `other`.
+static int set_cell_list(psl_diad* d, const int* cells, int64_t n, const float* u3, const std::vector<int>& other,
+                         const char* what, int** dev, float** dev_u, int64_t* dev_n, std::vector<int>* host) {
+  PSL_CHECK_ARG(n >= 0 && (n == 0 || (cells && u3)), std::string("bad ") + what + " arguments");
+  for (int64_t t = 0; t < n; ++t) PSL_CHECK_ARG(cells[t] >= 0 && cells[t] < d->n, std::string(what) + " cell out of range");
   if (n) {
-    seen.assign((size_t)d->n, 0);
+    std::vector<uint8_t> seen((size_t)d->n, 0);
+    for (int c : other) seen[c] = 2;
     for (int64_t t = 0; t < n; ++t) {
-      PSL_CHECK_ARG(!seen[cells[t]], "moving-bounceback cell listed twice");
+      PSL_CHECK_ARG(seen[cells[t]] != 1, std::string(what) + " cell listed twice");
+      PSL_CHECK_ARG(seen[cells[t]] != 2, std::string(what) + " cell is also a moving-wall or forced cell");
       seen[cells[t]] = 1;
     }
   }
-  int rc = upload_ints(&d->moving, cells, n);
+  *dev_n = 0;
+  host->clear();
+  int rc = upload_ints(dev, cells, n);
   if (rc == PSL_OK) {
-    if (d->moving_u) (void)hipFree(d->moving_u);
-    d->moving_u = nullptr;
+    if (*dev_u) (void)hipFree(*dev_u);
+    *dev_u = nullptr;
     if (n) {
-      if (hipMalloc(&d->moving_u, 3 * (size_t)n * sizeof(float)) != hipSuccess ||
-          hipMemcpy(d->moving_u, u3, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-        set_error("moving-bounceback upload failed");
+      if (hipMalloc(dev_u, 3 * (size_t)n * sizeof(float)) != hipSuccess ||
+          hipMemcpy(*dev_u, u3, 3 * (size_t)n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        set_error(std::string(what) + " upload failed");
         rc = PSL_E_HIP;
       }
     }
   }
-  d->n_moving = rc == PSL_OK ? n : 0;
+  if (rc == PSL_OK) {
+    *dev_n = n;
+    host->assign(cells, cells + n);
+  }
   return rc;
+}
+
+int psl_diad_set_moving_bounceback(psl_diad* d, const int* cells, int64_t n, const float* u3) {
+  PSL_CHECK_ARG(d, "null grid");
+  return set_cell_list(d, cells, n, u3, d->forced_host, "moving-bounceback", &d->moving, &d->moving_u, &d->n_moving,
+                       &d->moving_host);
+}
+
+int psl_diad_set_forced_velocity(psl_diad* d, const int* cells, int64_t n, const float* ut3) {
+  PSL_CHECK_ARG(d, "null grid");
+  return set_cell_list(d, cells, n, ut3, d->moving_host, "forced-velocity", &d->forced, &d->forced_u, &d->n_forced,
+                       &d->forced_host);
 }
 
 int psl_diad_apply_nonreflective_outlet_z(psl_diad* d) {
@@ -701,6 +836,7 @@ static int diad_recurse(psl_diad* const* lv, int nl, int l) {
   d->flipper = !d->flipper;
   int rc = PSL_ESO_DISPATCH(eso_update, d);
   if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_moving, d);
+  if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_forced, d);
   if (rc != PSL_OK || l == nl - 1) return rc;
   for (int s = 0; s < 2 && rc == PSL_OK; ++s) rc = diad_recurse(lv, nl, l + 1);
   if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_comm, d, lv[l + 1]);
@@ -725,7 +861,8 @@ int psl_diad_reduce_momentum_exchange(psl_diad* d, double out[3]) {
   PSL_CHECK_ARG(d && out, "null argument");
   const uint32_t nb = (uint32_t)((d->n + 255) / 256);
   const uint32_t nbm = (uint32_t)((d->n_moving + 255) / 256);
-  const size_t need = 3 * ((size_t)nb + nbm);
+  const uint32_t nbf = (uint32_t)((d->n_forced + 255) / 256);
+  const size_t need = 3 * ((size_t)nb + nbm + nbf);
   if (d->n_partials < need) {
     if (d->partials) PSL_HIP_TRY(hipFree(d->partials));
     d->partials = nullptr;
@@ -736,6 +873,9 @@ int psl_diad_reduce_momentum_exchange(psl_diad* d, double out[3]) {
   int rc = PSL_ESO_DISPATCH(eso_momentum, d, nb, nullptr, 0u, d->partials);
   if (rc == PSL_OK && nbm)
     rc = PSL_ESO_DISPATCH(eso_momentum, d, nbm, d->moving, (uint32_t)d->n_moving, d->partials + 3 * (size_t)nb);
+  if (rc == PSL_OK && nbf)
+    rc = PSL_ESO_DISPATCH(eso_momentum, d, nbf, d->forced, (uint32_t)d->n_forced,
+                          d->partials + 3 * ((size_t)nb + nbm));
   if (rc != PSL_OK) return rc;
   std::vector<double> h(need);
   PSL_HIP_TRY(hipMemcpyAsync(h.data(), d->partials, need * sizeof(double), hipMemcpyDeviceToHost, d->stream));
@@ -743,9 +883,35 @@ int psl_diad_reduce_momentum_exchange(psl_diad* d, double out[3]) {
   double acc[3] = {0.0, 0.0, 0.0}, mov[3] = {0.0, 0.0, 0.0};
   for (uint32_t b = 0; b < nb; ++b)
     for (int a = 0; a < 3; ++a) acc[a] += h[3 * (size_t)b + a];
-  for (uint32_t b = 0; b < nbm; ++b)
+  for (uint32_t b = 0; b < nbm + nbf; ++b)
     for (int a = 0; a < 3; ++a) mov[a] += h[3 * ((size_t)nb + b) + a];
   for (int a = 0; a < 3; ++a) out[a] = acc[a] - mov[a];
+  return PSL_OK;
+}
+
+int psl_diad_reduce_forcing(psl_diad* d, float rho_nominal_phys, double out[6]) {
+  PSL_CHECK_ARG(d && out, "null argument");
+  const uint32_t nb = (uint32_t)((d->n_forced + 255) / 256);
+  for (int a = 0; a < 6; ++a) out[a] = 0.0;
+  if (!nb) return PSL_OK;
+  const size_t need = 6 * (size_t)nb;
+  if (d->n_partials < need) {
+    if (d->partials) PSL_HIP_TRY(hipFree(d->partials));
+    d->partials = nullptr;
+    d->n_partials = 0;
+    PSL_HIP_TRY(hipMalloc(&d->partials, need * sizeof(double)));
+    d->n_partials = need;
+  }
+  const float h = d->info.res / 1000.f;
+  const float scale = rho_nominal_phys * h * h * h * h / (d->info.dtPhys * d->info.dtPhys);
+  const int rc = PSL_ESO_DISPATCH(eso_forcing_sum, d, nb, d->partials, scale);
+  if (rc != PSL_OK) return rc;
+  std::vector<double> hp(need);
+  PSL_HIP_TRY(hipMemcpyAsync(hp.data(), d->partials, need * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(d->stream));
+  for (uint32_t b = 0; b < nb; ++b)
+    for (int a = 0; a < 6; ++a) out[a] += hp[6 * (size_t)b + a];
+  for (int a = 3; a < 6; ++a) out[a] /= 1000.0;  // N mm -> N m
   return PSL_OK;
 }
 

@@ -164,6 +164,10 @@ struct psl_diad {
   int* moving = nullptr;     // moving-bounceback cells
   float* moving_u = nullptr; // their wall velocities (3 per cell)
   int64_t n_moving = 0;
+  int* forced = nullptr;     // forced-velocity cells
+  float* forced_u = nullptr; // their target velocities (3 per cell)
+  int64_t n_forced = 0;
+  std::vector<int> moving_host, forced_host;  // host copies of the two lists
   double* partials = nullptr;  // reduction scratch
   size_t n_partials = 0;
 };

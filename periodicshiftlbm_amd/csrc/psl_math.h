@@ -173,10 +173,13 @@ struct Axis {
 
 // Post-collision state -> populations.  ks_* are the non-zero central moments
 // of the post-collision distribution; writes v (full f, before any weight
-// offset is removed).
+// offset is removed).  With FIRST the first-order central moments ks100,
+// ks010, ks001 enter as well (non-zero only under a force).
+template <bool FIRST = false>
 PSL_HD PSL_INLINE void central_to_populations(float rho, float inv_rho, float ux, float uy, float uz, float ks200,
                                               float ks020, float ks002, float ks110, float ks101, float ks011,
-                                              float (&v)[27]) {
+                                              float (&v)[27], float ks100 = 0.f, float ks010 = 0.f,
+                                              float ks001 = 0.f) {
   // fourth and sixth order from the factorised cumulants (eqs 81-84)
   const float ks211 = (ks200 * ks011 + 2.f * ks101 * ks110) * inv_rho;
   const float ks121 = (ks020 * ks101 + 2.f * ks110 * ks011) * inv_rho;
@@ -193,10 +196,16 @@ PSL_HD PSL_INLINE void central_to_populations(float rho, float inv_rho, float ux
   const Axis X(ux), Y(uy), Z(uz);
   // x pass: p[x][b][c] for x in {a,b,c}, (b,c) = (y order, z order)
   float p[3][3][3];
-  X.even(rho, ks200, p[0][0][0], p[1][0][0], p[2][0][0]);
-  X.odd(ks101, p[0][0][1], p[1][0][1], p[2][0][1]);
+  if (FIRST) {
+    X.general(rho, ks100, ks200, p[0][0][0], p[1][0][0], p[2][0][0]);
+    X.general(ks001, ks101, 0.f, p[0][0][1], p[1][0][1], p[2][0][1]);
+  } else {
+    X.even(rho, ks200, p[0][0][0], p[1][0][0], p[2][0][0]);
+    X.odd(ks101, p[0][0][1], p[1][0][1], p[2][0][1]);
+  }
   X.even(ks002, ks202, p[0][0][2], p[1][0][2], p[2][0][2]);
-  X.odd(ks110, p[0][1][0], p[1][1][0], p[2][1][0]);
+  if (FIRST) X.general(ks010, ks110, 0.f, p[0][1][0], p[1][1][0], p[2][1][0]);
+  else X.odd(ks110, p[0][1][0], p[1][1][0], p[2][1][0]);
   X.even(ks011, ks211, p[0][1][1], p[1][1][1], p[2][1][1]);
   X.odd(ks112, p[0][1][2], p[1][1][2], p[2][1][2]);
   X.even(ks020, ks220, p[0][2][0], p[1][2][0], p[2][2][0]);
@@ -285,6 +294,49 @@ PSL_HD PSL_INLINE void collide_with_moments(float (&v)[27], Moments m, const Rel
 template <bool W_OFF>
 PSL_HD PSL_INLINE void collide(float (&v)[27], const Relax& rx, float cs) {
   collide_with_moments<W_OFF>(v, raw_moments(v), rx, cs);
+}
+
+// The forcing collision of one cell (M35Full/applyCollisionCustomized.h:5-17)
+// on full f with force (gx, gy, gz): the velocity moves by half the force,
+// u' = (rho u + g/2) / rho, the central moments are taken about u' (their
+// first order is then -g/2), the Smagorinsky rate sees f - feq(rho, u'), the
+// relaxation is the unforced one with u', and the backward step restores the
+// first-order central moments as +g/2, so the momentum gains g.
+PSL_HD PSL_INLINE void collide_forced(float (&v)[27], const Relax& rx, float cs, float gx, float gy, float gz) {
+  const Moments m = raw_moments(v);
+  const float rho = m.m000;
+  const float inv_rho = 1.f / rho;
+  const float ux = (m.m100 + 0.5f * gx) * inv_rho, uy = (m.m010 + 0.5f * gy) * inv_rho,
+              uz = (m.m001 + 0.5f * gz) * inv_rho;
+  const float k100 = m.m100 - rho * ux, k010 = m.m010 - rho * uy, k001 = m.m001 - rho * uz;
+  // second-order central moments about u' from the raw ones
+  const float k200 = m.m200 - ux * (2.f * m.m100 - rho * ux);
+  const float k020 = m.m020 - uy * (2.f * m.m010 - rho * uy);
+  const float k002 = m.m002 - uz * (2.f * m.m001 - rho * uz);
+  const float k110 = m.m110 - ux * m.m010 - uy * m.m100 + rho * ux * uy;
+  const float k101 = m.m101 - ux * m.m001 - uz * m.m100 + rho * ux * uz;
+  const float k011 = m.m011 - uy * m.m001 - uz * m.m010 + rho * uy * uz;
+  // f - feq(rho, u') has second moments m2 - rho (I/3 + u' u'^T)
+  const float omega = cs == 0.f ? rx.omega
+                                : omega_les(rho, inv_rho, m.m200 - rho * ux * ux, m.m020 - rho * uy * uy,
+                                            m.m002 - rho * uz * uz, m.m110 - rho * ux * uy, m.m101 - rho * ux * uz,
+                                            m.m011 - rho * uy * uz, rx.tau, cs);
+  const float h_rho = 0.5f * inv_rho;
+  const float Dxu = -omega * h_rho * (2.f * k200 - k020 - k002) - h_rho * (k200 + k020 + k002 - rho);
+  const float Dyv = Dxu + 3.f * omega * h_rho * (k200 - k020);
+  const float Dzw = Dxu + 3.f * omega * h_rho * (k200 - k002);
+  const float om1 = 1.f - omega;
+  const float ks110 = om1 * k110, ks101 = om1 * k101, ks011 = om1 * k011;
+  const float ux2Dxu = ux * ux * Dxu, uy2Dyv = uy * uy * Dyv, uz2Dzw = uz * uz * Dzw;
+  const float c3 = 3.f * rho * (1.f - 0.5f * omega);
+  const float e61 = om1 * (k200 - k020) - c3 * (ux2Dxu - uy2Dyv);
+  const float e64 = om1 * (k200 - k002) - c3 * (ux2Dxu - uz2Dzw);
+  const float e65 = rho - 1.5f * rho * (ux2Dxu + uy2Dyv + uz2Dzw);
+  const float ks200 = (1.f / 3.f) * (e61 + e64 + e65);
+  const float ks020 = (1.f / 3.f) * (e64 - 2.f * e61 + e65);
+  const float ks002 = (1.f / 3.f) * (e61 - 2.f * e64 + e65);
+  central_to_populations<true>(rho, inv_rho, ux, uy, uz, ks200, ks020, ks002, ks110, ks101, ks011, v, -k100, -k010,
+                               -k001);
 }
 
 // Central-moment rescaling between grid levels (include/gridRefinementFunctions.h:6-281,

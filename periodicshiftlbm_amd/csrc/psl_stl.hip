@@ -212,6 +212,12 @@ __global__ void k_mark_keys(const int* __restrict__ layers, int maxc, VoxParams 
   }
 }
 
+// packed (k, j, i) keys of a DIAD grid's cells from its ijk[3][n] table
+__global__ void k_pack_ijk(const int* __restrict__ ijk, int64_t n, uint64_t* __restrict__ keys) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+    keys[c] = pack_key(ijk[c], ijk[n + c], ijk[2 * n + c]);
+}
+
 __global__ void k_fill_int(int* a, size_t n, int v) {
   for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < n; c += (size_t)gridDim.x * blockDim.x) a[c] = v;
 }
@@ -426,6 +432,55 @@ int psl_stl_set_origin(psl_stl* s, float ox, float oy, float oz) {
   PSL_CHECK_ARG(s, "null stl");
   s->ox = ox; s->oy = oy; s->oz = oz;
   return PSL_OK;
+}
+
+// rotateSTLAlongX / Y / Z (include/STLFunctions.h:698-809): every vertex
+// rotated by `radians` about the axis through the STL's own origin,
+// right-handed, in FP32 with sinf / cosf of the angle; bounds follow.
+int psl_stl_rotate(psl_stl* s, int axis, float radians) {
+  PSL_CHECK_ARG(s && axis >= 0 && axis <= 2, "bad rotation");
+  const float sn = sinf(radians), cs = cosf(radians);
+  const int a = axis == 0 ? 1 : axis == 1 ? 2 : 0;  // the pair (a, b) turns a -> b
+  const int b = axis == 0 ? 2 : axis == 1 ? 0 : 1;
+  for (size_t v = 0; v < s->tri.size(); v += 3) {
+    const float pa = s->tri[v + a], pb = s->tri[v + b];
+    s->tri[v + a] = pa * cs - pb * sn;
+    s->tri[v + b] = pa * sn + pb * cs;
+  }
+  stl_bounds(s);
+  return upload_stl(s);
+}
+
+// ApplyMarkersInsideSTL (include/STLFunctions.h:482-696) on a DIAD grid's
+// cells: out[c] = 1 inside, 0 outside (the reference writes the value and its
+// complement), host array of the grid's n cells.
+int psl_diad_markers_inside_stl(const psl_diad* d, const psl_stl* s, uint8_t* out) {
+  PSL_CHECK_ARG(d && s && out, "null argument");
+  if (d->n == 0) return PSL_OK;
+  uint64_t* keys = nullptr;
+  uint8_t* m = nullptr;
+  int rc = PSL_OK;
+  if (hipMalloc(&keys, (size_t)d->n * sizeof(uint64_t)) != hipSuccess || hipMalloc(&m, (size_t)d->n) != hipSuccess ||
+      hipMemset(m, 0, (size_t)d->n) != hipSuccess) {
+    set_error("inside markers: allocation");
+    rc = PSL_E_HIP;
+  }
+  if (rc == PSL_OK) {
+    k_pack_ijk<<<grid1d((size_t)d->n), 256>>>(d->ijk, d->n, keys);
+    if (hipGetLastError() != hipSuccess) {
+      set_error("inside markers: key launch");
+      rc = PSL_E_HIP;
+    }
+  }
+  const psl_info& I = d->info;
+  if (rc == PSL_OK) rc = stl_mark_keys(s, I.ox, I.oy, I.oz, I.res, I.cellCountX, I.cellCountY, I.cellCountZ, keys, d->n, m);
+  if (rc == PSL_OK && hipMemcpy(out, m, (size_t)d->n, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("inside markers: copy");
+    rc = PSL_E_HIP;
+  }
+  if (keys) (void)hipFree(keys);
+  if (m) (void)hipFree(m);
+  return rc;
 }
 
 int psl_stl_get_triangles(const psl_stl* s, float* tri9) {

@@ -174,3 +174,106 @@ def test_moving_bounceback_pins(flip):
     assert np.max(np.abs(f_out.sum(0) - f_in.sum(0))) < 1e-12
     m_in, m_out = c @ f_in, c @ f_out
     assert np.max(np.abs(m_out - (-m_in + 2.0 * u.T))) < 1e-7
+
+
+# ------------------------------------------------------- forced velocity
+def test_forcing_collision_pins():
+    """The forcing collision conserves mass and adds the force g to the
+    momentum (first-order central moments -g/2 about u' = u + g/(2 rho),
+    restored as +g/2); with g = 0 it is the unforced collision bit for bit.
+    A dropped half-force shift, a sign flip of the restored first-order moments
+    or a velocity left unshifted each break the momentum balance."""
+    rng = np.random.default_rng(8)
+    c = np.array(O.velocity_set(), np.float64)
+    for cs in (0.0, 0.1):
+        for _ in range(20):
+            ru = np.array([1 + 0.02 * rng.standard_normal(), *(0.05 * rng.standard_normal(3))])
+            f = O.feq(ru) + 1e-3 * rng.standard_normal(27)
+            g = 2e-3 * rng.standard_normal(3)
+            out = O.collide_forced(f, 0.02, cs, g)
+            assert abs(out.sum() - f.sum()) < 1e-14
+            assert np.max(np.abs(c @ out - (c @ f + g))) < 1e-14
+            assert np.array_equal(O.collide_forced(f, 0.02, cs, np.zeros(3)), O.collide(f.copy(), 0.02, cs))
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_forced_cells_reach_their_target(flip):
+    """With g = rho (u_target - u) the momentum after the forcing collision is
+    rho u_target: every forced cell leaves the update at its target velocity
+    with its mass; slots of other cells are untouched."""
+    d = outlet_level()
+    nbr7 = np.ascontiguousarray(d["nbr"][:7], np.int32)
+    rng = np.random.default_rng(12 + flip)
+    ru = np.column_stack([1 + 0.01 * rng.standard_normal(d["n"]), 0.03 * rng.standard_normal((d["n"], 3))])
+    f = (O.feq(ru) + 1e-3 * rng.standard_normal((d["n"], 27))).T
+    raw = np.zeros((27, d["n"]))
+    E.eso_set_all(raw, nbr7, not flip, f)  # the previous update's writes = this step's reads
+    cells = np.sort(rng.choice(d["n"], 150, replace=False)).astype(np.int32)
+    ut = (0.04 * rng.standard_normal((cells.size, 3))).astype(np.float32)
+    before = raw.copy()
+    R, _ = O.esotwist_tables()
+    s = np.vstack([np.arange(d["n"])[None], d["nbr"][:7].astype(np.int64)])
+    f_in = np.stack([raw[q if flip else E.OPP[q], s[R[q]][cells]] for q in range(27)])  # read maps
+    O.eso_forced_velocity(raw, nbr7, flip, cells, ut, 0.02, 0.1)
+    out = E.eso_get_all(raw, nbr7, flip)[:, cells]
+    m = O.rho_u(np.ascontiguousarray(out.T))
+    assert np.max(np.abs(m[:, 0] - f_in.sum(0))) < 1e-13
+    assert np.max(np.abs(m[:, 1:] - ut)) < 1e-13
+    touched = np.zeros_like(raw, bool)
+    for q in range(27):
+        touched[q if flip else E.OPP[q], s[R[q]][cells]] = True
+    assert np.array_equal(raw[~touched], before[~touched])
+
+
+def test_forcing_sum_rigid_rotation():
+    """Cells at rest (rho = 1) driven towards a rigid rotation u_t = w e_z x r:
+    g = u_t, so the force is scale * sum u_t and the torque about z is
+    scale * w * sum (x^2 + y^2) / 1000, with scale = rho_phys (res/1000)^4 / dt^2."""
+    d = outlet_level()
+    nbr7 = np.ascontiguousarray(d["nbr"][:7], np.int32)
+    raw = np.zeros((27, d["n"]))
+    e = O.feq(np.array([1.0, 0.0, 0.0, 0.0]))
+    E.eso_set_all(raw, nbr7, False, np.repeat(e[:, None], d["n"], axis=1))
+    rng = np.random.default_rng(2)
+    cells = np.sort(rng.choice(d["n"], 300, replace=False)).astype(np.int32)
+    info = dict(res=100.0, ox=-550.0, oy=-550.0, oz=-1150.0, dt=0.01)
+    i, j, k = (a[cells].astype(np.float64) for a in d["ijk"])
+    x, y, z = (a * info["res"] + o for a, o in ((i, info["ox"]), (j, info["oy"]), (k, info["oz"])))
+    w = 1e-5
+    ut = np.column_stack([-w * y, w * x, np.zeros_like(x)]).astype(np.float32)
+    got = O.eso_forcing_sum(raw, nbr7, False, cells, ut, d["ijk"], info, 1000.0)
+    scale = 1000.0 * (info["res"] / 1000.0) ** 4 / info["dt"] ** 2
+    g = ut.astype(np.float64) * scale
+    assert np.allclose(got[:3], g.sum(0), rtol=1e-6, atol=1e-9)
+    assert np.isclose(got[5], scale * w * np.sum(x * x + y * y) / 1000.0, rtol=1e-6)
+    assert np.isclose(got[3], np.sum(-z * g[:, 1]) / 1000.0, rtol=1e-6, atol=1e-9)
+    assert np.isclose(got[4], np.sum(z * g[:, 0]) / 1000.0, rtol=1e-6, atol=1e-9)
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_forcing_sum_reads_the_next_step(flip):
+    """The forcing sum sees the populations the next update reads (read maps of
+    the other parity), not the cell's own last writes: on a random state it
+    equals scale * sum rho (u_t - u) over that gather, and differs from the
+    same sum over the write maps."""
+    d = outlet_level()
+    nbr7 = np.ascontiguousarray(d["nbr"][:7], np.int32)
+    rng = np.random.default_rng(30 + flip)
+    raw = 0.03 + 0.01 * rng.random((27, d["n"]))
+    cells = np.sort(rng.choice(d["n"], 100, replace=False)).astype(np.int32)
+    ut = (0.02 * rng.standard_normal((cells.size, 3))).astype(np.float32)
+    info = dict(res=1.0, ox=0.0, oy=0.0, oz=0.0, dt=1.0)
+    got = O.eso_forcing_sum(raw, nbr7, flip, cells, ut, d["ijk"], info, 1.0)
+    scale = 1.0 * (1.0 / 1000.0) ** 4
+    R, _ = O.esotwist_tables()
+    s = np.vstack([np.arange(d["n"])[None], d["nbr"][:7].astype(np.int64)])
+    nxt = np.stack([raw[E.OPP[q] if flip else q, s[R[q]][cells]] for q in range(27)])
+    own = E.eso_get_all(raw, nbr7, flip)[:, cells]
+    c = np.array(O.velocity_set(), np.float64)
+
+    def force(f):
+        rho = f.sum(0)
+        return (scale * (rho[:, None] * ut - (c @ f).T)).sum(0)
+
+    assert np.allclose(got[:3], force(nxt), rtol=1e-6, atol=0)  # scale is formed in FP32
+    assert not np.allclose(got[:3], force(own), rtol=1e-3, atol=0)

@@ -219,3 +219,133 @@ def test_moving_bounceback_matches_oracle(psl, storage):
     assert np.abs(fo).max() > 0
     tol = (1e-4 if storage == "f32" else 2e-2) * np.abs(fo).max()
     assert np.all(np.abs(fx - fo) <= tol)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_forced_velocity_matches_oracle(psl, storage):
+    """Forced-velocity cells (flagged bounceback, target velocities per cell)
+    beside moving walls and static walls, over three steps (streaming + local
+    update, then the level recursion on one level) against the oracle's update
+    followed by its moving-wall and forced-cell operators; then the forcing
+    sums (force and torque) and the drag, which leaves both lists out."""
+    P = psl
+    from tests.test_gpu_diad_builder import _gpu_build
+    from tests.test_diad_levels import outlet_level
+    base = dict(res=1.0, ox=-5.5, oy=-5.5, oz=-11.5, nu=0.02, dt=1e-3, nx=16, ny=12, nz=24)
+    rules = P.rules(cases.diad_sphere_rules(P, 1.6, 2),
+                    P.marker_rule(P.RULE_OUTLET, P.SHAPE_INDEX_BOX, idx=(0, -1, 0, -1, -1, -1), levels=[0]))
+    lv = _gpu_build(P, base, 2, 1, [], rules, nu_phys=2e-5)
+    d = outlet_level()
+    g = P.DIADGridStruct.from_levels(lv, 0, storage)
+    g.set_smagorinsky(0.1)
+    nbr7 = np.ascontiguousarray(d["nbr"][:7], np.int32)
+    rng = np.random.default_rng(21)
+    flags = cases.diad_sphere_flags(d["ijk"], d["bounceback"], (d["nx"], d["ny"], d["nz"]), 0)
+    interior = np.flatnonzero(flags == cases.FLUID)
+    picked = rng.choice(interior, 400, replace=False)
+    forced = np.sort(picked[:200]).astype(np.int32)
+    moving = np.sort(picked[200:300]).astype(np.int32)
+    flags[picked[300:]] = cases.BOUNCEBACK  # static walls
+    walls_only = flags.copy()
+    flags[forced] = cases.BOUNCEBACK
+    flags[moving] = cases.BOUNCEBACK
+    ut = (0.03 * rng.standard_normal((forced.size, 3))).astype(np.float32)
+    um = (0.02 * rng.standard_normal((moving.size, 3))).astype(np.float32)
+    g.set_flags(flags)
+    g.set_given_values(cases.channel_given_values(0.04))
+    g.set_moving_bounceback(moving, um)
+    g.set_forced_velocity(forced, ut)
+    with pytest.raises(P.PslError):  # a cell may not be in both lists
+        g.set_forced_velocity(np.concatenate([forced, moving[:1]]), np.vstack([ut, um[:1]]))
+    g.fill_equilibrium(ux=0.03)
+    raw = np.zeros((27, g.n), np.float32)
+    E.eso_set_all(raw, nbr7, False, g.get_f())
+    info = O.make_info(d["nx"], d["ny"], d["nz"], nu=0.02, cs=0.1)
+    ijk = np.ascontiguousarray(d["ijk"], np.int32)
+    flip = False
+    for step in range(3):
+        if step == 1:
+            P.updateGrid([g], 1)
+        else:
+            P.applyStreaming(g)
+            P.applyLocalCellUpdate(g)
+        flip = not flip
+        O.esotwist_update(raw, nbr7, ijk, info, flags, cases.channel_given_values(0.04).reshape(-1), flip)
+        O.eso_moving_bounceback(raw, nbr7, flip, moving, um)
+        O.eso_forced_velocity(raw, nbr7, flip, forced, ut, np.float32(0.02), np.float32(0.1))
+        ref = E.eso_get_all(raw, nbr7, flip)
+        got = g.get_f()
+        if storage == "f32":
+            err = _rel_err(got, ref)
+            print(f"  step {step + 1}: max rel err {err:.3e}", flush=True)
+            assert err < RTOL
+            # the forced cells leave the update at their targets
+            u = O.rho_u(np.ascontiguousarray(got[:, forced].T.astype(np.float64)))[:, 1:]
+            assert np.max(np.abs(u - ut)) < 5e-6
+        else:
+            err = float(np.max(np.abs(got.astype(np.float64) - ref)))
+            print(f"  step {step + 1}: max abs err {err:.3e} (f16 storage vs FP32 oracle)", flush=True)
+            assert err < 2e-3
+    lvl, _ = lv.info(0)
+    oinfo = dict(res=lvl.res, ox=lvl.ox, oy=lvl.oy, oz=lvl.oz, dt=lvl.dtPhys)
+    fo = O.eso_forcing_sum(raw, nbr7, flip, forced, ut, ijk, oinfo, 1000.0)
+    fg = g.forcing(1000.0)
+    print(f"  forcing GPU {fg} oracle {fo}", flush=True)
+    assert np.abs(fo).max() > 0
+    tol = (1e-4 if storage == "f32" else 2e-2) * np.abs(fo[:3]).max()
+    assert np.all(np.abs(fg[:3] - fo[:3]) <= tol)
+    ttol = (1e-4 if storage == "f32" else 2e-2) * np.abs(fo[3:]).max()
+    assert np.all(np.abs(fg[3:] - fo[3:]) <= ttol)
+    assert P.getTorque(g, 1000.0) == fg[5]
+    fx = g.momentum_exchange()
+    mo = O.eso_momentum_exchange(raw, nbr7, flip, walls_only)
+    print(f"  drag without moving and forced cells: GPU {fx} oracle {mo}", flush=True)
+    tol = (1e-4 if storage == "f32" else 2e-2) * np.abs(mo).max()
+    assert np.all(np.abs(fx - mo) <= tol)
+
+
+def _box_triangles(lo, hi):
+    """12 outward-wound triangles of an axis-aligned box."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float32)
+    faces = [(0, 2, 1), (0, 3, 2), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4),
+             (2, 3, 7), (2, 7, 6), (1, 2, 6), (1, 6, 5), (3, 0, 4), (3, 4, 7)]
+    return np.array([np.concatenate([v[a], v[b], v[c]]) for a, b, c in faces], np.float32)
+
+
+def test_rotating_stl_markers_match_oracle(psl):
+    """The M35 impeller loop: an STL blade rotated about z (rotateSTLAlongZ),
+    then ApplyMarkersInsideSTL on each DIAD level's cells, bit for bit against
+    the oracle's dense-box voxelizer sampled at the level's (i, j, k).  The
+    rotation matches the closed form, and turning back restores the blade."""
+    P = psl
+    lv, grids = cases.diad_sphere(P, "f32", n0=16, levels=2, span=1, radius=2.0)
+    tri0 = _box_triangles((-4.1, -0.7, -2.2), (4.1, 0.7, 2.2))
+    stl = P.STLStruct.from_triangles(tri0)
+    total = 0
+    for step, angle in enumerate((0.0, 0.3, 0.3, 0.45)):
+        if angle:
+            P.rotateSTLAlongZ(stl, angle)
+        tri = stl.triangles()
+        th = sum((0.0, 0.3, 0.3, 0.45)[: step + 1])
+        c, s = np.cos(th), np.sin(th)
+        v0 = tri0.reshape(-1, 3).astype(np.float64)
+        want_v = np.column_stack([v0[:, 0] * c - v0[:, 1] * s, v0[:, 0] * s + v0[:, 1] * c, v0[:, 2]])
+        assert np.max(np.abs(tri.reshape(-1, 3) - want_v)) < 1e-5
+        b = stl.bounds
+        assert np.allclose(b[:3], want_v.min(0), atol=1e-5) and np.allclose(b[3:], want_v.max(0), atol=1e-5)
+        for l, g in enumerate(grids):
+            info, _ = lv.info(l)
+            got = g.markers_inside_stl(stl)
+            dense = O.markers_inside_stl(tri, (0, 0, 0), (info.ox, info.oy, info.oz), info.res,
+                                         info.cellCountX, info.cellCountY, info.cellCountZ)
+            i, j, k = (a.astype(np.int64) for a in lv.array(l, "ijk"))
+            want = dense[(k * info.cellCountY + j) * info.cellCountX + i]
+            print(f"  angle {th:.2f} level {l}: {int(got.sum())} cells inside", flush=True)
+            assert np.array_equal(got, want)
+            total += int(got.sum())
+    assert total > 0
+    P.rotateSTLAlongZ(stl, -1.05)
+    assert np.max(np.abs(stl.triangles() - tri0)) < 1e-5
