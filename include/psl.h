@@ -426,6 +426,12 @@ int psl_scalar_apply_local_cell_update(psl_grid* grid);
  * bounceback cell. */
 int psl_section_cut(psl_grid* grid, int plane, int cut, float rho_nominal, float soundspeed, float* host_out,
                     int* dims);
+/* exportSectionCutPlotXY/ZY/ZX(GridStruct&, ScalarTransportStruct&)
+ * (include/plotter/exportSectionCutPlot.h:108-221): as psl_section_cut with a
+ * sixth float per pixel, the scalar sum f T / sum f (getScalarTransport,
+ * include/cellFunctions.h:228-235); needs psl_scalar_enable. */
+int psl_scalar_section_cut(psl_grid* grid, int plane, int cut, float rho_nominal, float soundspeed, float* host_out,
+                           int* dims);
 /* The same over DIAD levels (exportSectionCutPlotGeneral(std::vector<DIADGridStruct>&), lines 377-589):
  * the plane at the finest level whose plane has fewer than max_pixels pixels
  * (dims3 = rows, cols, levels used; the reference uses 20 000 000), 6 floats
@@ -444,6 +450,13 @@ int psl_diad_flow_report(psl_diad* const* levels, int nlevels, int plane, int cu
 /* /dev/shm/sim_data.bin format of the reference's plotters (exportSectionCutPlot.h:64-86):
  * int {plot_number, rows, cols, components}, then the floats. */
 int psl_write_sim_data(const char* path, int plot_number, int rows, int cols, int components, const float* data);
+/* export3DPlot(GridStruct&) (include/plotter/exportSectionCutPlot.h:591-672): every
+ * cell in (i, j, k) order with k fastest, 5 floats each (p, ux, uy, uz in m/s
+ * as in psl_section_cut, bounceback marker); host_out holds 5 N floats. */
+int psl_volume_export(psl_grid* grid, float rho_nominal, float soundspeed, float* host_out);
+/* Its file: int plot number, nx, ny, nz, components, then the floats. */
+int psl_write_volume_data(const char* path, int plot_number, int nx, int ny, int nz, int components,
+                          const float* data);
 /* historyData.bin (exportSectionCutPlot.h:994-1001): int count, then count floats. */
 int psl_write_history(const char* path, const float* values, int count);
 

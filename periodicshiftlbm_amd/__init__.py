@@ -219,6 +219,9 @@ _SIGS = [
     ("psl_diad_set_forced_velocity", _I, [_P, _P, ctypes.c_int64, _P]),
     ("psl_diad_reduce_forcing", _I, [_P, ctypes.c_float, _P]),
     ("psl_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
+    ("psl_scalar_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
+    ("psl_volume_export", _I, [_P, ctypes.c_float, ctypes.c_float, _P]),
+    ("psl_write_volume_data", _I, [ctypes.c_char_p, _I, _I, _I, _I, _I, _P]),
     ("psl_diad_section_cut", _I, [_P, _I, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
     ("psl_scalar_enable", _I, [_P, ctypes.c_float]),
     ("psl_scalar_set_given", _I, [_P, _P]),
@@ -753,10 +756,19 @@ def applyScalarTransportLocalCellUpdate(grid: "GridStruct", scalar: ScalarTransp
 PLANES = {"XY": 0, "ZY": 1, "ZX": 2}
 
 
-def section_cut(grid, plane: str, cut: int, rho_nominal: float = 1.225, soundspeed: float = 1.0) -> np.ndarray:
-    """exportSectionCutPlotGeneral on a GridStruct (rows x cols x 5) or a list of
+def section_cut(grid, plane: str, cut: int, rho_nominal: float = 1.225, soundspeed: float = 1.0,
+                scalar_transport: bool = False) -> np.ndarray:
+    """exportSectionCutPlotGeneral on a GridStruct (rows x cols x 5; x 6 with
+    scalar_transport, the last component the scalar) or a list of
     DIADGridStruct levels (rows x cols x 6, the last component the level)."""
     pl = PLANES[plane]
+    if scalar_transport:
+        dims = np.zeros(2, np.int32)
+        _check(lib().psl_scalar_section_cut(grid.handle, pl, cut, rho_nominal, soundspeed, None, dims.ctypes.data))
+        out = np.zeros((dims[0], dims[1], 6), np.float32)
+        _check(lib().psl_scalar_section_cut(grid.handle, pl, cut, rho_nominal, soundspeed, out.ctypes.data,
+                                            dims.ctypes.data))
+        return out
     if isinstance(grid, (list, tuple)):
         arr = (ctypes.c_void_p * len(grid))(*[g.handle.value for g in grid])
         dims = np.zeros(3, np.int32)
@@ -785,11 +797,25 @@ def getFlowReport(levels, plane: str, cut: int, bounds) -> dict:
 
 
 def exportSectionCutPlot(grid, plane: str, cut: int, plot_number: int, path: str = "/dev/shm/sim_data.bin",
-                         rho_nominal: float = 1.225, soundspeed: float = 1.0) -> np.ndarray:
-    """exportSectionCutPlotXY/ZY/ZX: the cut written in the plotters' sim_data.bin format."""
-    data = section_cut(grid, plane, cut, rho_nominal, soundspeed)
+                         rho_nominal: float = 1.225, soundspeed: float = 1.0,
+                         scalar_transport: bool = False) -> np.ndarray:
+    """exportSectionCutPlotXY/ZY/ZX: the cut written in the plotters' sim_data.bin
+    format (six components with scalar_transport, as plotterScalarTransport.py reads)."""
+    data = section_cut(grid, plane, cut, rho_nominal, soundspeed, scalar_transport)
     write_sim_data(path, plot_number, data)
     return data
+
+
+def export3DPlot(grid: GridStruct, plot_number: int, path: Optional[str] = "/dev/shm/sim_data.bin",
+                 rho_nominal: float = 1.225, soundspeed: float = 1.0) -> np.ndarray:
+    """export3DPlot: (nx, ny, nz, 5) of p, ux, uy, uz (m/s) and the bounceback
+    marker, written to path (None: not written) in the 3-D sim_data.bin format."""
+    I = grid.Info
+    out = np.zeros((I.cellCountX, I.cellCountY, I.cellCountZ, 5), np.float32)
+    _check(lib().psl_volume_export(grid.handle, rho_nominal, soundspeed, out.ctypes.data))
+    if path is not None:
+        _check(lib().psl_write_volume_data(path.encode(), plot_number, *out.shape, out.ctypes.data))
+    return out
 
 
 def write_sim_data(path: str, plot_number: int, data: np.ndarray):

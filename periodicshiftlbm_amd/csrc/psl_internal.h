@@ -195,6 +195,9 @@ int stl_mark_keys(const psl_stl* s, float gox, float goy, float goz, float res, 
 // implemented in psl_esotwist.hip: rho, ux, uy, uz per cell (4 x n floats,
 // device) on the grid's stream
 int diad_macro_device(psl_diad* d, float* out4n);
+// implemented in psl_scalar.hip: the scalar value of every cell (n floats,
+// device) on the grid's stream
+int scalar_field_device(psl_grid* g, float* out);
 // implemented in psl_grid.hip
 int grid_rebuild_flags(psl_grid* g);
 int ensure_partials(psl_grid* g, size_t n);

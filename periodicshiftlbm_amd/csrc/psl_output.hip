@@ -42,8 +42,28 @@ __device__ __forceinline__ void plane_components(int plane, float ux, float uy, 
 }
 
 // exportSectionCutPlotGeneral(GridStruct&) (lines 2-89), one thread per pixel
+// export3DPlot (lines 591-672): every cell in (i, j, k) order with k fastest,
+// five floats (p, ux, uy, uz in m/s, bounceback marker)
+__global__ void k_volume(const float* __restrict__ macro, const uint8_t* __restrict__ flags, int nx, int ny, int nz,
+                         float vel_scale, float p_scale, float* __restrict__ out) {
+  const size_t n = (size_t)nx * ny * nz;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n; o += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(o % nz);
+    const int j = (int)((o / nz) % ny);
+    const int i = (int)(o / ((size_t)nz * ny));
+    const size_t cell = ((size_t)k * ny + j) * nx + i;
+    float* d = out + 5 * o;
+    d[0] = (macro[cell] - 1.f) * p_scale;
+    d[1] = macro[n + cell] * vel_scale;
+    d[2] = macro[2 * n + cell] * vel_scale;
+    d[3] = macro[3 * n + cell] * vel_scale;
+    d[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK) ? 1.f : 0.f;
+  }
+}
+
+// With `scalar` (n floats) a sixth component, the scalar of the cell (lines 108-204).
 __global__ void k_cut_grid(const float* __restrict__ macro, const uint8_t* __restrict__ flags, int nx, int ny,
-                           size_t n, CutParams c, float* __restrict__ out) {
+                           size_t n, CutParams c, float* __restrict__ out, const float* __restrict__ scalar) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y;
   if (x >= c.h || y >= c.v) return;
@@ -56,12 +76,14 @@ __global__ void k_cut_grid(const float* __restrict__ macro, const uint8_t* __res
               uz = macro[3 * n + cell] * c.vel_scale;
   float uh, uv, un;
   plane_components(c.plane, ux, uy, uz, uh, uv, un);
-  float* o = out + 5 * ((size_t)y * c.h + x);
+  const int comps = scalar ? 6 : 5;
+  float* o = out + comps * ((size_t)y * c.h + x);
   o[0] = (rho - 1.f) * c.p_scale;
   o[1] = uh;
   o[2] = uv;
   o[3] = un;
   o[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK) ? 1.f : 0.f;
+  if (scalar) o[5] = scalar[cell];
 }
 
 // exportSectionCutPlotGeneral(std::vector<DIADGridStruct>&) (lines 377-573):
@@ -140,9 +162,10 @@ using namespace psl;
 
 extern "C" {
 
-int psl_section_cut(psl_grid* g, int plane, int cut, float rho_nominal, float soundspeed, float* host_out,
-                    int* dims) {
+static int grid_cut(psl_grid* g, int plane, int cut, float rho_nominal, float soundspeed, float* host_out, int* dims,
+                    bool with_scalar) {
   PSL_CHECK_ARG(g && dims && plane >= 0 && plane <= 2, "bad section-cut arguments");
+  PSL_CHECK_ARG(!with_scalar || g->scalar_T, "scalar transport not enabled");
   const psl_info& I = g->info;
   int h, v;
   plane_dims(plane, I.cellCountX, I.cellCountY, I.cellCountZ, h, v);
@@ -153,21 +176,24 @@ int psl_section_cut(psl_grid* g, int plane, int cut, float rho_nominal, float so
   dims[1] = h;
   if (!host_out) return PSL_OK;
   const size_t n = (size_t)I.cellCount;
+  const int comps = with_scalar ? 6 : 5;
   float* macro = nullptr;
   float* out = nullptr;
   int rc = PSL_OK;
-  if (hipMalloc(&macro, 4 * n * sizeof(float)) != hipSuccess ||
-      hipMalloc(&out, 5 * (size_t)h * v * sizeof(float)) != hipSuccess) {
+  if (hipMalloc(&macro, (with_scalar ? 5 : 4) * n * sizeof(float)) != hipSuccess ||
+      hipMalloc(&out, comps * (size_t)h * v * sizeof(float)) != hipSuccess) {
     set_error("section cut: allocation failed");
     rc = PSL_E_NOMEM;
   }
   if (rc == PSL_OK) rc = psl_get_macroscopic_device(g, macro);
+  if (rc == PSL_OK && with_scalar) rc = scalar_field_device(g, macro + 4 * n);
   if (rc == PSL_OK) {
     CutParams c{plane, cut, h, v, (I.res / 1000.f) / I.dtPhys, rho_nominal * soundspeed * soundspeed};
     dim3 grid((h + 255) / 256, v);
-    k_cut_grid<<<grid, 256, 0, g->stream>>>(macro, g->flags, I.cellCountX, I.cellCountY, n, c, out);
+    k_cut_grid<<<grid, 256, 0, g->stream>>>(macro, g->flags, I.cellCountX, I.cellCountY, n, c, out,
+                                            with_scalar ? macro + 4 * n : nullptr);
     if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(host_out, out, 5 * (size_t)h * v * sizeof(float), hipMemcpyDeviceToHost, g->stream) !=
+        hipMemcpyAsync(host_out, out, comps * (size_t)h * v * sizeof(float), hipMemcpyDeviceToHost, g->stream) !=
             hipSuccess ||
         hipStreamSynchronize(g->stream) != hipSuccess) {
       set_error("section cut: kernel or copy failed");
@@ -177,6 +203,45 @@ int psl_section_cut(psl_grid* g, int plane, int cut, float rho_nominal, float so
   if (macro) (void)hipFree(macro);
   if (out) (void)hipFree(out);
   return rc;
+}
+
+int psl_volume_export(psl_grid* g, float rho_nominal, float soundspeed, float* host_out) {
+  PSL_CHECK_ARG(g && host_out, "null argument");
+  const psl_info& I = g->info;
+  PSL_CHECK_ARG(I.dtPhys > 0.f, "dtPhys must be positive for physical velocities");
+  const size_t n = (size_t)I.cellCount;
+  float* macro = nullptr;
+  float* out = nullptr;
+  int rc = PSL_OK;
+  if (hipMalloc(&macro, 4 * n * sizeof(float)) != hipSuccess || hipMalloc(&out, 5 * n * sizeof(float)) != hipSuccess) {
+    set_error("volume export: allocation failed");
+    rc = PSL_E_NOMEM;
+  }
+  if (rc == PSL_OK) rc = psl_get_macroscopic_device(g, macro);
+  if (rc == PSL_OK) {
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 1u << 20);
+    k_volume<<<nb, 256, 0, g->stream>>>(macro, g->flags, I.cellCountX, I.cellCountY, I.cellCountZ,
+                                        (I.res / 1000.f) / I.dtPhys, rho_nominal * soundspeed * soundspeed, out);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(host_out, out, 5 * n * sizeof(float), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess) {
+      set_error("volume export: kernel or copy failed");
+      rc = PSL_E_HIP;
+    }
+  }
+  if (macro) (void)hipFree(macro);
+  if (out) (void)hipFree(out);
+  return rc;
+}
+
+int psl_section_cut(psl_grid* g, int plane, int cut, float rho_nominal, float soundspeed, float* host_out,
+                    int* dims) {
+  return grid_cut(g, plane, cut, rho_nominal, soundspeed, host_out, dims, false);
+}
+
+int psl_scalar_section_cut(psl_grid* g, int plane, int cut, float rho_nominal, float soundspeed, float* host_out,
+                           int* dims) {
+  return grid_cut(g, plane, cut, rho_nominal, soundspeed, host_out, dims, true);
 }
 
 }  // extern "C"
@@ -334,6 +399,27 @@ int psl_write_sim_data(const char* path, int plot_number, int rows, int cols, in
   const int header[4] = {plot_number, rows, cols, components};
   const size_t count = (size_t)rows * cols * components;
   const bool ok = std::fwrite(header, sizeof(int), 4, fp) == 4 && std::fwrite(data, sizeof(float), count, fp) == count;
+  std::fclose(fp);
+  if (!ok) {
+    set_error(std::string("short write to ") + path);
+    return PSL_E_IO;
+  }
+  return PSL_OK;
+}
+
+// The 3-D sim_data.bin of export3DPlot (exportSectionCutPlot.h:640-670): int
+// plot number, nx, ny, nz, components, then the floats.
+int psl_write_volume_data(const char* path, int plot_number, int nx, int ny, int nz, int components,
+                          const float* data) {
+  PSL_CHECK_ARG(path && data && nx > 0 && ny > 0 && nz > 0 && components > 0, "bad volume arguments");
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) {
+    set_error(std::string("cannot open ") + path);
+    return PSL_E_IO;
+  }
+  const int header[5] = {plot_number, nx, ny, nz, components};
+  const size_t count = (size_t)nx * ny * nz * components;
+  const bool ok = std::fwrite(header, sizeof(int), 5, fp) == 5 && std::fwrite(data, sizeof(float), count, fp) == count;
   std::fclose(fp);
   if (!ok) {
     set_error(std::string("short write to ") + path);

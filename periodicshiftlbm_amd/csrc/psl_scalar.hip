@@ -101,7 +101,36 @@ __global__ __launch_bounds__(256) void k_scalar_update(ScalarParams p) {
   }
 }
 
+// getScalarTransport (include/cellFunctions.h:228-235): sum f T / sum f per cell
+__global__ __launch_bounds__(256) void k_scalar_field(const float* __restrict__ f, const float* __restrict__ T,
+                                                      Shift27 sh, uint32_t n, float* __restrict__ out) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= n) return;
+  float fsum = 0.f, ft = 0.f;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    uint32_t s = cell + (uint32_t)sh.s[q];
+    if (s >= n) s -= n;
+    const float a = f[(size_t)q * n + s];
+    fsum += a;
+    ft += a * T[(size_t)q * n + s];
+  }
+  out[cell] = ft / fsum;
+}
+
 }  // namespace
+
+// the scalar value of every cell (n floats, device) on the grid's stream
+int scalar_field_device(psl_grid* g, float* out) {
+  PSL_CHECK_ARG(g && out && g->scalar_T, "scalar transport not enabled");
+  PSL_CHECK_ARG(g->storage == PSL_F32, "scalar transport needs FP32 distributions");
+  Shift27 sh;
+  for (int q = 0; q < 27; ++q) sh.s[q] = g->shift[q];
+  const uint32_t n = (uint32_t)g->info.cellCount;
+  k_scalar_field<<<(n + 255) / 256, 256, 0, g->stream>>>((const float*)g->f, g->scalar_T, sh, n, out);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
 }  // namespace psl
 
 using namespace psl;

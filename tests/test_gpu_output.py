@@ -62,6 +62,14 @@ def test_section_cut_grid(psl):
         for a, want in ((1, uh), (2, uv), (3, un)):
             assert np.max(np.abs(got[..., a] - want)) < 1e-5 * vel
         assert np.array_equal(got[..., 4], marker)
+    # export3DPlot: every cell, (i, j, k) order with k fastest
+    vol = P.export3DPlot(g, 3, path=None, rho_nominal=RHO0, soundspeed=CS)
+    assert vol.shape == (nx, ny, nz, 5)
+    order = cell.transpose(2, 1, 0)  # [i, j, k] -> cell
+    assert np.max(np.abs(vol[..., 0] - (rho[order] - 1.0) * RHO0 * CS * CS)) < 1e-5 * RHO0 * CS * CS
+    for a in range(3):
+        assert np.max(np.abs(vol[..., 1 + a] - u[a][order] * vel)) < 1e-5 * vel
+    assert np.array_equal(vol[..., 4], ((flags[order] & 7) == cases.BOUNCEBACK).astype(np.float32))
     g.close()
 
 
@@ -153,3 +161,46 @@ def test_flow_report_diad_levels(psl):
     for key in ("ux", "uy", "uz", "rho"):
         assert abs(rep[key] - want[key]) < 1e-6, key
     assert rep["areamm2"] == pytest.approx(want["areamm2"], rel=1e-12)
+
+
+def test_section_cut_scalar_transport(psl, tmp_path):
+    """The scalar-transport cut: the five components of the plain cut and, as a
+    sixth, sum f T / sum f of each cell, against the host from the grid's own
+    f and T (1e-5 relative); written as six components for the scalar plotter."""
+    P = psl
+    shape = nx, ny, nz = (48, 24, 24)
+    k3, j3, i3 = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    solid = ((i3 - 12.0) ** 2 + (j3 - 11.5) ** 2 + (k3 - 11.5) ** 2 <= 16.0).reshape(-1)
+    flags = cases.channel_flags(shape, solid)
+    info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=0.02, res=2.0, dtPhys=0.5)
+    g = P.GridStruct(info, "f32")
+    g.set_flags(flags)
+    g.set_given_values(cases.channel_given_values(0.04))
+    P.fill_equilibrium(g, ux=np.full(g.n, 0.04, np.float32))
+    st = P.ScalarTransportStruct(g, 0.7)
+    given = np.full(g.n, np.nan, np.float32)
+    given[i3.reshape(-1) == 0] = 1.0
+    st.set_given(given)
+    for _ in range(15):
+        P.applyStreaming(g)
+        P.applyScalarTransportLocalCellUpdate(g, st)
+    f = g.get_f().astype(np.float64)
+    T = st.get_T().astype(np.float64)
+    conc = (f * T).sum(0) / f.sum(0)
+    assert conc.max() > 0.5
+    cell = (k3 * ny + j3) * nx + i3
+    for plane, cut in (("XY", 11), ("ZY", 3), ("ZX", 12)):
+        got = P.section_cut(g, plane, cut, RHO0, CS, scalar_transport=True)
+        plain = P.section_cut(g, plane, cut, RHO0, CS)
+        sel = {"XY": cell[cut, :, :], "ZY": cell[:, :, cut].T, "ZX": cell[:, cut, :].T}[plane]
+        print(f"[cut] scalar {plane} {cut}: {got.shape}, max scalar {got[..., 5].max():.3f}", flush=True)
+        assert got.shape == sel.shape + (6,)
+        assert np.array_equal(got[..., :5], plain)
+        assert np.max(np.abs(got[..., 5] - conc[sel])) < 1e-5 * max(1.0, np.abs(conc).max())
+    path = str(tmp_path / "sim_data.bin")
+    data = P.exportSectionCutPlot(g, "ZY", 3, 7, path=path, rho_nominal=RHO0, soundspeed=CS, scalar_transport=True)
+    with open(path, "rb") as fh:
+        hdr = np.fromfile(fh, dtype=np.int32, count=4)
+        body = np.fromfile(fh, dtype=np.float32).reshape((hdr[1], hdr[2], hdr[3]))
+    assert list(hdr) == [7, ny, nz, 6] and np.array_equal(body, data)
+    g.close()

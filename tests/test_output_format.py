@@ -42,3 +42,19 @@ def test_writer_reports_bad_path():
         assert "cannot open" in str(e)
     else:
         raise AssertionError("no error for an unwritable path")
+
+
+def test_volume_data_round_trip(tmp_path):
+    """export3DPlot's file (exportSectionCutPlot.h:640-670): int32 plot number,
+    nx, ny, nz, components, then the cells in (i, j, k) order, k fastest."""
+    rng = np.random.default_rng(1)
+    data = rng.standard_normal((5, 4, 3, 5)).astype(np.float32)
+    path = str(tmp_path / "sim_data.bin")
+    P._check(P.lib().psl_write_volume_data(path.encode(), 9, 5, 4, 3, 5, data.ctypes.data))
+    with open(path, "rb") as f:
+        hdr = np.fromfile(f, dtype=np.int32, count=5)
+        back = np.fromfile(f, dtype=np.float32)
+    assert list(hdr) == [9, 5, 4, 3, 5] and back.size == data.size
+    # the reference's writer loops i, then j, then k innermost, 5 floats per cell
+    assert np.array_equal(back.reshape(5, 4, 3, 5)[2, 1, 0], data[2, 1, 0])
+    assert np.array_equal(back, data.reshape(-1))
