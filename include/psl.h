@@ -450,6 +450,14 @@ int psl_diad_flow_report(psl_diad* const* levels, int nlevels, int plane, int cu
 /* /dev/shm/sim_data.bin format of the reference's plotters (exportSectionCutPlot.h:64-86):
  * int {plot_number, rows, cols, components}, then the floats. */
 int psl_write_sim_data(const char* path, int plot_number, int rows, int cols, int components, const float* data);
+/* exportSectionCutPlotToiletPaperZ(GridStruct&) (include/plotter/exportSectionCutPlot.h:674-787):
+ * the cylinder of radius r (mm) about the z axis unrolled; dims[0] =
+ * (int)(2 pi r / res) rows along the circumference (row v at angle v res / r,
+ * the cell of getIJKCellIndexFromXYZ), dims[1] = cellCountZ columns; 5 floats
+ * per pixel (p, uz, tangential and radial velocity in m/s, bounceback
+ * marker).  host_out = NULL only reports dims.  A radius whose circle leaves
+ * the grid is rejected (the reference reads outside its arrays there). */
+int psl_cylinder_cut(psl_grid* grid, float r, float rho_nominal, float soundspeed, float* host_out, int* dims);
 /* export3DPlot(GridStruct&) (include/plotter/exportSectionCutPlot.h:591-672): every
  * cell in (i, j, k) order with k fastest, 5 floats each (p, ux, uy, uz in m/s
  * as in psl_section_cut, bounceback marker); host_out holds 5 N floats. */

@@ -221,6 +221,7 @@ _SIGS = [
     ("psl_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
     ("psl_scalar_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
     ("psl_volume_export", _I, [_P, ctypes.c_float, ctypes.c_float, _P]),
+    ("psl_cylinder_cut", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P]),
     ("psl_write_volume_data", _I, [ctypes.c_char_p, _I, _I, _I, _I, _I, _P]),
     ("psl_diad_section_cut", _I, [_P, _I, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
     ("psl_scalar_enable", _I, [_P, ctypes.c_float]),
@@ -804,6 +805,20 @@ def exportSectionCutPlot(grid, plane: str, cut: int, plot_number: int, path: str
     data = section_cut(grid, plane, cut, rho_nominal, soundspeed, scalar_transport)
     write_sim_data(path, plot_number, data)
     return data
+
+
+def exportSectionCutPlotToiletPaperZ(grid: GridStruct, r: float, plot_number: int,
+                                     path: Optional[str] = "/dev/shm/sim_data.bin", rho_nominal: float = 1.225,
+                                     soundspeed: float = 1.0) -> np.ndarray:
+    """The cylinder of radius r (mm) about z unrolled: rows x cellCountZ x 5 of
+    p, uz, u_tangential, u_radial (m/s) and the bounceback marker."""
+    dims = np.zeros(2, np.int32)
+    _check(lib().psl_cylinder_cut(grid.handle, r, rho_nominal, soundspeed, None, dims.ctypes.data))
+    out = np.zeros((dims[0], dims[1], 5), np.float32)
+    _check(lib().psl_cylinder_cut(grid.handle, r, rho_nominal, soundspeed, out.ctypes.data, dims.ctypes.data))
+    if path is not None:
+        write_sim_data(path, plot_number, out)
+    return out
 
 
 def export3DPlot(grid: GridStruct, plot_number: int, path: Optional[str] = "/dev/shm/sim_data.bin",

@@ -61,6 +61,29 @@ __global__ void k_volume(const float* __restrict__ macro, const uint8_t* __restr
   }
 }
 
+// exportSectionCutPlotToiletPaperZ(GridStruct&) (lines 674-787): the cylinder of
+// radius r about the z axis unrolled, rows along the circumference (row v at
+// angle v res / r, cell column col[v] = j nx + i and cos / sin of that angle
+// from the host), columns along k; five floats (p, uz, u_tangential,
+// u_radial, bounceback marker).
+__global__ void k_cut_cyl(const float* __restrict__ macro, const uint8_t* __restrict__ flags, size_t n, size_t plane,
+                          int nz, const int* __restrict__ col, const float2* __restrict__ rot, int rows,
+                          float vel_scale, float p_scale, float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int v = blockIdx.y;
+  if (k >= nz || v >= rows) return;
+  const size_t cell = (size_t)k * plane + (size_t)col[v];
+  const float ux = macro[n + cell] * vel_scale, uy = macro[2 * n + cell] * vel_scale,
+              uz = macro[3 * n + cell] * vel_scale;
+  const float c = rot[v].x, s = rot[v].y;
+  float* o = out + 5 * ((size_t)v * nz + k);
+  o[0] = (macro[cell] - 1.f) * p_scale;
+  o[1] = uz;
+  o[2] = -ux * s + uy * c;
+  o[3] = ux * c + uy * s;
+  o[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK) ? 1.f : 0.f;
+}
+
 // With `scalar` (n floats) a sixth component, the scalar of the cell (lines 108-204).
 __global__ void k_cut_grid(const float* __restrict__ macro, const uint8_t* __restrict__ flags, int nx, int ny,
                            size_t n, CutParams c, float* __restrict__ out, const float* __restrict__ scalar) {
@@ -231,6 +254,65 @@ int psl_volume_export(psl_grid* g, float rho_nominal, float soundspeed, float* h
   }
   if (macro) (void)hipFree(macro);
   if (out) (void)hipFree(out);
+  return rc;
+}
+
+int psl_cylinder_cut(psl_grid* g, float r, float rho_nominal, float soundspeed, float* host_out, int* dims) {
+  PSL_CHECK_ARG(g && dims && r > 0.f, "bad cylinder-cut arguments");
+  const psl_info& I = g->info;
+  PSL_CHECK_ARG(I.dtPhys > 0.f, "dtPhys must be positive for physical velocities");
+  const float PI = 3.14159265359f;
+  const int rows = (int)((2.0f * PI * r) / I.res);
+  PSL_CHECK_ARG(rows > 0, "radius below one cell");
+  dims[0] = rows;
+  dims[1] = I.cellCountZ;
+  // pixel rows -> cell columns, getIJKCellIndexFromXYZ (include/cellFunctions.h:32-37)
+  std::vector<int> col((size_t)rows);
+  std::vector<float2> rot((size_t)rows);
+  for (int v = 0; v < rows; ++v) {
+    const float theta = ((float)v * I.res) / r;
+    const float x = r * cosf(theta), y = r * sinf(theta);
+    const float fi = (x - I.ox) / I.res + 0.5f, fj = (y - I.oy) / I.res + 0.5f;
+    const int i = (int)fi, j = (int)fj;
+    PSL_CHECK_ARG(fi > -1.f && fj > -1.f && i < I.cellCountX && j < I.cellCountY, "cylinder leaves the grid");
+    col[v] = j * I.cellCountX + i;
+    rot[v] = make_float2(cosf(theta), sinf(theta));
+  }
+  if (!host_out) return PSL_OK;
+  const size_t n = (size_t)I.cellCount;
+  float* macro = nullptr;
+  float* out = nullptr;
+  int* dcol = nullptr;
+  float2* drot = nullptr;
+  int rc = PSL_OK;
+  if (hipMalloc(&macro, 4 * n * sizeof(float)) != hipSuccess ||
+      hipMalloc(&out, 5 * (size_t)rows * I.cellCountZ * sizeof(float)) != hipSuccess ||
+      hipMalloc(&dcol, (size_t)rows * sizeof(int)) != hipSuccess ||
+      hipMalloc(&drot, (size_t)rows * sizeof(float2)) != hipSuccess) {
+    set_error("cylinder cut: allocation failed");
+    rc = PSL_E_NOMEM;
+  }
+  if (rc == PSL_OK && (hipMemcpy(dcol, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                       hipMemcpy(drot, rot.data(), rot.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)) {
+    set_error("cylinder cut: upload failed");
+    rc = PSL_E_HIP;
+  }
+  if (rc == PSL_OK) rc = psl_get_macroscopic_device(g, macro);
+  if (rc == PSL_OK) {
+    dim3 grid((I.cellCountZ + 255) / 256, rows);
+    k_cut_cyl<<<grid, 256, 0, g->stream>>>(macro, g->flags, n, (size_t)I.cellCountX * I.cellCountY, I.cellCountZ,
+                                           dcol, drot, rows, (I.res / 1000.f) / I.dtPhys,
+                                           rho_nominal * soundspeed * soundspeed, out);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(host_out, out, 5 * (size_t)rows * I.cellCountZ * sizeof(float), hipMemcpyDeviceToHost,
+                       g->stream) != hipSuccess ||
+        hipStreamSynchronize(g->stream) != hipSuccess) {
+      set_error("cylinder cut: kernel or copy failed");
+      rc = PSL_E_HIP;
+    }
+  }
+  for (void* a : {(void*)macro, (void*)out, (void*)dcol, (void*)drot})
+    if (a) (void)hipFree(a);
   return rc;
 }
 

@@ -204,3 +204,47 @@ def test_section_cut_scalar_transport(psl, tmp_path):
         body = np.fromfile(fh, dtype=np.float32).reshape((hdr[1], hdr[2], hdr[3]))
     assert list(hdr) == [7, ny, nz, 6] and np.array_equal(body, data)
     g.close()
+
+
+def test_toilet_paper_cut(psl):
+    """The unrolled cylinder about z: row v at angle v res / r reads the cell
+    getIJKCellIndexFromXYZ names, column k; p, uz, the tangential and radial
+    velocity and the marker against host moments of the grid's populations.
+    A radius whose circle leaves the grid is rejected."""
+    P = psl
+    nx, ny, nz = 40, 36, 12
+    res = 2.0
+    k3, j3, i3 = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x3, y3 = i3 * res - 39.0, j3 * res - 35.0
+    solid = ((x3 - 8.0) ** 2 + y3 ** 2 <= 30.0).reshape(-1)
+    flags = cases.channel_flags((nx, ny, nz), solid)
+    info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=0.02, res=res, dtPhys=0.5,
+                        ox=-39.0, oy=-35.0, oz=0.0)
+    g = P.GridStruct(info, "f32")
+    g.set_flags(flags)
+    g.set_given_values(cases.channel_given_values(0.04))
+    P.fill_equilibrium(g, ux=np.full(g.n, 0.04, np.float32))
+    P.updateGrid([g], 10)
+    rho, u = _moments(g.get_f())
+    vel = (res / 1000.0) / info.dtPhys
+    for r in (12.5, 21.0):
+        got = P.exportSectionCutPlotToiletPaperZ(g, r, 1, path=None, rho_nominal=RHO0, soundspeed=CS)
+        rows = int(np.float32(2.0 * np.float32(3.14159265359) * np.float32(r)) / np.float32(res))
+        assert got.shape == (rows, nz, 5)
+        v = np.arange(rows, dtype=np.float32)
+        th = (v * np.float32(res)) / np.float32(r)
+        x, y = np.float32(r) * np.cos(th), np.float32(r) * np.sin(th)
+        i = ((x - np.float32(-39.0)) / np.float32(res) + np.float32(0.5)).astype(np.int64)
+        j = ((y - np.float32(-35.0)) / np.float32(res) + np.float32(0.5)).astype(np.int64)
+        cell = np.arange(nz)[None, :] * nx * ny + (j * nx + i)[:, None]
+        c, s = np.cos(th)[:, None], np.sin(th)[:, None]
+        ux, uy, uz = (u[a][cell] * vel for a in range(3))
+        print(f"[cut] toilet paper r = {r}: {got.shape}, marker pixels {int(got[..., 4].sum())}", flush=True)
+        assert np.max(np.abs(got[..., 0] - (rho[cell] - 1.0) * RHO0 * CS * CS)) < 1e-5 * RHO0 * CS * CS
+        assert np.max(np.abs(got[..., 1] - uz)) < 1e-5 * vel
+        assert np.max(np.abs(got[..., 2] - (-ux * s + uy * c))) < 2e-5 * vel
+        assert np.max(np.abs(got[..., 3] - (ux * c + uy * s))) < 2e-5 * vel
+        assert np.array_equal(got[..., 4], ((flags[cell] & 7) == cases.BOUNCEBACK).astype(np.float32))
+    with pytest.raises(P.PslError):
+        P.exportSectionCutPlotToiletPaperZ(g, 60.0, 1, path=None)
+    g.close()
