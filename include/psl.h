@@ -458,6 +458,16 @@ int psl_write_sim_data(const char* path, int plot_number, int rows, int cols, in
  * marker).  host_out = NULL only reports dims.  A radius whose circle leaves
  * the grid is rejected (the reference reads outside its arrays there). */
 int psl_cylinder_cut(psl_grid* grid, float r, float rho_nominal, float soundspeed, float* host_out, int* dims);
+/* The same over DIAD levels (exportSectionCutPlotToiletPaperZ(std::vector<DIADGridStruct>&),
+ * lines 789-993): the finest level whose unrolled plane has fewer than
+ * max_pixels pixels sets the pixel size (dims3 = rows, cols, levels used);
+ * cells within 1.5 half-widths of the cylinder paint their footprint at their
+ * polar angle, rows wrapping round the circumference, coarse to fine, the
+ * largest cell index winning where cells of one level share a pixel (the
+ * reference leaves that to the order of its writes); 6 floats per pixel (the
+ * five above and the level). */
+int psl_diad_cylinder_cut(psl_diad* const* levels, int nlevels, float r, float rho_nominal, float soundspeed,
+                          int64_t max_pixels, float* host_out, int* dims3);
 /* export3DPlot(GridStruct&) (include/plotter/exportSectionCutPlot.h:591-672): every
  * cell in (i, j, k) order with k fastest, 5 floats each (p, ux, uy, uz in m/s
  * as in psl_section_cut, bounceback marker); host_out holds 5 N floats. */

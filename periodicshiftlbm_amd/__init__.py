@@ -222,6 +222,7 @@ _SIGS = [
     ("psl_scalar_section_cut", _I, [_P, _I, _I, ctypes.c_float, ctypes.c_float, _P, _P]),
     ("psl_volume_export", _I, [_P, ctypes.c_float, ctypes.c_float, _P]),
     ("psl_cylinder_cut", _I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P]),
+    ("psl_diad_cylinder_cut", _I, [_P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
     ("psl_write_volume_data", _I, [ctypes.c_char_p, _I, _I, _I, _I, _I, _P]),
     ("psl_diad_section_cut", _I, [_P, _I, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_int64, _P, _P]),
     ("psl_scalar_enable", _I, [_P, ctypes.c_float]),
@@ -811,7 +812,19 @@ def exportSectionCutPlotToiletPaperZ(grid: GridStruct, r: float, plot_number: in
                                      path: Optional[str] = "/dev/shm/sim_data.bin", rho_nominal: float = 1.225,
                                      soundspeed: float = 1.0) -> np.ndarray:
     """The cylinder of radius r (mm) about z unrolled: rows x cellCountZ x 5 of
-    p, uz, u_tangential, u_radial (m/s) and the bounceback marker."""
+    p, uz, u_tangential, u_radial (m/s) and the bounceback marker; over a list
+    of DIAD levels rows x cols x 6, the last component the level."""
+    if isinstance(grid, (list, tuple)):
+        arr = (ctypes.c_void_p * len(grid))(*[g.handle.value for g in grid])
+        dims = np.zeros(3, np.int32)
+        _check(lib().psl_diad_cylinder_cut(arr, len(grid), r, rho_nominal, soundspeed, 20000000, None,
+                                           dims.ctypes.data))
+        out = np.zeros((dims[0], dims[1], 6), np.float32)
+        _check(lib().psl_diad_cylinder_cut(arr, len(grid), r, rho_nominal, soundspeed, 20000000, out.ctypes.data,
+                                           dims.ctypes.data))
+        if path is not None:
+            write_sim_data(path, plot_number, out)
+        return out
     dims = np.zeros(2, np.int32)
     _check(lib().psl_cylinder_cut(grid.handle, r, rho_nominal, soundspeed, None, dims.ctypes.data))
     out = np.zeros((dims[0], dims[1], 5), np.float32)

@@ -32,6 +32,11 @@ struct CutParams {
   float p_scale;    // rho_nominal * c_s^2
   int h0 = 0, v0 = 0;  // first pixel of a cropped window
   int raw = 0;         // 1: rho, ux, uy, uz in lattice units (flow reports)
+  // unrolled cylinder about z (plane ignored): radius r (mm), the finest
+  // level's spacing and the finest-level indices of the origin
+  int cyl = 0;
+  float r = 0.f, fres = 0.f;
+  int io = 0, jo = 0;
 };
 
 __device__ __forceinline__ void plane_components(int plane, float ux, float uy, float uz, float& uh, float& uv,
@@ -137,6 +142,39 @@ __global__ void k_cut_claim(const int* __restrict__ ijk, int64_t n, int cell_sca
   }
 }
 
+// exportSectionCutPlotToiletPaperZ(std::vector<DIADGridStruct>&) (lines 789-993):
+// the cells of one level near the cylinder of radius r claim the pixels of
+// their unrolled footprint (rows wrap around the circumference; largest cell
+// index wins), positions on the finest level's lattice.
+__global__ void k_cyl_claim(const int* __restrict__ ijk, int64_t n, int cell_scale, int target_scale, CutParams c,
+                            int* __restrict__ winner) {
+  const float PI = 3.14159265359f;
+  for (int64_t cell = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; cell < n;
+       cell += (int64_t)gridDim.x * blockDim.x) {
+    const int i = ijk[cell] * cell_scale, j = ijk[n + cell] * cell_scale, k = ijk[2 * n + cell] * cell_scale;
+    const float xc = ((float)i + 0.5f * cell_scale - (float)c.io) * c.fres;
+    const float yc = ((float)j + 0.5f * cell_scale - (float)c.jo) * c.fres;
+    const float dist = sqrtf(xc * xc + yc * yc);
+    const float half = cell_scale * c.fres * 0.5f;
+    if (dist < c.r - half * 1.5f || dist > c.r + half * 1.5f) continue;
+    float theta = atan2f(yc, xc);
+    if (theta < 0.f) theta += 2.0f * PI;
+    const int fv = (int)(theta * c.r / c.fres);
+    const int span = max(1, cell_scale / target_scale);
+    const int y0 = (fv - (span * target_scale) / 2) / target_scale;  // C division, towards zero
+    const int x0 = k / target_scale;
+    for (int dy = 0; dy < span; ++dy) {
+      int y = (y0 + dy) % c.v;
+      if (y < 0) y += c.v;
+      for (int dx = 0; dx < span; ++dx) {
+        const int x = x0 + dx;
+        if (x >= c.h) continue;
+        atomicMax(&winner[(size_t)y * c.h + x], (int)cell);
+      }
+    }
+  }
+}
+
 // the claimed pixels take the winning cell's fields (converted with its level)
 __global__ void k_cut_paint(const int* __restrict__ winner, const float* __restrict__ macro,
                             const uint8_t* __restrict__ flags, int64_t n, CutParams c, int level,
@@ -148,7 +186,15 @@ __global__ void k_cut_paint(const int* __restrict__ winner, const float* __restr
   const float rho = macro[cell], ux = macro[n + cell] * c.vel_scale, uy = macro[2 * n + cell] * c.vel_scale,
               uz = macro[3 * n + cell] * c.vel_scale;
   float* o = out + 6 * pix;
-  if (c.raw) {
+  if (c.cyl) {
+    // p, uz, tangential and radial velocity at the row's angle (lines 971-978)
+    const float theta = ((float)(pix / c.h) * (float)c.cut * c.fres) / c.r;
+    const float sn = sinf(theta), cs = cosf(theta);
+    o[0] = (rho - 1.f) * c.p_scale;
+    o[1] = uz;
+    o[2] = -ux * sn + uy * cs;
+    o[3] = ux * cs + uy * sn;
+  } else if (c.raw) {
     o[0] = rho;
     o[1] = ux;
     o[2] = uy;
@@ -365,7 +411,8 @@ int paint_levels(psl_diad* const* levels, int nlevels, const CutParams& base, in
       const int cell_scale = 1 << (nlevels - l - 1);
       if (hipMemsetAsync(winner, 0xff, pixels * sizeof(int), st) != hipSuccess) rc = PSL_E_HIP;
       const unsigned nb = (unsigned)std::min<int64_t>((d->n + 255) / 256, 1 << 20);
-      k_cut_claim<<<nb, 256, 0, st>>>(d->ijk, d->n, cell_scale, target_scale, c, winner);
+      if (c.cyl) k_cyl_claim<<<nb, 256, 0, st>>>(d->ijk, d->n, cell_scale, target_scale, c, winner);
+      else k_cut_claim<<<nb, 256, 0, st>>>(d->ijk, d->n, cell_scale, target_scale, c, winner);
       k_cut_paint<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(winner, macro, d->flags, d->n, c, l, out);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = PSL_E_HIP;
     }
@@ -403,6 +450,39 @@ int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cu
   dims3[2] = target;
   if (!host_out) return PSL_OK;
   CutParams c{plane, cut, h, v, 1.f, rho_nominal * soundspeed * soundspeed};
+  return paint_levels(levels, nlevels, c, 1 << (nlevels - target), host_out);
+}
+
+// exportSectionCutPlotToiletPaperZ over DIAD levels (lines 789-993); the target
+// scale rides in CutParams.cut for the paint kernel's row angles.
+int psl_diad_cylinder_cut(psl_diad* const* levels, int nlevels, float r, float rho_nominal, float soundspeed,
+                          int64_t max_pixels, float* host_out, int* dims3) {
+  PSL_CHECK_ARG(levels && nlevels >= 1 && dims3 && r > 0.f, "bad cylinder-cut arguments");
+  for (int l = 0; l < nlevels; ++l) PSL_CHECK_ARG(levels[l] && levels[l]->info.dtPhys > 0.f, "bad level");
+  const float PI = 3.14159265359f;
+  int target = nlevels, h = 0, v = 0;
+  while (target > 1) {
+    const psl_info& I = levels[target - 1]->info;
+    h = I.cellCountZ;
+    v = (int)((2.0f * PI * r) / I.res);
+    if ((int64_t)h * v < max_pixels) break;
+    --target;
+  }
+  const psl_info& T = levels[target - 1]->info;
+  h = T.cellCountZ;
+  v = (int)((2.0f * PI * r) / T.res);
+  PSL_CHECK_ARG(v > 0, "radius below one cell");
+  dims3[0] = v;
+  dims3[1] = h;
+  dims3[2] = target;
+  if (!host_out) return PSL_OK;
+  const psl_info& F = levels[nlevels - 1]->info;
+  CutParams c{0, 1 << (nlevels - target), h, v, 1.f, rho_nominal * soundspeed * soundspeed};
+  c.cyl = 1;
+  c.r = r;
+  c.fres = F.res;
+  c.io = (int)((0.f - F.ox) / F.res + 0.5f);  // getIJKCellIndexFromXYZ(0, 0, 0) on the finest level
+  c.jo = (int)((0.f - F.oy) / F.res + 0.5f);
   return paint_levels(levels, nlevels, c, 1 << (nlevels - target), host_out);
 }
 
