@@ -176,9 +176,12 @@ __global__ void k_cyl_claim(const int* __restrict__ ijk, int64_t n, int cell_sca
 }
 
 // the claimed pixels take the winning cell's fields (converted with its level)
+// `moving` (n bytes, or null) marks the cells flagged bounceback here that the
+// reference's marker leaves open: moving walls, and forced cells in the plots
+// (its flow reports count forced cells as blocked, include/flowReportFunctions.h:163).
 __global__ void k_cut_paint(const int* __restrict__ winner, const float* __restrict__ macro,
-                            const uint8_t* __restrict__ flags, int64_t n, CutParams c, int level,
-                            float* __restrict__ out) {
+                            const uint8_t* __restrict__ flags, const uint8_t* __restrict__ moving, int64_t n,
+                            CutParams c, int level, float* __restrict__ out) {
   const size_t pix = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (pix >= (size_t)c.h * c.v) return;
   const int cell = winner[pix];
@@ -207,8 +210,13 @@ __global__ void k_cut_paint(const int* __restrict__ winner, const float* __restr
     o[2] = uv;
     o[3] = un;
   }
-  o[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK) ? 1.f : 0.f;
+  o[4] = (flags && (flags[cell] & 7) == KIND_BOUNCEBACK && !(moving && moving[cell])) ? 1.f : 0.f;
   o[5] = (float)level;
+}
+
+__global__ void k_mark_list(const int* __restrict__ cells, int64_t m, uint8_t* __restrict__ mask) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < m) mask[cells[t]] = 1;
 }
 
 __global__ void k_cut_fill(float* __restrict__ out, size_t pixels, int raw) {
@@ -397,7 +405,11 @@ int paint_levels(psl_diad* const* levels, int nlevels, const CutParams& base, in
     psl_diad* d = levels[l];
     const psl_info& I = d->info;
     float* macro = nullptr;
-    if (hipMalloc(&macro, 4 * (size_t)d->n * sizeof(float)) != hipSuccess) {
+    uint8_t* moving = nullptr;
+    const bool open_forced = !base.raw && d->n_forced;
+    if (hipMalloc(&macro, 4 * (size_t)d->n * sizeof(float)) != hipSuccess ||
+        ((d->n_moving || open_forced) && hipMalloc(&moving, (size_t)d->n) != hipSuccess)) {
+      if (macro) (void)hipFree(macro);
       set_error("section cut: allocation failed");
       rc = PSL_E_NOMEM;
       break;
@@ -410,13 +422,22 @@ int paint_levels(psl_diad* const* levels, int nlevels, const CutParams& base, in
       c.vel_scale = base.raw ? 1.f : (I.res / 1000.f) / I.dtPhys;
       const int cell_scale = 1 << (nlevels - l - 1);
       if (hipMemsetAsync(winner, 0xff, pixels * sizeof(int), st) != hipSuccess) rc = PSL_E_HIP;
+      if (moving) {
+        if (hipMemsetAsync(moving, 0, (size_t)d->n, st) != hipSuccess) rc = PSL_E_HIP;
+        if (d->n_moving)
+          k_mark_list<<<(unsigned)((d->n_moving + 255) / 256), 256, 0, st>>>(d->moving, d->n_moving, moving);
+        if (open_forced)
+          k_mark_list<<<(unsigned)((d->n_forced + 255) / 256), 256, 0, st>>>(d->forced, d->n_forced, moving);
+      }
       const unsigned nb = (unsigned)std::min<int64_t>((d->n + 255) / 256, 1 << 20);
       if (c.cyl) k_cyl_claim<<<nb, 256, 0, st>>>(d->ijk, d->n, cell_scale, target_scale, c, winner);
       else k_cut_claim<<<nb, 256, 0, st>>>(d->ijk, d->n, cell_scale, target_scale, c, winner);
-      k_cut_paint<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(winner, macro, d->flags, d->n, c, l, out);
+      k_cut_paint<<<(unsigned)((pixels + 255) / 256), 256, 0, st>>>(winner, macro, d->flags, moving, d->n, c, l,
+                                                                    out);
       if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) rc = PSL_E_HIP;
     }
     (void)hipFree(macro);
+    if (moving) (void)hipFree(moving);
     if (rc == PSL_E_HIP) set_error("section cut: kernel failed");
   }
   if (rc == PSL_OK && pixels &&

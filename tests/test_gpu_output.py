@@ -327,3 +327,55 @@ def test_toilet_paper_cut_diad_levels(psl):
         assert len(np.unique(got[..., 5])) >= 2
         assert (~same).sum() <= 0.005 * same.size
         assert np.max(np.abs(got[same][:, 1:4] - want[same][:, 1:4])) < 1e-5
+
+
+def test_cut_markers_of_moving_and_forced_cells(psl):
+    """Moving-wall and forced cells are flagged bounceback here; the reference's
+    plot marker is its bounceback marker alone, so both paint as open in the
+    cuts, while its flow reports count forced cells as blocked
+    (include/flowReportFunctions.h:163).  Against the host painter with the
+    flags the reference would see."""
+    P = psl
+    lv, grids = cases.diad_sphere(P, "f32", n0=16, levels=3, span=1, radius=2.0)
+    for g in grids:
+        g.fill_equilibrium(ux=0.04)
+    P.updateGrid(grids, 3)
+    cut = 32
+    rng = np.random.default_rng(6)
+    levels_plot, levels_report = [], []
+    for l, g in enumerate(grids):
+        li, _ = lv.info(l)
+        ijk = lv.array(l, "ijk")
+        flags = cases.diad_sphere_flags(ijk, lv.array(l, "bounceback"), (li.cellCountX, li.cellCountY, li.cellCountZ), l)
+        plot_flags, report_flags = flags.copy(), flags.copy()
+        if l == 2:
+            on = np.flatnonzero((ijk[2] == cut) & (flags == cases.FLUID))
+            pick = rng.choice(on, 60, replace=False)
+            moving, forced = np.sort(pick[:30]).astype(np.int32), np.sort(pick[30:]).astype(np.int32)
+            dev = flags.copy()
+            dev[pick] = cases.BOUNCEBACK
+            g.set_flags(dev)
+            g.set_moving_bounceback(moving, np.zeros((30, 3), np.float32))
+            g.set_forced_velocity(forced, np.zeros((30, 3), np.float32))
+            report_flags[forced] = cases.BOUNCEBACK
+        levels_plot.append((g, li, ijk, plot_flags))
+        levels_report.append((g, P.InfoStruct(res=1000.0, dtPhys=1.0), ijk, report_flags))
+    got = P.section_cut(grids, "XY", cut, RHO0, CS)
+    v, h = got.shape[:2]
+    want = _paint(levels_plot, "XY", cut, 1, 3, h, v)
+    assert np.array_equal(got[..., 4], want[..., 4].astype(np.float32))
+    assert np.array_equal(got[..., 5], want[..., 5].astype(np.float32))
+    bounds = np.array([-6.0, -4.0, -100.0, 8.0, 5.0, 100.0], np.float32)
+    rep = P.getFlowReport(grids, "XY", cut, bounds)
+    fi, _ = lv.info(2)
+    f32 = np.float32
+    i0 = max(0, int(np.floor((bounds[0] - f32(fi.ox)) / f32(fi.res))))
+    i1 = min(fi.cellCountX, int(np.ceil((bounds[3] - f32(fi.ox)) / f32(fi.res))))
+    j0 = max(0, int(np.floor((bounds[1] - f32(fi.oy)) / f32(fi.res))))
+    j1 = min(fi.cellCountY, int(np.ceil((bounds[4] - f32(fi.oy)) / f32(fi.res))))
+    win = _paint(levels_report, "XY", cut, 1, 3, fi.cellCountX, fi.cellCountY)[j0:j1, i0:i1]
+    open_px = win[..., 4] == 0
+    print(f"[cut] moving/forced markers: plot open pixels {int((got[..., 4] == 0).sum())}, "
+          f"report open pixels {int(open_px.sum())}", flush=True)
+    assert rep["areamm2"] == pytest.approx(open_px.sum() * float(fi.res) ** 2, rel=1e-12)
+    assert abs(rep["ux"] - win[..., 1][open_px].mean()) < 1e-6
