@@ -286,6 +286,14 @@ inline void setMovingBounceback(DIADGridStruct& g, const std::vector<int>& cells
 inline void setForcedVelocity(DIADGridStruct& g, const std::vector<int>& cells, const std::vector<float>& ut3) {
   check(psl_diad_set_forced_velocity(g.handle(), cells.data(), (int64_t)cells.size(), ut3.data()));
 }
+// rotateSTLAlongZ (include/STLFunctions.h:774-809)
+inline void rotateSTLAlongZ(STLStruct& s, float radians) { check(psl_stl_rotate(s.handle(), 2, radians)); }
+// ApplyMarkersInsideSTL on a DIAD grid's cells (include/STLFunctions.h:482-696): 1 inside, 0 outside
+inline std::vector<uint8_t> ApplyMarkersInsideSTL(DIADGridStruct& g, STLStruct& s) {
+  std::vector<uint8_t> m((size_t)g.cellCount());
+  check(psl_diad_markers_inside_stl(g.handle(), s.handle(), m.data()));
+  return m;
+}
 // getTorque (M35Full/m35Main.cu:176-260): torque about z in N m of the forced cells
 inline double getTorque(DIADGridStruct& g, float rhoNominalPhys) {
   double out[6];
