@@ -416,6 +416,10 @@ int psl_scalar_set(psl_grid* grid, const float* host);
  * psl_apply_local_cell_update plus the scalar, in one launch.  Streaming is
  * psl_apply_streaming. */
 int psl_scalar_apply_local_cell_update(psl_grid* grid);
+/* applyZeroGradientOutletScalarTransportBC (scalarTransport/scalarTransportMain.cu:95-135):
+ * every non-bounceback cell of the last x layer takes the 27 scalar
+ * populations of its -x neighbour; called between streaming and the update. */
+int psl_scalar_apply_zero_gradient_outlet_x(psl_grid* grid);
 
 /* ----------------------------------------------------------- output */
 /* exportSectionCutPlotXY/ZY/ZX(GridStruct&) (include/plotter/exportSectionCutPlot.h:2-104):

@@ -99,6 +99,7 @@ def _declare(L):
                                                                  ctypes.c_longlong, P, P] + [ctypes.c_float] * 6 + [P]
         getattr(L, f"oracle_collide_forced_{suf}").argtypes = [P, RT, RT, P]
         getattr(L, f"oracle_scalar_update_{suf}").argtypes = [P, P, P, ctypes.POINTER(OracleInfo), P, P, P, RT]
+        getattr(L, f"oracle_scalar_outlet_x_{suf}").argtypes = [P, P, ctypes.POINTER(OracleInfo), P]
         getattr(L, f"oracle_scalar_collide_{suf}").argtypes = [P, P, ctypes.c_longlong, RT]
         getattr(L, f"oracle_eso_outlet_z_{suf}").argtypes = [P, ctypes.c_longlong, P, ctypes.c_int, P,
                                                              ctypes.c_longlong, P]
@@ -221,6 +222,11 @@ class Grid:
         getattr(lib(), f"oracle_scalar_update_{self.suf}")(self.f.ctypes.data, T.ctypes.data,
                                                           self.shift.ctypes.data, ctypes.byref(self.info),
                                                           _p(self.flags), self.bcvals.ctypes.data, _p(g), tau_t)
+
+    def scalar_outlet_x(self, T):
+        """applyZeroGradientOutletScalarTransportBC on the raw scalar array T (in place)."""
+        getattr(lib(), f"oracle_scalar_outlet_x_{self.suf}")(T.ctypes.data, self.shift.ctypes.data,
+                                                            ctypes.byref(self.info), _p(self.flags))
 
     def local_update(self):
         getattr(lib(), f"oracle_local_update_{self.suf}")(self.f.ctypes.data, self.shift.ctypes.data,

@@ -1135,6 +1135,20 @@ void eso_forcing_sum(const R* fa, long long n, const int* const* nbr, bool flipp
   for (int a = 3; a < 6; ++a) out6[a] = acc[a] / 1000.0;
 }
 
+// applyZeroGradientOutletScalarTransportBC (scalarTransport/scalarTransportMain.cu:95-135):
+// every non-bounceback cell of the last x layer takes the 27 scalar
+// populations of its -x neighbour, both read at their shifted slots.
+template <typename R>
+void scalar_outlet_x(R* Ta, const int* shift, const OracleInfo& I, const uint8_t* flags) {
+  const long long n = (long long)I.nx * I.ny * I.nz;
+  for (int k = 0; k < I.nz; ++k)
+    for (int j = 0; j < I.ny; ++j) {
+      const long long cw = ((long long)k * I.ny + j) * I.nx + (I.nx - 1), cr = cw - 1;
+      if (flags && (flags[cw] & 7) == K_BOUNCEBACK) continue;
+      for (int q = 0; q < 27; ++q) Ta[(long long)q * n + (cw + shift[q]) % n] = Ta[(long long)q * n + (cr + shift[q]) % n];
+    }
+}
+
 // ------------------------------------------------------------------- STL voxelizer
 // getRayHitYesNo: include/STLFunctions.h:184-245 (exact integer predicate)
 bool ray_hit(long long ax, long long ay, long long bx, long long by, long long cx, long long cy) {
@@ -1826,6 +1840,9 @@ ORACLE_PER_CELL(f64, double)
     const int* b[7];                                                                                           \
     for (int t = 0; t < 7; ++t) { a[t] = nbc7 + (size_t)t * nc; b[t] = nbf7 + (size_t)t * nf; }              \
     eso_to_fine(fc, nc, a, flip_c != 0, ff, nf, b, flip_f != 0, write, read, m);                              \
+  }                                                                                                            \
+  void oracle_scalar_outlet_x_##SUF(R* Ta, const int* shift, const OracleInfo* I, const uint8_t* flags) {     \
+    scalar_outlet_x(Ta, shift, *I, flags);                                                                     \
   }                                                                                                            \
   void oracle_scalar_update_##SUF(R* fa, R* Ta, const int* shift, const OracleInfo* I, const uint8_t* flags,   \
                                   const R* bcvals, const R* givenT, R tauT) {                                  \

@@ -230,6 +230,7 @@ _SIGS = [
     ("psl_scalar_get", _I, [_P, _P]),
     ("psl_scalar_set", _I, [_P, _P]),
     ("psl_scalar_apply_local_cell_update", _I, [_P]),
+    ("psl_scalar_apply_zero_gradient_outlet_x", _I, [_P]),
     ("psl_diad_flow_report", _I, [_P, _I, _I, _I, _P, _P]),
     ("psl_write_sim_data", _I, [ctypes.c_char_p, _I, _I, _I, _I, _P]),
     ("psl_write_history", _I, [ctypes.c_char_p, _P, _I]),
@@ -752,6 +753,12 @@ class ScalarTransportStruct:
 def applyScalarTransportLocalCellUpdate(grid: "GridStruct", scalar: ScalarTransportStruct):
     """applyLocalCellUpdate(GridStruct&, ScalarTransportStruct&)."""
     _check(lib().psl_scalar_apply_local_cell_update(grid.handle))
+
+
+def applyZeroGradientOutletScalarTransportBC(grid: "GridStruct", scalar: ScalarTransportStruct):
+    """scalarTransport/scalarTransportMain.cu:95-135: the last x layer copies the
+    scalar populations of the layer before it (between streaming and the update)."""
+    _check(lib().psl_scalar_apply_zero_gradient_outlet_x(grid.handle))
 
 
 # ------------------------------------------------------------ output

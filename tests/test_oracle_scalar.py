@@ -75,3 +75,32 @@ def test_fp32_instantiation_agrees_with_fp64():
     a = O.scalar_collide(f, T, 0.7)
     b = O.scalar_collide(f.astype(np.float32), T.astype(np.float32), 0.7)
     assert np.max(np.abs(a - b)) < 1e-5
+
+
+def test_zero_gradient_outlet_copies_the_layer_before():
+    """applyZeroGradientOutletScalarTransportBC in the oracle on a streamed grid
+    (non-zero shifts): in logical order, every non-bounceback cell of the last x
+    layer holds the populations of its -x neighbour afterwards; bounceback cells
+    and every other layer keep theirs."""
+    nx, ny, nz = 7, 5, 4
+    rng = np.random.default_rng(9)
+    flags = np.zeros(nx * ny * nz, np.uint8)
+    flags[rng.choice(nx * ny * nz, 30, replace=False)] = 1  # bounceback
+    g = O.Grid(nx, ny, nz, dtype=np.float64, flags=flags)
+    for _ in range(3):
+        g.stream()
+    assert np.any(g.shift != 0)
+    n = g.n
+    T = rng.random((27, n))
+
+    def logical(a):
+        return np.stack([a[q][(np.arange(n) + g.shift[q]) % n] for q in range(27)])
+
+    before = logical(T).reshape(27, nz, ny, nx)
+    g.scalar_outlet_x(T)
+    after = logical(T).reshape(27, nz, ny, nx)
+    bb = (flags.reshape(nz, ny, nx) & 7) == 1
+    last = ~bb[:, :, -1]
+    assert np.array_equal(after[:, :, :, -1][:, last], before[:, :, :, -2][:, last])
+    assert np.array_equal(after[:, :, :, -1][:, ~last], before[:, :, :, -1][:, ~last])
+    assert np.array_equal(after[:, :, :, :-1], before[:, :, :, :-1])
