@@ -794,6 +794,13 @@ def section_cut(grid, plane: str, cut: int, rho_nominal: float = 1.225, soundspe
     return out
 
 
+def getAvgInletPressure(grid: "GridStruct", rho_nominal: float = 1.225, soundspeed: float = 1.0) -> float:
+    """getAvgInletPressure (cascade/cascadeMain.cu:77-112): the pressure of the
+    mean density over the j = 0 plane, every cell counted; the plane comes from
+    the device section cut (pressure is linear in rho), averaged in FP64."""
+    return float(section_cut(grid, "ZX", 0, rho_nominal, soundspeed)[..., 0].astype(np.float64).mean())
+
+
 def getFlowReport(levels, plane: str, cut: int, bounds) -> dict:
     """getFlowReportXY/ZY/ZX over DIAD levels: mean ux, uy, uz, rho (lattice units)
     and the open area (mm^2) of the plane clipped to bounds (xmin, ymin, zmin,

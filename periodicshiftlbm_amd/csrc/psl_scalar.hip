@@ -119,23 +119,24 @@ __global__ __launch_bounds__(256) void k_scalar_field(const float* __restrict__ 
 }
 
 // applyZeroGradientOutletScalarTransportBC (scalarTransport/scalarTransportMain.cu:95-135):
-// one thread per (j, k) of the last x layer; a non-bounceback cell takes the 27
-// scalar populations of its -x neighbour at their shifted slots.  The written
-// and read cells are disjoint layers, so no slot is both read and written.
+// a non-bounceback cell of the last x layer takes the 27 scalar populations of
+// its -x neighbour at their shifted slots.  One thread per (q, j, k): the layer
+// is strided by nx in memory, so every copy is its own line and the launch
+// needs the 27x more waves to keep enough of them in flight.  The written and
+// read cells are disjoint layers, so no slot is both read and written.
 __global__ __launch_bounds__(256) void k_scalar_outlet_x(float* __restrict__ T, Shift27 sh, uint32_t n, int nx,
                                                          int ny, int nz, const uint8_t* __restrict__ flags) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  if (t >= (uint32_t)ny * nz) return;
-  const uint32_t j = t % ny, k = t / ny;
+  const uint32_t m = (uint32_t)ny * nz;
+  if (t >= 27u * m) return;
+  const uint32_t q = t / m, r = t - q * m;
+  const uint32_t j = r % ny, k = r / ny;
   const uint32_t cw = (k * ny + j) * nx + (nx - 1), cr = cw - 1;
   if (flags && (flags[cw] & 7) == KIND_BOUNCEBACK) return;
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    uint32_t sw = cw + (uint32_t)sh.s[q], sr = cr + (uint32_t)sh.s[q];
-    if (sw >= n) sw -= n;
-    if (sr >= n) sr -= n;
-    T[(size_t)q * n + sw] = T[(size_t)q * n + sr];
-  }
+  uint32_t sw = cw + (uint32_t)sh.s[q], sr = cr + (uint32_t)sh.s[q];
+  if (sw >= n) sw -= n;
+  if (sr >= n) sr -= n;
+  T[(size_t)q * n + sw] = T[(size_t)q * n + sr];
 }
 
 }  // namespace
@@ -210,7 +211,7 @@ int psl_scalar_apply_zero_gradient_outlet_x(psl_grid* g) {
   Shift27 sh;
   for (int q = 0; q < 27; ++q) sh.s[q] = g->shift[q];
   const psl_info& I = g->info;
-  const uint32_t m = (uint32_t)I.cellCountY * I.cellCountZ;
+  const uint32_t m = 27u * (uint32_t)I.cellCountY * I.cellCountZ;
   k_scalar_outlet_x<<<(m + 255) / 256, 256, 0, g->stream>>>(g->scalar_T, sh, (uint32_t)I.cellCount, I.cellCountX,
                                                             I.cellCountY, I.cellCountZ, g->flags);
   PSL_LAUNCH_CHECK();

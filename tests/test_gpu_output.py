@@ -62,6 +62,9 @@ def test_section_cut_grid(psl):
         for a, want in ((1, uh), (2, uv), (3, un)):
             assert np.max(np.abs(got[..., a] - want)) < 1e-5 * vel
         assert np.array_equal(got[..., 4], marker)
+    # getAvgInletPressure: the mean density over the j = 0 plane as a pressure
+    p_in = P.getAvgInletPressure(g, RHO0, CS)
+    assert abs(p_in - (rho[cell[:, 0, :]].mean() - 1.0) * RHO0 * CS * CS) < 1e-5 * RHO0 * CS * CS
     # export3DPlot: every cell, (i, j, k) order with k fastest
     vol = P.export3DPlot(g, 3, path=None, rho_nominal=RHO0, soundspeed=CS)
     assert vol.shape == (nx, ny, nz, 5)
