@@ -44,6 +44,8 @@ class OracleLevel:
         self.flip = False
         self.tables = {k: np.ascontiguousarray(d[k], np.int32) for k in ("c2f_read", "c2f_write", "f2c_write",
                                                                        "f2c_read")}
+        self.moving = None  # (cells, u3): moving-wall cells after each update
+        self.forced = None  # (cells, ut3): forced-velocity cells after each update
 
     def get(self):
         return eso_get_all(self.f, self.nbr7, self.flip)
@@ -62,6 +64,11 @@ def update_grid(levels, l=0):
     L = levels[l]
     L.flip = not L.flip
     O.esotwist_update(L.f, L.nbr7, L.ijk, L.info, L.flags, L.bcvals, L.flip)
+    if L.moving is not None:
+        O.eso_moving_bounceback(L.f, L.nbr7, L.flip, *L.moving)
+    if L.forced is not None:
+        R = L.f.dtype.type
+        O.eso_forced_velocity(L.f, L.nbr7, L.flip, *L.forced, R(L.info.nu), R(L.info.cs))
     if l < len(levels) - 1:
         update_grid(levels, l + 1)
         update_grid(levels, l + 1)
