@@ -824,17 +824,17 @@ def exportSectionCutPlot(grid, plane: str, cut: int, plot_number: int, path: str
 
 def exportSectionCutPlotToiletPaperZ(grid: GridStruct, r: float, plot_number: int,
                                      path: Optional[str] = "/dev/shm/sim_data.bin", rho_nominal: float = 1.225,
-                                     soundspeed: float = 1.0) -> np.ndarray:
+                                     soundspeed: float = 1.0, max_pixels: int = 20000000) -> np.ndarray:
     """The cylinder of radius r (mm) about z unrolled: rows x cellCountZ x 5 of
     p, uz, u_tangential, u_radial (m/s) and the bounceback marker; over a list
     of DIAD levels rows x cols x 6, the last component the level."""
     if isinstance(grid, (list, tuple)):
         arr = (ctypes.c_void_p * len(grid))(*[g.handle.value for g in grid])
         dims = np.zeros(3, np.int32)
-        _check(lib().psl_diad_cylinder_cut(arr, len(grid), r, rho_nominal, soundspeed, 20000000, None,
+        _check(lib().psl_diad_cylinder_cut(arr, len(grid), r, rho_nominal, soundspeed, max_pixels, None,
                                            dims.ctypes.data))
         out = np.zeros((dims[0], dims[1], 6), np.float32)
-        _check(lib().psl_diad_cylinder_cut(arr, len(grid), r, rho_nominal, soundspeed, 20000000, out.ctypes.data,
+        _check(lib().psl_diad_cylinder_cut(arr, len(grid), r, rho_nominal, soundspeed, max_pixels, out.ctypes.data,
                                            dims.ctypes.data))
         if path is not None:
             write_sim_data(path, plot_number, out)

@@ -37,6 +37,7 @@ struct CutParams {
   int cyl = 0;
   float r = 0.f, fres = 0.f;
   int io = 0, jo = 0;
+  int tscale = 1;  // finest-level cells per pixel
 };
 
 __device__ __forceinline__ void plane_components(int plane, float ux, float uy, float uz, float& uh, float& uv,
@@ -191,7 +192,7 @@ __global__ void k_cut_paint(const int* __restrict__ winner, const float* __restr
   float* o = out + 6 * pix;
   if (c.cyl) {
     // p, uz, tangential and radial velocity at the row's angle (lines 971-978)
-    const float theta = ((float)(pix / c.h) * (float)c.cut * c.fres) / c.r;
+    const float theta = ((float)(pix / c.h) * (float)c.tscale * c.fres) / c.r;
     const float sn = sinf(theta), cs = cosf(theta);
     o[0] = (rho - 1.f) * c.p_scale;
     o[1] = uz;
@@ -474,8 +475,7 @@ int psl_diad_section_cut(psl_diad* const* levels, int nlevels, int plane, int cu
   return paint_levels(levels, nlevels, c, 1 << (nlevels - target), host_out);
 }
 
-// exportSectionCutPlotToiletPaperZ over DIAD levels (lines 789-993); the target
-// scale rides in CutParams.cut for the paint kernel's row angles.
+// exportSectionCutPlotToiletPaperZ over DIAD levels (lines 789-993).
 int psl_diad_cylinder_cut(psl_diad* const* levels, int nlevels, float r, float rho_nominal, float soundspeed,
                           int64_t max_pixels, float* host_out, int* dims3) {
   PSL_CHECK_ARG(levels && nlevels >= 1 && dims3 && r > 0.f, "bad cylinder-cut arguments");
@@ -498,8 +498,9 @@ int psl_diad_cylinder_cut(psl_diad* const* levels, int nlevels, float r, float r
   dims3[2] = target;
   if (!host_out) return PSL_OK;
   const psl_info& F = levels[nlevels - 1]->info;
-  CutParams c{0, 1 << (nlevels - target), h, v, 1.f, rho_nominal * soundspeed * soundspeed};
+  CutParams c{0, 0, h, v, 1.f, rho_nominal * soundspeed * soundspeed};
   c.cyl = 1;
+  c.tscale = 1 << (nlevels - target);
   c.r = r;
   c.fres = F.res;
   c.io = (int)((0.f - F.ox) / F.res + 0.5f);  // getIJKCellIndexFromXYZ(0, 0, 0) on the finest level

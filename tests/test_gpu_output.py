@@ -330,6 +330,19 @@ def test_toilet_paper_cut_diad_levels(psl):
         assert len(np.unique(got[..., 5])) >= 2
         assert (~same).sum() <= 0.005 * same.size
         assert np.max(np.abs(got[same][:, 1:4] - want[same][:, 1:4])) < 1e-5
+    # a pixel budget the finest plane exceeds: the middle level sets the pixel size
+    r = 4.1
+    got = P.exportSectionCutPlotToiletPaperZ(grids, r, 1, path=None, rho_nominal=RHO0, soundspeed=CS, max_pixels=2000)
+    mid, _ = lv.info(1)
+    v, h = got.shape[:2]
+    assert v == int(np.float32(2.0 * np.float32(3.14159265359) * np.float32(r)) / np.float32(mid.res))
+    assert h == mid.cellCountZ
+    want = _paint_cyl(levels, r, 2, 3, h, v, fin.res, io, jo)
+    same = (got[..., 5] == want[..., 5]) & (got[..., 4] == want[..., 4])
+    print(f"[cut] DIAD toilet paper r = {r} at the middle level's pixels: {got.shape}, "
+          f"{int((~same).sum())} of {same.size} pixels differ", flush=True)
+    assert (~same).sum() <= 0.005 * same.size
+    assert np.max(np.abs(got[same][:, 1:4] - want[same][:, 1:4])) < 1e-5
 
 
 def test_cut_markers_of_moving_and_forced_cells(psl):
