@@ -766,7 +766,7 @@ PLANES = {"XY": 0, "ZY": 1, "ZX": 2}
 
 
 def section_cut(grid, plane: str, cut: int, rho_nominal: float = 1.225, soundspeed: float = 1.0,
-                scalar_transport: bool = False) -> np.ndarray:
+                scalar_transport: bool = False, max_pixels: int = 20000000) -> np.ndarray:
     """exportSectionCutPlotGeneral on a GridStruct (rows x cols x 5; x 6 with
     scalar_transport, the last component the scalar) or a list of
     DIADGridStruct levels (rows x cols x 6, the last component the level)."""
@@ -781,10 +781,10 @@ def section_cut(grid, plane: str, cut: int, rho_nominal: float = 1.225, soundspe
     if isinstance(grid, (list, tuple)):
         arr = (ctypes.c_void_p * len(grid))(*[g.handle.value for g in grid])
         dims = np.zeros(3, np.int32)
-        _check(lib().psl_diad_section_cut(arr, len(grid), pl, cut, rho_nominal, soundspeed, 20000000, None,
+        _check(lib().psl_diad_section_cut(arr, len(grid), pl, cut, rho_nominal, soundspeed, max_pixels, None,
                                           dims.ctypes.data))
         out = np.zeros((dims[0], dims[1], 6), np.float32)
-        _check(lib().psl_diad_section_cut(arr, len(grid), pl, cut, rho_nominal, soundspeed, 20000000,
+        _check(lib().psl_diad_section_cut(arr, len(grid), pl, cut, rho_nominal, soundspeed, max_pixels,
                                           out.ctypes.data, dims.ctypes.data))
         return out
     dims = np.zeros(2, np.int32)

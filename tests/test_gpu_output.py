@@ -124,6 +124,14 @@ def test_section_cut_diad_levels(psl, plane, cut):
     assert np.array_equal(got[..., 4], want[..., 4].astype(np.float32))
     assert np.max(np.abs(got[..., 0] - want[..., 0])) < 1e-5 * RHO0 * CS * CS
     assert np.max(np.abs(got[..., 1:4] - want[..., 1:4])) < 1e-5 * 1.0
+    # a pixel budget below the finest plane: the middle level sets the pixel size
+    budget = got.shape[0] * got.shape[1]
+    small = P.section_cut(grids, plane, cut, RHO0, CS, max_pixels=budget)
+    assert small.shape[0] * 2 == got.shape[0] and small.shape[1] * 2 == got.shape[1]
+    want2 = _paint(levels, plane, cut, 2, 3, small.shape[1], small.shape[0])
+    assert np.array_equal(small[..., 5], want2[..., 5].astype(np.float32))
+    assert np.array_equal(small[..., 4], want2[..., 4].astype(np.float32))
+    assert np.max(np.abs(small[..., 1:4] - want2[..., 1:4])) < 1e-5 * 1.0
 
 
 def test_flow_report_diad_levels(psl):
