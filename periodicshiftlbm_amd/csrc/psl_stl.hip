@@ -108,17 +108,26 @@ __device__ inline TriLattice tri_lattice(const float* t, const VoxParams& p) {
   return L;
 }
 
+// One wave per triangle: the lanes stride over the columns of its footprint
+// (a large triangle covers hundreds of columns, which one thread would walk
+// alone).  The per-column counts and the sorted layers do not depend on the
+// order of the atomics.
+constexpr int VOX_WAVES = 4;  // waves (triangles) per 256-thread block
+
 // K14: hits per column
 __global__ void k_count_hits(const float* __restrict__ tri, int ntri, VoxParams p, int* __restrict__ counter) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * VOX_WAVES + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (t >= ntri) return;
   const TriLattice L = tri_lattice(tri + 9 * (size_t)t, p);
-  for (long long j = L.jmin; j <= L.jmax; ++j)
-    for (long long i = L.imin; i <= L.imax; ++i) {
-      const long long rk = i * 100, rl = j * 100;
-      if (ray_hit(L.ak - rk, L.al - rl, L.bk - rk, L.bl - rl, L.ck - rk, L.cl - rl))
-        atomicAdd(&counter[(size_t)j * p.nx + i], 1);
-    }
+  const long long w = L.imax - L.imin + 1, h = L.jmax - L.jmin + 1;
+  if (w <= 0 || h <= 0) return;
+  for (long long f = lane; f < w * h; f += 64) {
+    const long long j = L.jmin + f / w, i = L.imin + f % w;
+    const long long rk = i * 100, rl = j * 100;
+    if (ray_hit(L.ak - rk, L.al - rl, L.bk - rk, L.bl - rl, L.ck - rk, L.cl - rl))
+      atomicAdd(&counter[(size_t)j * p.nx + i], 1);
+  }
 }
 
 __global__ void k_max_count(const int* __restrict__ counter, size_t n, int* out) {
@@ -132,23 +141,26 @@ __global__ void k_max_count(const int* __restrict__ counter, size_t n, int* out)
 // K16: hit cell index per triangle and column
 __global__ void k_hit_index(const float* __restrict__ tri, int ntri, VoxParams p, int* __restrict__ counter,
                             int* __restrict__ layers, int maxc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * VOX_WAVES + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (t >= ntri) return;
   const TriLattice L = tri_lattice(tri + 9 * (size_t)t, p);
   const size_t ncol = (size_t)p.nx * p.ny;
-  for (long long j = L.jmin; j <= L.jmax; ++j)
-    for (long long i = L.imin; i <= L.imax; ++i) {
-      const long long rk = i * 100, rl = j * 100;
-      if (!ray_hit(L.ak - rk, L.al - rl, L.bk - rk, L.bl - rl, L.ck - rk, L.cl - rl)) continue;
-      const size_t col = (size_t)j * p.nx + i;
-      const int w = atomicAdd(&counter[col], 1);
-      const float rx = (int)i * p.res;
-      const float ry = (int)j * p.res;
-      const float rz = ray_hit_z(L.ax, L.ay, L.az, L.bx, L.by, L.bz, L.cx, L.cy, L.cz, rx, ry);
-      int k = (int)fmaxf(0.0f, ceilf(rz / p.res));
-      k = min(k, p.nz);
-      if (w < maxc) layers[(size_t)w * ncol + col] = k;
-    }
+  const long long fw = L.imax - L.imin + 1, fh = L.jmax - L.jmin + 1;
+  if (fw <= 0 || fh <= 0) return;
+  for (long long f = lane; f < fw * fh; f += 64) {
+    const long long j = L.jmin + f / fw, i = L.imin + f % fw;
+    const long long rk = i * 100, rl = j * 100;
+    if (!ray_hit(L.ak - rk, L.al - rl, L.bk - rk, L.bl - rl, L.ck - rk, L.cl - rl)) continue;
+    const size_t col = (size_t)j * p.nx + i;
+    const int w = atomicAdd(&counter[col], 1);
+    const float rx = (int)i * p.res;
+    const float ry = (int)j * p.res;
+    const float rz = ray_hit_z(L.ax, L.ay, L.az, L.bx, L.by, L.bz, L.cx, L.cy, L.cz, rx, ry);
+    int k = (int)fmaxf(0.0f, ceilf(rz / p.res));
+    k = min(k, p.nz);
+    if (w < maxc) layers[(size_t)w * ncol + col] = k;
+  }
 }
 
 // K17: per-column insertion sort (the result does not depend on the atomic order)
@@ -317,7 +329,7 @@ int stl_layers(const psl_stl* s, const VoxParams& p, int** layers_out, int* maxc
   if (rc == PSL_OK) {
     (void)hipMemset(counter, 0, ncol * sizeof(int));
     (void)hipMemset(dmax, 0, sizeof(int));
-    if (s->count > 0) k_count_hits<<<(s->count + 255) / 256, 256>>>(s->d_tri, s->count, p, counter);
+    if (s->count > 0) k_count_hits<<<(s->count + VOX_WAVES - 1) / VOX_WAVES, 256>>>(s->d_tri, s->count, p, counter);
     k_max_count<<<grid1d(ncol), 256>>>(counter, ncol, dmax);
     if (hipGetLastError() != hipSuccess || hipMemcpy(&maxc, dmax, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
       fail("count");
@@ -327,7 +339,8 @@ int stl_layers(const psl_stl* s, const VoxParams& p, int** layers_out, int* maxc
   if (rc == PSL_OK) {
     k_fill_int<<<grid1d((size_t)maxc * ncol), 256>>>(layers, (size_t)maxc * ncol, p.nz);
     (void)hipMemset(counter, 0, ncol * sizeof(int));
-    if (s->count > 0) k_hit_index<<<(s->count + 255) / 256, 256>>>(s->d_tri, s->count, p, counter, layers, maxc);
+    if (s->count > 0)
+      k_hit_index<<<(s->count + VOX_WAVES - 1) / VOX_WAVES, 256>>>(s->d_tri, s->count, p, counter, layers, maxc);
     dim3 g2((p.nx + 255) / 256, p.ny);
     k_sort_columns<<<g2, 256>>>(layers, maxc, p);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) fail("hits");
