@@ -15,7 +15,15 @@
 //   PS shifter ............... paper Table 1 D2Q9 map + brute-force streaming
 //   feq / moments ............ exact moment identities
 //   collision ................ mass/momentum conservation, rest-state fixed point,
-//                              cube-symmetry equivariance, shear-wave decay rate
+//                              cube-symmetry equivariance, shear-wave decay rate;
+//                              non-linear part: every post-collision cumulant of
+//                              order 3-6, computed from its definition (log of
+//                              the moment generating function), is zero;
+//                              second-order central moments relax by (1 - omega)
+//                              (off-diagonal at any u, deviatoric at u = 0, trace
+//                              to rho); diagonal shear and sound waves decay at
+//                              the same rate with and without a uniform
+//                              background flow (the Galilean corrections)
 //   LES omega ................ closed form (Cs=0, Pi_neq=0)
 //   boundary conditions ...... bit-exact vs the reference's compiled headers
 //                              (tests/golden/bc_vectors.npz) + moment identities

@@ -26,6 +26,10 @@
 #include <cmath>
 #include <numeric>
 
+// The shared headers end in contraction mode fast; from here on nothing is
+// fused, whatever the command line says.
+#pragma clang fp contract(off)
+
 namespace psl {
 namespace {
 

@@ -19,6 +19,10 @@
 #include <fstream>
 #include <limits>
 
+// The shared headers end in contraction mode fast; from here on nothing is
+// fused, whatever the command line says.
+#pragma clang fp contract(off)
+
 namespace psl {
 
 __host__ __device__ inline bool ray_hit(long long ax, long long ay, long long bx, long long by, long long cx,
