@@ -16,6 +16,7 @@ import pytest
 
 import oracle as O
 from tests import diad_dense
+from tests.stl_shapes import tilted_box
 
 import periodicshiftlbm_amd as P
 from periodicshiftlbm_amd import cases
@@ -47,10 +48,24 @@ def _case(name):
         r = P.rules(P.marker_rule(P.RULE_BOUNCEBACK, P.SHAPE_INDEX_BOX, idx=(0, -1, 0, -1, 0, 0)),
                     P.marker_rule(P.RULE_BOUNCEBACK, P.SHAPE_INDEX_BOX, idx=(0, -1, 0, -1, -1, -1)))
         return base, 1, 0, [(tri, (0.0, 0.0, 0.0))], r
+    if name == "ties_3":
+        # cells of 0.1 from a non-dyadic origin: rule faces and the tilted top
+        # face of an STL box fall on cell positions, where a fused
+        # `i * res + ox` or hit height rounds the other way (an oracle built
+        # with contraction gets other level sizes here)
+        base = dict(res=0.1, ox=-1.2, oy=-1.08, oz=-1.32, nu=0.01, dt=1.0, nx=24, ny=20, nz=20)
+        r = P.rules(
+            P.marker_rule(P.RULE_BOUNCEBACK, P.SHAPE_SPHERE_LE, p=(0.0, 0.0, 0.0, 0.3)),
+            P.marker_rule(P.RULE_REFINEMENT, P.SHAPE_BOX, p=(-0.6, -0.5, -0.5, 0.7, 0.5, 0.6), levels=[0]),
+            P.marker_rule(P.RULE_REFINEMENT, P.SHAPE_SPHERE_LT, p=(0.05, 0.0, 0.0, 0.5), levels=[0, 1]),
+            P.marker_rule(P.RULE_OUTLET, P.SHAPE_INDEX_BOX, idx=(-1, -1, 0, -1, 0, -1), levels=[0]),
+        )
+        tri = tilted_box(-0.85, 0.35, 0.15, 0.75, -0.35, 0.05, 1.0, 0.5)
+        return base, 3, 1, [(tri, (0.0, 0.0, 0.0))], r
     raise KeyError(name)
 
 
-CASES = ["sphere_rules_3", "stl_2", "stl_walls_1"]
+CASES = ["sphere_rules_3", "stl_2", "stl_walls_1", "ties_3"]
 
 
 @pytest.fixture(scope="module")
