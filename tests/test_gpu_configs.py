@@ -21,10 +21,13 @@
   the acoustic start-up transient in which the instantaneous drag crosses
   zero.  FP32 and FP16 must be within 1 % of the FP64 mean; BF16 (8-bit
   significand) is reported and bounded at 5 %.
-* Two-level sphere drag (the two-level configuration scaled by 1/4, Re =
-  1000): FP32 and FP16 storage through the whole recursion (coarse step, two
-  fine steps, coarse-fine communication) against the FP64 oracle's
-  recursion, mean fine-grid drag within 1 %.
+* Two-level sphere drag (the two-level configuration scaled by 1/4, at Re =
+  1000 and at the configuration's own Re = 1e5, where tau = 0.50001 and the
+  Smagorinsky rate sets the relaxation): FP32 and FP16 storage through the
+  whole recursion (coarse step, two fine steps, coarse-fine communication)
+  against the FP64 oracle's recursion, mean fine-grid drag within 1 %, and
+  the mean lift (y force, zero for the symmetric flow up to the start-up
+  asymmetry of the voxelized sphere) within 1 % of the drag.
 """
 import numpy as np
 import pytest
@@ -154,17 +157,18 @@ TWO_LEVEL_SCALED = {"shape": (64, 32, 32), "center": (16.0, 16.0, 16.0), "diamet
                     "sub": (8, 4, 4, 40, 28, 28)}
 
 
-def _scaled_two_level(P, storage):
+def _scaled_two_level(P, storage, reynolds):
     c = TWO_LEVEL_SCALED
-    return cases.two_level_sphere(storage, reynolds=1000.0, sub=c["sub"], shape=c["shape"], center=c["center"],
+    return cases.two_level_sphere(storage, reynolds=reynolds, sub=c["sub"], shape=c["shape"], center=c["center"],
                                   diameter=c["diameter"])
 
 
-@pytest.fixture(scope="module")
-def fp64_two_level_drag(psl, oracle_mod):
-    """Mean fine-grid drag of the FP64 oracle's two-level recursion."""
+@pytest.fixture(scope="module", params=[1000.0, 1e5], ids=["re1e3", "re1e5"])
+def fp64_two_level_drag(request, psl, oracle_mod):
+    """Mean fine-grid force (x, y) of the FP64 oracle's two-level recursion."""
     P, O = psl, oracle_mod
-    (gc, gf), (lo, hi) = _scaled_two_level(P, "f32")
+    reynolds = request.param
+    (gc, gf), (lo, hi) = _scaled_two_level(P, "f32", reynolds)
     sub = TWO_LEVEL_SCALED["sub"]
     oc = O.Grid(gc.Info.cellCountX, gc.Info.cellCountY, gc.Info.cellCountZ, dtype=np.float64, nu=gc.Info.nu,
                 cs=0.17, flags=gc.get_flags(), bcvals=cases.channel_given_values(0.04), sub=sub)
@@ -175,6 +179,7 @@ def fp64_two_level_drag(psl, oracle_mod):
     oc.fill_equilibrium()
     of.fill_equilibrium()
     first, last, every = WINDOW
+    print(f"[two-level FP64 oracle] Re={reynolds:g} coarse steps={last}", flush=True)
     samples = []
     for s in range(1, last + 1):
         oc.step(periodic_y=True)
@@ -182,26 +187,29 @@ def fp64_two_level_drag(psl, oracle_mod):
         of.step()
         O.coarse_fine_comm(oc, of)
         if s >= first and (s - first) % every == 0:
-            samples.append(float(of.momentum_exchange(lo, hi)[0]))
-    return lo, hi, float(np.mean(samples))
+            samples.append(of.momentum_exchange(lo, hi)[:2].astype(np.float64))
+    return reynolds, lo, hi, np.mean(samples, axis=0)
 
 
 @pytest.mark.parametrize("storage", ["f32", "f16"])
 def test_two_level_sphere_drag_vs_fp64(psl, fp64_two_level_drag, storage):
     P = psl
-    lo, hi, mean64 = fp64_two_level_drag
-    (gc, gf), _ = _scaled_two_level(P, storage)
+    reynolds, lo, hi, mean64 = fp64_two_level_drag
+    (gc, gf), _ = _scaled_two_level(P, storage, reynolds)
     first, last, every = WINDOW
-    print(f"[two-level-drag-{storage}] coarse {TWO_LEVEL_SCALED['shape']} sub {TWO_LEVEL_SCALED['sub']} "
-          f"steps={last}", flush=True)
+    print(f"[two-level-drag-{storage}] Re={reynolds:g} coarse {TWO_LEVEL_SCALED['shape']} "
+          f"sub {TWO_LEVEL_SCALED['sub']} steps={last}", flush=True)
     P.updateGrid([gc, gf], first)
-    samples = [float(gf.momentum_exchange(lo, hi)[0])]
+    samples = [gf.momentum_exchange(lo, hi)[:2]]
     while len(samples) < (last - first) // every + 1:
         P.updateGrid([gc, gf], every)
-        samples.append(float(gf.momentum_exchange(lo, hi)[0]))
-    mean = float(np.mean(samples))
-    rel = abs(mean / mean64 - 1)
-    print(f"  mean Fx {mean:.6e} FP64 {mean64:.6e} rel {rel:.2e}", flush=True)
+        samples.append(gf.momentum_exchange(lo, hi)[:2])
+    mean = np.mean(samples, axis=0)
+    drag_err = abs(mean[0] / mean64[0] - 1)
+    lift_err = abs(mean[1] - mean64[1]) / abs(mean64[0])
+    print(f"  mean Fx {mean[0]:.6e} FP64 {mean64[0]:.6e} rel {drag_err:.2e}; mean Fy {mean[1]:.3e} "
+          f"FP64 {mean64[1]:.3e} (difference {lift_err:.2e} of the drag)", flush=True)
     gc.close()
     gf.close()
-    assert rel < 0.01
+    assert drag_err < 0.01
+    assert lift_err < 0.01
