@@ -50,6 +50,29 @@ def _compile(src: str) -> str:
     return obj
 
 
+EXAMPLES = os.path.join(HERE, "examples")
+EXAMPLE_BIN = os.path.join(EXAMPLES, "bin")
+
+
+def example_sources():
+    return sorted(f for f in os.listdir(EXAMPLES) if f.endswith(".cpp"))
+
+
+def build_example(src: str, out_dir: str = EXAMPLE_BIN) -> str:
+    """One host driver of examples/ against include/psl_facade.hpp and
+    libpsl_amd.so (g++ only: the facade is plain C++ over the C ABI).  The
+    rpath is relative to the binary, so the tree can move."""
+    os.makedirs(out_dir, exist_ok=True)
+    exe = os.path.join(out_dir, src[:-4])
+    rpath = os.path.relpath(HERE, out_dir)
+    cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I", os.path.join(REPO, "include"),
+           os.path.join(EXAMPLES, src), "-L", HERE, "-lpsl_amd", f"-Wl,-rpath,$ORIGIN/{rpath}", "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return exe
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -60,8 +83,10 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    for src in example_sources():
+        build_example(src)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {len(example_sources())} example drivers in {EXAMPLE_BIN}")
     return LIB
 
 

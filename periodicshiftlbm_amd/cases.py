@@ -284,6 +284,96 @@ def two_level_sphere(storage: str = "f16", reynolds: float = 1e5, u_inlet: float
     return grids, (lo, hi)
 
 
+# ------------------------------------------------ nested periodic-shift sphere
+def reference_sphere_levels(levels: int = 4, cells: int = 220, reynolds: float = 1e6, u_inlet: float = 0.07):
+    """The nested-box sphere of flowAroundSphere/flowAroundSphereMainSingle.cu:
+    a sphere of 2000 mm at (4000, 11000, 11000) mm in a cube of 22000 mm with
+    `cells` coarse cells per edge (220: the driver's 100 mm), and `levels`
+    grids, each 2x finer, nested by the driver's placement rule (lines
+    186-213: progress = (l / (L-1))^0.2 sets the box, the fine origin is
+    ox_c + iStart res_c - res_f / 2).  Markers (lines 41-67): bounceback inside
+    the sphere (r^2 <= R^2 at iCell * res + ox), given u on the coarse
+    grid's j, k and i = 0 faces, given rho at i = Nx - 1; ghost box faces on
+    the finer grids.  Every expression is evaluated in FP32 in the driver's
+    order.  Returns a list of (InfoStruct fields dict, flag bytes) and the
+    finest level's drag box (lo, hi) of getSphereDrag (lines 99-112)."""
+    f32 = np.float32
+    D = f32(2000.0)
+    R = f32(0.5 * 2000.0)
+    res0 = f32(22000.0 / cells)
+    ux = f32(u_inlet)
+    nu_phys = f32(ux * (D / f32(1000.0)) / f32(reynolds))
+    dt0 = f32(f32(ux / ux) * f32(res0 / f32(1000)))
+    domain = f32(f32(11.0) * D)
+    sx, sy, sz = f32(2.0) * D, f32(5.5) * D, f32(5.5) * D
+    n0 = int(math.ceil(float(domain / res0)))
+
+    def nu_of(dt, res):
+        r = f32(res / f32(1000.0))
+        return f32(f32(dt * nu_phys) / f32(r * r))
+
+    infos = [dict(gridID=0, res=res0, dtPhys=dt0, nu=nu_of(dt0, res0), cellCountX=n0, cellCountY=n0, cellCountZ=n0,
+                  ox=f32(0), oy=f32(0), oz=f32(0))]
+    for lev in range(1, levels):
+        c = infos[-1]
+        res = f32(c["res"] * f32(0.5))
+        dt = f32(c["dtPhys"] * f32(0.5))
+        progress = f32(np.power(f32(f32(lev) / f32(levels - 1)), f32(0.2)))
+        x_start = f32(f32(progress * f32(1.4)) * D)
+        x_end = f32(f32(f32(1) - progress) * domain) + f32(f32(progress * f32(2.6)) * D)
+        y_start = f32(f32(progress * f32(4.9)) * D)
+        c["iSubgridStart"] = int(f32(f32(f32(x_start - c["ox"]) / c["res"]) + f32(0.5)))
+        c["iSubgridEnd"] = int(f32(f32(f32(x_end - c["ox"]) / c["res"]) + f32(0.5)))
+        c["jSubgridStart"] = int(f32(f32(f32(y_start - c["oy"]) / c["res"]) + f32(0.5)))
+        c["jSubgridEnd"] = c["jSubgridStart"] + (c["iSubgridEnd"] - c["iSubgridStart"])
+        c["kSubgridStart"] = c["jSubgridStart"]
+        c["kSubgridEnd"] = c["jSubgridEnd"]
+        half = f32(res * f32(0.5))
+        infos.append(dict(
+            gridID=lev, res=res, dtPhys=dt, nu=nu_of(dt, res),
+            ox=f32(f32(c["ox"] + f32(f32(c["iSubgridStart"]) * c["res"])) - half),
+            oy=f32(f32(c["oy"] + f32(f32(c["jSubgridStart"]) * c["res"])) - half),
+            oz=f32(f32(c["oz"] + f32(f32(c["kSubgridStart"]) * c["res"])) - half),
+            cellCountX=2 * (c["iSubgridEnd"] - c["iSubgridStart"]),
+            cellCountY=2 * (c["jSubgridEnd"] - c["jSubgridStart"]),
+            cellCountZ=2 * (c["kSubgridEnd"] - c["kSubgridStart"])))
+    out = []
+    for info in infos:
+        nx, ny, nz = info["cellCountX"], info["cellCountY"], info["cellCountZ"]
+        k, j, i = np.meshgrid(np.arange(nz, dtype=np.int32), np.arange(ny, dtype=np.int32),
+                              np.arange(nx, dtype=np.int32), indexing="ij")
+        x = i.astype(f32) * info["res"] + info["ox"]
+        y = j.astype(f32) * info["res"] + info["oy"]
+        z = k.astype(f32) * info["res"] + info["oz"]
+        r2 = (x - sx) * (x - sx) + (y - sy) * (y - sy) + (z - sz) * (z - sz)
+        solid = r2 <= R * R
+        if info["gridID"] != 0:
+            face = (i == 0) | (i == nx - 1) | (j == 0) | (j == ny - 1) | (k == 0) | (k == nz - 1)
+            flags = np.where(solid, BOUNCEBACK, np.where(face, GHOST, FLUID))
+        else:
+            flags = np.full(solid.shape, FLUID)
+            flags[i == nx - 1] = GIVEN_RHO
+            flags[i == 0] = GIVEN_U
+            flags[(k == 0) | (k == nz - 1)] = GIVEN_U
+            flags[(j == 0) | (j == ny - 1)] = GIVEN_U
+            flags[solid] = BOUNCEBACK
+        out.append((info, flags.astype(np.uint8).reshape(-1)))
+    fin = infos[-1]
+    lo = [int(f32(f32(sx - R - fin["ox"]) / fin["res"])), int(f32(f32(sy - R - fin["oy"]) / fin["res"])),
+          int(f32(f32(sz - R - fin["oz"]) / fin["res"]))]
+    hi = [int(f32(f32(sx + R - fin["ox"]) / fin["res"])) + 2, int(f32(f32(sy + R - fin["oy"]) / fin["res"])) + 2,
+          int(f32(f32(sz + R - fin["oz"]) / fin["res"])) + 2]
+    return out, (lo, hi)
+
+
+def reference_sphere_drag_coefficient(fx: float, levels_info, u_inlet: float = 0.07) -> float:
+    """C_D of the finest level's lattice force: -8 F / (U^2 pi D^2) with D in
+    finest cells (the driver's physical form reduces to this because its
+    physical velocity equals the lattice velocity)."""
+    d = 2000.0 / float(levels_info[-1]["res"])
+    return -8.0 * fx / (u_inlet * u_inlet * math.pi * d * d)
+
+
 # ------------------------------------------------------------ DIAD sphere
 def diad_sphere_flags(ijk: np.ndarray, bounceback: np.ndarray, shape, level: int) -> np.ndarray:
     """Flag bytes of one level of the wall-refined sphere channel
