@@ -142,6 +142,18 @@ int psl_grid_device_f(psl_grid* grid, void** dev_ptr, size_t* bytes);
  * accumulated in FP64 (getSphereDrag, flowAroundSphereMainSingle.cu:99-156; getLift,
  * NACA0012/NACA0012Main.cu:96-133).  lo/hi may be NULL for the whole grid. */
 int psl_reduce_momentum_exchange(psl_grid* grid, const int* lo3, const int* hi3, double* out3);
+/* psl_reduce_momentum_exchange without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * per-step drag loops, harnesses that forbid hipMalloc).  Its per-block partials live in a
+ * fixed per-device array of PSL_NOALLOC_MAX_BOX_CELLS / 256 * 3 + 4 doubles (1.5 MiB),
+ * held until the process ends; a box of more than PSL_NOALLOC_MAX_BOX_CELLS cells is
+ * rejected with PSL_E_ARG.  Same kernels and block count as psl_reduce_momentum_exchange,
+ * so the same bits.  Call with the grid's device current.  Every call on one device
+ * shares that device's array: calls that overlap on one device (other host threads,
+ * other streams) race. */
+#define PSL_NOALLOC_MAX_BOX_CELLS 16777216
+#define PSL_NOALLOC_MAX_DEVICES 64
+int psl_reduce_momentum_exchange_noalloc(psl_grid* grid, const int* lo3, const int* hi3, double* out3);
 /* Total mass sum f over updated cells (not ghost / covered / bounceback), FP64. */
 int psl_reduce_mass(psl_grid* grid, double* out);
 

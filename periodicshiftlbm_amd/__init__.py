@@ -139,6 +139,7 @@ _SIGS = [
     ("psl_get_macroscopic", _I, [_P, _P, _P, _P, _P]),
     ("psl_grid_device_f", _I, [_P, _P, _P]),
     ("psl_reduce_momentum_exchange", _I, [_P, _P, _P, _P]),
+    ("psl_reduce_momentum_exchange_noalloc", _I, [_P, _P, _P, _P]),
     ("psl_reduce_mass", _I, [_P, _P]),
     ("psl_stl_read", _I, [ctypes.c_char_p, _P]),
     ("psl_stl_from_triangles", _I, [_P, _I, _P]),
@@ -410,11 +411,15 @@ class GridStruct:
     def set_stream(self, stream_ptr: int):
         _check(lib().psl_grid_set_stream(self._h, ctypes.c_void_p(stream_ptr)))
 
-    def momentum_exchange(self, lo=None, hi=None) -> np.ndarray:
+    def momentum_exchange(self, lo=None, hi=None, noalloc: bool = False) -> np.ndarray:
+        """FP64 momentum exchange over the bounceback cells of [lo, hi).  With
+        noalloc the library's fixed per-device scratch is used instead of a
+        buffer it allocates (boxes up to PSL_NOALLOC_MAX_BOX_CELLS cells)."""
         out = np.zeros(3, np.float64)
         lo_a = None if lo is None else np.ascontiguousarray(lo, np.int32)
         hi_a = None if hi is None else np.ascontiguousarray(hi, np.int32)
-        _check(lib().psl_reduce_momentum_exchange(self._h, _ptr(lo_a), _ptr(hi_a), out.ctypes.data))
+        fn = lib().psl_reduce_momentum_exchange_noalloc if noalloc else lib().psl_reduce_momentum_exchange
+        _check(fn(self._h, _ptr(lo_a), _ptr(hi_a), out.ctypes.data))
         return out
 
     def profile(self, enable: bool):

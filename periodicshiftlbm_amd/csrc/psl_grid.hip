@@ -645,16 +645,6 @@ template <class St> struct MacroFn {
     return PSL_OK;
   }
 };
-template <class St> struct MomentumFn {
-  static int run(psl_grid* g, const int* lo, const int* hi, uint32_t nb) {
-    const psl_info& I = g->info;
-    k_momentum_exchange<St><<<nb, 256, 0, g->stream>>>((const typename St::T*)g->f, shift_of(g), ncells(g), g->flags,
-                                                       I.cellCountX, I.cellCountY, lo[0], lo[1], lo[2], hi[0] - lo[0],
-                                                       hi[1] - lo[1], hi[2] - lo[2], g->partials);
-    PSL_LAUNCH_CHECK();
-    return PSL_OK;
-  }
-};
 template <class St> struct MassFn {
   static int run(psl_grid* g, uint32_t nb) {
     k_mass<St><<<nb, 256, 0, g->stream>>>((const typename St::T*)g->f, shift_of(g), ncells(g), g->flags, g->partials);
@@ -952,29 +942,110 @@ int psl_grid_device_f(psl_grid* g, void** dev_ptr, size_t* bytes) {
   return PSL_OK;
 }
 
-int psl_reduce_momentum_exchange(psl_grid* g, const int* lo3, const int* hi3, double* out3) {
-  PSL_CHECK_ARG(g && out3, "null argument");
+// The box [lo, hi) of a momentum-exchange call clipped to the grid; false when
+// it is empty (or the grid has no flags), with the result then zero.
+static bool momentum_box(const psl_grid* g, const int* lo3, const int* hi3, int (&lo)[3], int (&hi)[3],
+                         uint32_t& nb) {
   const psl_info& I = g->info;
-  int lo[3] = {0, 0, 0}, hi[3] = {I.cellCountX, I.cellCountY, I.cellCountZ};
+  lo[0] = lo[1] = lo[2] = 0;
+  hi[0] = I.cellCountX;
+  hi[1] = I.cellCountY;
+  hi[2] = I.cellCountZ;
   if (lo3) for (int a = 0; a < 3; ++a) lo[a] = std::max(0, lo3[a]);
   if (hi3) {
     hi[0] = std::min(I.cellCountX, hi3[0]);
     hi[1] = std::min(I.cellCountY, hi3[1]);
     hi[2] = std::min(I.cellCountZ, hi3[2]);
   }
-  out3[0] = out3[1] = out3[2] = 0.0;
-  if (!g->flags || hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2]) return PSL_OK;
-  const uint64_t total = (uint64_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
-  const uint32_t nb = blocks_for(total);
-  int rc = ensure_partials(g, 3ull * nb);
-  if (rc) return rc;
-  rc = dispatch_storage<MomentumFn>(g->storage, g, (const int*)lo, (const int*)hi, nb);
-  if (rc) return rc;
-  k_sum_partials<3><<<1, 256, 0, g->stream>>>(g->partials, nb, g->result);
+  if (!g->flags || hi[0] <= lo[0] || hi[1] <= lo[1] || hi[2] <= lo[2]) return false;
+  nb = blocks_for((uint64_t)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]));
+  return true;
+}
+
+// Per-block partials into `partials` (3 nb doubles), their fixed-order sum
+// into `result` (3 doubles), and the copy to the host: the kernels and block
+// count of both momentum-exchange entry points, so they return the same bits.
+static int momentum_exchange_into(psl_grid* g, const int (&lo)[3], const int (&hi)[3], uint32_t nb, double* partials,
+                                  double* result, double* out3) {
+  const psl_info& I = g->info;
+#define PSL_MOMENTUM_LAUNCH(ST)                                                                                   \
+  k_momentum_exchange<ST><<<nb, 256, 0, g->stream>>>((const typename ST::T*)g->f, shift_of(g), ncells(g), g->flags, \
+                                                     I.cellCountX, I.cellCountY, lo[0], lo[1], lo[2], hi[0] - lo[0],  \
+                                                     hi[1] - lo[1], hi[2] - lo[2], partials)
+  if (g->storage == PSL_F32) PSL_MOMENTUM_LAUNCH(StF32);
+  else if (g->storage == PSL_F16DEV) PSL_MOMENTUM_LAUNCH(StF16);
+  else if (g->storage == PSL_BF16DEV) PSL_MOMENTUM_LAUNCH(StBF16);
+  else {
+    set_error("unknown storage");
+    return PSL_E_ARG;
+  }
+#undef PSL_MOMENTUM_LAUNCH
   PSL_LAUNCH_CHECK();
-  PSL_HIP_TRY(hipMemcpyAsync(out3, g->result, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  k_sum_partials<3><<<1, 256, 0, g->stream>>>(partials, nb, result);
+  PSL_LAUNCH_CHECK();
+  PSL_HIP_TRY(hipMemcpyAsync(out3, result, 3 * sizeof(double), hipMemcpyDeviceToHost, g->stream));
   PSL_HIP_TRY(hipStreamSynchronize(g->stream));
   return PSL_OK;
+}
+
+int psl_reduce_momentum_exchange(psl_grid* g, const int* lo3, const int* hi3, double* out3) {
+  PSL_CHECK_ARG(g && out3, "null argument");
+  int lo[3], hi[3];
+  uint32_t nb = 0;
+  out3[0] = out3[1] = out3[2] = 0.0;
+  if (!momentum_box(g, lo3, hi3, lo, hi, nb)) return PSL_OK;
+  int rc = ensure_partials(g, 3ull * nb);
+  if (rc) return rc;
+  return momentum_exchange_into(g, lo, hi, nb, g->partials, g->result, out3);
+}
+
+// ------------------------------------------- momentum exchange, no allocation
+// Scratch of psl_reduce_momentum_exchange_noalloc: 3 FP64 partials per block
+// of 256 box cells, then the 3 sums.  Every device the library runs on holds
+// its own copy until the process ends.  In host code `scratch` is only a
+// symbol: its device address comes from hipGetSymbolAddress, once per device.
+constexpr uint32_t NOALLOC_MAX_BLOCKS = PSL_NOALLOC_MAX_BOX_CELLS / 256;
+static __device__ __align__(256) double scratch[3 * NOALLOC_MAX_BLOCKS + 4];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[PSL_NOALLOC_MAX_DEVICES] = {};
+
+static int scratch_of_current_device(double** out) {
+  int dev = -1;
+  PSL_HIP_TRY(hipGetDevice(&dev));
+  PSL_CHECK_ARG(dev >= 0 && dev < PSL_NOALLOC_MAX_DEVICES, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return PSL_E_HIP;
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return PSL_OK;
+}
+
+int psl_reduce_momentum_exchange_noalloc(psl_grid* g, const int* lo3, const int* hi3, double* out3) {
+  PSL_CHECK_ARG(g && out3, "null argument");
+  int lo[3], hi[3];
+  uint32_t nb = 0;
+  out3[0] = out3[1] = out3[2] = 0.0;
+  if (!momentum_box(g, lo3, hi3, lo, hi, nb)) return PSL_OK;
+  if (nb > NOALLOC_MAX_BLOCKS) {
+    set_error("psl_reduce_momentum_exchange_noalloc: box of " + std::to_string((uint64_t)nb * 256) +
+              " cells (rounded up to 256) exceeds the scratch limit of " +
+              std::to_string((uint64_t)PSL_NOALLOC_MAX_BOX_CELLS) + " cells");
+    return PSL_E_ARG;
+  }
+  double* sp = nullptr;
+  const int rc = scratch_of_current_device(&sp);
+  if (rc) return rc;
+  // every block writes its partials and the sum kernel overwrites the result,
+  // so nothing left by an earlier call is read
+  return momentum_exchange_into(g, lo, hi, nb, sp, sp + 3ull * NOALLOC_MAX_BLOCKS, out3);
 }
 
 int psl_reduce_mass(psl_grid* g, double* out) {
