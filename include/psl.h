@@ -171,6 +171,14 @@ int psl_grid_profile_read(psl_grid* grid, double* total_ms, int* launches);
  * them back, no collision, `launches` times.  The data-movement ceiling of the
  * local update at that size and storage (no reference counterpart). */
 int psl_probe_ps_pattern(int nx, int ny, int nz, int storage, int launches, double* gbs);
+/* The same access pattern taken apart (no collision, on a scratch allocation of
+ * 27 rows of nx*ny*nz elements): `streams` populations (1, 2, 4, 8, 16 or 27); mode 0 reads then
+ * writes in place, 1 only reads, 2 only writes; cells_per_lane 1, 2 or 4 consecutive
+ * cells per lane as one access (16-bit storage pairs cells only with
+ * line_aligned); line_aligned rounds the shifts down to whole 256-byte runs.
+ * gbs = bytes moved per launch / mean launch time. */
+int psl_probe_ps_access(int nx, int ny, int nz, int storage, int streams, int mode, int cells_per_lane,
+                        int line_aligned, int launches, double* gbs);
 int psl_device_copy(void* dst, const void* src, size_t bytes);
 /* Device buffer helpers for callers without a device allocator. */
 int psl_device_alloc(size_t bytes, void** out);
@@ -206,18 +214,27 @@ int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* bou
 /* ------------------------------------------------------- batched grids */
 /* Many independent grids advanced together (no reference counterpart: the
  * reference steps one grid at a time).  Each grid keeps its own flags, given
- * values, Cs and storage (all grids share the storage); one step of all grids
- * is a single replay of a captured hipGraph: a device shift update, one bulk
- * launch over the blocks of every grid and one launch over all boundary lists
- * (or one fused launch).  Every grid's result is bit-identical to stepping it
- * alone.  Grids need flags and must not use per-cell Cs, periodic copies or
- * sub-grids.  The grids' own handles stay valid (get_f, reductions,
- * single-grid steps); changes to a grid's flags, values, nu or Cs are picked
- * up at the next psl_batch_step.  Destroy the batch before its grids. */
+ * values, Cs (uniform or per cell) and storage (all grids share the storage);
+ * one step of all grids is a device shift update, one bulk launch over the
+ * blocks of every grid and one launch over all boundary lists (or one fused
+ * launch).  Steps are captured in hipGraphs of one step and of
+ * steps_per_graph steps (default 1000) and replayed, so nsteps steps take
+ * nsteps / steps_per_graph + nsteps % steps_per_graph graph launches.  Every
+ * grid's result is bit-identical to stepping it alone.  Grids need flags and
+ * must not use periodic copies, sub-grids or a learned closure (checked at
+ * every psl_batch_step).  The grids' own handles stay valid (get_f,
+ * reductions, single-grid steps); changes to a grid's flags, values, nu or Cs
+ * are picked up at the next psl_batch_step.  Destroy the batch before its
+ * grids. */
 typedef struct psl_batch psl_batch;
 int psl_batch_create(psl_grid* const* grids, int count, psl_batch** out);
 int psl_batch_destroy(psl_batch* batch);
 int psl_batch_step(psl_batch* batch, int nsteps);
+/* Steps captured in the multi-step graph, 1 .. PSL_BATCH_MAX_STEPS_PER_GRAPH (1: one
+ * graph launch per step). */
+#define PSL_BATCH_MAX_STEPS_PER_GRAPH 10000
+int psl_batch_set_steps_per_graph(psl_batch* batch, int steps);
+int psl_batch_get_steps_per_graph(const psl_batch* batch, int* steps);
 /* order 0: grids in the given order, 1: largest grid first (block order of the launch). */
 int psl_batch_set_tuning(psl_batch* batch, int tuning, int order);
 int psl_batch_get_tuning(const psl_batch* batch, int* tuning, int* order);
@@ -269,6 +286,13 @@ int psl_les_targets(const float* coarse, int m, float nu, float* feat, float* ta
 /* Use the network in the grid's bulk update (NULL: back to Smagorinsky).  The
  * network must outlive its use; the fused-boundary variant is not used. */
 int psl_grid_set_closure(psl_grid* grid, const psl_mlp* mlp);
+/* Evaluate the grid's network every `steps` local updates (default 1).  The evaluating
+ * update stores each cell's network output as FP16 and relaxes with that rounded value;
+ * the updates between read it back and relax with the same eddy viscosity (the network's
+ * cost is paid once per `steps`).  The next update after this call evaluates. */
+int psl_grid_set_closure_every(psl_grid* grid, int steps);
+/* The held network output per cell (FP32 host array of cellCount values). */
+int psl_grid_get_closure_y(psl_grid* grid, float* host);
 
 /* --------------------------------------------- esotwist (DIAD) grids */
 /* DIADGridStruct (include/types.h:166-171) from host cell tables: ijk3 = 3 arrays of n

@@ -54,6 +54,8 @@ def lib():
 
 def _declare(L):
     P = ctypes.c_void_p
+    L.oracle_build_flags.restype = ctypes.c_char_p
+    L.oracle_build_flags.argtypes = []
     L.oracle_velocity_set.argtypes = [P, P, P]
     for suf, R in (("f32", ctypes.c_float), ("f64", ctypes.c_double)):
         getattr(L, f"oracle_feq_{suf}").argtypes = [P, P, ctypes.c_longlong]
@@ -458,3 +460,8 @@ def diad_build(base, levels, span, nu_phys, stls=(), rules=None):
         return out
     finally:
         L.oracle_diad_free(h)
+
+
+def build_flags() -> str:
+    """The compiler flags the oracle library was built with."""
+    return lib().oracle_build_flags().decode()

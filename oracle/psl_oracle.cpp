@@ -44,6 +44,10 @@
 #include <string>
 #include <vector>
 
+#ifndef PSL_ORACLE_FLAGS
+#define PSL_ORACLE_FLAGS "unknown"
+#endif
+
 namespace {
 
 // ---------------------------------------------------------------- velocity set
@@ -1922,6 +1926,9 @@ void oracle_markers_inside_stl(const float* tri, int ntri, const float* stl_orig
 }
 
 int oracle_check_stl_edges(const float* tri, int ntri, int* counts) { return check_stl_edges(tri, ntri, counts); }
+
+// Compiler flags of this build (the CPU baseline reports them).
+const char* oracle_build_flags() { return PSL_ORACLE_FLAGS; }
 
 int oracle_esotwist_tables(int* read_slot, int* write_slot) {
   for (int q = 0; q < 27; ++q) {

@@ -172,6 +172,8 @@ _SIGS = [
     ("psl_batch_create", _I, [_P, _I, _P]),
     ("psl_batch_destroy", _I, [_P]),
     ("psl_batch_step", _I, [_P, _I]),
+    ("psl_batch_set_steps_per_graph", _I, [_P, _I]),
+    ("psl_batch_get_steps_per_graph", _I, [_P, _P]),
     ("psl_batch_set_tuning", _I, [_P, _I, _I]),
     ("psl_batch_get_tuning", _I, [_P, _P, _P]),
     ("psl_batch_autotune", _I, [_P, _I, _P, _P, _P]),
@@ -188,9 +190,12 @@ _SIGS = [
     ("psl_les_filter", _I, [_P, _P, _P, _P, _I, _I, _I, _I, _P]),
     ("psl_les_targets", _I, [_P, _I, ctypes.c_float, _P, _P]),
     ("psl_grid_set_closure", _I, [_P, _P]),
+    ("psl_grid_set_closure_every", _I, [_P, _I]),
+    ("psl_grid_get_closure_y", _I, [_P, _P]),
     ("psl_get_macroscopic_device", _I, [_P, _P]),
     ("psl_grid_time_update_kernel", _I, [_P, _I, _P, _P]),
     ("psl_probe_ps_pattern", _I, [_I, _I, _I, _I, _I, _P]),
+    ("psl_probe_ps_access", _I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _P]),
     ("psl_diad_create", _I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     ("psl_diad_destroy", _I, [_P]),
     ("psl_diad_set_flags", _I, [_P, _P]),
@@ -402,6 +407,17 @@ class GridStruct:
         self._closure = mlp  # keep the network alive while the grid uses it
         _check(lib().psl_grid_set_closure(self._h, None if mlp is None else mlp.handle))
 
+    def set_closure_every(self, steps: int):
+        """Evaluate the network every `steps` local updates and hold its output
+        (FP16 per cell) in between; 1 evaluates it every update."""
+        _check(lib().psl_grid_set_closure_every(self._h, int(steps)))
+
+    def closure_y(self) -> np.ndarray:
+        """The held network output per cell (after a step with set_closure_every > 1)."""
+        out = np.zeros(self.n, np.float32)
+        _check(lib().psl_grid_get_closure_y(self._h, out.ctypes.data))
+        return out
+
     def device_f(self):
         p = ctypes.c_void_p()
         nbytes = ctypes.c_size_t()
@@ -460,7 +476,8 @@ class GridStruct:
 
 
 class Batch:
-    """Many GridStructs advanced together by one hipGraph replay per step."""
+    """Many GridStructs advanced together by replays of captured hipGraphs
+    (one of a single step, one of steps_per_graph steps)."""
 
     def __init__(self, grids: Sequence[GridStruct]):
         self.grids = list(grids)  # keep the grids alive
@@ -481,6 +498,16 @@ class Batch:
 
     def step(self, n: int = 1):
         _check(lib().psl_batch_step(self._h, n))
+
+    @property
+    def steps_per_graph(self) -> int:
+        v = ctypes.c_int()
+        _check(lib().psl_batch_get_steps_per_graph(self._h, ctypes.byref(v)))
+        return v.value
+
+    @steps_per_graph.setter
+    def steps_per_graph(self, steps: int):
+        _check(lib().psl_batch_set_steps_per_graph(self._h, int(steps)))
 
     def set_tuning(self, tuning: int, order: int = 0):
         _check(lib().psl_batch_set_tuning(self._h, tuning, order))
@@ -567,15 +594,19 @@ class DIADGridStruct:
 
     def set_moving_bounceback(self, cells, u3):
         """Moving-wall cells (flagged BOUNCEBACK) and their velocities (n x 3)."""
-        c = np.ascontiguousarray(cells, np.int32)
+        c = np.ascontiguousarray(cells, np.int32).reshape(-1)
         u = np.ascontiguousarray(u3, np.float32).reshape(-1)
+        if u.size != 3 * c.size:
+            raise ValueError(f"{c.size} moving-wall cells need {3 * c.size} velocity components, got {u.size}")
         _check(lib().psl_diad_set_moving_bounceback(self._h, c.ctypes.data, c.size, u.ctypes.data))
 
     def set_forced_velocity(self, cells, ut3):
         """Forced-velocity cells (flagged BOUNCEBACK) and their target
         velocities (n x 3, lattice units)."""
-        c = np.ascontiguousarray(cells, np.int32)
+        c = np.ascontiguousarray(cells, np.int32).reshape(-1)
         u = np.ascontiguousarray(ut3, np.float32).reshape(-1)
+        if u.size != 3 * c.size:
+            raise ValueError(f"{c.size} forced cells need {3 * c.size} velocity components, got {u.size}")
         _check(lib().psl_diad_set_forced_velocity(self._h, c.ctypes.data, c.size, u.ctypes.data))
 
     def forcing(self, rho_nominal_phys: float) -> np.ndarray:
@@ -1037,6 +1068,23 @@ def probe_ps_pattern(nx: int, ny: int, nz: int, storage="f32", launches: int = 2
     require_device()
     g = ctypes.c_double()
     _check(lib().psl_probe_ps_pattern(nx, ny, nz, storage, launches, ctypes.byref(g)))
+    return g.value
+
+
+PROBE_MODES = {"inplace": 0, "read": 1, "write": 2}
+
+
+def probe_ps_access(nx: int, ny: int, nz: int, storage="f32", streams: int = 27, mode: str = "inplace",
+                    cells_per_lane: int = 1, line_aligned: bool = False, launches: int = 20) -> float:
+    """HBM rate (GB/s of bytes moved) of the periodic-shift access pattern taken
+    apart: `streams` populations, read + write in place / read only / write
+    only, several consecutive cells per lane, shifts rounded to whole lines."""
+    if isinstance(storage, str):
+        storage = STORAGE_NAMES[storage]
+    require_device()
+    g = ctypes.c_double()
+    _check(lib().psl_probe_ps_access(nx, ny, nz, storage, int(streams), PROBE_MODES[mode], int(cells_per_lane),
+                                     int(bool(line_aligned)), launches, ctypes.byref(g)))
     return g.value
 
 

@@ -16,9 +16,10 @@
 //
 // The per-cell work is bulk_update / boundary_update, the same functions on
 // the same StepParams / BoundaryParams as the single-grid kernels, so every
-// grid's result is bit-identical to stepping it alone.  One step is captured
-// once in a hipGraph and replayed, so a step costs one graph launch whatever
-// the number of grids.  The block order of the grids (as given, or largest
+// grid's result is bit-identical to stepping it alone.  Steps are captured
+// in hipGraphs, one of a single step and one of steps_per_graph (default
+// 1000) consecutive steps, and replayed: 5000 steps are five graph launches
+// whatever the number of grids.  The block order of the grids (as given, or largest
 // first) and the kernel variant are chosen by psl_batch_autotune.
 #include "psl_internal.h"
 #include "psl_stream.h"
@@ -118,8 +119,11 @@ struct psl_batch {
   BatchEntry* d_order = nullptr;
   uint32_t total_blocks = 0, total_bblocks = 0, total_bnd = 0, total_fused = 0;
   int64_t total_cells = 0;
-  hipGraph_t graph = nullptr;
+  hipGraph_t graph = nullptr;  // one step
   hipGraphExec_t exec = nullptr;
+  int steps_per_graph = 1000;  // steps captured in graph_k
+  hipGraph_t graph_k = nullptr;
+  hipGraphExec_t exec_k = nullptr;
   // what the captured graph was built from
   std::vector<StepParams> h_params;
   std::vector<BoundaryParams> h_bparams;
@@ -128,11 +132,19 @@ struct psl_batch {
 
 namespace {
 
-void free_batch_device(psl_batch* b) {
+void free_graphs(psl_batch* b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->exec_k) (void)hipGraphExecDestroy(b->exec_k);
+  if (b->graph_k) (void)hipGraphDestroy(b->graph_k);
   b->exec = nullptr;
   b->graph = nullptr;
+  b->exec_k = nullptr;
+  b->graph_k = nullptr;
+}
+
+void free_batch_device(psl_batch* b) {
+  free_graphs(b);
   void* ptrs[] = {b->d_params,       b->d_bparams,      b->d_fbase,         b->d_delta,          b->d_shift,
                   b->d_block_domain, b->d_block_first, b->d_bblock_domain, b->d_bblock_first, b->d_order};
   for (void* p : ptrs)
@@ -200,6 +212,36 @@ int upload(T** dst, const std::vector<T>& src) {
   return PSL_OK;
 }
 
+// Capture `steps` consecutive steps into one graph (the kernels read all
+// state through device tables, so replaying it advances every grid `steps`
+// times) and instantiate it.
+int capture_steps(psl_batch* b, int steps, hipGraph_t* graph_out, hipGraphExec_t* exec_out) {
+  hipStream_t cap = nullptr;
+  PSL_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipStream_t saved = b->stream;
+  b->stream = cap;
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  int rc = PSL_OK;
+  for (int s = 0; e == hipSuccess && rc == PSL_OK && s < steps; ++s) rc = enqueue_dispatch(b);
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(cap, &graph);
+  b->stream = saved;
+  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess || e2 != hipSuccess || rc != PSL_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    set_error("batch graph capture failed");
+    return rc ? rc : PSL_E_HIP;
+  }
+  const hipError_t e3 = hipGraphInstantiate(exec_out, graph, nullptr, nullptr, 0);
+  if (e3 != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    set_error(std::string("batch graph instantiation failed: ") + hipGetErrorString(e3));
+    return PSL_E_HIP;
+  }
+  *graph_out = graph;
+  return PSL_OK;
+}
+
 // Host-side StepParams / BoundaryParams of a grid (the stream bases are
 // filled in on the device each step).
 StepParams step_params_of(const psl_grid* g) {
@@ -210,7 +252,7 @@ StepParams step_params_of(const psl_grid* g) {
   p.tau = tau;
   p.omega = 1.f / tau;
   p.cs = g->cs;
-  p.cs_cell = nullptr;
+  p.cs_cell = g->cs_cell;
   p.flags = g->flags;
   return p;
 }
@@ -290,26 +332,7 @@ int build_batch(psl_batch* b) {
   if (!rc) rc = upload(&b->d_order, fused);
   if (!rc) rc = upload(&b->d_shift, std::vector<int>((size_t)nd * 27, 0));
   if (rc) return rc;
-  // capture one step; the graph's kernels read all state through device tables
-  hipStream_t cap = nullptr;
-  PSL_HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-  hipStream_t saved = b->stream;
-  b->stream = cap;
-  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-  rc = PSL_OK;
-  if (e == hipSuccess) rc = enqueue_dispatch(b);
-  hipGraph_t graph = nullptr;
-  hipError_t e2 = hipStreamEndCapture(cap, &graph);
-  b->stream = saved;
-  (void)hipStreamDestroy(cap);
-  if (e != hipSuccess || e2 != hipSuccess || rc != PSL_OK) {
-    if (graph) (void)hipGraphDestroy(graph);
-    set_error("batch graph capture failed");
-    return rc ? rc : PSL_E_HIP;
-  }
-  b->graph = graph;
-  PSL_HIP_TRY(hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0));
-  return PSL_OK;
+  return capture_steps(b, 1, &b->graph, &b->exec);
 }
 
 // True when a grid changed since the graph was captured (new flags, boundary
@@ -321,18 +344,17 @@ bool batch_stale(const psl_batch* b) {
     const BoundaryParams q = boundary_params_of(g);
     const StepParams& P = b->h_params[d];
     const BoundaryParams& Q = b->h_bparams[d];
-    if (p.n != P.n || p.tau != P.tau || p.cs != P.cs || p.flags != P.flags || q.list != Q.list || q.n != Q.n ||
-        q.bc != Q.bc || g->f != b->h_fbase[d] || g->cs_cell || g->periodic_copies)
+    if (p.n != P.n || p.tau != P.tau || p.cs != P.cs || p.cs_cell != P.cs_cell || p.flags != P.flags ||
+        q.list != Q.list || q.n != Q.n || q.bc != Q.bc || g->f != b->h_fbase[d])
       return true;
   }
   return false;
 }
 
 // A grid the batch kernels can advance: flags set (the boundary list is
-// built from them), no per-cell Cs, no periodic copies, no sub-grid.
+// built from them), no learned closure, no periodic copies, no sub-grid.
 const char* unbatchable(const psl_grid* g) {
   if (!g->flags) return "batch grids need flags (psl_grid_set_flags)";
-  if (g->cs_cell) return "batch grids cannot use per-cell Cs";
   if (g->closure) return "batch grids cannot use the learned closure";
   if (g->periodic_copies) return "batch grids cannot use periodic copies";
   if (g->info.iSubgridEnd - g->info.iSubgridStart > 4) return "batch grids cannot carry a sub-grid";
@@ -392,20 +414,31 @@ int psl_batch_destroy(psl_batch* b) {
 int psl_batch_step(psl_batch* b, int nsteps) {
   PSL_CHECK_ARG(b && nsteps >= 0, "bad batch step");
   if (nsteps == 0) return PSL_OK;
-  if (batch_stale(b)) {
-    for (const psl_grid* g : b->grids) {
-      const char* why = unbatchable(g);
-      if (why) {
-        set_error(why);
-        return PSL_E_ARG;
-      }
+  // a closure, periodic copies or a sub-grid set on a grid after the batch
+  // was created are refused here, whatever the captured graph holds
+  for (const psl_grid* g : b->grids) {
+    const char* why = unbatchable(g);
+    if (why) {
+      set_error(why);
+      return PSL_E_ARG;
     }
+  }
+  if (batch_stale(b)) {
     int rc = build_batch(b);
     if (rc) return rc;
   }
   int rc = upload_shifts(b);
   if (rc) return rc;
-  for (int s = 0; s < nsteps; ++s) PSL_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  // whole graphs of steps_per_graph steps, then single steps
+  const int k = b->steps_per_graph;
+  if (k > 1 && nsteps >= k && !b->exec_k) {
+    rc = capture_steps(b, k, &b->graph_k, &b->exec_k);
+    if (rc) return rc;
+  }
+  int left = nsteps;
+  if (k > 1)
+    for (; left >= k; left -= k) PSL_HIP_TRY(hipGraphLaunch(b->exec_k, b->stream));
+  for (; left > 0; --left) PSL_HIP_TRY(hipGraphLaunch(b->exec, b->stream));
   for (psl_grid* g : b->grids)
     for (int s = 0; s < nsteps; ++s) psl_apply_streaming(g);
   return PSL_OK;
@@ -418,6 +451,24 @@ int psl_batch_set_tuning(psl_batch* b, int tuning, int order) {
   b->tuning = tuning;
   b->order = order;
   return build_batch(b);
+}
+
+int psl_batch_set_steps_per_graph(psl_batch* b, int steps) {
+  PSL_CHECK_ARG(b && steps >= 1 && steps <= PSL_BATCH_MAX_STEPS_PER_GRAPH, "steps per graph out of range");
+  if (steps == b->steps_per_graph) return PSL_OK;
+  PSL_HIP_TRY(hipStreamSynchronize(b->stream));
+  if (b->exec_k) (void)hipGraphExecDestroy(b->exec_k);
+  if (b->graph_k) (void)hipGraphDestroy(b->graph_k);
+  b->exec_k = nullptr;
+  b->graph_k = nullptr;
+  b->steps_per_graph = steps;
+  return PSL_OK;
+}
+
+int psl_batch_get_steps_per_graph(const psl_batch* b, int* steps) {
+  PSL_CHECK_ARG(b && steps, "null argument");
+  *steps = b->steps_per_graph;
+  return PSL_OK;
 }
 
 int psl_batch_get_tuning(const psl_batch* b, int* tuning, int* order) {

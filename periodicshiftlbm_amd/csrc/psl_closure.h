@@ -204,7 +204,8 @@ __device__ __forceinline__ float closure_wave(const MlpNet* __restrict__ net, co
 // (they run it on rest-state inputs and discard the result).
 template <class St, bool FLAGS, int TUNE>
 __device__ __forceinline__ void bulk_update_closure(const StepParams& p, uint32_t block,
-                                                    const MlpNet* __restrict__ net, float nu, TanhTable& tt) {
+                                                    const MlpNet* __restrict__ net, float nu, TanhTable& tt,
+                                                    uint16_t* __restrict__ y_out) {
   tanh_table_fill(tt);  // before any lane may leave (barrier)
   const uint32_t block_first = block * 256u;
   const uint32_t cell = block_first + threadIdx.x;
@@ -229,16 +230,20 @@ __device__ __forceinline__ void bulk_update_closure(const StepParams& p, uint32_
   const Moments m = raw_moments_words<St>(r);
   float raw[MLP_IN];
   cell_closure_features<St::W_OFF>(m, raw);
-  const float y = closure_wave(net, tt, raw);
+  float y = closure_wave(net, tt, raw);
   if (!active) return;
+  if (y_out) {
+    // held for the next steps as FP16; this step uses the same rounded value
+    const _Float16 h = (_Float16)y;
+    y_out[cell] = __builtin_bit_cast(uint16_t, h);
+    y = (float)h;
+  }
   float v[27];
   if (kind == KIND_BOUNCEBACK) {
     words_to_float<St>(r, v);
     bounceback(v);
   } else {
-    const float tau = 3.f * (nu + MLP_NU_SCALE * fmaxf(y, 0.f)) + 0.5f;
-    Relax rx{1.f / tau, tau, 0.f};
-    collide_with_moments<St::W_OFF>(v, m, rx, 0.f);
+    collide_with_moments<St::W_OFF>(v, m, closure_relax(nu, MLP_NU_SCALE, y), 0.f);
   }
   store27<St, TUNE>(p, w_first, straddle, cell, v);
 }

@@ -624,8 +624,23 @@ int psl_diad_destroy(psl_diad* d) {
   return PSL_OK;
 }
 
+// Moving-wall and forced cells are updated by their own list kernels and must
+// be skipped by the local update, so they must be flagged bounceback.
+static int lists_are_bounceback(const psl_diad* d, const uint8_t* flags) {
+  for (const std::vector<int>* list : {&d->moving_host, &d->forced_host})
+    for (int c : *list)
+      if (!flags || (flags[c] & 7) != KIND_BOUNCEBACK) {
+        set_error("moving-wall and forced-velocity cells must be flagged PSL_BOUNCEBACK (cell " + std::to_string(c) +
+                  ")");
+        return PSL_E_ARG;
+      }
+  return PSL_OK;
+}
+
 int psl_diad_set_flags(psl_diad* d, const uint8_t* host) {
   PSL_CHECK_ARG(d, "null grid");
+  const int rc = lists_are_bounceback(d, host);
+  if (rc) return rc;
   if (!host) {
     if (d->flags) PSL_HIP_TRY(hipFree(d->flags));
     d->flags = nullptr;
@@ -782,6 +797,15 @@ static int set_cell_list(psl_diad* d, const int* cells, int64_t n, const float* 
       PSL_CHECK_ARG(seen[cells[t]] != 2, std::string(what) + " cell is also a moving-wall or forced cell");
       seen[cells[t]] = 1;
     }
+  }
+  if (n) {
+    // the listed cells must be flagged bounceback (flags as the device holds them)
+    PSL_CHECK_ARG(d->flags, std::string(what) + " cells need flags (psl_diad_set_flags) first");
+    std::vector<uint8_t> fl((size_t)d->n);
+    PSL_HIP_TRY(hipMemcpy(fl.data(), d->flags, (size_t)d->n, hipMemcpyDeviceToHost));
+    for (int64_t t = 0; t < n; ++t)
+      PSL_CHECK_ARG((fl[cells[t]] & 7) == KIND_BOUNCEBACK,
+                    std::string(what) + " cell " + std::to_string(cells[t]) + " is not flagged PSL_BOUNCEBACK");
   }
   *dev_n = 0;
   host->clear();

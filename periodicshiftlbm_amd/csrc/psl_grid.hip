@@ -72,9 +72,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
 // Bulk cells with the learned closure in place of the Smagorinsky rate.
 template <class St, bool FLAGS, int TUNE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ps_bulk_closure(
-    StepParams p, const MlpNet* __restrict__ net, float nu) {
+    StepParams p, const MlpNet* __restrict__ net, float nu, uint16_t* __restrict__ y_out) {
   __shared__ TanhTable tt;
-  bulk_update_closure<St, FLAGS, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, net, nu, tt);
+  bulk_update_closure<St, FLAGS, TUNE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, net, nu, tt,
+                                       y_out);
 }
 
 // One workgroup per entry of the fused order (fused_order): boundary-list
@@ -508,9 +509,9 @@ static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, c
 
 template <class St, int TUNE>
 static void launch_closure(bool flags, uint32_t nb, hipStream_t stream, const StepParams& p, const MlpNet* net,
-                           float nu) {
-  if (flags) k_ps_bulk_closure<St, true, TUNE><<<nb, 256, 0, stream>>>(p, net, nu);
-  else k_ps_bulk_closure<St, false, TUNE><<<nb, 256, 0, stream>>>(p, net, nu);
+                           float nu, uint16_t* y_out) {
+  if (flags) k_ps_bulk_closure<St, true, TUNE><<<nb, 256, 0, stream>>>(p, net, nu, y_out);
+  else k_ps_bulk_closure<St, false, TUNE><<<nb, 256, 0, stream>>>(p, net, nu, y_out);
 }
 
 template <class St, int TUNE>
@@ -539,7 +540,10 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
   p.cs_cell = g->cs_cell;
   p.flags = g->flags;
   p.xcd_remap = (g->tuning & PSL_TUNE_XCD_REMAP) ? 1u : 0u;
-  const bool les = g->cs != 0.f || g->cs_cell != nullptr;
+  p.nut_y = nullptr;
+  p.nu = g->info.nu;
+  p.nut_scale = MLP_NU_SCALE;
+  bool les = g->cs != 0.f || g->cs_cell != nullptr;
   BoundaryParams b;
   b.list = g->bc_list;
   b.n = g->flags ? g->n_bc : 0u;
@@ -548,7 +552,21 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
   b.nz = g->info.cellCountZ;
   b.bc = g->bc_dev;
   const MlpNet* net = psl_mlp_device_net(g->closure);
-  const bool fused = b.n && (g->tuning & PSL_TUNE_FUSED_BOUNDARY) && !net;
+  // a closure evaluated every closure_every steps: the evaluating step stores
+  // the network output per cell, the steps between relax with the stored one
+  uint16_t* y_out = nullptr;
+  if (net && g->closure_every > 1) {
+    if (g->closure_count % (uint64_t)g->closure_every == 0) {
+      y_out = g->closure_y;
+    } else {
+      p.nut_y = g->closure_y;
+      les = true;
+      net = nullptr;
+    }
+    ++g->closure_count;
+  }
+  const bool held = p.nut_y != nullptr;
+  const bool fused = b.n && (g->tuning & PSL_TUNE_FUSED_BOUNDARY) && !net && !held;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (g->profile) {
     PSL_HIP_TRY(hipEventCreate(&ev0));
@@ -561,10 +579,10 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
     const bool fl = g->flags != nullptr;
     const float nu = g->info.nu;
     switch (nt) {
-      case 0: launch_closure<St, 0>(fl, nb, g->stream, p, net, nu); break;
-      case 1: launch_closure<St, 1>(fl, nb, g->stream, p, net, nu); break;
-      case 2: launch_closure<St, 2>(fl, nb, g->stream, p, net, nu); break;
-      default: launch_closure<St, 3>(fl, nb, g->stream, p, net, nu); break;
+      case 0: launch_closure<St, 0>(fl, nb, g->stream, p, net, nu, y_out); break;
+      case 1: launch_closure<St, 1>(fl, nb, g->stream, p, net, nu, y_out); break;
+      case 2: launch_closure<St, 2>(fl, nb, g->stream, p, net, nu, y_out); break;
+      default: launch_closure<St, 3>(fl, nb, g->stream, p, net, nu, y_out); break;
     }
   } else if (fused) {
     const uint32_t nf = (uint32_t)g->fused_host.size();
@@ -577,7 +595,9 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
   } else {
     const bool fl = g->flags != nullptr;
     // the listed cells are staged by the bulk kernel whenever there are any
-    const Stage sg{b.n ? g->stage_buf : nullptr, g->stage_base};
+    // (not between closure evaluations: the closure path updates them from
+    // their slots)
+    const Stage sg{b.n && !held ? g->stage_buf : nullptr, g->stage_base};
     switch (nt) {
       case 0: launch_bulk<St, 0>(fl, les, nb, g->stream, p, sg); break;
       case 1: launch_bulk<St, 1>(fl, les, nb, g->stream, p, sg); break;
@@ -591,7 +611,7 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
     g->prof_events.emplace_back(ev0, ev1);
   }
   if (b.n && !fused && !dominant_only) {
-    if (net) k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->stream>>>(p, b);
+    if (net || held) k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->stream>>>(p, b);
     else k_ps_boundary_staged<St><<<(b.n + 63) / 64, 64, 0, g->stream>>>(p, b, g->stage_buf);
     PSL_LAUNCH_CHECK();
   }
@@ -734,6 +754,7 @@ int psl_grid_destroy(psl_grid* g) {
   if (g->stage_base) (void)hipFree(g->stage_base);
   if (g->cs_cell) (void)hipFree(g->cs_cell);
   if (g->partials) (void)hipFree(g->partials);
+  if (g->closure_y) (void)hipFree(g->closure_y);
   if (g->result) (void)hipFree(g->result);
   if (g->bc_dev) (void)hipFree(g->bc_dev);
   if (g->scalar_T) (void)hipFree(g->scalar_T);
@@ -1241,6 +1262,30 @@ int psl_grid_autotune(psl_grid* g, int steps, int* best, double* times_ms) {
 int psl_grid_set_closure(psl_grid* g, const psl_mlp* m) {
   PSL_CHECK_ARG(g, "null grid");
   g->closure = m;
+  g->closure_count = 0;  // the next step evaluates the network
+  return PSL_OK;
+}
+
+int psl_grid_set_closure_every(psl_grid* g, int steps) {
+  PSL_CHECK_ARG(g && steps >= 1, "closure_every must be at least 1");
+  if (steps > 1 && !g->closure_y) PSL_HIP_TRY(hipMalloc(&g->closure_y, (size_t)g->info.cellCount * sizeof(uint16_t)));
+  g->closure_every = steps;
+  g->closure_count = 0;
+  return PSL_OK;
+}
+
+int psl_grid_get_closure_y(psl_grid* g, float* host) {
+  PSL_CHECK_ARG(g && host, "null argument");
+  PSL_CHECK_ARG(g->closure_y, "the grid holds no network output (psl_grid_set_closure_every with steps > 1)");
+  const size_t n = (size_t)g->info.cellCount;
+  std::vector<uint16_t> h(n);
+  PSL_HIP_TRY(hipMemcpyAsync(h.data(), g->closure_y, n * sizeof(uint16_t), hipMemcpyDeviceToHost, g->stream));
+  PSL_HIP_TRY(hipStreamSynchronize(g->stream));
+  for (size_t c = 0; c < n; ++c) {
+    _Float16 v;
+    memcpy(&v, &h[c], sizeof(v));
+    host[c] = (float)v;
+  }
   return PSL_OK;
 }
 

@@ -123,6 +123,9 @@ struct psl_grid {
   int64_t census[3] = {0, 0, 0};  // bulk, boundary-list, skipped cells
   int tuning = PSL_TUNE_XCD_REMAP;  // bulk-kernel variant (psl_grid_set_tuning / autotune)
   const psl_mlp* closure = nullptr;  // learned closure (null: Smagorinsky)
+  int closure_every = 1;          // network evaluated every closure_every local updates
+  uint64_t closure_count = 0;     // local updates since the closure was set
+  uint16_t* closure_y = nullptr;  // held network output per cell (FP16 bits), closure_every > 1
   float* scalar_T = nullptr;      // scalar-transport populations (27 * N FP32, same layout as f)
   float* scalar_given = nullptr;  // prescribed scalar per cell (NaN: free), or null
   float tau_t = 1.f;              // scalar relaxation time

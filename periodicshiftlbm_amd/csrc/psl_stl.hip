@@ -415,7 +415,23 @@ int psl_stl_read(const char* path, psl_stl** out) {
     set_error("truncated STL header");
     return PSL_E_IO;
   }
-  std::vector<float> tri(9 * (size_t)count);
+  // the header's count must match the file (84 + 50 bytes per triangle)
+  // before anything is allocated for it
+  file.seekg(0, std::ios::end);
+  const std::streamoff size = file.tellg();
+  if (count > (uint32_t)std::numeric_limits<int>::max() || size != (std::streamoff)(84 + 50 * (uint64_t)count)) {
+    set_error("STL file " + std::string(path) + ": " + std::to_string(count) + " triangles in the header, " +
+              std::to_string((long long)size) + " bytes on disk (binary STL needs 84 + 50 per triangle)");
+    return PSL_E_IO;
+  }
+  file.seekg(84, std::ios::beg);
+  std::vector<float> tri;
+  try {
+    tri.resize(9 * (size_t)count);
+  } catch (const std::bad_alloc&) {
+    set_error("out of host memory for the STL triangles");
+    return PSL_E_NOMEM;
+  }
   for (uint32_t t = 0; t < count; ++t) {
     char rec[50];
     file.read(rec, 50);

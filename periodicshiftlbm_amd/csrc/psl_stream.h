@@ -32,7 +32,19 @@ struct StepParams {
   float cs;
   const float* cs_cell;
   const uint8_t* flags;
+  // learned closure held between network evaluations: per-cell network
+  // output y (FP16 bits); fluid cells relax with tau = 3 (nu + nut_scale
+  // max(y, 0)) + 1/2 and no Smagorinsky rate.  Null otherwise.
+  const uint16_t* nut_y;
+  float nu;
+  float nut_scale;
 };
+
+// tau of a cell under the learned closure's eddy viscosity (network output y)
+__device__ __forceinline__ Relax closure_relax(float nu, float nut_scale, float y) {
+  const float tau = 3.f * (nu + nut_scale * fmaxf(y, 0.f)) + 0.5f;
+  return Relax{1.f / tau, tau, 0.f};
+}
 
 __device__ __forceinline__ uint32_t wave_first_cell(uint32_t block_first) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -370,6 +382,10 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
     Relax rx{p.omega, p.tau, p.cs};
     float cs = p.cs;
     if (LES && p.cs_cell) cs = ld<float, false>(p.cs_cell + cell);
+    if (LES && p.nut_y) {
+      rx = closure_relax(p.nu, p.nut_scale, (float)__builtin_bit_cast(_Float16, ld<uint16_t, false>(p.nut_y + cell)));
+      cs = 0.f;
+    }
     // the collision reads only the moments; v receives the post-collision values
     collide_with_moments<St::W_OFF>(v, raw_moments_words<St>(r), rx, LES ? cs : 0.f);
   }

@@ -125,11 +125,13 @@ class Workflow:
     training -> coarse LES comparisons."""
 
     def __init__(self, P, n: int = 512, r: int = 4, k_peak: float = 8.0, u_rms: float = 0.04,
-                 nu_fine: float = 5e-4, cs_fine: float = 0.1, storage: str = "f16", seed: int = 7, log=print):
+                 nu_fine: float = 5e-4, cs_fine: float = 0.1, storage: str = "f16", seed: int = 7, log=print,
+                 closure_every: int = 1):
         self.P, self.n, self.r, self.m = P, n, r, n // r
         self.k_peak, self.u_rms, self.nu_fine, self.cs_fine = k_peak, u_rms, nu_fine, cs_fine
         self.nu_coarse = nu_fine / r
         self.storage, self.seed, self.log = storage, seed, log
+        self.closure_every = closure_every  # network evaluated every this many coarse steps
         self.snapshots = {}   # fine step -> (coarse rho, u) host arrays of the filtered field
         self.feats, self.tgts = [], []  # device buffers per snapshot
         self.timings = {}
@@ -205,6 +207,7 @@ class Workflow:
         if model == "mlp":
             net = self.scaled_net(param)
             g.set_closure(net)
+            g.set_closure_every(self.closure_every)
         later = sorted(k for k in self.snapshots if k > start)[every - 1::every]
         errs = {}
         done = start
@@ -277,9 +280,11 @@ class Workflow:
                      + (f" (param {out[k]['param']})" if "param" in out[k] else ""))
         return out
 
-    def cost_ratio(self, n: int = 256, launches: int = 50, storage: str = "f32", cs: float = 0.17):
+    def cost_ratio(self, n: int = 256, launches: int = 48, storage: str = "f32", cs: float = 0.17):
         """Mean bulk-kernel time with the network / with Smagorinsky on a periodic
-        n^3 box (launches back to back between two events)."""
+        n^3 box (launches back to back between two events; with the network
+        evaluated every closure_every steps, the mean over evaluating and
+        holding launches)."""
         P = self.P
         u = [np.zeros((n, n, n), np.float32)] * 3
         times = {}
@@ -287,6 +292,7 @@ class Workflow:
             g = periodic_grid(P, n, self.nu_coarse, storage, u, cs)
             if model == "mlp":
                 g.set_closure(self.net)
+                g.set_closure_every(self.closure_every)
             P.updateGrid([g], 2)
             times[model], _ = g.time_update_kernel(launches)
             g.close()

@@ -40,3 +40,19 @@ def test_oracle_library_is_test_only():
     path = P.LIB_PATH
     data = open(path, "rb").read()
     assert b"oracle_" not in data
+
+
+def test_stl_reader_rejects_inconsistent_header_without_gpu(tmp_path):
+    """A binary STL whose triangle count disagrees with its size (or exceeds
+    the int range) is refused before anything is allocated for it."""
+    import struct
+
+    import periodicshiftlbm_amd as P
+    L = P.lib()
+    h = ctypes.c_void_p()
+    for count, records in ((4_000_000_000, 1), (3, 2)):
+        path = tmp_path / f"bad{count}.stl"
+        path.write_bytes(b"\0" * 80 + struct.pack("<I", count) + b"\0" * (50 * records))
+        rc = L.psl_stl_read(str(path).encode(), ctypes.byref(h))
+        assert rc == -4, rc  # PSL_E_IO
+        assert b"bytes on disk" in L.psl_last_error()

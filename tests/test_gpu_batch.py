@@ -133,3 +133,53 @@ def test_batch_rejects_mixed_storage(psl):
     c = cases.domain_grid(doms[1], "f16")[0]
     with pytest.raises(RuntimeError):
         P.Batch([a, c])
+
+
+def test_batch_multi_step_graphs_and_per_cell_cs(psl):
+    """Graphs of several captured steps (steps_per_graph) advance every grid as
+    the same number of single steps, and a grid with a per-cell Smagorinsky
+    field (getSmagorinskyConstant per cell) keeps it inside the batch."""
+    P = psl
+    doms = _domains()
+    rng = np.random.default_rng(5)
+    cs_fields = {1: rng.uniform(0.0, 0.3, int(np.prod(doms[1]["shape"]))).astype(np.float32)}
+    batched, single = _grids(doms, "f16"), _grids(doms, "f16")
+    for i, cs in cs_fields.items():
+        batched[i].set_smagorinsky(cs)
+        single[i].set_smagorinsky(cs)
+    b = P.Batch(batched)
+    assert b.steps_per_graph == 1000
+    b.steps_per_graph = 7
+    print("[batch multi-step graphs] 7 steps per graph, 17 steps", flush=True)
+    b.step(17)  # two 7-step graphs and three single steps
+    for g in single:
+        P.updateGrid([g], 17)
+    _assert_identical(doms, batched, single, "7-step graphs")
+    b.steps_per_graph = 1
+    b.step(4)
+    for g in single:
+        P.updateGrid([g], 4)
+    _assert_identical(doms, batched, single, "one-step graphs")
+    # the per-cell field matters: the same grid with uniform Cs ends elsewhere
+    alt = _grids(doms[1:2], "f16")[0]
+    P.updateGrid([alt], 21)
+    assert np.count_nonzero(alt.get_f() != batched[1].get_f()) > 0
+    with pytest.raises(P.PslError):
+        b.steps_per_graph = 0
+    b.close()
+
+
+def test_batch_refuses_grid_changed_to_unbatchable(psl):
+    """Periodic copies set on a grid after the batch was created are refused
+    at the next step instead of being ignored by the captured graph."""
+    P = psl
+    doms = _domains()[:2]
+    grids = _grids(doms, "f32")
+    b = P.Batch(grids)
+    b.step(1)
+    grids[0].set_periodic_copies(y=True)
+    with pytest.raises(P.PslError):
+        b.step(1)
+    grids[0].set_periodic_copies()
+    b.step(1)
+    b.close()

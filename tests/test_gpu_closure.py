@@ -141,8 +141,10 @@ def test_les_filter_and_targets_match_oracle(psl):
     assert ferr < 1e-4
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_constant_network_equals_viscosity_shift(psl, storage):
+@pytest.mark.parametrize("storage,every", [("f32", 1), ("f16", 1), ("f16", 3)])
+def test_constant_network_equals_viscosity_shift(psl, storage, every):
+    """With the network evaluated every `every` steps and held in between (y0 =
+    2 is exact in FP16), every step relaxes with nu + 1e-3 y0."""
     P = psl
     n = 16
     shape = cases.tgv_shape(n)
@@ -159,11 +161,12 @@ def test_constant_network_equals_viscosity_shift(psl, storage):
         g.set_periodic_copies(x=True, y=True)
         if closure is not None:
             g.set_closure(closure)
+            g.set_closure_every(every)
         P.updateGrid([g], 10)
         grids.append(g)
     a, b = grids[0].get_f(), grids[1].get_f()
     err = float(np.max(np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-3)))
-    print(f"[closure-const-{storage}] max rel err {err:.3e}", flush=True)
+    print(f"[closure-const-{storage}-every{every}] max rel err {err:.3e}", flush=True)
     assert err < 1e-5
 
 
@@ -199,6 +202,62 @@ def test_closure_collision_matches_oracle(psl, oracle_mod):
     print(f"[closure-collision] nu_t range {C.NU_SCALE * max(y.min(), 0):.2e}..{C.NU_SCALE * y.max():.2e}; "
           f"max rel err {err:.3e}", flush=True)
     assert err < 1e-5
+
+
+def test_held_closure_matches_oracle(psl, oracle_mod):
+    """The network evaluated every third update: the evaluating update stores
+    each cell's output as FP16 and relaxes with it; the next update relaxes a
+    new state with the stored outputs, not with the network on the new
+    features.  Both against the oracle collision with the per-cell nu_t of
+    the stored outputs (1e-5 relative); the stored outputs against the
+    network on the oracle's features (bf16 inference, FP16 storage)."""
+    P, O = psl, oracle_mod
+    rng = np.random.default_rng(16)
+    shape = (16, 8, 8)
+    nc = int(np.prod(shape))
+    nu = 3e-4
+    info = P.InfoStruct(cellCountX=shape[0], cellCountY=shape[1], cellCountZ=shape[2], nu=nu)
+    g = P.GridStruct(info, "f32")
+
+    def state():
+        rho = rng.uniform(0.97, 1.03, nc)
+        u = rng.uniform(-0.05, 0.05, (3, nc))
+        f = O.feq(np.stack([rho, *u], axis=1))
+        return (f * (1 + rng.normal(0, 0.01, f.shape))).astype(np.float32)
+
+    def expected(f, y):
+        exp = np.empty(f.shape, np.float64)
+        for cidx in range(nc):
+            exp[cidx] = O.collide(f[cidx:cidx + 1], nu + C.NU_SCALE * max(float(y[cidx]), 0.0), 0.0)[0]
+        return exp
+
+    def rel(a, b):
+        return float(np.max(np.abs(a.astype(np.float64) - b) / np.maximum(np.abs(b), 1e-3)))
+
+    f1 = state()
+    raw = _features_from_f(f1.astype(np.float64))
+    mean, std = raw.mean(0), raw.std(0) + 1e-12
+    p = C.init_params(6) + rng.normal(0, 0.1, C.N_PARAM)
+    p[-1] = 1.0
+    net = P.MLP(p.astype(np.float32), mean.astype(np.float32), std.astype(np.float32))
+    g.set_closure(net)
+    g.set_closure_every(3)
+    g.set_f(np.ascontiguousarray(f1.T))
+    P.applyLocalCellUpdate(g)  # evaluates and stores
+    y_held = g.closure_y()
+    y_net = net.eval(raw.astype(np.float32))
+    assert np.max(np.abs(y_held - y_net)) <= 0.03 * np.max(np.abs(y_net)) + 0.02  # bf16 inference bound
+    e1 = rel(g.get_f().T, expected(f1, y_held))
+    f2 = state()
+    g.set_f(np.ascontiguousarray(f2.T))
+    P.applyLocalCellUpdate(g)  # holds: the stored outputs, not the network on f2
+    e2 = rel(g.get_f().T, expected(f2, y_held))
+    y2 = net.eval(_features_from_f(f2.astype(np.float64)).astype(np.float32))
+    e2_fresh = rel(g.get_f().T, expected(f2, y2))
+    print(f"[closure-held] evaluating update {e1:.3e}, held update {e2:.3e} (vs a fresh evaluation {e2_fresh:.3e})",
+          flush=True)
+    assert e1 < 1e-5 and e2 < 1e-5
+    assert e2_fresh > 10 * e2  # the held update did not evaluate the network
 
 
 def _features_from_f(f):

@@ -257,6 +257,15 @@ def test_forced_velocity_matches_oracle(psl, storage):
     g.set_forced_velocity(forced, ut)
     with pytest.raises(P.PslError):  # a cell may not be in both lists
         g.set_forced_velocity(np.concatenate([forced, moving[:1]]), np.vstack([ut, um[:1]]))
+    # listed cells must stay flagged bounceback: flags that make them fluid are
+    # refused (and the grid keeps its flags), as is a fluid cell in a list
+    with pytest.raises(P.PslError):
+        g.set_flags(walls_only)
+    with pytest.raises(P.PslError):
+        g.set_forced_velocity(np.concatenate([forced, np.setdiff1d(interior, picked)[:1]]), np.vstack([ut, ut[:1]]))
+    with pytest.raises(ValueError):  # three velocity components per cell
+        g.set_forced_velocity(forced, ut[:-1])
+    g.set_forced_velocity(forced, ut)
     g.fill_equilibrium(ux=0.03)
     raw = np.zeros((27, g.n), np.float32)
     E.eso_set_all(raw, nbr7, False, g.get_f())

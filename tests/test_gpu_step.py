@@ -137,6 +137,37 @@ def test_sphere_channel_matches_oracle(psl, oracle_mod):
     assert np.all(np.abs(fg - fo) <= 1e-5 * np.max(np.abs(fo)))
 
 
+def test_per_cell_smagorinsky_matches_oracle(psl, oracle_mod):
+    """getSmagorinskyConstant evaluated per cell (flowAroundSphereMainSingle.cu:79-82
+    returns one constant; the drivers may vary it in space): a random Cs field
+    on the sphere channel, boundary-list cells included, against the oracle
+    with the same field."""
+    P, O = psl, oracle_mod
+    shape = (40, 20, 20)
+    flags = _channel(shape, 4, (10, 10, 10))
+    bc = cases.channel_given_values(0.04)
+    n = int(np.prod(shape))
+    cs = np.random.default_rng(3).uniform(0.0, 0.3, n).astype(np.float32)
+    nx, ny, nz = shape
+    info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=0.002)
+    g = P.GridStruct(info, "f32")
+    og = O.Grid(nx, ny, nz, dtype=np.float32, nu=0.002, cs=0.0, flags=flags, bcvals=bc, cs_cell=cs)
+    g.set_flags(flags)
+    g.set_given_values(bc)
+    g.set_smagorinsky(cs)
+    _init_same(g, og, np.ones(n), np.full(n, 0.03), np.zeros(n), np.zeros(n))
+    g.set_periodic_copies(y=True)
+    steps = 25
+    print(f"[per-cell Cs] shape={shape} steps={steps}", flush=True)
+    P.updateGrid([g], steps)
+    for _ in range(steps):
+        og.step(periodic_y=True)
+    err = _rel_err(g.get_f(), og.logical_f())
+    print(f"  max rel err {err:.3e}")
+    assert err < RTOL
+    g.close()
+
+
 def test_all_boundary_normals_match_oracle(psl, oracle_mod):
     """Every box face, edge and corner with restore/MBBC of all four kinds."""
     P, O = psl, oracle_mod
