@@ -35,6 +35,7 @@ sys.path.insert(0, REPO)
 METRIC = "MLUPS + % HBM roofline, D3Q27 fused step: plain FP32 vs FP16-store/batched/MLP"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = {"f32": 216, "f16": 108, "bf16": 108}
+INFINITY_CACHE_BYTES = 256 << 20  # MI355X last-level (MALL) cache
 
 
 def log(*a):
@@ -87,9 +88,11 @@ def time_steps(P, g, steps: int, warmup: int):
     return time.perf_counter() - t0
 
 
-def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
+def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16", avg_steps: int = 2500,
+              avg_every: int = 100):
     """64 sphere-in-channel domains of uneven size advanced by one hipGraph
-    replay per step; aggregate MLUPS and per-domain drag coefficient."""
+    replay per step; aggregate MLUPS, its HBM fraction, and per-domain drag
+    coefficient averaged over avg_steps steps after the timed run."""
     doms = cases.batch_domains()
     grids = [cases.domain_grid(d, storage)[0] for d in doms]
     b = P.Batch(grids)
@@ -105,18 +108,100 @@ def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16"):
     b.step(steps)
     P.synchronize()
     el = time.perf_counter() - t0
-    cd = []
-    for d, g in zip(doms, grids):
-        lo, hi = cases.sphere_box_of(d["center"], d["diameter"])
-        fx = g.momentum_exchange(lo, hi)[0]
-        cd.append(cases.drag_coefficient(fx, d["u_inlet"], d["diameter"]))
+    # drag: after the timed steps, sampled every avg_every steps over avg_steps
+    # more (untimed), averaged per domain
+    boxes = [cases.sphere_box_of(d["center"], d["diameter"]) for d in doms]
+    samples = []
+    for _ in range(avg_steps // avg_every):
+        b.step(avg_every)
+        samples.append([g.momentum_exchange(lo, hi)[0] for g, (lo, hi) in zip(grids, boxes)])
+    fx = np.mean(np.array(samples), axis=0) if samples else np.zeros(len(doms))
+    cd = [cases.drag_coefficient(f, d["u_inlet"], d["diameter"]) for f, d in zip(fx, doms)]
     b.close()
     for g in grids:
         g.close()
-    return {"mlups": cells * steps / el / 1e6, "ms_per_step": el * 1e3 / steps, "domains": len(doms),
+    mlups = cells * steps / el / 1e6
+    achieved = mlups * 1e6 * BYTES_PER_UPDATE[storage] / 1e9
+    return {"mlups": mlups, "ms_per_step": el * 1e3 / steps, "domains": len(doms),
             "cells": cells, "boundary_cells": nbnd, "steps": steps, "storage": storage, "autotune": tune,
-            "drag_coefficient": [round(float(c), 5) for c in cd],
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": BYTES_PER_UPDATE[storage],
+                         "note": "whole step (all kernels of the replayed graph) over all domains"},
+            "working_set_bytes": cells * 27 * (4 if storage == "f32" else 2),
+            "drag_coefficient_mean": [round(float(c), 5) for c in cd],
+            "drag_average": {"after_step": steps, "steps": avg_steps, "every": avg_every},
             "L": [d["L"] for d in doms], "re": [round(d["re"], 1) for d in doms]}
+
+
+def drag_history_leg(P, cases, storages, steps: int, every: int, tuning=None):
+    """The sphere configuration's drag history (flowAroundSphereMainSingle.cu:
+    240-252 records C_D every step): `steps` steps from rest per storage, the
+    momentum-exchange drag coefficient every `every` steps, and its mean over
+    the second half of the run.  The history in the output is thinned to every
+    200 steps."""
+    out = {}
+    lo, hi = cases.sphere_box()
+    for st in storages:
+        g, _, _, _ = build_sphere_grid(P, cases, st)
+        if tuning and st in tuning:
+            g.tuning = tuning[st]
+        fx = []
+        t0 = time.perf_counter()
+        for _ in range(steps // every):
+            P.updateGrid([g], every)
+            fx.append(float(g.momentum_exchange(lo, hi)[0]))
+        el = time.perf_counter() - t0
+        g.close()
+        cd = np.array([cases.drag_coefficient(f) for f in fx])
+        thin = max(1, 200 // every)
+        out[st] = {"steps": steps, "every": every, "seconds": el,
+                   "mean_cd_second_half": float(np.mean(cd[len(cd) // 2:])),
+                   "std_cd_second_half": float(np.std(cd[len(cd) // 2:])),
+                   "cd_every_200_steps": [round(float(c), 5) for c in cd[thin - 1::thin]]}
+    if "f32" in out:
+        for st in storages:
+            if st != "f32":
+                out[st]["mean_cd_rel_to_f32"] = out[st]["mean_cd_second_half"] / out["f32"]["mean_cd_second_half"] - 1
+    out["note"] = ("D = 32 cells; the reference publishes C_D(Re=1000) = 0.514 at D/res = 160 "
+                   "(flowAroundSphere/resultsAllPlotter.py:188-191), a finer grid")
+    return out
+
+
+def access_probes(P, n: int = 256):
+    """The periodic-shift access pattern taken apart on an n^3 grid (no
+    collision): stream count, reads against writes, line alignment of the
+    streams, cells per lane.  GB/s of bytes moved per launch."""
+    rows = []
+
+    def run(storage, streams, mode, cpl, aligned):
+        gbs = P.probe_ps_access(n, n, n, storage, streams, mode, cpl, aligned)
+        rows.append({"storage": storage, "streams": streams, "mode": mode, "cells_per_lane": cpl,
+                     "line_aligned": aligned, "GBs": gbs})
+        log(f"  {storage} {mode:>7} streams {streams:2d} cells/lane {cpl} {'aligned' if aligned else 'shifted':>7}: "
+            f"{gbs:7.0f} GB/s")
+
+    for storage in ("f32", "f16"):
+        for streams in (1, 2, 4, 8, 16, 27):
+            run(storage, streams, "inplace", 1, False)
+        for mode in ("read", "write"):
+            for aligned in (False, True):
+                run(storage, 27, mode, 1, aligned)
+        run(storage, 27, "inplace", 1, True)
+        for cpl in (2, 4):
+            if storage == "f32":
+                run(storage, 27, "inplace", cpl, False)
+            run(storage, 27, "inplace", cpl, True)
+    return rows
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def copy_probe(P, mib: int = 2048, reps: int = 20):
@@ -157,6 +242,8 @@ def large_box_leg(P, cases, n: int, steps: int):
         g.close()
         out[storage]["pattern_limit_GBs"] = P.probe_ps_pattern(nx, ny, nz, storage)
         out[storage]["frac_of_pattern_limit"] = gbs / out[storage]["pattern_limit_GBs"]
+    out["f16_over_f32"] = out["f16"]["mlups"] / out["f32"]["mlups"]
+    out["note"] = "working sets of 1.8 GB (FP32) and 0.9 GB (FP16): HBM-resident, beyond the 256 MB Infinity Cache"
     return out
 
 
@@ -164,8 +251,8 @@ def esotwist_leg(P, n: int, steps: int):
     """The esotwist update (DIAD grids, include/applyLocalCellUpdate.h:77-171) on a
     dense, fully periodic n^3 box: every cell stored with its seven neighbour
     indices (periodicity through the indices), uniform flow, no Smagorinsky.
-    Algorithmic bytes per cell: 27 reads + 27 writes of the storage and the
-    seven 4-byte neighbour indices."""
+    Algorithmic bytes per cell: 27 reads + 27 writes of the storage (216 / 108
+    B); the seven 4-byte neighbour indices are reported as overhead."""
     out = {}
     ijk, nbr = P.DIADGridStruct.dense_box(n, n, n)
     for storage in ("f32", "f16"):
@@ -182,10 +269,12 @@ def esotwist_leg(P, n: int, steps: int):
             P.applyLocalCellUpdate(g)
         P.synchronize()
         el = time.perf_counter() - t0
-        per_cell = 54 * (4 if storage == "f32" else 2) + 28
+        per_cell = BYTES_PER_UPDATE[storage]
         gbs = per_cell * g.n * steps / el / 1e9
         out[storage] = {"cells": g.n, "mlups": g.n * steps / el / 1e6, "ms_per_step": el * 1e3 / steps,
-                        "bytes_per_cell": per_cell, "achieved_GBs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS}
+                        "bytes_per_cell": per_cell, "achieved_GBs": gbs, "roofline_frac": gbs / HBM_PEAK_GBS,
+                        "overhead_bytes_per_cell": 28,
+                        "achieved_GBs_with_overhead": (per_cell + 28) * g.n * steps / el / 1e9}
         g.close()
     return out
 
@@ -226,8 +315,10 @@ def diad_leg(P, cases, levels: int, steps: int, storage: str = "f16"):
     `levels` levels, wall refinement span 3, Re = 1e6, Cs = 0.1): the device
     builder's time and cell counts, then `steps` coarse steps of the level
     recursion.  MLUPS counts sum_l N_l 2^l cell updates per coarse step; the
-    bytes per update are the storage's 54 values plus the seven 4-byte
-    neighbour indices and the flag byte."""
+    algorithmic bytes per update are the storage's 54 values (216 / 108 B); the
+    seven 4-byte neighbour indices and the flag byte (29 B) are overhead,
+    reported apart, and the interface transfers are in the step's time but not
+    in its bytes."""
     t0 = time.perf_counter()
     lv, grids = cases.diad_sphere_reference(P, storage, levels=levels)
     setup_s = time.perf_counter() - t0
@@ -239,7 +330,7 @@ def diad_leg(P, cases, levels: int, steps: int, storage: str = "f16"):
     P.updateGrid(grids, steps)
     P.synchronize()
     el = time.perf_counter() - t0
-    per_update = 54 * (4 if storage == "f32" else 2) + 28 + 1
+    per_update = BYTES_PER_UPDATE[storage]
     mlups = updates * steps / el / 1e6
     force = grids[-1].momentum_exchange()
     out = {"mlups": mlups, "ms_per_coarse_step": el * 1e3 / steps, "coarse_steps": steps, "storage": storage,
@@ -247,6 +338,8 @@ def diad_leg(P, cases, levels: int, steps: int, storage: str = "f16"):
            "build_ms": lv.build_ms, "setup_s": setup_s, "bytes_per_update": per_update,
            "achieved_GBs": mlups * 1e6 * per_update / 1e9,
            "roofline_frac": mlups * 1e6 * per_update / 1e9 / HBM_PEAK_GBS,
+           "overhead_bytes_per_update": 29,
+           "achieved_GBs_with_overhead": mlups * 1e6 * (per_update + 29) / 1e9,
            "finest_force_after_run": [float(x) for x in force]}
     for g in grids:
         g.close()
@@ -331,6 +424,7 @@ def cpu_baseline(cases, flags, nu, cs, shape, budget_s: float):
             break
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": nx * ny * nz * steps / el / 1e6, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "build": "g++ " + O.build_flags(),
             "sample": f"{steps} FP32 oracle steps of the same 256x128x128 sphere workload ({el:.1f} s, OpenMP)"}
 
 
@@ -347,6 +441,8 @@ def main():
                     help="use this local-update variant (bits: 1 NT loads, 2 NT stores, 4 XCD order, 8 fused "
                          "boundary) instead of autotuning")
     ap.add_argument("--batch-steps", type=int, default=5000, help="steps of the 64-domain batch (0: skip)")
+    ap.add_argument("--drag-steps", type=int, default=20000,
+                    help="steps of the sphere drag history per storage (0: skip)")
     ap.add_argument("--two-level-steps", type=int, default=200, help="coarse steps of the two-level sphere (0: skip)")
     ap.add_argument("--large-n", type=int, default=256, help="edge of the large periodic box (0: skip)")
     ap.add_argument("--scalar-steps", type=int, default=100,
@@ -357,6 +453,8 @@ def main():
     ap.add_argument("--diad-storages", default="f16,f32", help="storages of the DIAD sphere leg")
     ap.add_argument("--closure-n", type=int, default=512,
                     help="fine grid edge of the learned-closure leg (0: skip)")
+    ap.add_argument("--access-probes", action="store_true",
+                    help="only run the access-pattern probes on a 256^3 grid and print their JSON line")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times) for counter calibration")
     args = ap.parse_args()
@@ -369,6 +467,12 @@ def main():
     from periodicshiftlbm_amd import cases
 
     P.require_device()
+    if args.access_probes:
+        log("[bench] periodic-shift access probes, 256^3")
+        cp = copy_probe(P)
+        log(f"  device copy of 2 GiB: {cp['GBs']:.0f} GB/s")
+        print(json.dumps({"access_probes": access_probes(P), "hbm_copy_probe": cp}), flush=True)
+        return
     if args.copy_calibration_mb:
         nbytes = args.copy_calibration_mb << 20
         a, b = P.DeviceBuffer(nbytes), P.DeviceBuffer(nbytes)
@@ -417,6 +521,7 @@ def main():
                          "traffic": measured_traffic(kernel, storage)},
             "drag_force_x_after_run": fx,
             "tuning": g.tuning, "autotune_ms": tune_times,
+            "working_set_bytes": ncell * (27 * (4 if storage == "f32" else 2) + 1),
         }
         log(f"  {mlups:.1f} MLUPS, {kernel} {avg_ms * 1e3:.1f} us ({achieved:.0f} GB/s, "
             f"{achieved / HBM_PEAK_GBS * 100:.1f}% of HBM peak, {achieved / pattern * 100:.0f}% of the access "
@@ -448,11 +553,22 @@ def main():
                          "achieved_GBs": v["roofline"]["achieved"], "kernel": v["kernel"],
                          "pattern_limit_GBs": v["pattern_limit_GBs"], "frac_of_pattern_limit": v["frac_of_pattern_limit"],
                          "kernel_ms": v["bulk_kernel_ms"],
-                         "tuning": v["tuning"], "autotune_ms": v["autotune_ms"]}
+                         "tuning": v["tuning"], "autotune_ms": v["autotune_ms"],
+                         "working_set_bytes": v["working_set_bytes"],
+                         "cache_resident": v["working_set_bytes"] < INFINITY_CACHE_BYTES}
                      for k, v in results.items()},
     }
     if "f32" in results and "f16" in results:
         out["variants"]["f16_over_f32"] = results["f16"]["mlups"] / results["f32"]["mlups"]
+        out["variants"]["f16_over_f32_note"] = ("sphere: the FP16 working set fits the 256 MB Infinity Cache; "
+                                                "large_box.f16_over_f32 is the HBM-resident ratio")
+    if args.drag_steps:
+        log(f"[bench] sphere drag history, {args.drag_steps} steps per storage")
+        dh = drag_history_leg(P, cases, args.storages.split(","), args.drag_steps, 20,
+                              {k: v["tuning"] for k, v in results.items()})
+        log("  " + ", ".join(f"{k}: mean C_D {v['mean_cd_second_half']:.4f} ({v['seconds']:.1f} s)"
+                             for k, v in dh.items() if isinstance(v, dict)))
+        out["variants"]["drag_history"] = dh
     if args.batch_steps:
         log(f"[bench] batch of 64 domains, {args.batch_steps} steps")
         bl = batch_leg(P, cases, args.batch_steps, 0 if (args.no_autotune or args.tuning is not None) else 3)
@@ -466,7 +582,8 @@ def main():
         lb = large_box_leg(P, cases, args.large_n, 100)
         log("  " + ", ".join(f"{k}: {v['mlups']:.0f} MLUPS, {v['achieved_GBs']:.0f} GB/s "
                              f"({v['frac_of_pattern_limit'] * 100:.0f} % of the access pattern's "
-                             f"{v['pattern_limit_GBs']:.0f} GB/s)" for k, v in lb.items()))
+                             f"{v['pattern_limit_GBs']:.0f} GB/s)" for k, v in lb.items() if isinstance(v, dict))
+            + f"; FP16 / FP32 {lb['f16_over_f32']:.2f}")
         out["variants"]["large_box"] = lb
         log(f"[bench] esotwist (DIAD) dense periodic box {args.large_n}^3")
         eso = esotwist_leg(P, args.large_n, 50)
