@@ -393,6 +393,21 @@ int psl_diad_coarse_fine_communication(psl_diad* coarse, psl_diad* fine);
  * streaming + local update of level l, two steps of level l+1, then the
  * communication of l and l+1; nsteps coarse steps. */
 int psl_diad_update_grid(psl_diad* const* levels, int nlevels, int nsteps);
+/* Interface records of a level: psl_diad_update_grid keeps, for the cells the level
+ * transfers read, a contiguous copy of the populations a local update leaves them with,
+ * and the transfers read those instead of 27 scattered slots per cell.  Child records
+ * (the children of the coarser level's fine-to-coarse entries) are written by the second
+ * of the level's two steps in the recursion, the one the transfer to the coarser level
+ * follows; source records (the cells this level's coarse-to-fine entries read) by every
+ * update.  psl_diad_apply_local_cell_update writes both.  Cells a transfer rewrites after
+ * the update, and bounceback cells, are read from their slots; set_f, fill_equilibrium,
+ * the outlet and a transfer into the level mark the records stale until the next update.
+ * count: records of this level (0 before the first psl_diad_update_grid of its level list);
+ * fresh: 1 when every one holds the last local update's values. */
+int psl_diad_interface_records(const psl_diad* grid, int64_t* count, int* fresh);
+/* Records whose values differ from the populations their cells hold now (0 right after a
+ * local update of the level; a consistency check). */
+int psl_diad_verify_interface_records(psl_diad* grid, int64_t* mismatches);
 /* outletCellArray (include/types.h:171) and kMinNbrArray (include/types.h:161, n
  * entries) from host arrays; psl_diad_create_level sets them for the levels
  * whose outlet rules marked cells. */

@@ -218,6 +218,8 @@ _SIGS = [
     ("psl_diad_set_interface", _I, [_P, _P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_int64]),
     ("psl_diad_coarse_fine_communication", _I, [_P, _P]),
     ("psl_diad_update_grid", _I, [_P, _I, _I]),
+    ("psl_diad_interface_records", _I, [_P, _P, _P]),
+    ("psl_diad_verify_interface_records", _I, [_P, _P]),
     ("psl_diad_reduce_momentum_exchange", _I, [_P, _P]),
     ("psl_diad_set_outlet", _I, [_P, _P, ctypes.c_int64, _P]),
     ("psl_diad_apply_nonreflective_outlet_z", _I, [_P]),
@@ -591,6 +593,19 @@ class DIADGridStruct:
         a = [np.ascontiguousarray(x, np.int32) for x in (c2f_read, c2f_write, f2c_write, f2c_read)]
         _check(lib().psl_diad_set_interface(self._h, fine.handle, a[0].ctypes.data, a[1].ctypes.data, a[0].size,
                                             a[2].ctypes.data, a[3].ctypes.data, a[2].size))
+
+    def interface_records(self):
+        """(record count, fresh) of this level's interface records."""
+        n = ctypes.c_int64()
+        fresh = ctypes.c_int()
+        _check(lib().psl_diad_interface_records(self._h, ctypes.byref(n), ctypes.byref(fresh)))
+        return n.value, bool(fresh.value)
+
+    def verify_interface_records(self) -> int:
+        """Records that differ from the populations their cells hold now."""
+        n = ctypes.c_int64()
+        _check(lib().psl_diad_verify_interface_records(self._h, ctypes.byref(n)))
+        return n.value
 
     def set_moving_bounceback(self, cells, u3):
         """Moving-wall cells (flagged BOUNCEBACK) and their velocities (n x 3)."""

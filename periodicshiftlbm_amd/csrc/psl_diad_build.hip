@@ -991,6 +991,7 @@ int psl_diad_create_level(const psl_diad_levels* levels, int level, int storage,
   const int64_t n = L.n;
   PSL_CHECK_ARG(n > 0, "empty level");
   psl_diad* d = new psl_diad();
+  d->rec_version = diad_next_version();
   d->info = L.info;
   d->info.gridID = (float)level;
   d->info.cellCount = (int)n;

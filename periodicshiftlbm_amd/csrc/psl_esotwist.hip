@@ -12,6 +12,7 @@
 // Layout: fArray(q, cell) at f[q * n + cell]; ijk[3][n]; nbr[7][n].
 #include "psl_internal.h"
 
+#include <atomic>
 #include <cstring>
 
 namespace psl {
@@ -29,7 +30,92 @@ struct EsoParams {
   const int* ijk;  // 3 x n
   const int* nbr;  // 7 x n
   const float4* bc;  // 4 given-value slots (device)
+  // interface records the local update writes (null: none): child records,
+  // read by the transfer to the coarser level, and source records, read by
+  // the transfer to the finer level
+  void* rec;                  // child records
+  const uint64_t* rec_mask;   // per 64 cells: cells with a child record
+  const uint32_t* rec_base;   // per 64 cells: child records below
+  void* srec;                 // source records
+  const uint64_t* srec_mask;
+  const uint32_t* srec_base;
 };
+
+// Interface records: the post-collision storage words of one cell, in q
+// order, padded to ESO_REC_STRIDE elements (64 B in 16-bit storage, 128 B in
+// FP32) and written with 16-byte stores.
+constexpr int ESO_REC_STRIDE = 32;
+
+template <class St>
+__device__ __forceinline__ void rec_store(void* rec, uint32_t idx, const float (&v)[27]) {
+  typedef typename St::T T;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  u4* r = (u4*)((T*)rec + (size_t)idx * ESO_REC_STRIDE);
+  if constexpr (sizeof(T) == 2) {
+    uint32_t w[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t lo = 2 * k < 27 ? (uint32_t)__builtin_bit_cast(uint16_t, St::store(v[2 * k])) : 0u;
+      const uint32_t hi = 2 * k + 1 < 27 ? (uint32_t)__builtin_bit_cast(uint16_t, St::store(v[2 * k + 1])) : 0u;
+      w[k] = lo | (hi << 16);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = u4{w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]};
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      u4 x;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) x[c] = 4 * k + c < 27 ? __builtin_bit_cast(uint32_t, St::store(v[4 * k + c])) : 0u;
+      r[k] = x;
+    }
+  }
+}
+
+// Full values of a record (the lattice weights added back for 16-bit storage).
+template <class St>
+__device__ __forceinline__ void rec_load(const void* rec, uint32_t idx, float (&v)[27]) {
+  typedef typename St::T T;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const u4* r = (const u4*)((const T*)rec + (size_t)idx * ESO_REC_STRIDE);
+  if constexpr (sizeof(T) == 2) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const u4 x = r[k];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int q = 8 * k + 2 * c;
+        if (q < 27) v[q] = St::load(__builtin_bit_cast(T, (uint16_t)(x[c] & 0xffffu)));
+        if (q + 1 < 27) v[q + 1] = St::load(__builtin_bit_cast(T, (uint16_t)(x[c] >> 16)));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const u4 x = r[k];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        // the element goes through a scalar first: a bit cast applied to a
+        // vector element directly compiles to a read of element 0 here
+        const uint32_t w = x[c];
+        if (4 * k + c < 27) v[4 * k + c] = St::load(__builtin_bit_cast(T, w));
+      }
+    }
+  }
+  if (St::W_OFF) {
+#pragma unroll
+    for (int q = 0; q < 27; ++q) v[q] += weight(q);
+  }
+}
+
+// Record index of `cell`, or -1 when it has none.
+__device__ __forceinline__ int rec_index(const uint64_t* __restrict__ mask, const uint32_t* __restrict__ base,
+                                         uint32_t cell) {
+  const uint64_t m = mask[cell >> 6];
+  const uint32_t lane = cell & 63u;
+  if (!((m >> lane) & 1u)) return -1;
+  return (int)(base[cell >> 6] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)));
+}
 
 template <class St>
 __device__ __forceinline__ void eso_slots(const EsoParams& p, uint32_t cell, uint32_t (&slot)[8]) {
@@ -77,6 +163,14 @@ __global__ __launch_bounds__(256) void k_eso_update(typename St::T* __restrict__
   for (int q = 0; q < 27; ++q) {
     const int fi = FLIP ? opp(q) : q;
     f[(size_t)fi * p.n + slot[ESO_WRITE[q]]] = St::store(v[q]);
+  }
+  if (p.rec) {
+    const int r = rec_index(p.rec_mask, p.rec_base, cell);
+    if (r >= 0) rec_store<St>(p.rec, (uint32_t)r, v);
+  }
+  if (p.srec) {
+    const int r = rec_index(p.srec_mask, p.srec_base, cell);
+    if (r >= 0) rec_store<St>(p.srec, (uint32_t)r, v);
   }
 }
 
@@ -154,57 +248,145 @@ __device__ __forceinline__ void eso_put(typename St::T* __restrict__ f, const Es
   }
 }
 
+// Slot of population q in the write maps (ESO_WRITE) for a q that varies
+// across lanes: the table packed three bits per entry.
+constexpr uint64_t pack3(const int (&t)[27], int lo, int hi) {
+  uint64_t r = 0;
+  for (int q = hi - 1; q >= lo; --q) r = (r << 3) | (uint64_t)t[q];
+  return r;
+}
+constexpr uint64_t ESO_WRITE_LO = pack3(ESO_WRITE, 0, 21), ESO_WRITE_HI = pack3(ESO_WRITE, 21, 27);
+__device__ __forceinline__ int write_slot_of(int q) {
+  return q < 21 ? (int)((ESO_WRITE_LO >> (3 * q)) & 7u) : (int)((ESO_WRITE_HI >> (3 * (q - 21))) & 7u);
+}
+
+// Storage index of population q (any lane-varying q) of `cell` as its last
+// update left it (the element eso_get reads).
+template <bool FLIP>
+__device__ __forceinline__ size_t eso_at(const EsoParams& p, uint32_t cell, int q) {
+  const int ws = write_slot_of(q);
+  const uint32_t slot = ws == 0 ? cell : (uint32_t)p.nbr[(size_t)(ws - 1) * p.n + cell];
+  return (size_t)(FLIP ? opp(q) : q) * p.n + slot;
+}
+
 // writeToCoarseGridInterface(DIAD): include/gridRefinementFunctions.h:540-622.
-// Eight lanes per interface cell, one per fine child (the read cell and its
-// +i +j +k +ij +ik +jk +ijk neighbours): each lane gathers its child, the
-// group sums the eight in child order through lane shuffles (the reference's
-// sequential order, so every lane holds the same sum), and lane 0 rescales
-// fine -> coarse and writes the coarse cell.  Eight independent gathers per
-// cell keep enough loads in flight on the thin interface lists.
+// Eight lanes per interface cell.  Lane w owns populations 4w .. 4w+3 (lane
+// 7 none): it sums them over the eight fine children (the read cell and its
+// +i +j +k +ij +ik +jk +ijk neighbours) in child order, the reference's
+// sequential order, reading each child's record (one 8- or 16-byte load) or,
+// for a child without one, its slots.  27 lane shuffles then give every lane
+// the 27 averages, each lane rescales fine -> coarse (identical in every
+// lane) and writes its own populations of the coarse cell.
 template <class St, bool FC, bool FF>
 __global__ __launch_bounds__(256) void k_eso_to_coarse(typename St::T* __restrict__ fc, EsoParams pc,
                                                        const typename St::T* __restrict__ ff, EsoParams pf,
                                                        const int* __restrict__ write, const int* __restrict__ read,
-                                                       uint32_t m) {
+                                                       uint32_t m, const int* __restrict__ child_rec) {
+  typedef typename St::T T;
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   const uint32_t g = t >> 3, w = t & 7;
   const bool valid = g < m;  // the eight lanes of a group agree
-  float v[27];
-  if (valid) {
-    const uint32_t rc = (uint32_t)read[g];
-    const uint32_t child = w == 0 ? rc : (uint32_t)pf.nbr[(size_t)(w - 1) * pf.n + rc];
-    eso_get<St, FF>(ff, pf, child, v);
-  } else {
+  const int q0 = 4 * (int)w;
+  float s[4] = {0.f, 0.f, 0.f, 0.f};
+  // the coarse cell and its neighbour slots, loaded ahead of the children so
+  // that the stores at the end wait for nothing
+  const uint32_t wc = valid ? (uint32_t)write[g] : 0u;
+  uint32_t slot[8];
+  if (valid) eso_slots<St>(pc, wc, slot);
+  if (valid && w < 7) {
+    // the eight children's records (two 16-byte loads), -1 for a child without one
+    int r[8];
+    if (child_rec) {
+      typedef int i4 __attribute__((ext_vector_type(4)));
+      const i4 a = *(const i4*)(child_rec + 8 * (size_t)g);
+      const i4 b = *(const i4*)(child_rec + 8 * (size_t)g + 4);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) v[q] = 0.f;
+      for (int c = 0; c < 4; ++c) {
+        const int ac = a[c], bc = b[c];
+        r[c] = ac;
+        r[c + 4] = bc;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) r[c] = -1;
+    }
+    bool all = true;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) all &= r[c] >= 0;
+    float x[8][4];
+    if (all) {
+      // every child has a record: eight independent loads, all in flight
+      // (elements q0 .. q0+3; the padding beyond 26 is never used)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const T* e = (const T*)pf.rec + (size_t)r[c] * ESO_REC_STRIDE + q0;
+        if constexpr (sizeof(T) == 2) {
+          typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+          const u2 y = *(const u2*)e;
+          const uint32_t y0 = y[0], y1 = y[1];
+          x[c][0] = St::load(__builtin_bit_cast(T, (uint16_t)(y0 & 0xffffu)));
+          x[c][1] = St::load(__builtin_bit_cast(T, (uint16_t)(y0 >> 16)));
+          x[c][2] = St::load(__builtin_bit_cast(T, (uint16_t)(y1 & 0xffffu)));
+          x[c][3] = St::load(__builtin_bit_cast(T, (uint16_t)(y1 >> 16)));
+        } else {
+          typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+          const u4 y = *(const u4*)e;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const uint32_t yk = y[k];
+            x[c][k] = St::load(__builtin_bit_cast(T, yk));
+          }
+        }
+      }
+    } else {
+      // some child without a record: every child from its slots
+      const uint32_t rc = (uint32_t)read[g];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t child = c == 0 ? rc : (uint32_t)pf.nbr[(size_t)(c - 1) * pf.n + rc];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[c][k] = q0 + k < 27 ? St::load(ff[eso_at<FF>(pf, child, q0 + k)]) : 0.f;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s[k] += St::W_OFF ? x[c][k] + weight(q0 + k) : x[c][k];
   }
   float acc[27];
 #pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    float a = 0.f;
-#pragma unroll
-    for (int src = 0; src < 8; ++src) a += __shfl(v[q], src, 8);
-    acc[q] = a * 0.125f;
-  }
-  if (!valid || w != 0) return;
+  for (int q = 0; q < 27; ++q) acc[q] = __shfl(s[q & 3], q >> 2, 8) * 0.125f;
+  if (!valid || w == 7) return;
   rescale(acc, false);
-  eso_put<St, FC>(fc, pc, (uint32_t)write[g], acc);
+  // population q is stored by lane q / 4; q is a constant in each store, so
+  // acc stays in registers
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if (w == (uint32_t)(q >> 2)) {
+      float x = acc[q];
+      if (St::W_OFF) x -= weight(q);
+      fc[(size_t)(FC ? opp(q) : q) * pc.n + slot[ESO_WRITE[q]]] = St::store(x);
+    }
+  }
 }
 
 // writeToFineGridInterface(DIAD): include/gridRefinementFunctions.h:624-702.
-// Eight lanes per interface cell: each reads the coarse cell (the same lines),
-// rescales coarse -> fine (identical in every lane) and writes one child: the
-// write cell or its +i +j +k +ij +ik +jk +ijk neighbour.
+// Eight lanes per interface cell: each reads the coarse cell (its record, or
+// its slots; the same lines for all eight), rescales coarse -> fine
+// (identical in every lane) and writes one child: the write cell or its +i +j
+// +k +ij +ik +jk +ijk neighbour.
 template <class St, bool FC, bool FF>
 __global__ __launch_bounds__(256) void k_eso_to_fine(const typename St::T* __restrict__ fc, EsoParams pc,
                                                      typename St::T* __restrict__ ff, EsoParams pf,
                                                      const int* __restrict__ write, const int* __restrict__ read,
-                                                     uint32_t m) {
+                                                     uint32_t m, const int* __restrict__ src_rec) {
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   const uint32_t g = t >> 3, w = t & 7;
   if (g >= m) return;
   float v[27];
-  eso_get<St, FC>(fc, pc, (uint32_t)read[g], v);
+  const int r = src_rec ? src_rec[g] : -1;
+  if (r >= 0) rec_load<St>(pc.srec, (uint32_t)r, v);
+  else eso_get<St, FC>(fc, pc, (uint32_t)read[g], v);
   rescale(v, true);
   const uint32_t wc = (uint32_t)write[g];
   const uint32_t child = w == 0 ? wc : (uint32_t)pf.nbr[(size_t)(w - 1) * pf.n + wc];
@@ -423,11 +605,121 @@ __global__ __launch_bounds__(256) void k_eso_momentum(const typename St::T* __re
   if (threadIdx.x < 3) partials[3 * (size_t)blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// ------------------------------------------------------- interface records
+// Which cells get a record: the children of every fine-to-coarse entry on
+// the finer level (mark bit 0: child records) and the source of every
+// coarse-to-fine entry on the coarser level (bit 1: source records).  Cells a
+// transfer writes after the level's update (the level's own fine-to-coarse
+// targets) are left to the slot path, as are cells the local update skips
+// (bounceback).
+__global__ void k_rec_mark_children(const int* __restrict__ read, int64_t m, const int* __restrict__ nbr, int64_t n,
+                                    uint8_t* __restrict__ mark) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 8 * m) return;
+  const int64_t g = t >> 3;
+  const int w = (int)(t & 7);
+  const int rc = read[g];
+  const int child = w == 0 ? rc : nbr[(int64_t)(w - 1) * n + rc];
+  mark[child] |= 1;
+}
+
+__global__ void k_rec_mark_list(const int* __restrict__ cells, int64_t m, uint8_t* __restrict__ mark, uint8_t bit,
+                                bool set) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  if (set) mark[cells[t]] |= bit;
+  else mark[cells[t]] = 0;
+}
+
+// one wave per 64 cells: the record mask and the record count of the group
+// for the cells whose mark has `bit`
+__global__ __launch_bounds__(256) void k_rec_mask(const uint8_t* __restrict__ mark, uint8_t bit,
+                                                  const uint8_t* __restrict__ flags, int64_t n,
+                                                  uint64_t* __restrict__ mask, uint32_t* __restrict__ count) {
+  const int64_t cell = blockIdx.x * 256ll + threadIdx.x;
+  bool keep = false;
+  if (cell < n) keep = (mark[cell] & bit) && (!flags || (flags[cell] & 7) != KIND_BOUNCEBACK);
+  const uint64_t b = __ballot(keep);
+  if ((threadIdx.x & 63) == 0 && cell < n) {
+    mask[cell >> 6] = b;
+    count[cell >> 6] = (uint32_t)__popcll(b);
+  }
+}
+
+// exclusive prefix sum of nw counts in one workgroup of 1024 threads, each
+// owning a contiguous chunk; total[0] = the sum
+__global__ __launch_bounds__(1024) void k_rec_scan(const uint32_t* __restrict__ count, int64_t nw,
+                                                   uint32_t* __restrict__ base, uint32_t* __restrict__ total) {
+  __shared__ uint32_t part[1024];
+  const int64_t chunk = (nw + 1023) / 1024;
+  const int64_t lo = threadIdx.x * chunk, hi = min(nw, lo + chunk);
+  uint32_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += count[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < 1024; ++i) {
+      const uint32_t x = part[i];
+      part[i] = acc;
+      acc += x;
+    }
+    total[0] = acc;
+  }
+  __syncthreads();
+  uint32_t acc = part[threadIdx.x];
+  for (int64_t i = lo; i < hi; ++i) {
+    base[i] = acc;
+    acc += count[i];
+  }
+}
+
+__global__ void k_rec_index_children(const int* __restrict__ read, int64_t m, const int* __restrict__ nbr, int64_t n,
+                                     const uint64_t* __restrict__ mask, const uint32_t* __restrict__ base,
+                                     int* __restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 8 * m) return;
+  const int64_t g = t >> 3;
+  const int w = (int)(t & 7);
+  const int rc = read[g];
+  const int child = w == 0 ? rc : nbr[(int64_t)(w - 1) * n + rc];
+  out[t] = rec_index(mask, base, (uint32_t)child);
+}
+
+__global__ void k_rec_index_list(const int* __restrict__ cells, int64_t m, const uint64_t* __restrict__ mask,
+                                 const uint32_t* __restrict__ base, int* __restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  out[t] = rec_index(mask, base, (uint32_t)cells[t]);
+}
+
+// Records that differ from the populations the cell's slots hold now (both
+// in the storage words, so a fresh record matches bit for bit).
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_rec_verify(const typename St::T* __restrict__ f, EsoParams p,
+                                                    const void* __restrict__ rec, const uint64_t* __restrict__ mask,
+                                                    const uint32_t* __restrict__ base,
+                                                    unsigned long long* __restrict__ bad) {
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  if (cell >= p.n) return;
+  const int r = rec_index(mask, base, cell);
+  if (r < 0) return;
+  float a[27], b[27];
+  eso_get<St, FLIP>(f, p, cell, a);
+  rec_load<St>(rec, (uint32_t)r, b);
+  int nbad = 0;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) nbad += __builtin_bit_cast(uint32_t, a[q]) != __builtin_bit_cast(uint32_t, b[q]);
+  if (nbad) atomicAdd(bad, 1ull);
+}
+
 }  // namespace psl
 
 using namespace psl;
 
-static EsoParams eso_params(const psl_diad* d) {
+// child_records: whether a local update with these parameters writes the
+// child records (the source records are written whenever they exist)
+static EsoParams eso_params(const psl_diad* d, bool child_records = true) {
   EsoParams p;
   p.n = (uint32_t)d->n;
   p.nx = d->info.cellCountX;
@@ -441,16 +733,159 @@ static EsoParams eso_params(const psl_diad* d) {
   p.ijk = d->ijk;
   p.nbr = d->nbr;
   p.bc = d->bc_dev;
+  p.rec = child_records && d->child.n ? d->child.rec : nullptr;
+  p.rec_mask = d->child.mask;
+  p.rec_base = d->child.base;
+  p.srec = d->source.n ? d->source.rec : nullptr;
+  p.srec_mask = d->source.mask;
+  p.srec_base = d->source.base;
   return p;
 }
 
+// Record versions are unique across all grids of the process, so a key can
+// never match a level that was destroyed and replaced at the same address.
+uint64_t psl::diad_next_version() {
+  static std::atomic<uint64_t> next{1};
+  return next.fetch_add(1) + 1;
+}
+
+// The populations changed by something other than a local update: no record
+// of the level holds them until the next update.
+static void records_stale(psl_diad* d) {
+  d->child.fresh = false;
+  d->source.fresh = false;
+}
+
+static void records_changed(psl_diad* d) {
+  d->rec_version = diad_next_version();
+  records_stale(d);
+  d->rec_key = 0;  // no transfer uses this level's records until they are rebuilt
+}
+
+static void free_record_set(psl_diad::Records& r) {
+  for (void* a : {(void*)r.mask, (void*)r.base, r.rec})
+    if (a) (void)hipFree(a);
+  r = psl_diad::Records();
+}
+
+static void free_records(psl_diad* d) {
+  free_record_set(d->child);
+  free_record_set(d->source);
+  for (void* a : {(void*)d->f2c_child_rec, (void*)d->c2f_src_rec})
+    if (a) (void)hipFree(a);
+  d->f2c_child_rec = nullptr;
+  d->c2f_src_rec = nullptr;
+  d->rec_key = 0;
+  d->rec_fine = nullptr;
+}
+
+// Mask, record bases and record storage of the cells of one level whose mark
+// has `bit` (stream-ordered on st; synchronizes st for the record count).
+static int build_record_set(psl_diad* d, const uint8_t* mark, uint8_t bit, hipStream_t st, psl_diad::Records& r) {
+  const int64_t nw = (d->n + 63) / 64;
+  uint32_t* count = nullptr;
+  uint32_t* total = nullptr;
+  uint32_t ntot = 0;
+  int rc = PSL_OK;
+  if (hipMalloc(&r.mask, (size_t)nw * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc(&r.base, (size_t)nw * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&count, (size_t)nw * sizeof(uint32_t)) != hipSuccess || hipMalloc(&total, sizeof(uint32_t)) != hipSuccess) {
+    rc = PSL_E_HIP;
+  } else {
+    k_rec_mask<<<(unsigned)((d->n + 255) / 256), 256, 0, st>>>(mark, bit, d->flags, d->n, r.mask, count);
+    k_rec_scan<<<1, 1024, 0, st>>>(count, nw, r.base, total);
+    if (hipMemcpyAsync(&ntot, total, sizeof(uint32_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = PSL_E_HIP;
+  }
+  if (count) (void)hipFree(count);
+  if (total) (void)hipFree(total);
+  r.n = ntot;
+  if (rc == PSL_OK && ntot && hipMalloc(&r.rec, (size_t)ntot * ESO_REC_STRIDE * storage_bytes(d->storage)) != hipSuccess)
+    rc = PSL_E_HIP;
+  return rc;
+}
+
+// What the records of a level list depend on: every level's version and the
+// level order.
+static uint64_t records_key(psl_diad* const* lv, int nl, int l) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t x) { h = (h ^ x) * 1099511628211ull; };
+  mix((uint64_t)nl);
+  mix((uint64_t)l);
+  for (int k = 0; k < nl; ++k) {
+    mix((uint64_t)(uintptr_t)lv[k]);
+    mix(lv[k]->rec_version);
+  }
+  return h;
+}
+
+// (Re)build the interface records of every level of the list when anything
+// they depend on changed since they were built.
+static int diad_prepare_records(psl_diad* const* lv, int nl) {
+  if (nl < 2) return PSL_OK;
+  bool stale = false;
+  for (int l = 0; l < nl; ++l) stale |= lv[l]->rec_key != records_key(lv, nl, l);
+  if (!stale) return PSL_OK;
+  const hipStream_t st = lv[0]->stream;
+  std::vector<uint8_t*> mark(nl, nullptr);
+  int rc = PSL_OK;
+  auto grid1d = [](int64_t k) { return (unsigned)((k + 255) / 256); };
+  for (int l = 0; l < nl && rc == PSL_OK; ++l) {
+    free_records(lv[l]);
+    if (hipMalloc(&mark[l], (size_t)lv[l]->n) != hipSuccess || hipMemsetAsync(mark[l], 0, (size_t)lv[l]->n, st) != hipSuccess)
+      rc = PSL_E_HIP;
+  }
+  for (int l = 0; l + 1 < nl && rc == PSL_OK; ++l) {
+    psl_diad *c = lv[l], *f = lv[l + 1];
+    if (c->n_f2c) k_rec_mark_children<<<grid1d(8 * c->n_f2c), 256, 0, st>>>(c->f2c_read, c->n_f2c, f->nbr, f->n, mark[l + 1]);
+    if (c->n_c2f) k_rec_mark_list<<<grid1d(c->n_c2f), 256, 0, st>>>(c->c2f_read, c->n_c2f, mark[l], 2, true);
+  }
+  for (int l = 0; l < nl && rc == PSL_OK; ++l) {
+    psl_diad* d = lv[l];
+    if (l + 1 < nl && d->n_f2c) k_rec_mark_list<<<grid1d(d->n_f2c), 256, 0, st>>>(d->f2c_write, d->n_f2c, mark[l], 0, false);
+    rc = build_record_set(d, mark[l], 1, st, d->child);
+    if (rc == PSL_OK) rc = build_record_set(d, mark[l], 2, st, d->source);
+  }
+  for (int l = 0; l + 1 < nl && rc == PSL_OK; ++l) {
+    psl_diad *c = lv[l], *f = lv[l + 1];
+    c->rec_fine = f;
+    if (c->n_f2c) {
+      if (hipMalloc(&c->f2c_child_rec, 8 * (size_t)c->n_f2c * sizeof(int)) != hipSuccess) rc = PSL_E_HIP;
+      else
+        k_rec_index_children<<<grid1d(8 * c->n_f2c), 256, 0, st>>>(c->f2c_read, c->n_f2c, f->nbr, f->n, f->child.mask,
+                                                                  f->child.base, c->f2c_child_rec);
+    }
+    if (c->n_c2f && rc == PSL_OK) {
+      if (hipMalloc(&c->c2f_src_rec, (size_t)c->n_c2f * sizeof(int)) != hipSuccess) rc = PSL_E_HIP;
+      else
+        k_rec_index_list<<<grid1d(c->n_c2f), 256, 0, st>>>(c->c2f_read, c->n_c2f, c->source.mask, c->source.base,
+                                                           c->c2f_src_rec);
+    }
+  }
+  if (rc == PSL_OK && hipGetLastError() != hipSuccess) rc = PSL_E_HIP;
+  if (rc == PSL_OK && hipStreamSynchronize(st) != hipSuccess) rc = PSL_E_HIP;
+  for (uint8_t* m : mark)
+    if (m) (void)hipFree(m);
+  if (rc) {
+    for (int l = 0; l < nl; ++l) free_records(lv[l]);
+    set_error("DIAD interface records: setup failed");
+    return rc;
+  }
+  for (int l = 0; l < nl; ++l) lv[l]->rec_key = records_key(lv, nl, l);
+  return PSL_OK;
+}
+
 template <class St>
-static int eso_update(psl_diad* d) {
-  const EsoParams p = eso_params(d);
+static int eso_update(psl_diad* d, bool child_records) {
+  const EsoParams p = eso_params(d, child_records);
   const uint32_t nb = (uint32_t)((d->n + 255) / 256);
   if (d->flipper) k_eso_update<St, true><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p);
   else k_eso_update<St, false><<<nb, 256, 0, d->stream>>>((typename St::T*)d->f, p);
   PSL_LAUNCH_CHECK();
+  // the records this update wrote hold its values; the others are behind
+  d->child.fresh = p.rec != nullptr;
+  d->source.fresh = p.srec != nullptr;
   return PSL_OK;
 }
 
@@ -478,15 +913,21 @@ template <class St, bool FC, bool FF>
 static int eso_comm_t(psl_diad* c, psl_diad* f) {
   typedef typename St::T T;
   const EsoParams pc = eso_params(c), pf = eso_params(f);
+  // records are read only when both levels' last updates wrote them for
+  // tables built with this pair
+  const bool built = c->rec_key && f->rec_key && c->rec_fine == f;
+  const int* child_rec = built && f->child.fresh ? c->f2c_child_rec : nullptr;
+  const int* src_rec = built && c->source.fresh ? c->c2f_src_rec : nullptr;
   if (c->n_f2c) {
     k_eso_to_coarse<St, FC, FF><<<(unsigned)((8 * c->n_f2c + 255) / 256), 256, 0, c->stream>>>(
-        (T*)c->f, pc, (const T*)f->f, pf, c->f2c_write, c->f2c_read, (uint32_t)c->n_f2c);
+        (T*)c->f, pc, (const T*)f->f, pf, c->f2c_write, c->f2c_read, (uint32_t)c->n_f2c, child_rec);
     PSL_LAUNCH_CHECK();
   }
   if (c->n_c2f) {
     k_eso_to_fine<St, FC, FF><<<(unsigned)((8 * c->n_c2f + 255) / 256), 256, 0, c->stream>>>(
-        (const T*)c->f, pc, (T*)f->f, pf, c->c2f_write, c->c2f_read, (uint32_t)c->n_c2f);
+        (const T*)c->f, pc, (T*)f->f, pf, c->c2f_write, c->c2f_read, (uint32_t)c->n_c2f, src_rec);
     PSL_LAUNCH_CHECK();
+    records_stale(f);  // fine cells were rewritten after the fine update
   }
   return PSL_OK;
 }
@@ -573,6 +1014,21 @@ static int eso_macro(psl_diad* d, float* out) {
   return PSL_OK;
 }
 
+template <class St>
+static int rec_verify(psl_diad* d, unsigned long long* bad) {
+  const EsoParams p = eso_params(d);
+  const uint32_t nb = (uint32_t)((d->n + 255) / 256);
+  for (const psl_diad::Records* r : {&d->child, &d->source}) {
+    if (!r->n) continue;
+    if (d->flipper)
+      k_rec_verify<St, true><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, r->rec, r->mask, r->base, bad);
+    else
+      k_rec_verify<St, false><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, r->rec, r->mask, r->base, bad);
+  }
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
 #define PSL_ESO_DISPATCH(fn, ...)                                         \
   (d->storage == PSL_F32 ? fn<StF32>(__VA_ARGS__)                         \
                          : d->storage == PSL_F16DEV ? fn<StF16>(__VA_ARGS__) : fn<StBF16>(__VA_ARGS__))
@@ -586,6 +1042,7 @@ int psl_diad_create(const psl_info* info, int storage, int64_t n, const int* ijk
   for (int64_t t = 0; t < 7 * n; ++t)
     PSL_CHECK_ARG(nbr7[t] >= 0 && nbr7[t] < n, "neighbour index out of range");
   psl_diad* d = new psl_diad();
+  d->rec_version = diad_next_version();
   d->info = *info;
   d->storage = storage;
   d->n = n;
@@ -620,6 +1077,7 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->moving_u) (void)hipFree(d->moving_u);
   if (d->forced_u) (void)hipFree(d->forced_u);
   if (d->partials) (void)hipFree(d->partials);
+  free_records(d);
   delete d;
   return PSL_OK;
 }
@@ -641,6 +1099,7 @@ int psl_diad_set_flags(psl_diad* d, const uint8_t* host) {
   PSL_CHECK_ARG(d, "null grid");
   const int rc = lists_are_bounceback(d, host);
   if (rc) return rc;
+  records_changed(d);
   if (!host) {
     if (d->flags) PSL_HIP_TRY(hipFree(d->flags));
     d->flags = nullptr;
@@ -671,6 +1130,7 @@ int psl_diad_set_stream(psl_diad* d, void* s) {
 
 int psl_diad_fill_equilibrium(psl_diad* d, const float* rho, const float* ux, const float* uy, const float* uz) {
   PSL_CHECK_ARG(d, "null grid");
+  records_stale(d);
   const size_t n = (size_t)d->n;
   float* dv[4] = {nullptr, nullptr, nullptr, nullptr};
   const float* hv[4] = {rho, ux, uy, uz};
@@ -695,7 +1155,7 @@ int psl_diad_apply_streaming(psl_diad* d) {
 
 int psl_diad_apply_local_cell_update(psl_diad* d) {
   PSL_CHECK_ARG(d, "null grid");
-  int rc = PSL_ESO_DISPATCH(eso_update, d);
+  int rc = PSL_ESO_DISPATCH(eso_update, d, true);
   if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_moving, d);
   if (rc != PSL_OK) return rc;
   return PSL_ESO_DISPATCH(eso_forced, d);
@@ -727,6 +1187,7 @@ int psl_diad_get_f(psl_diad* d, float* host) {
 
 int psl_diad_set_f(psl_diad* d, const float* host) {
   PSL_CHECK_ARG(d && host, "null argument");
+  records_stale(d);
   const size_t bytes = 27ull * d->n * sizeof(float);
   float* buf = nullptr;
   PSL_HIP_TRY(hipMalloc(&buf, bytes));
@@ -763,6 +1224,7 @@ int psl_diad_set_interface(psl_diad* coarse, const psl_diad* fine, const int* c2
   for (int64_t t = 0; t < n_f2c; ++t)
     PSL_CHECK_ARG(f2c_write[t] >= 0 && f2c_write[t] < coarse->n && f2c_read[t] >= 0 && f2c_read[t] < fine->n,
                   "fine-to-coarse index out of range");
+  records_changed(coarse);
   int rc = upload_ints(&coarse->c2f_read, c2f_read, n_c2f);
   if (rc == PSL_OK) rc = upload_ints(&coarse->c2f_write, c2f_write, n_c2f);
   if (rc == PSL_OK) rc = upload_ints(&coarse->f2c_write, f2c_write, n_f2c);
@@ -777,6 +1239,7 @@ int psl_diad_set_outlet(psl_diad* d, const int* outlet_cells, int64_t n_outlet, 
   for (int64_t t = 0; t < n_outlet; ++t) PSL_CHECK_ARG(outlet_cells[t] >= 0 && outlet_cells[t] < d->n, "outlet cell out of range");
   if (n_outlet)
     for (int64_t c = 0; c < d->n; ++c) PSL_CHECK_ARG(kmin[c] >= 0 && kmin[c] < d->n, "-k neighbour out of range");
+  records_changed(d);
   int rc = upload_ints(&d->outlet, outlet_cells, n_outlet);
   if (rc == PSL_OK) rc = upload_ints(&d->kmin, kmin, n_outlet ? d->n : 0);
   d->n_outlet = rc == PSL_OK ? n_outlet : 0;
@@ -842,6 +1305,9 @@ int psl_diad_set_forced_velocity(psl_diad* d, const int* cells, int64_t n, const
 
 int psl_diad_apply_nonreflective_outlet_z(psl_diad* d) {
   PSL_CHECK_ARG(d, "null grid");
+  // the outlet stores where the next step reads, which includes populations
+  // of the outlet cells' neighbours: the level's records no longer hold them
+  records_stale(d);
   return PSL_ESO_DISPATCH(eso_outlet, d);
 }
 
@@ -854,17 +1320,45 @@ int psl_diad_coarse_fine_communication(psl_diad* coarse, psl_diad* fine) {
 }
 
 // updateGrid(std::vector<DIADGridStruct>&, level)
-// (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:200-209)
-static int diad_recurse(psl_diad* const* lv, int nl, int l) {
+// (flowAroundSphereDIAD/flowAroundSphereMainSingleDIAD.cu:200-209).  `last`:
+// this is the second of the level's two steps, after which the coarser
+// level's transfer reads the child records, so only this update writes them.
+static int diad_recurse(psl_diad* const* lv, int nl, int l, bool last) {
   psl_diad* d = lv[l];
   d->flipper = !d->flipper;
-  int rc = PSL_ESO_DISPATCH(eso_update, d);
+  int rc = PSL_ESO_DISPATCH(eso_update, d, last);
   if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_moving, d);
   if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_forced, d);
   if (rc != PSL_OK || l == nl - 1) return rc;
-  for (int s = 0; s < 2 && rc == PSL_OK; ++s) rc = diad_recurse(lv, nl, l + 1);
+  for (int s = 0; s < 2 && rc == PSL_OK; ++s) rc = diad_recurse(lv, nl, l + 1, s == 1);
   if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_comm, d, lv[l + 1]);
   return rc;
+}
+
+int psl_diad_verify_interface_records(psl_diad* d, int64_t* mismatches) {
+  PSL_CHECK_ARG(d && mismatches, "null argument");
+  *mismatches = 0;
+  if (!d->rec_key || !(d->child.n + d->source.n)) return PSL_OK;
+  unsigned long long* bad = nullptr;
+  PSL_HIP_TRY(hipMalloc(&bad, sizeof(unsigned long long)));
+  int rc = hipMemsetAsync(bad, 0, sizeof(unsigned long long), d->stream) == hipSuccess ? PSL_OK : PSL_E_HIP;
+  if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(rec_verify, d, bad);
+  unsigned long long h = 0;
+  if (rc == PSL_OK && (hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+                       hipStreamSynchronize(d->stream) != hipSuccess))
+    rc = PSL_E_HIP;
+  (void)hipFree(bad);
+  *mismatches = (int64_t)h;
+  return rc;
+}
+
+int psl_diad_interface_records(const psl_diad* d, int64_t* count, int* fresh) {
+  PSL_CHECK_ARG(d && count, "null argument");
+  *count = d->rec_key ? d->child.n + d->source.n : 0;
+  // every record of the level holds the last local update's values
+  if (fresh)
+    *fresh = d->rec_key && (!d->child.n || d->child.fresh) && (!d->source.n || d->source.fresh) ? 1 : 0;
+  return PSL_OK;
 }
 
 int psl_diad_update_grid(psl_diad* const* levels, int nlevels, int nsteps) {
@@ -874,8 +1368,10 @@ int psl_diad_update_grid(psl_diad* const* levels, int nlevels, int nsteps) {
     PSL_CHECK_ARG(levels[l]->storage == levels[0]->storage, "levels must share the storage type");
     PSL_CHECK_ARG(levels[l]->stream == levels[0]->stream, "levels must share the stream");
   }
+  const int rc0 = diad_prepare_records(levels, nlevels);
+  if (rc0 != PSL_OK) return rc0;
   for (int s = 0; s < nsteps; ++s) {
-    const int rc = diad_recurse(levels, nlevels, 0);
+    const int rc = diad_recurse(levels, nlevels, 0, true);
     if (rc != PSL_OK) return rc;
   }
   return PSL_OK;

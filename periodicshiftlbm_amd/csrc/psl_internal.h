@@ -173,6 +173,25 @@ struct psl_diad {
   std::vector<int> moving_host, forced_host;  // host copies of the two lists
   double* partials = nullptr;  // reduction scratch
   size_t n_partials = 0;
+  // interface records (psl_esotwist.hip): the local update also writes the
+  // post-collision populations of the cells a level transfer reads, one
+  // contiguous record per cell, so the transfers read whole lines.  Child
+  // records serve the transfer to the coarser level (the children of its
+  // fine-to-coarse entries), source records the transfer to the finer level
+  // (the sources of this level's coarse-to-fine entries).
+  struct Records {
+    uint64_t* mask = nullptr;  // per 64 cells: the cells with a record
+    uint32_t* base = nullptr;  // per 64 cells: records below them
+    void* rec = nullptr;       // ESO_REC_STRIDE storage elements per record
+    int64_t n = 0;
+    bool fresh = false;        // they hold the last local update's values
+  };
+  Records child, source;
+  int* f2c_child_rec = nullptr;        // (coarser level) 8 per f2c entry: the child's record or -1
+  int* c2f_src_rec = nullptr;          // (coarser level) per c2f entry: the source's record or -1
+  uint64_t rec_version = 1;            // bumped by every change the records depend on
+  uint64_t rec_key = 0;                // versions and neighbours the records were built for
+  const psl_diad* rec_fine = nullptr;  // the finer level f2c_child_rec indexes into
 };
 
 struct psl_stl {
@@ -195,6 +214,9 @@ __host__ __device__ inline uint64_t pack_key(int i, int j, int k) {
 // spacing res, box nx x ny x nz) into out[c] for n cells given as packed keys
 int stl_mark_keys(const psl_stl* s, float gox, float goy, float goz, float res, int nx, int ny, int nz,
                   const uint64_t* keys, int64_t n, uint8_t* out);
+// implemented in psl_esotwist.hip: a version number for a DIAD grid's
+// interface records, unique in the process
+uint64_t diad_next_version();
 // implemented in psl_esotwist.hip: rho, ux, uy, uz per cell (4 x n floats,
 // device) on the grid's stream
 int diad_macro_device(psl_diad* d, float* out4n);

@@ -58,6 +58,10 @@ def test_diad_levels_match_oracle(psl):
         P.updateGrid(grids, 2)
         for _ in range(2):
             E.update_grid(ol)
+        # the recursion's transfers read the interface records the updates wrote
+        recs = [g.interface_records()[0] for g in grids]
+        print(f"  interface records per level {recs}", flush=True)
+        assert all(r > 0 for r in recs)
         for l, (g, o) in enumerate(zip(grids, ol)):
             assert g.esotwistFlipper == o.flip
             err = _rel_err(g.get_f(), o.get())
@@ -79,6 +83,29 @@ def test_diad_levels_match_oracle(psl):
     scale = np.array([2.0 * (np.abs(c)[:, None] * np.abs(f[:, bb])).sum() for c in (cx, cy, cz)])
     print(f"  FP64 from the populations {ref}", flush=True)
     assert np.all(np.abs(fx - ref) <= 4e-6 * scale)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_interface_records_hold_each_update(psl, storage):
+    """Every record equals, word for word, what its cell's slots hold right
+    after a local update of its level; a state set from the host is never read
+    from records."""
+    P = psl
+    lv, grids = cases.diad_sphere(P, storage, n0=16, levels=3, span=1, radius=2.0)
+    for g in grids:
+        g.fill_equilibrium(ux=0.04)
+    P.updateGrid(grids, 1)
+    for l, g in enumerate(grids):
+        n, _ = g.interface_records()
+        P.applyStreaming(g)
+        P.applyLocalCellUpdate(g)
+        bad = g.verify_interface_records()
+        print(f"[records] {storage} level {l}: {n} records, {bad} differ after an update", flush=True)
+        assert n > 0
+        assert g.interface_records() == (n, True)
+        assert bad == 0
+        g.set_f(g.get_f())
+        assert g.interface_records() == (n, False)
 
 
 def test_diad_communication_random_state(psl):
