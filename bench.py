@@ -90,19 +90,25 @@ def time_steps(P, g, steps: int, warmup: int):
 
 def batch_leg(P, cases, steps: int, autotune_steps: int, storage: str = "f16", avg_steps: int = 2500,
               avg_every: int = 100):
-    """64 sphere-in-channel domains of uneven size advanced by one hipGraph
-    replay per step; aggregate MLUPS, its HBM fraction, and per-domain drag
-    coefficient averaged over avg_steps steps after the timed run."""
+    """64 sphere-in-channel domains of uneven size advanced by replays of a
+    hipGraph of 1000 captured steps; aggregate MLUPS, its HBM fraction, and
+    per-domain drag coefficient averaged over avg_steps steps after the timed
+    run.  The autotuner searches kernel variant, grid order, cells per thread
+    and tile order; the multi-step graph is captured in an untimed warm-up."""
     doms = cases.batch_domains()
     grids = [cases.domain_grid(d, storage)[0] for d in doms]
     b = P.Batch(grids)
-    cells, blocks, nbnd = b.census()
     tune = None
     if autotune_steps:
-        t, o, times = b.autotune(autotune_steps)
-        tune = {"tuning": t, "order": o, "ms_per_step": [round(float(x), 4) for x in times]}
-        for g in grids:  # restart from rest after tuning
-            P.fill_equilibrium(g)
+        (t, o, cpt, tile), times = b.autotune_layouts(autotune_steps)
+        tune = {"tuning": t, "order": o, "cells_per_thread": cpt, "tile_order": tile,
+                "ms_per_step": {f"tiles{ti}_cpt{c + 1}_order{od}": [round(float(x), 4) for x in times[ti, c, od]]
+                                for ti in range(2) for c in range(2) for od in range(2)}}
+    cells, blocks, nbnd = b.census()
+    if steps >= b.steps_per_graph:
+        b.step(b.steps_per_graph)  # captures the multi-step graph
+    for g in grids:  # restart from rest after tuning and warm-up
+        P.fill_equilibrium(g)
     P.synchronize()
     t0 = time.perf_counter()
     b.step(steps)

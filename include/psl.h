@@ -242,6 +242,22 @@ int psl_batch_get_tuning(const psl_batch* batch, int* tuning, int* order);
  * steps each; the grids advance) and keeps the fastest; times_ms[2 * PSL_TUNE_VARIANTS]
  * receives ms per step at order * PSL_TUNE_VARIANTS + variant. */
 int psl_batch_autotune(psl_batch* batch, int steps, int* best_tuning, int* best_order, double* times_ms);
+/* Layout of the two-launch bulk kernel: cells_per_thread 1 or 2 (a workgroup updates
+ * 256 or 512 consecutive cells of one grid), tile_order 0 (each grid's tiles in turn)
+ * or 1 (interleaved across grids: tile t of every grid that has one, then t + 1).  Cells
+ * are stored in the reference's order (k, j, i) whatever the layout: periodic-shift
+ * streaming shifts the linear cell index, so the tiles, not the cells, are reordered.
+ * Results are bit-identical at every layout. */
+int psl_batch_set_layout(psl_batch* batch, int cells_per_thread, int tile_order);
+int psl_batch_get_layout(const psl_batch* batch, int* cells_per_thread, int* tile_order);
+/* psl_batch_autotune at the four layouts (the fused variants, which keep their own
+ * workgroup order, at layout 1/0 only) and keeps the fastest configuration:
+ * best[4] = {variant, order, cells_per_thread, tile_order}; times_ms (optional,
+ * PSL_BATCH_LAYOUT_CONFIGS entries) receives ms per step at
+ * ((tile_order * 2 + cells_per_thread - 1) * 2 + order) * PSL_TUNE_VARIANTS + variant,
+ * -1 where a configuration is not run. */
+#define PSL_BATCH_LAYOUT_CONFIGS (8 * PSL_TUNE_VARIANTS)
+int psl_batch_autotune_layouts(psl_batch* batch, int steps, int best[4], double* times_ms);
 int psl_batch_census(const psl_batch* batch, int64_t* cells, int* blocks, int* boundary_cells);
 
 /* ----------------------------------------------- learned LES closure */

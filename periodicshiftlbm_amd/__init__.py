@@ -177,6 +177,9 @@ _SIGS = [
     ("psl_batch_set_tuning", _I, [_P, _I, _I]),
     ("psl_batch_get_tuning", _I, [_P, _P, _P]),
     ("psl_batch_autotune", _I, [_P, _I, _P, _P, _P]),
+    ("psl_batch_set_layout", _I, [_P, _I, _I]),
+    ("psl_batch_get_layout", _I, [_P, _P, _P]),
+    ("psl_batch_autotune_layouts", _I, [_P, _I, _P, _P]),
     ("psl_batch_census", _I, [_P, _P, _P, _P]),
     ("psl_device_upload", _I, [_P, _P, ctypes.c_size_t]),
     ("psl_device_download", _I, [_P, _P, ctypes.c_size_t]),
@@ -525,6 +528,26 @@ class Batch:
         times = np.zeros(2 * TUNE_VARIANTS, np.float64)
         _check(lib().psl_batch_autotune(self._h, steps, ctypes.byref(t), ctypes.byref(o), times.ctypes.data))
         return t.value, o.value, times
+
+    def set_layout(self, cells_per_thread: int = 1, tile_order: int = 0):
+        """Bulk tiles of 256 * cells_per_thread cells; tile_order 0: each grid's
+        tiles in turn, 1: interleaved across grids."""
+        _check(lib().psl_batch_set_layout(self._h, int(cells_per_thread), int(tile_order)))
+
+    @property
+    def layout(self):
+        c, t = ctypes.c_int(), ctypes.c_int()
+        _check(lib().psl_batch_get_layout(self._h, ctypes.byref(c), ctypes.byref(t)))
+        return c.value, t.value
+
+    def autotune_layouts(self, steps: int = 10):
+        """Variant, grid order, cells per thread and tile order, all searched:
+        ((variant, order, cells_per_thread, tile_order), times) with times of
+        shape (tile_order, cells_per_thread - 1, order, variant), -1 where not run."""
+        best = np.zeros(4, np.int32)
+        times = np.zeros(8 * TUNE_VARIANTS, np.float64)
+        _check(lib().psl_batch_autotune_layouts(self._h, steps, best.ctypes.data, times.ctypes.data))
+        return tuple(int(x) for x in best), times.reshape(2, 2, 2, TUNE_VARIANTS)
 
     def census(self):
         c, b, n = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()

@@ -8,8 +8,10 @@
 //   k_batch_advance   applyStreaming of every grid on the device: the shift
 //                     update and the per-population bases of the step, written
 //                     into each grid's StepParams;
-//   k_batch_bulk      one launch over the concatenated bulk blocks of all
-//                     grids (block -> grid table);
+//   k_batch_bulk      one launch over the bulk tiles of all grids (tile ->
+//                     grid and tile -> block tables): 256 or 512 cells per
+//                     workgroup (one or two cells per thread), the grids'
+//                     tiles in turn or interleaved across grids;
 //   k_batch_boundary  one launch over the concatenated boundary-list blocks;
 //   k_batch_fused     both in one launch, each grid's boundary workgroups
 //                     interleaved with its bulk blocks (a tuning variant).
@@ -20,7 +22,8 @@
 // in hipGraphs, one of a single step and one of steps_per_graph (default
 // 1000) consecutive steps, and replayed: 5000 steps are five graph launches
 // whatever the number of grids.  The block order of the grids (as given, or largest
-// first) and the kernel variant are chosen by psl_batch_autotune.
+// first) and the kernel variant are chosen by psl_batch_autotune, and with
+// them the cells per thread and the tile order by psl_batch_autotune_layouts.
 #include "psl_internal.h"
 #include "psl_stream.h"
 
@@ -54,13 +57,17 @@ __global__ void k_batch_advance(StepParams* __restrict__ params, const void* con
   P.thr[q] = P.n - (uint32_t)s;
 }
 
-template <class St, int TUNE>
+// One tile of CPT consecutive 256-cell blocks of grid tile_domain[t], its
+// tile index tile_index[t] within the grid; each thread updates one cell of
+// every block in turn.
+template <class St, int TUNE, int CPT>
 __global__ __launch_bounds__(256) void k_batch_bulk(const StepParams* __restrict__ params,
-                                                    const uint32_t* __restrict__ block_domain,
-                                                    const uint32_t* __restrict__ block_first, uint32_t remap) {
-  const uint32_t b = remap ? xcd_block_chunked<64>(blockIdx.x, gridDim.x) : blockIdx.x;
-  const uint32_t d = block_domain[b];
-  bulk_update<St, true, true, TUNE>(params[d], b - block_first[d]);
+                                                    const uint32_t* __restrict__ tile_domain,
+                                                    const uint32_t* __restrict__ tile_index, uint32_t remap) {
+  const uint32_t t = remap ? xcd_block_chunked<64>(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint32_t d = tile_domain[t];
+  const uint32_t first = tile_index[t] * CPT;
+  for (int k = 0; k < CPT; ++k) bulk_update<St, true, true, TUNE>(params[d], first + k);
 }
 
 template <class St>
@@ -106,14 +113,16 @@ struct psl_batch {
   int storage = PSL_F32;
   int tuning = PSL_TUNE_XCD_REMAP;
   int order = 0;  // 0: as given, 1: largest grid first
+  int cpt = 1;         // cells per thread of the two-launch bulk kernel (1 or 2)
+  int tile_order = 0;  // 0: each grid's tiles in turn, 1: interleaved across grids
   hipStream_t stream = nullptr;
   StepParams* d_params = nullptr;
   BoundaryParams* d_bparams = nullptr;
   const void** d_fbase = nullptr;
   int* d_delta = nullptr;
   int* d_shift = nullptr;
-  uint32_t* d_block_domain = nullptr;
-  uint32_t* d_block_first = nullptr;
+  uint32_t* d_block_domain = nullptr;  // bulk tile -> grid
+  uint32_t* d_block_tile = nullptr;    // bulk tile -> tile index within its grid
   uint32_t* d_bblock_domain = nullptr;
   uint32_t* d_bblock_first = nullptr;
   BatchEntry* d_order = nullptr;
@@ -145,8 +154,8 @@ void free_graphs(psl_batch* b) {
 
 void free_batch_device(psl_batch* b) {
   free_graphs(b);
-  void* ptrs[] = {b->d_params,       b->d_bparams,      b->d_fbase,         b->d_delta,          b->d_shift,
-                  b->d_block_domain, b->d_block_first, b->d_bblock_domain, b->d_bblock_first, b->d_order};
+  void* ptrs[] = {b->d_params,       b->d_bparams,     b->d_fbase,         b->d_delta,          b->d_shift,
+                  b->d_block_domain, b->d_block_tile, b->d_bblock_domain, b->d_bblock_first, b->d_order};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b->d_params = nullptr;
@@ -155,7 +164,7 @@ void free_batch_device(psl_batch* b) {
   b->d_delta = nullptr;
   b->d_shift = nullptr;
   b->d_block_domain = nullptr;
-  b->d_block_first = nullptr;
+  b->d_block_tile = nullptr;
   b->d_bblock_domain = nullptr;
   b->d_bblock_first = nullptr;
   b->d_order = nullptr;
@@ -179,14 +188,23 @@ int enqueue_step(psl_batch* b) {
     }
 #undef PSL_FUSED
   } else {
-#define PSL_BULK(V)                                                                                             \
-  k_batch_bulk<St, V><<<b->total_blocks, 256, 0, b->stream>>>(b->d_params, b->d_block_domain, b->d_block_first, \
-                                                              remap)
-    switch (nt) {
-      case 0: PSL_BULK(0); break;
-      case 1: PSL_BULK(1); break;
-      case 2: PSL_BULK(2); break;
-      default: PSL_BULK(3); break;
+#define PSL_BULK(V, C)                                                                                          \
+  k_batch_bulk<St, V, C><<<b->total_blocks, 256, 0, b->stream>>>(b->d_params, b->d_block_domain, b->d_block_tile, \
+                                                                 remap)
+    if (b->cpt == 2) {
+      switch (nt) {
+        case 0: PSL_BULK(0, 2); break;
+        case 1: PSL_BULK(1, 2); break;
+        case 2: PSL_BULK(2, 2); break;
+        default: PSL_BULK(3, 2); break;
+      }
+    } else {
+      switch (nt) {
+        case 0: PSL_BULK(0, 1); break;
+        case 1: PSL_BULK(1, 1); break;
+        case 2: PSL_BULK(2, 1); break;
+        default: PSL_BULK(3, 1); break;
+      }
     }
 #undef PSL_BULK
     if (b->total_bnd)
@@ -282,14 +300,34 @@ int build_batch(psl_batch* b) {
   std::vector<BoundaryParams> bparams(nd);
   std::vector<const void*> fbase(nd);
   std::vector<int> delta((size_t)nd * 27);
-  std::vector<uint32_t> block_domain, block_first(nd), bblock_domain, bblock_first(nd);
+  std::vector<uint32_t> block_domain, block_tile, bblock_domain, bblock_first(nd);
   b->total_cells = 0;
   b->total_bnd = 0;
+  // bulk tiles of 256 * cpt cells: each grid's in turn, or round-robin
+  // across the grids (tile t of every grid that has one, then t + 1)
+  const uint64_t tile_cells = 256ull * (uint64_t)b->cpt;
+  std::vector<uint32_t> ntile(nd);
+  uint32_t max_tiles = 0;
+  for (int d = 0; d < nd; ++d) {
+    ntile[d] = (uint32_t)((b->grids[d]->info.cellCount + tile_cells - 1) / tile_cells);
+    max_tiles = std::max(max_tiles, ntile[d]);
+  }
+  if (b->tile_order == 1) {
+    for (uint32_t t = 0; t < max_tiles; ++t)
+      for (int d : order)
+        if (t < ntile[d]) {
+          block_domain.push_back((uint32_t)d);
+          block_tile.push_back(t);
+        }
+  } else {
+    for (int d : order)
+      for (uint32_t t = 0; t < ntile[d]; ++t) {
+        block_domain.push_back((uint32_t)d);
+        block_tile.push_back(t);
+      }
+  }
   for (int d : order) {
     const psl_grid* g = b->grids[d];
-    const uint32_t nbk = (uint32_t)((g->info.cellCount + 255) / 256);
-    block_first[d] = (uint32_t)block_domain.size();
-    block_domain.insert(block_domain.end(), nbk, (uint32_t)d);
     const uint32_t nbb = (g->n_bc + 255u) / 256u;
     bblock_first[d] = (uint32_t)bblock_domain.size();
     bblock_domain.insert(bblock_domain.end(), nbb, (uint32_t)d);
@@ -326,7 +364,7 @@ int build_batch(psl_batch* b) {
   if (!rc) rc = upload(&b->d_fbase, fbase);
   if (!rc) rc = upload(&b->d_delta, delta);
   if (!rc) rc = upload(&b->d_block_domain, block_domain);
-  if (!rc) rc = upload(&b->d_block_first, block_first);
+  if (!rc) rc = upload(&b->d_block_tile, block_tile);
   if (!rc) rc = upload(&b->d_bblock_domain, bblock_domain);
   if (!rc) rc = upload(&b->d_bblock_first, bblock_first);
   if (!rc) rc = upload(&b->d_order, fused);
@@ -453,6 +491,23 @@ int psl_batch_set_tuning(psl_batch* b, int tuning, int order) {
   return build_batch(b);
 }
 
+int psl_batch_set_layout(psl_batch* b, int cells_per_thread, int tile_order) {
+  PSL_CHECK_ARG(b, "null batch");
+  PSL_CHECK_ARG(cells_per_thread == 1 || cells_per_thread == 2, "cells per thread must be 1 or 2");
+  PSL_CHECK_ARG(tile_order == 0 || tile_order == 1, "tile order must be 0 (grids in turn) or 1 (interleaved)");
+  PSL_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->cpt = cells_per_thread;
+  b->tile_order = tile_order;
+  return build_batch(b);
+}
+
+int psl_batch_get_layout(const psl_batch* b, int* cells_per_thread, int* tile_order) {
+  PSL_CHECK_ARG(b, "null batch");
+  if (cells_per_thread) *cells_per_thread = b->cpt;
+  if (tile_order) *tile_order = b->tile_order;
+  return PSL_OK;
+}
+
 int psl_batch_set_steps_per_graph(psl_batch* b, int steps) {
   PSL_CHECK_ARG(b && steps >= 1 && steps <= PSL_BATCH_MAX_STEPS_PER_GRAPH, "steps per graph out of range");
   if (steps == b->steps_per_graph) return PSL_OK;
@@ -516,6 +571,51 @@ int psl_batch_autotune(psl_batch* b, int steps, int* best_tuning, int* best_orde
   if (best_tuning) *best_tuning = bt;
   if (best_order) *best_order = bo;
   return psl_batch_set_tuning(b, bt, bo);
+}
+
+// The autotune above at every layout: cells per thread 1 and 2 and both tile
+// orders for the two-launch variants (the fused variants have their own
+// workgroup order and one cell per thread, so they are timed at layout 1/0
+// only).  times_ms[PSL_BATCH_LAYOUT_CONFIGS]: ms per step at
+// ((tile_order * 2 + cpt - 1) * 2 + order) * PSL_TUNE_VARIANTS + variant, -1
+// where a configuration is not run.
+int psl_batch_autotune_layouts(psl_batch* b, int steps, int best[4], double* times_ms) {
+  PSL_CHECK_ARG(b && steps > 0 && best, "bad autotune arguments");
+  double bt = 1e300;
+  int bv = b->tuning, bo = b->order, bc = b->cpt, btile = b->tile_order;
+  if (times_ms)
+    for (int i = 0; i < PSL_BATCH_LAYOUT_CONFIGS; ++i) times_ms[i] = -1.0;
+  double part[2 * PSL_TUNE_VARIANTS];
+  for (int tile = 0; tile < 2; ++tile)
+    for (int c = 1; c <= 2; ++c) {
+      int rc = psl_batch_set_layout(b, c, tile);
+      if (!rc) rc = psl_batch_autotune(b, steps, nullptr, nullptr, part);
+      if (rc) return rc;
+      for (int order = 0; order < 2; ++order)
+        for (int v = 0; v < PSL_TUNE_VARIANTS; ++v) {
+          const bool fused = (v & PSL_TUNE_FUSED_BOUNDARY) != 0;
+          if (fused && (c != 1 || tile != 0)) continue;  // the same launch as at layout 1/0
+          const double per = part[order * PSL_TUNE_VARIANTS + v];
+          if (times_ms) times_ms[((tile * 2 + c - 1) * 2 + order) * PSL_TUNE_VARIANTS + v] = per;
+          if (per < bt) {
+            bt = per;
+            bv = v;
+            bo = order;
+            bc = c;
+            btile = tile;
+          }
+        }
+    }
+  best[0] = bv;
+  best[1] = bo;
+  best[2] = bc;
+  best[3] = btile;
+  PSL_HIP_TRY(hipStreamSynchronize(b->stream));
+  b->tuning = bv;
+  b->order = bo;
+  b->cpt = bc;
+  b->tile_order = btile;
+  return build_batch(b);
 }
 
 int psl_batch_census(const psl_batch* b, int64_t* cells, int* blocks, int* boundary_cells) {

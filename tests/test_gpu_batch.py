@@ -2,7 +2,8 @@
 
 A batch advances every grid by one hipGraph replay per step, with the same
 per-cell device code as the single-grid kernels.  Each grid must end exactly
-(bit for bit, every storage type, every kernel variant and block order) where
+(bit for bit, every storage type, every kernel variant, block order, number
+of cells per thread and tile order) where
 the same number of single-grid updateGrid steps puts it; the single-grid
 path is pinned against the oracle by the other GPU tests, and one batched
 domain is compared with the oracle directly here (1e-5 relative, FP32).
@@ -68,6 +69,29 @@ def test_batch_matches_single_grid_steps(psl, storage):
         for g in single:
             P.updateGrid([g], 3)
         _assert_identical(doms, batched, single, f"tuning {tuning} order {order}")
+    # the layouts of the two-launch bulk kernel: two cells per thread and the
+    # tiles interleaved across grids
+    for cpt, tile in ((2, 0), (1, 1), (2, 1)):
+        for tuning in (0, P.TUNE_XCD_REMAP | P.TUNE_NT_LOAD):
+            b.set_tuning(tuning, 1)
+            b.set_layout(cpt, tile)
+            assert b.layout == (cpt, tile)
+            b.step(3)
+            for g in single:
+                P.updateGrid([g], 3)
+            _assert_identical(doms, batched, single, f"layout cpt {cpt} tiles {tile} tuning {tuning}")
+    # the full search: 4 layouts x 2 orders x 16 variants, one warm-up step
+    # and one timed step each (the grids advance), fused variants at layout 1/0 only
+    best, times = b.autotune_layouts(steps=1)
+    assert best[2] in (1, 2) and best[3] in (0, 1)
+    assert int(np.count_nonzero(times >= 0)) == 8 * 16 - 3 * 2 * 8
+    for g in single:
+        P.updateGrid([g], 4 * 2 * 16 * 2)
+    _assert_identical(doms, batched, single, f"after the layout search (best {best})")
+    b.step(5)
+    for g in single:
+        P.updateGrid([g], 5)
+    _assert_identical(doms, batched, single, f"best layout {best}")
     b.close()
 
 

@@ -1,8 +1,12 @@
 """The benchmark configurations at parity-test size, against the oracle.
 
 * Taylor-Green vortex, 64^3 fully periodic, FP32, 1000 steps (the
-  CPU-runnable configuration at full size), against the FP32 oracle (the
-  reference's own mixed FP32/FP64 expressions) and the FP64 oracle.  After
+  CPU-runnable configuration at full size), on both layouts: the reference's
+  65 x 65 x 64 periodic-shift grid with a duplicated layer and x/y copies
+  (include/boundaryConditions/periodicXY.h), and the true-periodic 64^3
+  esotwist torus (periodicity through the neighbour indices, no copies),
+  against the FP32 oracle (the reference's own mixed FP32/FP64 expressions)
+  and the FP64 oracle.  After
   1000 steps two FP32 evaluations of the same step drift apart by FP32
   roundoff alone: the FP32 oracle's kinetic energy is 1.35e-5 below the FP64
   oracle's, and the GPU's (restructured FP32 collision) 7.5e-6 above it.  So:
@@ -12,7 +16,9 @@
     - kinetic energy and enstrophy every 100 steps within 2e-5 relative of
       the FP64 oracle (the GPU's enstrophy drift reaches 1.02e-5 at step
       1000, the FP32 oracle's 1.58e-5), and no further from it than the FP32
-      oracle is.
+      oracle is, on both layouts;
+    - the torus's populations after 1000 steps within 2e-5 relative of the
+      periodic-shift grid's on the 64^3 unique cells.
 * Sphere drag with f - w stored in FP16 / BF16 (and FP32) against an FP64
   oracle run of the same channel: the sphere configuration scaled by 1/4
   (64 x 32 x 32, icosphere diameter 8 voxelized by the STL path, U = 0.04,
@@ -61,29 +67,54 @@ def test_taylor_green_64_1000_steps(psl, oracle_mod):
     o64.fill_equilibrium(rho, ux, uy, uz)
     g.set_f(o32.logical_f().astype(np.float32))
     g.set_periodic_copies(x=True, y=True)
-    print(f"[tgv64] shape={shape} 1000 steps", flush=True)
-    worst_gpu, worst_o32 = 0.0, 0.0
+    # the true-periodic layout: the 64^3 unique cells as an esotwist torus,
+    # started from the same bits
+    sel = np.zeros((nz, ny, nx), bool)
+    sel[:, :n, :n] = True
+    sel = sel.reshape(-1)
+    ijk, nbr = P.DIADGridStruct.dense_box(n, n, n)
+    torus = P.DIADGridStruct(P.InfoStruct(cellCountX=n, cellCountY=n, cellCountZ=n, nu=nu), ijk, nbr, "f32")
+    torus.set_f(np.ascontiguousarray(o32.logical_f().astype(np.float32)[:, sel]))
+
+    def torus_stats():
+        ro = O.rho_u(np.ascontiguousarray(torus.get_f().T.astype(np.float64)))
+        full = np.zeros((4, nx * ny * nz))
+        full[:, sel] = ro.T
+        return cases.kinetic_energy_enstrophy(*full, n)
+
+    print(f"[tgv64] shape={shape} and a {n}^3 torus, 1000 steps", flush=True)
+    worst_gpu, worst_torus, worst_o32 = 0.0, 0.0, 0.0
     for block in range(10):
         P.updateGrid([g], 100)
         for _ in range(100):
+            P.applyStreaming(torus)
+            P.applyLocalCellUpdate(torus)
             o32.step(periodic_x=True, periodic_y=True)
             o64.step(periodic_x=True, periodic_y=True)
         r, a, b, c = g.macroscopic()
         ke_g, en_g = cases.kinetic_energy_enstrophy(r, a, b, c, n)
+        ke_t, en_t = torus_stats()
         ke_32, en_32 = _stats(O, o32.logical_f(), n)
         ke_64, en_64 = _stats(O, o64.logical_f(), n)
         eg = max(abs(ke_g / ke_64 - 1), abs(en_g / en_64 - 1))
+        et = max(abs(ke_t / ke_64 - 1), abs(en_t / en_64 - 1))
         e32 = max(abs(ke_32 / ke_64 - 1), abs(en_32 / en_64 - 1))
-        worst_gpu, worst_o32 = max(worst_gpu, eg), max(worst_o32, e32)
-        print(f"  step {100 * (block + 1)}: KE {ke_g:.7e} vs FP64 {ke_g / ke_64 - 1:+.2e} (FP32 oracle "
-              f"{ke_32 / ke_64 - 1:+.2e}); enstrophy vs FP64 {en_g / en_64 - 1:+.2e} (FP32 oracle "
-              f"{en_32 / en_64 - 1:+.2e})", flush=True)
-    err = _rel_err(g.get_f(), o32.logical_f())
-    print(f"  f max rel err vs FP32 oracle after 1000 steps {err:.3e}; KE/enstrophy vs FP64: gpu {worst_gpu:.2e}, "
-          f"FP32 oracle {worst_o32:.2e}")
-    assert worst_gpu < 2e-5
-    assert worst_gpu <= worst_o32
+        worst_gpu, worst_torus, worst_o32 = max(worst_gpu, eg), max(worst_torus, et), max(worst_o32, e32)
+        print(f"  step {100 * (block + 1)}: KE {ke_g:.7e} vs FP64 {ke_g / ke_64 - 1:+.2e} (torus "
+              f"{ke_t / ke_64 - 1:+.2e}, FP32 oracle {ke_32 / ke_64 - 1:+.2e}); enstrophy vs FP64 "
+              f"{en_g / en_64 - 1:+.2e} (torus {en_t / en_64 - 1:+.2e}, FP32 oracle {en_32 / en_64 - 1:+.2e})",
+              flush=True)
+    f_ps = g.get_f()
+    err = _rel_err(f_ps, o32.logical_f())
+    err_torus = _rel_err(torus.get_f(), f_ps[:, sel])
+    print(f"  f max rel err vs FP32 oracle after 1000 steps {err:.3e}; torus vs periodic shift {err_torus:.3e}; "
+          f"KE/enstrophy vs FP64: periodic shift {worst_gpu:.2e}, torus {worst_torus:.2e}, FP32 oracle "
+          f"{worst_o32:.2e}")
+    assert worst_gpu < 2e-5 and worst_torus < 2e-5
+    assert worst_gpu <= worst_o32 and worst_torus <= worst_o32
     assert err < 2e-5
+    assert err_torus < 2e-5
+    torus.close()
 
 
 SCALED = {"shape": (64, 32, 32), "center": (16.0, 16.0, 16.0), "diameter": 8.0}
