@@ -64,9 +64,10 @@ __global__ __launch_bounds__(64) void k_ps_boundary_staged(StepParams p, Boundar
 
 // (held at the plain kernel's 7 waves per SIMD: without the bound the staging
 // branch raises the register count to 5 waves)
-template <class St, bool FLAGS, bool LES, int TUNE, bool STAGE>
+template <class St, bool FLAGS, bool LES, int TUNE, bool STAGE, bool HELD = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_ps_bulk(StepParams p, Stage sg) {
-  bulk_update<St, FLAGS, LES, TUNE, STAGE>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x, sg);
+  bulk_update<St, FLAGS, LES, TUNE, STAGE, HELD>(p, p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x,
+                                                 sg);
 }
 
 // Bulk cells with the learned closure in place of the Smagorinsky rate.
@@ -494,8 +495,13 @@ static void drop_profile_events(psl_grid* g) {
 }
 
 template <class St, int TUNE>
-static void launch_bulk(bool flags, bool les, uint32_t nb, hipStream_t stream, const StepParams& p, const Stage& sg) {
-  if (sg.buf) {
+static void launch_bulk(bool flags, bool les, bool held, uint32_t nb, hipStream_t stream, const StepParams& p,
+                        const Stage& sg) {
+  if (held) {
+    // between learned-closure evaluations (never staged)
+    if (flags) k_ps_bulk<St, true, true, TUNE, false, true><<<nb, 256, 0, stream>>>(p, sg);
+    else k_ps_bulk<St, false, true, TUNE, false, true><<<nb, 256, 0, stream>>>(p, sg);
+  } else if (sg.buf) {
     if (les) k_ps_bulk<St, true, true, TUNE, true><<<nb, 256, 0, stream>>>(p, sg);
     else k_ps_bulk<St, true, false, TUNE, true><<<nb, 256, 0, stream>>>(p, sg);
   } else if (flags) {
@@ -599,10 +605,10 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
     // their slots)
     const Stage sg{b.n && !held ? g->stage_buf : nullptr, g->stage_base};
     switch (nt) {
-      case 0: launch_bulk<St, 0>(fl, les, nb, g->stream, p, sg); break;
-      case 1: launch_bulk<St, 1>(fl, les, nb, g->stream, p, sg); break;
-      case 2: launch_bulk<St, 2>(fl, les, nb, g->stream, p, sg); break;
-      default: launch_bulk<St, 3>(fl, les, nb, g->stream, p, sg); break;
+      case 0: launch_bulk<St, 0>(fl, les, held, nb, g->stream, p, sg); break;
+      case 1: launch_bulk<St, 1>(fl, les, held, nb, g->stream, p, sg); break;
+      case 2: launch_bulk<St, 2>(fl, les, held, nb, g->stream, p, sg); break;
+      default: launch_bulk<St, 3>(fl, les, held, nb, g->stream, p, sg); break;
     }
   }
   PSL_LAUNCH_CHECK();

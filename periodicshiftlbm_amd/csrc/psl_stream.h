@@ -34,7 +34,7 @@ struct StepParams {
   const uint8_t* flags;
   // learned closure held between network evaluations: per-cell network
   // output y (FP16 bits); fluid cells relax with tau = 3 (nu + nut_scale
-  // max(y, 0)) + 1/2 and no Smagorinsky rate.  Null otherwise.
+  // max(y, 0)) + 1/2 and no Smagorinsky rate (read by the HELD update only).
   const uint16_t* nut_y;
   float nu;
   float nut_scale;
@@ -336,8 +336,10 @@ __device__ __forceinline__ void boundary_update_staged(const StepParams& p, cons
 
 // Fluid and bounceback cells of one block in cell order; with STAGE the
 // listed box-face cells of the block are loaded too and written to their
-// staging records (their update is the boundary kernel's).
-template <class St, bool FLAGS, bool LES, int TUNE, bool STAGE = false>
+// staging records (their update is the boundary kernel's).  HELD: a step
+// between evaluations of a learned closure, relaxing with the network output
+// stored per cell (p.nut_y) and no Smagorinsky term.
+template <class St, bool FLAGS, bool LES, int TUNE, bool STAGE = false, bool HELD = false>
 __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block, const Stage& sg = Stage{}) {
   const uint32_t block_first = block * 256u;
   const uint32_t cell = block_first + threadIdx.x;
@@ -349,6 +351,10 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
   int kind = KIND_FLUID;
   if (FLAGS) kind = ld<uint8_t, false>(p.flags + cell) & 7;
   if (kind > (STAGE ? KIND_GIVEN_RHOU : KIND_BOUNCEBACK)) return;
+  // the stored network output is loaded ahead of the populations, so that it
+  // is in flight with them rather than after them
+  uint16_t y_held = 0;
+  if constexpr (HELD) y_held = ld<uint16_t, false>(p.nut_y + cell);
   Word<St> r[27];
   load27_raw<St, TUNE>(p, w_first, straddle, cell, r);
   float v[27];
@@ -378,14 +384,15 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
     // the swap commutes with "f - w" (opposite directions share a weight)
     words_to_float<St>(r, v);
     bounceback(v);
+  } else if constexpr (HELD) {
+    // the rate from the stored network output; the constant zero removes the
+    // Smagorinsky term from this instantiation
+    const float y = (float)__builtin_bit_cast(_Float16, y_held);
+    collide_with_moments<St::W_OFF>(v, raw_moments_words<St>(r), closure_relax(p.nu, p.nut_scale, y), 0.f);
   } else {
     Relax rx{p.omega, p.tau, p.cs};
     float cs = p.cs;
     if (LES && p.cs_cell) cs = ld<float, false>(p.cs_cell + cell);
-    if (LES && p.nut_y) {
-      rx = closure_relax(p.nu, p.nut_scale, (float)__builtin_bit_cast(_Float16, ld<uint16_t, false>(p.nut_y + cell)));
-      cs = 0.f;
-    }
     // the collision reads only the moments; v receives the post-collision values
     collide_with_moments<St::W_OFF>(v, raw_moments_words<St>(r), rx, LES ? cs : 0.f);
   }
