@@ -211,6 +211,24 @@ int psl_grid_time_update_kernel(psl_grid* grid, int launches, double* ms_per_lau
 /* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
 int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
 
+/* ------------------------------------------------ restart state, health */
+/* No reference counterpart (the reference never writes fArray and has no guard against
+ * a diverging run).  The restart state is a 256-byte header (magic "PSLSTATE", version,
+ * kind, storage, sizes, shifter, closure step count) followed by the storage words of f
+ * exactly as the device holds them (FP32, or the 16-bit f - w words), then the scalar
+ * populations (27 x N FP32) if psl_scalar_enable was called and the held network output
+ * (N FP16) if the grid evaluates a learned closure every k > 1 steps.  A grid with the same
+ * info, storage and settings loaded from it continues bit for bit.  Flags, given values
+ * and Cs are configuration, not state.  bytes must equal psl_grid_state_bytes; loading
+ * checks the header against the grid (PSL_E_ARG on any mismatch). */
+int psl_grid_state_bytes(const psl_grid* grid, int64_t* bytes);
+int psl_grid_save_state(psl_grid* grid, void* host, int64_t bytes);
+int psl_grid_load_state(psl_grid* grid, const void* host, int64_t bytes);
+/* Cells with a non-finite population or a density <= 0 (density = the sum of the cell's
+ * populations as the next local update reads them), and the least and greatest density
+ * of the other cells (+inf / -inf when there are none); rho_min / rho_max may be NULL. */
+int psl_grid_check_health(psl_grid* grid, int64_t* bad_cells, double* rho_min, double* rho_max);
+
 /* ------------------------------------------------------- batched grids */
 /* Many independent grids advanced together (no reference counterpart: the
  * reference steps one grid at a time).  Each grid keeps its own flags, given
@@ -424,6 +442,13 @@ int psl_diad_interface_records(const psl_diad* grid, int64_t* count, int* fresh)
 /* Records whose values differ from the populations their cells hold now (0 right after a
  * local update of the level; a consistency check). */
 int psl_diad_verify_interface_records(psl_diad* grid, int64_t* mismatches);
+/* Restart state and health of an esotwist grid, as psl_grid_*_state / psl_grid_check_health:
+ * the header carries the esotwist parity (esotwistFlipper) instead of a shifter, then the
+ * storage words of f; loading marks the level's interface records stale. */
+int psl_diad_state_bytes(const psl_diad* grid, int64_t* bytes);
+int psl_diad_save_state(psl_diad* grid, void* host, int64_t bytes);
+int psl_diad_load_state(psl_diad* grid, const void* host, int64_t bytes);
+int psl_diad_check_health(psl_diad* grid, int64_t* bad_cells, double* rho_min, double* rho_max);
 /* outletCellArray (include/types.h:171) and kMinNbrArray (include/types.h:161, n
  * entries) from host arrays; psl_diad_create_level sets them for the levels
  * whose outlet rules marked cells. */

@@ -222,6 +222,14 @@ _SIGS = [
     ("psl_diad_coarse_fine_communication", _I, [_P, _P]),
     ("psl_diad_update_grid", _I, [_P, _I, _I]),
     ("psl_diad_interface_records", _I, [_P, _P, _P]),
+    ("psl_diad_state_bytes", _I, [_P, _P]),
+    ("psl_diad_save_state", _I, [_P, _P, ctypes.c_int64]),
+    ("psl_diad_load_state", _I, [_P, _P, ctypes.c_int64]),
+    ("psl_diad_check_health", _I, [_P, _P, _P, _P]),
+    ("psl_grid_state_bytes", _I, [_P, _P]),
+    ("psl_grid_save_state", _I, [_P, _P, ctypes.c_int64]),
+    ("psl_grid_load_state", _I, [_P, _P, ctypes.c_int64]),
+    ("psl_grid_check_health", _I, [_P, _P, _P, _P]),
     ("psl_diad_verify_interface_records", _I, [_P, _P]),
     ("psl_diad_reduce_momentum_exchange", _I, [_P, _P]),
     ("psl_diad_set_outlet", _I, [_P, _P, ctypes.c_int64, _P]),
@@ -282,6 +290,19 @@ def require_device():
 
 def _ptr(a):
     return None if a is None else a.ctypes.data
+
+
+def _state_bytes(state) -> np.ndarray:
+    """A restart state given as a byte array or a file path."""
+    if isinstance(state, (str, os.PathLike)):
+        return np.fromfile(state, np.uint8)
+    return np.ascontiguousarray(state, np.uint8).reshape(-1)
+
+
+def _health(fn, h):
+    bad, lo, hi = ctypes.c_int64(), ctypes.c_double(), ctypes.c_double()
+    _check(fn(h, ctypes.byref(bad), ctypes.byref(lo), ctypes.byref(hi)))
+    return bad.value, lo.value, hi.value
 
 
 # ---------------------------------------------------------------- grids
@@ -399,6 +420,27 @@ class GridStruct:
         out = DeviceBuffer(4 * self.n * 4)
         _check(lib().psl_get_macroscopic_device(self._h, out.ptr))
         return out
+
+    def state(self) -> np.ndarray:
+        """The restart state (psl_grid_save_state): header, f as stored, and the
+        scalar populations / held closure output where the grid has them."""
+        n = ctypes.c_int64()
+        _check(lib().psl_grid_state_bytes(self._h, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        _check(lib().psl_grid_save_state(self._h, buf.ctypes.data, buf.size))
+        return buf
+
+    def save_state(self, path):
+        self.state().tofile(path)
+
+    def load_state(self, state):
+        """Restore a state from state() or save_state (array or path)."""
+        a = _state_bytes(state)
+        _check(lib().psl_grid_load_state(self._h, a.ctypes.data, a.size))
+
+    def check_health(self):
+        """(cells with a non-finite population or rho <= 0, min rho, max rho of the others)."""
+        return _health(lib().psl_grid_check_health, self._h)
 
     def time_update_kernel(self, launches: int = 50):
         """(mean ms, kernel name) of the dominant update kernel launched back to back."""
@@ -623,6 +665,25 @@ class DIADGridStruct:
         fresh = ctypes.c_int()
         _check(lib().psl_diad_interface_records(self._h, ctypes.byref(n), ctypes.byref(fresh)))
         return n.value, bool(fresh.value)
+
+    def state(self) -> np.ndarray:
+        """The restart state (psl_diad_save_state): header with the parity, f as stored."""
+        n = ctypes.c_int64()
+        _check(lib().psl_diad_state_bytes(self._h, ctypes.byref(n)))
+        buf = np.empty(n.value, np.uint8)
+        _check(lib().psl_diad_save_state(self._h, buf.ctypes.data, buf.size))
+        return buf
+
+    def save_state(self, path):
+        self.state().tofile(path)
+
+    def load_state(self, state):
+        a = _state_bytes(state)
+        _check(lib().psl_diad_load_state(self._h, a.ctypes.data, a.size))
+
+    def check_health(self):
+        """(cells with a non-finite population or rho <= 0, min rho, max rho of the others)."""
+        return _health(lib().psl_diad_check_health, self._h)
 
     def verify_interface_records(self) -> int:
         """Records that differ from the populations their cells hold now."""

@@ -31,7 +31,7 @@ EXTRA = {
 }
 SOURCES = ["psl_grid.hip", "psl_refine.hip", "psl_stl.hip", "psl_esotwist.hip", "psl_batch.hip", "psl_closure.hip",
            "psl_diad_build.hip", "psl_output.hip",
-           "psl_scalar.hip", "psl_probe.hip"]
+           "psl_scalar.hip", "psl_probe.hip", "psl_state.hip"]
 
 
 def hipcc() -> str:

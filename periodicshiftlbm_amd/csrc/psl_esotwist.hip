@@ -12,7 +12,9 @@
 // Layout: fArray(q, cell) at f[q * n + cell]; ijk[3][n]; nbr[7][n].
 #include "psl_internal.h"
 
+#include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstring>
 
 namespace psl {
@@ -713,6 +715,52 @@ __global__ __launch_bounds__(256) void k_rec_verify(const typename St::T* __rest
   if (nbad) atomicAdd(bad, 1ull);
 }
 
+// Per block: cells whose populations (as their last update left them) hold a
+// non-finite value or sum to rho <= 0, and the least and greatest density of
+// the other cells.
+template <class St, bool FLIP>
+__global__ __launch_bounds__(256) void k_eso_health(const typename St::T* __restrict__ f, EsoParams p,
+                                                    uint32_t* __restrict__ bad, float2* __restrict__ range) {
+  __shared__ uint32_t sb[256];
+  __shared__ float slo[256], shi[256];
+  const uint32_t cell = blockIdx.x * 256u + threadIdx.x;
+  uint32_t b = 0;
+  float lo = INFINITY, hi = -INFINITY;
+  if (cell < p.n) {
+    float v[27];
+    eso_get<St, FLIP>(f, p, cell, v);
+    float rho = 0.f;
+    bool finite = true;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      finite = finite && __builtin_isfinite(v[q]);
+      rho += v[q];
+    }
+    if (!finite || !(rho > 0.f)) {
+      b = 1;
+    } else {
+      lo = rho;
+      hi = rho;
+    }
+  }
+  sb[threadIdx.x] = b;
+  slo[threadIdx.x] = lo;
+  shi[threadIdx.x] = hi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sb[threadIdx.x] += sb[threadIdx.x + w];
+      slo[threadIdx.x] = fminf(slo[threadIdx.x], slo[threadIdx.x + w]);
+      shi[threadIdx.x] = fmaxf(shi[threadIdx.x], shi[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bad[blockIdx.x] = sb[0];
+    range[blockIdx.x] = make_float2(slo[0], shi[0]);
+  }
+}
+
 }  // namespace psl
 
 using namespace psl;
@@ -1025,6 +1073,15 @@ static int rec_verify(psl_diad* d, unsigned long long* bad) {
     else
       k_rec_verify<St, false><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, r->rec, r->mask, r->base, bad);
   }
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
+}
+
+template <class St>
+static int eso_health(psl_diad* d, uint32_t nb, uint32_t* bad, float2* range) {
+  const EsoParams p = eso_params(d);
+  if (d->flipper) k_eso_health<St, true><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, bad, range);
+  else k_eso_health<St, false><<<nb, 256, 0, d->stream>>>((const typename St::T*)d->f, p, bad, range);
   PSL_LAUNCH_CHECK();
   return PSL_OK;
 }
@@ -1350,6 +1407,77 @@ int psl_diad_verify_interface_records(psl_diad* d, int64_t* mismatches) {
   (void)hipFree(bad);
   *mismatches = (int64_t)h;
   return rc;
+}
+
+int psl_diad_state_bytes(const psl_diad* d, int64_t* bytes) {
+  PSL_CHECK_ARG(d && bytes, "null argument");
+  *bytes = (int64_t)sizeof(StateHeader) + 27 * d->n * (int64_t)storage_bytes(d->storage);
+  return PSL_OK;
+}
+
+int psl_diad_save_state(psl_diad* d, void* host, int64_t bytes) {
+  PSL_CHECK_ARG(d && host, "null argument");
+  int64_t want = 0;
+  (void)psl_diad_state_bytes(d, &want);
+  PSL_CHECK_ARG(bytes == want, "state buffer size differs from psl_diad_state_bytes");
+  StateHeader h;
+  state_header_init(h, STATE_DIAD, d->storage, d->n, d->info);
+  h.flipper = d->flipper ? 1 : 0;
+  memcpy(host, &h, sizeof(h));
+  PSL_HIP_TRY(hipStreamSynchronize(d->stream));
+  PSL_HIP_TRY(hipMemcpy((char*)host + sizeof(h), d->f, (size_t)(want - (int64_t)sizeof(h)), hipMemcpyDeviceToHost));
+  return PSL_OK;
+}
+
+int psl_diad_load_state(psl_diad* d, const void* host, int64_t bytes) {
+  PSL_CHECK_ARG(d && host && bytes >= (int64_t)sizeof(StateHeader), "bad state arguments");
+  StateHeader h;
+  memcpy(&h, host, sizeof(h));
+  int64_t want = 0;
+  (void)psl_diad_state_bytes(d, &want);
+  const int rc = state_header_check(h, STATE_DIAD, d->storage, d->n, d->info, bytes, want);
+  if (rc) return rc;
+  PSL_CHECK_ARG(h.sections == 0 && (h.flipper == 0 || h.flipper == 1), "malformed esotwist state");
+  PSL_HIP_TRY(hipStreamSynchronize(d->stream));
+  PSL_HIP_TRY(hipMemcpy(d->f, (const char*)host + sizeof(h), (size_t)(want - (int64_t)sizeof(h)),
+                        hipMemcpyHostToDevice));
+  d->flipper = h.flipper != 0;
+  records_stale(d);
+  return PSL_OK;
+}
+
+int psl_diad_check_health(psl_diad* d, int64_t* bad_cells, double* rho_min, double* rho_max) {
+  PSL_CHECK_ARG(d && bad_cells, "null argument");
+  const uint32_t nb = (uint32_t)((d->n + 255) / 256);
+  uint32_t* bad = nullptr;
+  float2* range = nullptr;
+  PSL_HIP_TRY(hipMalloc(&bad, (size_t)nb * sizeof(uint32_t)));
+  int rc = hipMalloc(&range, (size_t)nb * sizeof(float2)) == hipSuccess ? PSL_OK : PSL_E_HIP;
+  if (rc == PSL_OK) rc = PSL_ESO_DISPATCH(eso_health, d, nb, bad, range);
+  std::vector<uint32_t> hb(rc == PSL_OK ? nb : 0);
+  std::vector<float2> hr(rc == PSL_OK ? nb : 0);
+  if (rc == PSL_OK &&
+      (hipMemcpyAsync(hb.data(), bad, nb * sizeof(uint32_t), hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+       hipMemcpyAsync(hr.data(), range, nb * sizeof(float2), hipMemcpyDeviceToHost, d->stream) != hipSuccess ||
+       hipStreamSynchronize(d->stream) != hipSuccess))
+    rc = PSL_E_HIP;
+  (void)hipFree(bad);
+  if (range) (void)hipFree(range);
+  if (rc) {
+    set_error("health check failed");
+    return rc;
+  }
+  int64_t nbad = 0;
+  double lo = INFINITY, hi = -INFINITY;
+  for (uint32_t b = 0; b < nb; ++b) {
+    nbad += hb[b];
+    lo = std::min(lo, (double)hr[b].x);
+    hi = std::max(hi, (double)hr[b].y);
+  }
+  *bad_cells = nbad;
+  if (rho_min) *rho_min = lo;
+  if (rho_max) *rho_max = hi;
+  return PSL_OK;
 }
 
 int psl_diad_interface_records(const psl_diad* d, int64_t* count, int* fresh) {

@@ -214,6 +214,31 @@ __host__ __device__ inline uint64_t pack_key(int i, int j, int k) {
 // spacing res, box nx x ny x nz) into out[c] for n cells given as packed keys
 int stl_mark_keys(const psl_stl* s, float gox, float goy, float goz, float res, int nx, int ny, int nz,
                   const uint64_t* keys, int64_t n, uint8_t* out);
+// Restart state (psl_state.hip, psl_esotwist.hip): this header, then the
+// storage words of f (27 x n, q-major, as held on the device), then the
+// sections the header lists.
+struct StateHeader {
+  char magic[8];           // "PSLSTATE"
+  uint32_t version;        // STATE_VERSION
+  uint32_t kind;           // STATE_PS or STATE_DIAD
+  int32_t storage;
+  int32_t sections;        // STATE_SCALAR | STATE_CLOSURE_Y
+  int64_t n;               // cells
+  int32_t nx, ny, nz;
+  int32_t flipper;         // esotwist parity (DIAD)
+  int32_t shift[27];       // shifter (periodic shift)
+  int32_t closure_every;   // network evaluated every closure_every updates
+  uint64_t closure_count;  // updates since the closure was set
+  uint8_t pad[88];
+};
+static_assert(sizeof(StateHeader) == 256, "state header layout");
+constexpr uint32_t STATE_VERSION = 1, STATE_PS = 0, STATE_DIAD = 1;
+constexpr int32_t STATE_SCALAR = 1;     // 27 x n FP32 scalar populations follow f
+constexpr int32_t STATE_CLOSURE_Y = 2;  // n FP16 held network outputs follow
+// a header for kind/storage/n/dims; its validation against a grid on load
+void state_header_init(StateHeader& h, uint32_t kind, int storage, int64_t n, const psl_info& info);
+int state_header_check(const StateHeader& h, uint32_t kind, int storage, int64_t n, const psl_info& info,
+                       int64_t bytes, int64_t expect_bytes);
 // implemented in psl_esotwist.hip: a version number for a DIAD grid's
 // interface records, unique in the process
 uint64_t diad_next_version();
