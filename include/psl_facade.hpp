@@ -143,6 +143,33 @@ inline void momentumExchange(GridStruct& g, const int (&lo)[3], const int (&hi)[
   check(psl_reduce_momentum_exchange(g.handle(), lo, hi, out));
 }
 
+// Restart state (no reference counterpart): the populations as stored on the
+// device with the shifter; a grid with the same settings continues from it bit
+// for bit.
+inline std::vector<uint8_t> saveState(GridStruct& g) {
+  int64_t n = 0;
+  check(psl_grid_state_bytes(g.handle(), &n));
+  std::vector<uint8_t> s((size_t)n);
+  check(psl_grid_save_state(g.handle(), s.data(), n));
+  return s;
+}
+inline void loadState(GridStruct& g, const std::vector<uint8_t>& s) {
+  check(psl_grid_load_state(g.handle(), s.data(), (int64_t)s.size()));
+}
+
+// Cells with a non-finite population or density <= 0, and the density range
+// of the others (no reference counterpart: the reference has no guard).
+struct Health {
+  int64_t badCells = 0;
+  double rhoMin = 0.0, rhoMax = 0.0;
+  bool ok() const { return badCells == 0; }
+};
+inline Health checkHealth(GridStruct& g) {
+  Health h;
+  check(psl_grid_check_health(g.handle(), &h.badCells, &h.rhoMin, &h.rhoMax));
+  return h;
+}
+
 // ------------------------------------------------------------- geometry
 class STLStruct {
  public:
@@ -277,6 +304,22 @@ inline void updateGrid(std::vector<DIADGridStruct*>& grids, int steps = 1) {
 // getSphereDrag(DIADGridStruct&), all components, FP64
 inline void momentumExchange(DIADGridStruct& g, double (&out)[3]) {
   check(psl_diad_reduce_momentum_exchange(g.handle(), out));
+}
+// restart state and health of an esotwist grid (the parity in place of the shifter)
+inline std::vector<uint8_t> saveState(DIADGridStruct& g) {
+  int64_t n = 0;
+  check(psl_diad_state_bytes(g.handle(), &n));
+  std::vector<uint8_t> s((size_t)n);
+  check(psl_diad_save_state(g.handle(), s.data(), n));
+  return s;
+}
+inline void loadState(DIADGridStruct& g, const std::vector<uint8_t>& s) {
+  check(psl_diad_load_state(g.handle(), s.data(), (int64_t)s.size()));
+}
+inline Health checkHealth(DIADGridStruct& g) {
+  Health h;
+  check(psl_diad_check_health(g.handle(), &h.badCells, &h.rhoMin, &h.rhoMax));
+  return h;
 }
 // movingBounceback cells with their wall velocities (cells flagged PSL_BOUNCEBACK)
 inline void setMovingBounceback(DIADGridStruct& g, const std::vector<int>& cells, const std::vector<float>& u3) {

@@ -167,6 +167,15 @@ int main(int argc, char** argv) {
       const double lapTime = std::chrono::duration<double>(t1 - t0).count();
       std::printf("Finished iteration %d\nDrag %.6e\nDrag coefficient %.6f\nGLUPS: %.3f\n", iteration, drag,
                   historyDragCoefficient[iteration], (double)cellUpdatesPerIteration * iterationChunk / lapTime / 1e9);
+      // a diverged run stops here instead of plotting NaN fields (the reference has no guard)
+      for (int level = 0; level < gridLevelCount; level++) {
+        const Health h = checkHealth(*levels[level]);
+        if (!h.ok()) {
+          std::printf("Grid %d: %lld cells with non-finite populations or rho <= 0; stopping\n", level,
+                      (long long)h.badCells);
+          return 2;
+        }
+      }
       for (int level = gridLevelCount - 2; level >= 0; level--) fillCoarseGridFromFine(*levels[level], *levels[level + 1]);
       for (int level = 0; level < gridLevelCount; level++)
         exportSectionCutPlot(*levels[level], XY, levels[level]->Info.cellCountZ / 2, iteration + level, rhoNominalPhys,

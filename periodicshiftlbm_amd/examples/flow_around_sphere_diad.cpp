@@ -113,6 +113,15 @@ int main(int argc, char** argv) {
       const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       const double cd = -8.0 * f[0] / (uxInlet * uxInlet * M_PI * Dfine * Dfine);
       std::printf("iteration %d: C_D %.4f, %.2f GLUPS\n", it, cd, updates * (double)iterationChunk / s / 1e9);
+      // a diverged run stops here instead of writing NaN drag (the reference has no guard)
+      for (size_t l = 0; l < grids.size(); ++l) {
+        const Health h = checkHealth(*grids[l]);
+        if (!h.ok()) {
+          std::printf("level %zu: %lld cells with non-finite populations or rho <= 0; stopping\n", l,
+                      (long long)h.badCells);
+          return 2;
+        }
+      }
       t0 = std::chrono::steady_clock::now();
     }
   }
