@@ -1012,7 +1012,8 @@ int psl_diad_create_level(const psl_diad_levels* levels, int level, int storage,
   k_flags_from_bb<<<blocks_for(n), TPB>>>(L.bb.p, n, d->flags);
   if (hipGetLastError() != hipSuccess) return fail("tables");
   // the first seven neighbour columns (+i +j +k +ij +ik +jk +ijk)
-  if (hipMemcpy(d->nbr, L.nbr.p, 7ull * (size_t)n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess)
+  if (hipMemcpy(d->nbr, L.nbr.p, 7ull * (size_t)n * sizeof(int), hipMemcpyDeviceToDevice) != hipSuccess ||
+      diad_build_adjacency(d) != PSL_OK)
     return fail("neighbours");
   const int64_t nc = L.cnt[A_C2F_READ], nf = L.cnt[A_F2C_WRITE];
   if (nc) {

@@ -31,6 +31,7 @@ struct EsoParams {
   const uint8_t* flags;
   const int* ijk;  // 3 x n
   const int* nbr;  // 7 x n
+  const uint8_t* adj;  // per cell: which of +i +ij +ik +ijk are the next cell (see eso_slots_adj)
   const float4* bc;  // 4 given-value slots (device)
   // interface records the local update writes (null: none): child records,
   // read by the transfer to the coarser level, and source records, read by
@@ -126,6 +127,43 @@ __device__ __forceinline__ void eso_slots(const EsoParams& p, uint32_t cell, uin
   for (int t = 0; t < 7; ++t) slot[t + 1] = (uint32_t)p.nbr[(size_t)t * p.n + cell];
 }
 
+// The eight slots of a cell for the local update from three neighbour columns
+// instead of seven.  Cells are sorted by (k, j, i), so the +i, +ij, +ik and
+// +ijk neighbours are nearly always the cell after the cell itself, +j, +k and
+// +jk; adjacency bit t says so.  Where a bit is clear the neighbour is read
+// from its column; the lanes whose bits are set read the column's first entry
+// instead (one shared line), so no lane waits on a branch.
+__device__ __forceinline__ void eso_slots_adj(const EsoParams& p, uint32_t cell, uint32_t (&slot)[8]) {
+  const size_t n = p.n;
+  const uint32_t m = p.adj[cell];
+  const uint32_t j = (uint32_t)p.nbr[n + cell], k = (uint32_t)p.nbr[2 * n + cell], jk = (uint32_t)p.nbr[5 * n + cell];
+  const uint32_t xi = (uint32_t)p.nbr[(m & 1u) ? 0 : cell];
+  const uint32_t xij = (uint32_t)p.nbr[(m & 2u) ? 3 * n : 3 * n + cell];
+  const uint32_t xik = (uint32_t)p.nbr[(m & 4u) ? 4 * n : 4 * n + cell];
+  const uint32_t xijk = (uint32_t)p.nbr[(m & 8u) ? 6 * n : 6 * n + cell];
+  slot[0] = cell;
+  slot[1] = (m & 1u) ? cell + 1 : xi;
+  slot[2] = j;
+  slot[3] = k;
+  slot[4] = (m & 2u) ? j + 1 : xij;
+  slot[5] = (m & 4u) ? k + 1 : xik;
+  slot[6] = jk;
+  slot[7] = (m & 8u) ? jk + 1 : xijk;
+}
+
+__global__ void k_eso_adjacency(const int* __restrict__ nbr, uint32_t n, uint8_t* __restrict__ adj) {
+  const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+  if (c >= n) return;
+  const size_t N = n;
+  const int j = nbr[N + c], k = nbr[2 * N + c], jk = nbr[5 * N + c];
+  uint8_t m = 0;
+  if (nbr[c] == (int)c + 1) m |= 1;
+  if (nbr[3 * N + c] == j + 1) m |= 2;
+  if (nbr[4 * N + c] == k + 1) m |= 4;
+  if (nbr[6 * N + c] == jk + 1) m |= 8;
+  adj[c] = m;
+}
+
 template <class St, bool FLIP>
 __global__ __launch_bounds__(256) void k_eso_update(typename St::T* __restrict__ f, EsoParams p) {
   typedef typename St::T T;
@@ -136,7 +174,7 @@ __global__ __launch_bounds__(256) void k_eso_update(typename St::T* __restrict__
   if (kind == KIND_BOUNCEBACK) return;  // reflected implicitly by the maps
   if (kind == KIND_GHOST || kind == KIND_COVERED) kind = KIND_GIVEN_NONE;
   uint32_t slot[8];
-  eso_slots<St>(p, cell, slot);
+  eso_slots_adj(p, cell, slot);
   float v[27];
 #pragma unroll
   for (int q = 0; q < 27; ++q) {
@@ -780,6 +818,7 @@ static EsoParams eso_params(const psl_diad* d, bool child_records = true) {
   p.flags = d->flags;
   p.ijk = d->ijk;
   p.nbr = d->nbr;
+  p.adj = d->adj;
   p.bc = d->bc_dev;
   p.rec = child_records && d->child.n ? d->child.rec : nullptr;
   p.rec_mask = d->child.mask;
@@ -788,6 +827,15 @@ static EsoParams eso_params(const psl_diad* d, bool child_records = true) {
   p.srec_mask = d->source.mask;
   p.srec_base = d->source.base;
   return p;
+}
+
+int psl::diad_build_adjacency(psl_diad* d) {
+  if (d->adj) (void)hipFree(d->adj);
+  d->adj = nullptr;
+  PSL_HIP_TRY(hipMalloc(&d->adj, (size_t)d->n));
+  k_eso_adjacency<<<(unsigned)((d->n + 255) / 256), 256>>>(d->nbr, (uint32_t)d->n, d->adj);
+  PSL_LAUNCH_CHECK();
+  return PSL_OK;
 }
 
 // Record versions are unique across all grids of the process, so a key can
@@ -1108,12 +1156,14 @@ int psl_diad_create(const psl_info* info, int storage, int64_t n, const int* ijk
       hipMemset(d->bc_dev, 0, 4 * sizeof(float4)) != hipSuccess || hipMalloc(&d->ijk, 3ull * n * sizeof(int)) != hipSuccess ||
       hipMalloc(&d->nbr, 7ull * n * sizeof(int)) != hipSuccess ||
       hipMemcpy(d->ijk, ijk3, 3ull * n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(d->nbr, nbr7, 7ull * n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(d->nbr, nbr7, 7ull * n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      diad_build_adjacency(d) != PSL_OK) {
     set_error("DIAD allocation / upload failed");
     if (d->f) (void)hipFree(d->f);
     if (d->bc_dev) (void)hipFree(d->bc_dev);
     if (d->ijk) (void)hipFree(d->ijk);
     if (d->nbr) (void)hipFree(d->nbr);
+    if (d->adj) (void)hipFree(d->adj);
     delete d;
     return PSL_E_NOMEM;
   }
@@ -1127,6 +1177,7 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->f) (void)hipFree(d->f);
   if (d->ijk) (void)hipFree(d->ijk);
   if (d->nbr) (void)hipFree(d->nbr);
+  if (d->adj) (void)hipFree(d->adj);
   if (d->flags) (void)hipFree(d->flags);
   if (d->bc_dev) (void)hipFree(d->bc_dev);
   for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read, d->outlet, d->kmin, d->moving, d->forced})

@@ -150,6 +150,9 @@ struct psl_diad {
   void* f = nullptr;
   int* ijk = nullptr;
   int* nbr = nullptr;
+  // per cell: bit 0 set when the +i neighbour is the next cell in memory,
+  // bits 1, 2, 3 when +ij, +ik, +ijk follow +j, +k, +jk (psl_esotwist.hip)
+  uint8_t* adj = nullptr;
   uint8_t* flags = nullptr;
   float4* bc_dev = nullptr;
   float cs = 0.f;
@@ -239,6 +242,9 @@ constexpr int32_t STATE_CLOSURE_Y = 2;  // n FP16 held network outputs follow
 void state_header_init(StateHeader& h, uint32_t kind, int storage, int64_t n, const psl_info& info);
 int state_header_check(const StateHeader& h, uint32_t kind, int storage, int64_t n, const psl_info& info,
                        int64_t bytes, int64_t expect_bytes);
+// implemented in psl_esotwist.hip: the adjacency bits of a DIAD grid from its
+// neighbour table (d->nbr set), allocated into d->adj
+int diad_build_adjacency(psl_diad* d);
 // implemented in psl_esotwist.hip: a version number for a DIAD grid's
 // interface records, unique in the process
 uint64_t diad_next_version();
