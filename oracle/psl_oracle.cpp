@@ -569,7 +569,11 @@ inline long long shifted(long long cell, int shift, long long N) {
   return s;
 }
 
-// applyStreaming(GridStruct&): include/applyStreaming.h:11-31
+// applyStreaming(GridStruct&): include/applyStreaming.h:11-31.  The reference
+// wraps the new shift at most twice, which is the reduction modulo N whenever
+// the step's offset |cx + nx cy + nx ny cz| <= 2N (every grid with ny nz >= 2);
+// on thinner grids (ny = nz = 1, or a single cell) it leaves the shift outside
+// [0, N) and would index outside fArray, so here the shift is reduced modulo N.
 void stream_shift(int* shift, int nx, int ny, int nz) {
   const long long N = (long long)nx * ny * nz;
   for (int q = 0; q < 27; ++q) {
@@ -577,10 +581,8 @@ void stream_shift(int* shift, int nx, int ny, int nz) {
     s -= CX[q];
     s -= (long long)nx * CY[q];
     s -= (long long)ny * nx * CZ[q];
+    s %= N;
     if (s < 0) s += N;
-    if (s < 0) s += N;
-    if (s >= N) s -= N;
-    if (s >= N) s -= N;
     shift[q] = (int)s;
   }
 }

@@ -46,10 +46,9 @@ __global__ void k_batch_advance(StepParams* __restrict__ params, const void* con
   const int64_t N = P.n;
   int64_t s = shift[t];
   s -= delta[t];
+  // reduced modulo N (also on grids thinner than two wraps allow)
+  s %= N;
   if (s < 0) s += N;
-  if (s < 0) s += N;
-  if (s >= N) s -= N;
-  if (s >= N) s -= N;
   shift[t] = (int)s;
   T* row = (T*)fbase[d] + (size_t)q * P.n + s;
   P.lo[q] = row;

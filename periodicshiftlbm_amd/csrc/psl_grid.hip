@@ -873,11 +873,11 @@ int psl_apply_streaming(psl_grid* g) {
     s -= CX[q];
     s -= nx * CY[q];
     s -= nxy * CZ[q];
-    // two wraps allow cellCountZ = 1 (include/applyStreaming.h:24-27)
+    // reduced modulo N: the reference's two wraps (include/applyStreaming.h:24-27)
+    // are the same on every grid with cellCountY * cellCountZ >= 2, and leave
+    // the shift outside [0, N) on thinner ones
+    s %= N;
     if (s < 0) s += N;
-    if (s < 0) s += N;
-    if (s >= N) s -= N;
-    if (s >= N) s -= N;
     g->shift[q] = (int)s;
   }
   return PSL_OK;

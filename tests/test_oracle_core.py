@@ -585,3 +585,24 @@ def test_ray_hit_predicate_kat():
     t1 = (-3, -1, 3, 1, -3, 5)
     t2 = (-3, -1, 3, -5, 3, 1)
     assert O.ray_hit(*t1) != O.ray_hit(*t2)
+
+
+def test_stream_shift_closed_form_on_thin_grids():
+    """After t streaming steps population q sits shifted by -t d_q modulo N,
+    d_q = cx + nx cy + nx ny cz (include/applyStreaming.h:11-31 applied t
+    times).  The reference wraps at most twice per step, which equals this
+    whenever |d_q| <= 2N; on grids one cell thick in y and z (and a single
+    cell) it would leave [0, N), so the shift is reduced modulo N there."""
+    import numpy as np
+
+    import oracle as O
+    cx, cy, cz = (np.asarray(c, np.int64) for c in O.velocity_set())
+    for shape in [(1, 1, 1), (7, 1, 1), (2, 1, 1), (3, 2, 1), (5, 3, 2), (64, 64, 1), (1, 1, 64)]:
+        nx, ny, nz = shape
+        n = nx * ny * nz
+        d = cx + nx * cy + nx * ny * cz
+        s = np.zeros(27, np.int32)
+        for t in range(1, 40):
+            s = O.stream_shift(s, *shape)
+            assert np.all((s >= 0) & (s < n)), shape
+            assert np.array_equal(s, np.mod(-t * d, n)), (shape, t)
