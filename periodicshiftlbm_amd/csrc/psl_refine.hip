@@ -328,7 +328,8 @@ template <class St> struct CommFn {
   static int run(psl_grid* c, psl_grid* f, bool full) { return communicate<St>(c, f, full); }
 };
 
-static int comm_dispatch(psl_grid* c, psl_grid* f, bool full) {
+// Refuses a coarse/fine pair the transfers cannot serve; launches nothing.
+static int comm_check(const psl_grid* c, const psl_grid* f) {
   if (!c || !f) {
     set_error("null grid");
     return PSL_E_ARG;
@@ -349,6 +350,21 @@ static int comm_dispatch(psl_grid* c, psl_grid* f, bool full) {
     set_error("sub-grid outside the coarse grid");
     return PSL_E_ARG;
   }
+  // the fine-to-coarse ring is one coarse cell inside each face of the
+  // sub-grid, so along every axis the coarse grid extends in the sub-grid must
+  // span at least two coarse cells
+  if ((C.cellCountX > 1 && C.iSubgridEnd - C.iSubgridStart < 2) ||
+      (C.cellCountY > 1 && C.jSubgridEnd - C.jSubgridStart < 2) ||
+      (C.cellCountZ > 1 && C.kSubgridEnd - C.kSubgridStart < 2)) {
+    set_error("sub-grid must span at least two coarse cells along every axis of the coarse grid");
+    return PSL_E_ARG;
+  }
+  return PSL_OK;
+}
+
+static int comm_dispatch(psl_grid* c, psl_grid* f, bool full) {
+  const int rc = comm_check(c, f);
+  if (rc) return rc;
   if (c->stream != f->stream) {
     // order the fine grid's pending work before the transfer on the coarse stream
     hipEvent_t ev;
@@ -395,6 +411,11 @@ int psl_fill_coarse_from_fine(psl_grid* coarse, psl_grid* fine) { return comm_di
 int psl_update_grid(psl_grid* const* levels, int nlevels, int nsteps) {
   PSL_CHECK_ARG(levels && nlevels > 0 && nsteps >= 0, "bad level list");
   for (int l = 0; l < nlevels; ++l) PSL_CHECK_ARG(levels[l], "null level");
+  // every coarse/fine pair is checked before the first launch
+  for (int l = 0; l + 1 < nlevels; ++l) {
+    const int rc = comm_check(levels[l], levels[l + 1]);
+    if (rc) return rc;
+  }
   for (int s = 0; s < nsteps; ++s) {
     const int rc = update_level(levels, nlevels, 0);
     if (rc) return rc;

@@ -61,6 +61,28 @@ def _two_level(P, O, nu=0.01, cs=0.0):
     return gc, gf, oc, of
 
 
+@pytest.mark.parametrize("sub", [(6, 5, 4, 7, 15, 13), (6, 5, 4, 18, 6, 13), (6, 5, 4, 18, 15, 5)])
+def test_sub_grid_one_coarse_cell_wide_is_refused(psl, sub):
+    """A sub-grid one coarse cell wide along an axis the coarse grid extends in
+    would put its fine-to-coarse ring outside the sub-grid (fine cells past the
+    fine box); the transfers, the coarse fill and the recursion refuse it
+    before launching a transfer."""
+    P = psl
+    ci = P.InfoStruct(cellCountX=24, cellCountY=20, cellCountZ=18, nu=0.01, res=1.0,
+                      iSubgridStart=sub[0], jSubgridStart=sub[1], kSubgridStart=sub[2],
+                      iSubgridEnd=sub[3], jSubgridEnd=sub[4], kSubgridEnd=sub[5])
+    fi = P.InfoStruct(gridID=1, cellCountX=2 * (sub[3] - sub[0]), cellCountY=2 * (sub[4] - sub[1]),
+                      cellCountZ=2 * (sub[5] - sub[2]), nu=0.02, res=0.5)
+    gc, gf = P.GridStruct(ci, "f32"), P.GridStruct(fi, "f32")
+    print(f"[thin sub-grid] {sub}", flush=True)
+    for call in (lambda: P.applyCoarseFineGridCommunication(gc, gf), lambda: P.fillCoarseGridFromFine(gc, gf),
+                 lambda: P.updateGrid([gc, gf], 1)):
+        with pytest.raises(RuntimeError, match="two coarse cells"):
+            call()
+    gc.close()
+    gf.close()
+
+
 def test_coarse_fine_communication_matches_oracle(psl, oracle_mod):
     P, O = psl, oracle_mod
     gc, gf, oc, of = _two_level(P, O)
