@@ -58,8 +58,27 @@ __global__ __launch_bounds__(256) void k_ps_boundary(StepParams p, BoundaryParam
 
 // The listed cells from the records k_ps_bulk<..., STAGE = true> wrote.
 template <class St>
-__global__ __launch_bounds__(64) void k_ps_boundary_staged(StepParams p, BoundaryParams b, const float* stage) {
-  boundary_update_staged<St>(p, b, stage, blockIdx.x * 64u + threadIdx.x);
+__global__ __launch_bounds__(64) void k_ps_boundary_staged(StagedParams p) {
+  typedef typename St::T T;
+  __shared__ float4 given[4];
+  const uint32_t t = blockIdx.x * 64u + threadIdx.x;
+  const bool live = t < p.nlist;
+  // the record (issued first) and the given-value table are in flight together
+  float v[27];
+  uint32_t cell;
+  int fl;
+  load_staged<St>(p.stage, live ? t : p.nlist - 1, v, cell, fl);
+  if (threadIdx.x < 4) given[threadIdx.x] = p.bc[threadIdx.x];
+  __syncthreads();
+  if (!live) return;
+  const float cs = p.cs_cell ? ld<float, false>(p.cs_cell + cell) : p.cs;
+  T* f = (T*)p.f;
+  boundary_core<St>(cell, fl, v, given[(fl >> 6) & 3], p.nx, p.ny, p.nz, Relax{p.omega, p.tau, p.cs}, cs,
+                    [&](int q, T w) {
+                      uint32_t slot = cell + (uint32_t)p.s[q];
+                      if (slot >= p.n) slot -= p.n;
+                      st<T, false>(f + (size_t)q * p.n + slot, w);
+                    });
 }
 
 // (held at the plain kernel's 7 waves per SIMD: without the bound the staging
@@ -618,7 +637,23 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
   }
   if (b.n && !fused && !dominant_only) {
     if (net || held) k_ps_boundary<St><<<blocks_for(b.n), 256, 0, g->stream>>>(p, b);
-    else k_ps_boundary_staged<St><<<(b.n + 63) / 64, 64, 0, g->stream>>>(p, b, g->stage_buf);
+    else {
+      StagedParams sp;
+      sp.f = g->f;
+      sp.stage = g->stage_buf;
+      sp.cs_cell = g->cs_cell;
+      sp.bc = g->bc_dev;
+      sp.n = p.n;
+      sp.nlist = b.n;
+      for (int q = 0; q < 27; ++q) sp.s[q] = g->shift[q];
+      sp.omega = p.omega;
+      sp.tau = p.tau;
+      sp.cs = p.cs;
+      sp.nx = b.nx;
+      sp.ny = b.ny;
+      sp.nz = b.nz;
+      k_ps_boundary_staged<St><<<(b.n + 63) / 64, 64, 0, g->stream>>>(sp);
+    }
     PSL_LAUNCH_CHECK();
   }
   return PSL_OK;

@@ -256,42 +256,73 @@ struct BoundaryParams {
 // storages, to which the boundary kernel adds w exactly as when it loads the
 // slots itself, so both paths compute identical results).  The
 // boundary kernel then reads the records instead of 27 strided slots, whose
-// lines it would otherwise fetch again for 2-4 useful bytes each.  base[w] is
-// the number of listed cells below cell 64 w.
-constexpr int STAGE_STRIDE = 28;
+// lines it would otherwise fetch again for 2-4 useful bytes each.  Words 27
+// and 28 of a record hold the cell index and its flag byte (integer bits) and
+// the record is padded to 128 bytes, one cache line: the boundary kernel's
+// only per-cell load is its record.  base[w] is the number of listed cells
+// below cell 64 w.
+constexpr int STAGE_STRIDE = 32;
 struct Stage {
   float* buf;
   const uint32_t* base;
 };
 
+// Arguments of the staged boundary kernel, kept compact (the slot of
+// population q of cell c is (c + s[q]) mod n in row q of f) so that its few
+// scalar loads are issued together at the start; with the 27 row pointers and
+// thresholds of StepParams they were fetched one cache line after another.
+struct StagedParams {
+  void* f;
+  const float* stage;
+  const float* cs_cell;
+  const float4* bc;
+  uint32_t n;
+  uint32_t nlist;
+  int s[27];
+  float omega, tau, cs;
+  int nx, ny, nz;
+};
+
 // Restore + MBBC + collision of listed cell `cell` (flags fl) whose full f is
-// in v, and the in-place store (include/applyLocalCellUpdate.h:46-71).
+// in v, and the in-place store through store(q, word) (include/
+// applyLocalCellUpdate.h:46-71).
+template <class St, class Store>
+__device__ __forceinline__ void boundary_core(uint32_t cell, int fl, float (&v)[27], float4 given, int nx, int ny,
+                                              int nz, const Relax& rx, float cs, Store&& store) {
+  const int kind = fl & 7;
+  const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
+  const int k = (int)(cell / nxy);
+  const uint32_t rem = cell - (uint32_t)k * nxy;
+  const int j = (int)(rem / (uint32_t)nx);
+  const int i = (int)(rem - (uint32_t)j * (uint32_t)nx);
+  const int code = outer_normal_code(i, j, k, nx, ny, nz, fl);
+  boundary_cell(v, kind, code, given.x, given.y, given.z, given.w);
+  collide<false>(v, rx, cs);
+#pragma unroll
+  for (int q = 0; q < 27; ++q) store(q, St::store(St::W_OFF ? v[q] - weight(q) : v[q]));
+}
+
+template <class St>
+__device__ __forceinline__ void boundary_finish_given(const StepParams& p, const BoundaryParams& b, uint32_t cell,
+                                                      int fl, float (&v)[27], float4 given) {
+  typedef typename St::T T;
+  const float cs = p.cs_cell ? ld<float, false>(p.cs_cell + cell) : p.cs;
+  boundary_core<St>(cell, fl, v, given, b.nx, b.ny, b.nz, Relax{p.omega, p.tau, p.cs}, cs, [&](int q, T w) {
+    T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
+    st<T, false>(base + cell, w);
+  });
+}
+
+// Given (rho, u) of the cell's slot: one indexed load from the device table
+// (a select chain over the four slots was miscompiled for divergent slots, so
+// no branch or select is used here).
 template <class St>
 __device__ __forceinline__ void boundary_finish(const StepParams& p, const BoundaryParams& b, uint32_t cell, int fl,
                                                 float (&v)[27]) {
-  typedef typename St::T T;
-  const int kind = fl & 7;
-  const uint32_t nxy = (uint32_t)b.nx * (uint32_t)b.ny;
-  const int k = (int)(cell / nxy);
-  const uint32_t rem = cell - (uint32_t)k * nxy;
-  const int j = (int)(rem / (uint32_t)b.nx);
-  const int i = (int)(rem - (uint32_t)j * (uint32_t)b.nx);
-  const int code = outer_normal_code(i, j, k, b.nx, b.ny, b.nz, fl);
-  // given (rho, u) of the cell's slot: one indexed load from the device table
-  // (a select chain over the four slots was miscompiled for divergent slots,
-  // so no branch or select is used here)
   const float* g = (const float*)(b.bc + ((fl >> 6) & 3));
-  boundary_cell(v, kind, code, ld<float, false>(g), ld<float, false>(g + 1), ld<float, false>(g + 2),
-                ld<float, false>(g + 3));
-  Relax rx{p.omega, p.tau, p.cs};
-  const float cs = p.cs_cell ? ld<float, false>(p.cs_cell + cell) : p.cs;
-  collide<false>(v, rx, cs);
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    T* base = (T*)(cell >= p.thr[q] ? p.hi[q] : p.lo[q]);
-    const float o = St::W_OFF ? v[q] - weight(q) : v[q];
-    st<T, false>(base + cell, St::store(o));
-  }
+  boundary_finish_given<St>(p, b, cell, fl, v,
+                            make_float4(ld<float, false>(g), ld<float, false>(g + 1), ld<float, false>(g + 2),
+                                        ld<float, false>(g + 3)));
 }
 
 // Listed cell t, loaded from its 27 slots.
@@ -311,27 +342,26 @@ __device__ __forceinline__ void boundary_update(const StepParams& p, const Bound
   boundary_finish<St>(p, b, cell, fl, v);
 }
 
-// Listed cell t, loaded from its staged record.
+// Listed cell t, loaded from its staged record (populations, cell index and
+// flag byte in one line).  The caller's workgroup has the given-value table in
+// `given` (LDS), so the record is the cell's only global load.
 template <class St>
-__device__ __forceinline__ void boundary_update_staged(const StepParams& p, const BoundaryParams& b,
-                                                       const float* stage, uint32_t t) {
-  if (t >= b.n) return;
-  const uint32_t cell = ld<uint32_t, false>(b.list + t);
-  const int fl = ld<uint8_t, false>(p.flags + cell);
+__device__ __forceinline__ void load_staged(const float* stage, uint32_t t, float (&v)[27], uint32_t& cell, int& fl) {
   typedef float f4 __attribute__((ext_vector_type(4)));
   const PSL_GLOBAL f4* r = (const PSL_GLOBAL f4*)(stage + (size_t)t * STAGE_STRIDE);
-  float v[27];
+  f4 x[STAGE_STRIDE / 4];
 #pragma unroll
-  for (int i = 0; i < STAGE_STRIDE / 4; ++i) {
-    const f4 x = r[i];
+  for (int i = 0; i < STAGE_STRIDE / 4; ++i) x[i] = r[i];
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (4 * i + c < 27) v[4 * i + c] = x[c];
-  }
-#pragma unroll
-  for (int q = 0; q < 27; ++q)
+  for (int q = 0; q < 27; ++q) {
+    v[q] = x[q >> 2][q & 3];
     if (St::W_OFF) v[q] += weight(q);
-  boundary_finish<St>(p, b, cell, fl, v);
+  }
+  // copied to scalars before the casts (a cast of a vector element in place
+  // reads the wrong element)
+  const float wc = x[6][3], wf = x[7][0];
+  cell = __builtin_bit_cast(uint32_t, wc);
+  fl = (int)__builtin_bit_cast(uint32_t, wf);
 }
 
 // Fluid and bounceback cells of one block in cell order; with STAGE the
@@ -348,8 +378,9 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
   // all 64 lanes take part in the ballot, so every population is tested
   const bool straddle = wave_straddles(p, w_first);
   if (cell >= p.n) return;
-  int kind = KIND_FLUID;
-  if (FLAGS) kind = ld<uint8_t, false>(p.flags + cell) & 7;
+  int fl = 0;
+  if (FLAGS) fl = ld<uint8_t, false>(p.flags + cell);
+  const int kind = FLAGS ? fl & 7 : KIND_FLUID;
   if (kind > (STAGE ? KIND_GIVEN_RHOU : KIND_BOUNCEBACK)) return;
   // the stored network output is loaded ahead of the populations, so that it
   // is in flight with them rather than after them
@@ -366,14 +397,18 @@ __device__ __forceinline__ void bulk_update(const StepParams& p, uint32_t block,
         words_to_float<St>(r, v);
         const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
         const uint32_t t = ld<uint32_t, false>(sg.base + (w_first >> 6)) + rank;
-        // seven 16-byte stores per record (one lane of the wave is active)
+        // eight 16-byte stores per record (the listed lanes are active)
         typedef float f4 __attribute__((ext_vector_type(4)));
         PSL_GLOBAL f4* rec = (PSL_GLOBAL f4*)(sg.buf + (size_t)t * STAGE_STRIDE);
+        const float wc = __builtin_bit_cast(float, cell), wf = __builtin_bit_cast(float, (uint32_t)fl);
 #pragma unroll
         for (int i = 0; i < STAGE_STRIDE / 4; ++i) {
           f4 x;
 #pragma unroll
-          for (int c = 0; c < 4; ++c) x[c] = 4 * i + c < 27 ? v[4 * i + c] : 0.f;
+          for (int c = 0; c < 4; ++c) {
+            const int q = 4 * i + c;
+            x[c] = q < 27 ? v[q] : q == 27 ? wc : q == 28 ? wf : 0.f;
+          }
           rec[i] = x;
         }
         return;
