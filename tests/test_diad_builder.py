@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from tests import diad_dense
+from tests import diad_dense, random_diad
 from tests.stl_shapes import tilted_box
 
 import periodicshiftlbm_amd as P
@@ -103,6 +103,26 @@ def test_oracle_matches_dense_restatement(built, name):
 @pytest.mark.parametrize("name", CASES)
 def test_builder_properties(built, name):
     lv, (base, levels, span, stls, r) = built[name]
+    _check_properties(lv, levels)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_cases_oracle_matches_dense_and_properties(seed):
+    """Seeded random sphere walls, refinement and outlet rules over two or
+    three levels (tests/random_diad.py): the oracle equals the dense
+    restatement on every array and has the construction's properties."""
+    base, levels, span, stls, r = random_diad.case(seed)
+    lv = O.diad_build(base, levels, span, 1e-3, stls, r)
+    dense = diad_dense.build(O, base, levels, span, 1e-3, stls, r)
+    print(f"  random {seed}: levels {[d['n'] for d in lv]}")
+    for lev, (a, b) in enumerate(zip(lv, dense)):
+        assert a["n"] == b["n"] > 0
+        for key in O.DIAD_ARRAYS:
+            assert np.array_equal(a[key], b[key]), (lev, key)
+    _check_properties(lv, levels)
+
+
+def _check_properties(lv, levels):
     for lev, d in enumerate(lv):
         i, j, k = d["ijk"].astype(np.int64)
         key = (k * d["ny"] + j) * d["nx"] + i

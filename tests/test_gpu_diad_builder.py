@@ -12,6 +12,7 @@ single level with STL plus index-box walls.
 import numpy as np
 import pytest
 
+from tests import random_diad
 from tests.test_diad_builder import CASES, _case, _rules_sphere
 
 pytestmark = pytest.mark.gpu
@@ -52,6 +53,15 @@ def _compare(P, O, name, base, levels, span, stls, rules):
 def test_builder_matches_oracle(psl, oracle_mod, name):
     base, levels, span, stls, rules = _case(name)
     _compare(psl, oracle_mod, name, base, levels, span, stls, rules)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_builder_random_cases_match_oracle(psl, oracle_mod, seed):
+    """Seeded random sphere walls, refinement and outlet rules over two or
+    three levels (tests/random_diad.py), pinned on the CPU against the dense
+    restatement."""
+    base, levels, span, stls, rules = random_diad.case(seed)
+    _compare(psl, oracle_mod, f"random {seed}", base, levels, span, stls, rules)
 
 
 def test_builder_four_levels(psl, oracle_mod):
