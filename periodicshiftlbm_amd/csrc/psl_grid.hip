@@ -122,26 +122,25 @@ __constant__ int c_pos_y[9] = {6, 12, 13, 15, 17, 19, 22, 24, 26};
 __constant__ int c_neg_y[9] = {5, 11, 14, 16, 18, 20, 21, 23, 25};
 
 template <class T>
-__global__ __launch_bounds__(256) void k_periodic(T* __restrict__ f, Shift27 sh, uint32_t n, int nx, int ny, int nz,
-                                                  int axis) {
-  // one thread per (surface cell, copied population): 18 independent copies;
-  // the surface coordinate runs fastest, so for the y copies (rows along x) a
-  // wave moves one contiguous run of each population
-  const int na = axis == 0 ? ny : nx;
-  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-  if (t >= (uint64_t)na * (uint64_t)nz * 18u) return;
-  const int a = (int)(t % (uint64_t)na);
-  const uint64_t rest = t / (uint64_t)na;
-  const int c = (int)(rest % 18u);
-  const int k = (int)(rest / 18u);
-  const uint32_t plane = (uint32_t)k * (uint32_t)nx * (uint32_t)ny;
+__global__ __launch_bounds__(256) void k_periodic(T* __restrict__ f, Shift27 sh, uint32_t n, int nx, int ny,
+                                                  uint32_t row_blocks, int axis) {
+  // one thread per (surface cell, copied population): blockIdx.y is the
+  // copied population (18), blockIdx.x the 256-cell run of a surface row and
+  // the row's k; the surface coordinate runs fastest, so for the y copies
+  // (rows along x) a wave moves one contiguous run of each population
+  const uint32_t na = axis == 0 ? (uint32_t)ny : (uint32_t)nx;
+  const uint32_t k = blockIdx.x / row_blocks;  // workgroup-uniform
+  const uint32_t a = (blockIdx.x - k * row_blocks) * 256u + threadIdx.x;
+  if (a >= na) return;
+  const int c = (int)blockIdx.y;
+  const uint32_t plane = k * (uint32_t)nx * (uint32_t)ny;
   uint32_t left, right;
   if (axis == 0) {
-    left = plane + (uint32_t)a * nx;
-    right = left + nx - 1;
+    left = plane + a * (uint32_t)nx;
+    right = left + (uint32_t)nx - 1u;
   } else {
     left = plane + a;
-    right = plane + (uint32_t)(ny - 1) * nx + a;
+    right = plane + (uint32_t)(ny - 1) * (uint32_t)nx + a;
   }
   const bool pos = c < 9;
   const int q = pos ? (axis == 0 ? c_pos_x[c] : c_pos_y[c]) : (axis == 0 ? c_neg_x[c - 9] : c_neg_y[c - 9]);
@@ -717,13 +716,15 @@ template <class St> struct MassFn {
 static int periodic_copy(psl_grid* g, int axis) {
   const psl_info& I = g->info;
   const int na = axis == 0 ? I.cellCountY : I.cellCountX;
-  const uint32_t grid = blocks_for((uint64_t)na * I.cellCountZ * 18u);
+  const uint32_t row_blocks = blocks_for((uint64_t)na);
+  // row_blocks * nz <= cellCount / 256 + nz < 2^31 (cells are below 2^31)
+  const dim3 grid(row_blocks * (uint32_t)I.cellCountZ, 18);
   if (g->storage == PSL_F32)
     k_periodic<float><<<grid, 256, 0, g->stream>>>((float*)g->f, shift_of(g), ncells(g), I.cellCountX, I.cellCountY,
-                                                   I.cellCountZ, axis);
+                                                   row_blocks, axis);
   else
     k_periodic<uint16_t><<<grid, 256, 0, g->stream>>>((uint16_t*)g->f, shift_of(g), ncells(g), I.cellCountX,
-                                                      I.cellCountY, I.cellCountZ, axis);
+                                                      I.cellCountY, row_blocks, axis);
   PSL_LAUNCH_CHECK();
   return PSL_OK;
 }
