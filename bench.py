@@ -239,12 +239,14 @@ def large_box_leg(P, cases, n: int, steps: int):
         g = P.GridStruct(info, storage)
         g.set_periodic_copies(x=True, y=True)
         P.fill_equilibrium(g, rho, ux, uy, uz)
+        g.autotune(5)  # variant and occupancy cap (the decaying flow advances a few steps)
         el = time_steps(P, g, steps, 10)
         ms, _ = g.time_update_kernel(50)
         gbs = BYTES_PER_UPDATE[storage] * g.n / (ms * 1e-3) / 1e9
         out[storage] = {"cells": g.n, "f_bytes": g.n * 27 * (4 if storage == "f32" else 2),
                         "mlups": g.n * steps / el / 1e6, "bulk_kernel_ms": ms, "achieved_GBs": gbs,
-                        "roofline_frac": gbs / HBM_PEAK_GBS, "tuning": g.tuning}
+                        "roofline_frac": gbs / HBM_PEAK_GBS, "tuning": g.tuning,
+                        "occupancy": g.occupancy}
         g.close()
         out[storage]["pattern_limit_GBs"] = P.probe_ps_pattern(nx, ny, nz, storage)
         out[storage]["frac_of_pattern_limit"] = gbs / out[storage]["pattern_limit_GBs"]
@@ -516,7 +518,8 @@ def main():
             best, tt = g.autotune(10)
             tune_times = {int(v): float(t) for v, t in enumerate(tt)}
             P.fill_equilibrium(g)  # restart from rest after tuning
-            log(f"  autotune: variant {best} ({', '.join(f'{v}:{t * 1e3:.1f}us' for v, t in tune_times.items())})")
+            log(f"  autotune: variant {best}, workgroups per CU {g.occupancy or 'uncapped'} "
+                f"({', '.join(f'{v}:{t * 1e3:.1f}us' for v, t in tune_times.items())})")
         el = time_steps(P, g, args.steps, args.warmup)
         mlups = ncell * args.steps / el / 1e6
         bulk, nbc, skipped = g.census()
@@ -539,7 +542,7 @@ def main():
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": measured_traffic(kernel, storage)},
             "drag_force_x_after_run": fx,
-            "tuning": g.tuning, "autotune_ms": tune_times,
+            "tuning": g.tuning, "occupancy": g.occupancy, "autotune_ms": tune_times,
             "working_set_bytes": ncell * (27 * (4 if storage == "f32" else 2) + 1),
         }
         log(f"  {mlups:.1f} MLUPS, {kernel} {avg_ms * 1e3:.1f} us ({achieved:.0f} GB/s, "
@@ -572,7 +575,7 @@ def main():
                          "achieved_GBs": v["roofline"]["achieved"], "kernel": v["kernel"],
                          "pattern_limit_GBs": v["pattern_limit_GBs"], "frac_of_pattern_limit": v["frac_of_pattern_limit"],
                          "kernel_ms": v["bulk_kernel_ms"],
-                         "tuning": v["tuning"], "autotune_ms": v["autotune_ms"],
+                         "tuning": v["tuning"], "occupancy": v["occupancy"], "autotune_ms": v["autotune_ms"],
                          "working_set_bytes": v["working_set_bytes"],
                          "cache_resident": v["working_set_bytes"] < INFINITY_CACHE_BYTES}
                      for k, v in results.items()},

@@ -198,9 +198,22 @@ enum {
 };
 int psl_grid_set_tuning(psl_grid* grid, int tuning);
 int psl_grid_get_tuning(const psl_grid* grid, int* tuning);
-/* Autotuner: times all PSL_TUNE_VARIANTS variants on this grid (`steps` local
- * updates each after one warm-up; the state advances) and keeps the fastest;
- * best and times_ms[PSL_TUNE_VARIANTS] (mean ms per local update) are optional. */
+/* Cap on the workgroups of the bulk update kernel that share a compute unit:
+ * 0 (no cap: the kernel's register budget allows 7) or 2-6.  The launch then
+ * reserves that share of the CU's LDS (the kernel uses none), so fewer
+ * workgroups run per CU at once and the HBM sees fewer row streams in flight.
+ * Results are the same bits at every cap; only the speed changes.  No
+ * reference counterpart (a launch-shape setting, like the variants above). */
+int psl_grid_set_occupancy(psl_grid* grid, int workgroups_per_cu);
+int psl_grid_get_occupancy(const psl_grid* grid, int* workgroups_per_cu);
+/* Autotuner: times all PSL_TUNE_VARIANTS variants on this grid (`steps` whole
+ * steps each after one warm-up: streaming, the grid's periodic copies and the
+ * local update, so the rows are misaligned as in a run; the state and the
+ * shifter advance) and keeps the fastest;
+ * then, for a variant whose bulk kernel is a launch of its own, times the
+ * occupancy caps 5, 4 and 3 and keeps one if it is faster than no cap by more
+ * than 0.5 % (psl_grid_get_occupancy tells which); best and
+ * times_ms[PSL_TUNE_VARIANTS] (mean ms per step, no cap) are optional. */
 int psl_grid_autotune(psl_grid* grid, int steps, int* best, double* times_ms);
 /* Mean duration of the grid's dominant update kernel (kernel = 0: k_ps_bulk,
  * 1: k_ps_fused, 2: the bulk kernel with the learned closure), launched

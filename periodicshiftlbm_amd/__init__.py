@@ -165,6 +165,8 @@ _SIGS = [
     ("psl_grid_cell_census", _I, [_P, _P, _P, _P]),
     ("psl_grid_set_tuning", _I, [_P, _I]),
     ("psl_grid_get_tuning", _I, [_P, _P]),
+    ("psl_grid_set_occupancy", _I, [_P, _I]),
+    ("psl_grid_get_occupancy", _I, [_P, _P]),
     ("psl_grid_autotune", _I, [_P, _I, _P, _P]),
     ("psl_device_copy", _I, [_P, _P, ctypes.c_size_t]),
     ("psl_device_alloc", _I, [ctypes.c_size_t, _P]),
@@ -504,8 +506,20 @@ class GridStruct:
     def tuning(self, v: int):
         _check(lib().psl_grid_set_tuning(self._h, int(v)))
 
+    @property
+    def occupancy(self) -> int:
+        """Cap on the bulk kernel's workgroups per CU (0: none; 2-6)."""
+        v = ctypes.c_int()
+        _check(lib().psl_grid_get_occupancy(self._h, ctypes.byref(v)))
+        return v.value
+
+    @occupancy.setter
+    def occupancy(self, v: int):
+        _check(lib().psl_grid_set_occupancy(self._h, int(v)))
+
     def autotune(self, steps: int = 20):
-        """Time the local-update variants on this grid and keep the fastest (state advances)."""
+        """Time the local-update variants on this grid and keep the fastest, then
+        the bulk kernel's occupancy caps (state advances)."""
         best = ctypes.c_int()
         times = np.zeros(TUNE_VARIANTS, np.float64)
         _check(lib().psl_grid_autotune(self._h, steps, ctypes.byref(best), times.ctypes.data))

@@ -513,21 +513,21 @@ static void drop_profile_events(psl_grid* g) {
 }
 
 template <class St, int TUNE>
-static void launch_bulk(bool flags, bool les, bool held, uint32_t nb, hipStream_t stream, const StepParams& p,
-                        const Stage& sg) {
+static void launch_bulk(bool flags, bool les, bool held, uint32_t nb, size_t lds, hipStream_t stream,
+                        const StepParams& p, const Stage& sg) {
   if (held) {
     // between learned-closure evaluations (never staged)
-    if (flags) k_ps_bulk<St, true, true, TUNE, false, true><<<nb, 256, 0, stream>>>(p, sg);
-    else k_ps_bulk<St, false, true, TUNE, false, true><<<nb, 256, 0, stream>>>(p, sg);
+    if (flags) k_ps_bulk<St, true, true, TUNE, false, true><<<nb, 256, lds, stream>>>(p, sg);
+    else k_ps_bulk<St, false, true, TUNE, false, true><<<nb, 256, lds, stream>>>(p, sg);
   } else if (sg.buf) {
-    if (les) k_ps_bulk<St, true, true, TUNE, true><<<nb, 256, 0, stream>>>(p, sg);
-    else k_ps_bulk<St, true, false, TUNE, true><<<nb, 256, 0, stream>>>(p, sg);
+    if (les) k_ps_bulk<St, true, true, TUNE, true><<<nb, 256, lds, stream>>>(p, sg);
+    else k_ps_bulk<St, true, false, TUNE, true><<<nb, 256, lds, stream>>>(p, sg);
   } else if (flags) {
-    if (les) k_ps_bulk<St, true, true, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
-    else k_ps_bulk<St, true, false, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
+    if (les) k_ps_bulk<St, true, true, TUNE, false><<<nb, 256, lds, stream>>>(p, sg);
+    else k_ps_bulk<St, true, false, TUNE, false><<<nb, 256, lds, stream>>>(p, sg);
   } else {
-    if (les) k_ps_bulk<St, false, true, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
-    else k_ps_bulk<St, false, false, TUNE, false><<<nb, 256, 0, stream>>>(p, sg);
+    if (les) k_ps_bulk<St, false, true, TUNE, false><<<nb, 256, lds, stream>>>(p, sg);
+    else k_ps_bulk<St, false, false, TUNE, false><<<nb, 256, lds, stream>>>(p, sg);
   }
 }
 
@@ -623,10 +623,10 @@ static int launch_local_update(psl_grid* g, bool dominant_only = false) {
     // their slots)
     const Stage sg{b.n && !held ? g->stage_buf : nullptr, g->stage_base};
     switch (nt) {
-      case 0: launch_bulk<St, 0>(fl, les, held, nb, g->stream, p, sg); break;
-      case 1: launch_bulk<St, 1>(fl, les, held, nb, g->stream, p, sg); break;
-      case 2: launch_bulk<St, 2>(fl, les, held, nb, g->stream, p, sg); break;
-      default: launch_bulk<St, 3>(fl, les, held, nb, g->stream, p, sg); break;
+      case 0: launch_bulk<St, 0>(fl, les, held, nb, g->bulk_lds, g->stream, p, sg); break;
+      case 1: launch_bulk<St, 1>(fl, les, held, nb, g->bulk_lds, g->stream, p, sg); break;
+      case 2: launch_bulk<St, 2>(fl, les, held, nb, g->bulk_lds, g->stream, p, sg); break;
+      default: launch_bulk<St, 3>(fl, les, held, nb, g->bulk_lds, g->stream, p, sg); break;
     }
   }
   PSL_LAUNCH_CHECK();
@@ -1261,42 +1261,106 @@ int psl_grid_get_tuning(const psl_grid* g, int* tuning) {
 // updates with the grid's own data; the distributions advance by
 // 8 * steps updates) and keep the fastest.  times_ms (optional, 8 entries)
 // receives the mean bulk-kernel time of each variant.
+int psl_grid_set_occupancy(psl_grid* g, int workgroups_per_cu) {
+  PSL_CHECK_ARG(g, "null grid");
+  PSL_CHECK_ARG(workgroups_per_cu == 0 || (workgroups_per_cu >= 2 && workgroups_per_cu <= 6),
+                "workgroups per CU must be 0 (no cap) or 2-6");
+  size_t lds = 0;
+  if (workgroups_per_cu) {
+    int dev = 0, per_cu = 0, per_block = 0;
+    PSL_HIP_TRY(hipGetDevice(&dev));
+    PSL_HIP_TRY(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    PSL_HIP_TRY(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    // whole KiB below the CU's share: the allocation's rounding up then still
+    // leaves room for exactly that many workgroups
+    lds = ((size_t)per_cu / (size_t)workgroups_per_cu) & ~(size_t)1023;
+    PSL_CHECK_ARG(lds > 0 && lds <= (size_t)per_block && (size_t)per_cu / lds == (size_t)workgroups_per_cu,
+                  "this device's LDS cannot express that cap");
+  }
+  g->occupancy = workgroups_per_cu;
+  g->bulk_lds = lds;
+  return PSL_OK;
+}
+
+int psl_grid_get_occupancy(const psl_grid* g, int* workgroups_per_cu) {
+  PSL_CHECK_ARG(g && workgroups_per_cu, "null argument");
+  *workgroups_per_cu = g->occupancy;
+  return PSL_OK;
+}
+
 int psl_grid_autotune(psl_grid* g, int steps, int* best, double* times_ms) {
   PSL_CHECK_ARG(g && steps > 0, "bad autotune arguments");
-  // whole local updates (bulk + boundary list) are timed, so that the fused
-  // and the two-launch forms compare on equal terms
-  const int saved = g->tuning;
+  // whole steps are timed, so that the fused and the two-launch forms compare
+  // on equal terms
+  const int saved = g->tuning, saved_occ = g->occupancy;
   double best_t = 1e300;
   int best_v = saved;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   PSL_HIP_TRY(hipEventCreate(&e0));
   PSL_HIP_TRY(hipEventCreate(&e1));
-  int rc = PSL_OK;
+  int rc = psl_grid_set_occupancy(g, 0);
+  // one whole step (streaming, periodic copies, local update): timed with the
+  // streaming, the rows are misaligned as in a run (repeated updates at one
+  // shift keep them where they are, at zero shift all line-aligned, where
+  // the non-temporal variants win and lose by a third once the rows move)
+  auto step = [&]() {
+    int r = psl_apply_streaming(g);
+    if (r == PSL_OK && (g->periodic_copies & 1)) r = psl_apply_bc_periodic_x(g);
+    if (r == PSL_OK && (g->periodic_copies & 2)) r = psl_apply_bc_periodic_y(g);
+    if (r == PSL_OK) r = psl_apply_local_cell_update(g);
+    return r;
+  };
+  // mean ms of `steps` steps after one warm-up, at the current settings
+  auto time_updates = [&](double& mean) {
+    int r = step();
+    if (r == PSL_OK && hipEventRecord(e0, g->stream) != hipSuccess) r = PSL_E_HIP;
+    for (int s = 0; s < steps && r == PSL_OK; ++s) r = step();
+    if (r == PSL_OK && hipEventRecord(e1, g->stream) != hipSuccess) r = PSL_E_HIP;
+    float ms = 0.f;
+    if (r == PSL_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      r = PSL_E_HIP;
+    mean = (double)ms / steps;
+    return r;
+  };
   for (int v = 0; v < PSL_TUNE_VARIANTS && rc == PSL_OK; ++v) {
     g->tuning = v;
-    rc = psl_apply_local_cell_update(g);  // warm
-    if (rc == PSL_OK && hipEventRecord(e0, g->stream) != hipSuccess) rc = PSL_E_HIP;
-    for (int s = 0; s < steps && rc == PSL_OK; ++s) rc = psl_apply_local_cell_update(g);
-    if (rc == PSL_OK && hipEventRecord(e1, g->stream) != hipSuccess) rc = PSL_E_HIP;
-    float ms = 0.f;
-    if (rc == PSL_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-      rc = PSL_E_HIP;
+    double mean = 0.0;
+    rc = time_updates(mean);
     if (rc != PSL_OK) break;
-    const double mean = (double)ms / steps;
     if (times_ms) times_ms[v] = mean;
     if (mean < best_t) {
       best_t = mean;
       best_v = v;
     }
   }
+  // occupancy caps for the chosen variant when its bulk kernel has a launch of
+  // its own; a cap is kept only if it wins by more than the timing noise
+  int best_occ = 0;
+  if (rc == PSL_OK && !(best_v & PSL_TUNE_FUSED_BOUNDARY)) {
+    g->tuning = best_v;
+    const double base_t = best_t;
+    for (const int occ : {5, 4, 3}) {
+      if (psl_grid_set_occupancy(g, occ) != PSL_OK) continue;  // not expressible on this device
+      double mean = 0.0;
+      rc = time_updates(mean);
+      if (rc != PSL_OK) break;
+      if (mean < best_t && mean < 0.995 * base_t) {
+        best_t = mean;
+        best_occ = occ;
+      }
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc != PSL_OK) {
     g->tuning = saved;
+    (void)psl_grid_set_occupancy(g, saved_occ);
     if (rc == PSL_E_HIP) set_error("autotune timing failed");
     return rc;
   }
   g->tuning = best_v;
+  rc = psl_grid_set_occupancy(g, best_occ);
+  if (rc) return rc;
   if (best) *best = best_v;
   return PSL_OK;
 }

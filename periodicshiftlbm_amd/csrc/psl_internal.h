@@ -122,6 +122,8 @@ struct psl_grid {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
   int64_t census[3] = {0, 0, 0};  // bulk, boundary-list, skipped cells
   int tuning = PSL_TUNE_XCD_REMAP;  // bulk-kernel variant (psl_grid_set_tuning / autotune)
+  int occupancy = 0;               // bulk-kernel workgroups per CU cap (0: none)
+  size_t bulk_lds = 0;             // dynamic LDS a bulk launch reserves for that cap
   const psl_mlp* closure = nullptr;  // learned closure (null: Smagorinsky)
   int closure_every = 1;          // network evaluated every closure_every local updates
   uint64_t closure_count = 0;     // local updates since the closure was set

@@ -83,6 +83,7 @@ def test_size_and_argument_limits_without_gpu():
     nbr = (ctypes.c_int * 14)(*([1, 0] * 6 + [2, 0]))
     assert L.psl_diad_create(ctypes.byref(ci), P.F32, 2, ijk, nbr, ctypes.byref(h)) == E_ARG
     assert b"neighbour" in L.psl_last_error()
+    assert L.psl_grid_set_occupancy(None, 3) == E_ARG
     # layout, closure interval and restart entry points refuse null handles and bad ranges
     assert L.psl_batch_set_layout(None, 1, 0) == E_ARG
     assert L.psl_grid_set_closure_every(None, 2) == E_ARG

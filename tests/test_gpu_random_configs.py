@@ -11,7 +11,7 @@ the channel flags do; axes under three cells thick have no listed faces.
 
 * FP32: 8 steps from a random non-equilibrium state against the FP32 oracle,
   element-wise within 1e-5 relative (the tolerance of the other step tests),
-  and every local-update variant on the same bits.
+  and every local-update variant and bulk occupancy cap on the same bits.
 * FP16 / BF16 storage of f - w: the staged and fused variants on the same
   bits, and within 1e-3 (FP16) / 1e-2 (BF16) absolute of the FP64 oracle.
 """
@@ -54,6 +54,15 @@ def test_random_config_fp32_matches_oracle(psl, oracle_mod, seed):
             ref = f
         else:
             assert np.array_equal(f.view(np.uint32), ref.view(np.uint32)), f"variant {v}"
+    # the occupancy caps reserve LDS only: the same bits
+    for occ in (2, 3, 4, 5, 6):
+        g = rc.grid(P, c, "f32")
+        g.set_f(f0)
+        g.occupancy = occ
+        assert g.occupancy == occ
+        P.updateGrid([g], steps)
+        assert np.array_equal(g.get_f().view(np.uint32), ref.view(np.uint32)), f"occupancy {occ}"
+        g.close()
 
 
 @pytest.mark.parametrize("storage,bound", [("f16", 1e-3), ("bf16", 1e-2)])

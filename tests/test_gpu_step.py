@@ -352,3 +352,29 @@ def test_small_channel_with_boundary_cells_matches_oracle(psl, oracle_mod):
         else:
             assert np.array_equal(f.view(np.uint32), ref.view(np.uint32)), f"variant {v}"
         g.close()
+
+
+def test_occupancy_cap_arguments_and_autotune(psl):
+    """psl_grid_set_occupancy takes 0 (no cap) or 2-6 workgroups per CU; the
+    autotuner leaves a cap that the grid reports, and the tuned grid steps on
+    the same bits as an uncapped one."""
+    P = psl
+    d = cases.batch_domains(count=1, seed=3, l_min=24, l_max=24)[0]
+    g, _ = cases.domain_grid(d, "f32")
+    for bad in (-1, 1, 7, 64):
+        with pytest.raises(RuntimeError, match="workgroups per CU"):
+            g.occupancy = bad
+    assert g.occupancy == 0
+    best, _ = g.autotune(3)
+    occ = g.occupancy
+    print(f"[occupancy] autotune: variant {best}, cap {occ}", flush=True)
+    assert occ in (0, 3, 4, 5)
+    f0 = g.get_f()
+    ref, _ = cases.domain_grid(d, "f32")
+    ref.tuning = best
+    for x in (g, ref):
+        x.set_f(f0)
+        P.updateGrid([x], 6)
+    assert np.array_equal(g.get_f().view(np.uint32), ref.get_f().view(np.uint32))
+    g.close()
+    ref.close()
