@@ -419,18 +419,55 @@ template <class St, bool FC, bool FF>
 __global__ __launch_bounds__(256) void k_eso_to_fine(const typename St::T* __restrict__ fc, EsoParams pc,
                                                      typename St::T* __restrict__ ff, EsoParams pf,
                                                      const int* __restrict__ write, const int* __restrict__ read,
-                                                     uint32_t m, const int* __restrict__ src_rec) {
+                                                     uint32_t m, const int* __restrict__ src_rec,
+                                                     const uint32_t* __restrict__ child_slots) {
+  typedef typename St::T T;
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   const uint32_t g = t >> 3, w = t & 7;
   if (g >= m) return;
+  // the child's slot cells: one contiguous 32-byte read from the pair's table
+  // (issued first, in flight with the source), else through the columns
+  uint32_t slot[8];
+  if (child_slots) {
+    const uint4* cs = (const uint4*)(child_slots + (size_t)t * 8);
+    const uint4 a = cs[0], b = cs[1];
+    slot[0] = a.x; slot[1] = a.y; slot[2] = a.z; slot[3] = a.w;
+    slot[4] = b.x; slot[5] = b.y; slot[6] = b.z; slot[7] = b.w;
+  }
   float v[27];
   const int r = src_rec ? src_rec[g] : -1;
   if (r >= 0) rec_load<St>(pc.srec, (uint32_t)r, v);
   else eso_get<St, FC>(fc, pc, (uint32_t)read[g], v);
   rescale(v, true);
+  if (!child_slots) {
+    const uint32_t wc = (uint32_t)write[g];
+    const uint32_t child = w == 0 ? wc : (uint32_t)pf.nbr[(size_t)(w - 1) * pf.n + wc];
+    eso_slots<St>(pf, child, slot);
+  }
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    const int fi = FF ? opp(q) : q;
+    float x = v[q];
+    if (St::W_OFF) x -= weight(q);
+    ff[(size_t)fi * pf.n + slot[ESO_WRITE[q]]] = St::store(x);
+  }
+}
+
+// The table k_eso_to_fine reads: for entry g and child w (child 0 the entry's
+// fine cell, children 1-7 its +i, +j, +k, +ij, +ik, +jk, +ijk neighbours), the
+// child's eight slot cells, in (entry, child, slot) order.
+__global__ __launch_bounds__(256) void k_eso_c2f_slots(EsoParams pf, const int* __restrict__ write, uint32_t m,
+                                                       uint32_t* __restrict__ out) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t g = t >> 3, w = t & 7;
+  if (g >= m) return;
   const uint32_t wc = (uint32_t)write[g];
   const uint32_t child = w == 0 ? wc : (uint32_t)pf.nbr[(size_t)(w - 1) * pf.n + wc];
-  eso_put<St, FF>(ff, pf, child, v);
+  uint32_t slot[8];
+  eso_slots<StF32>(pf, child, slot);
+  uint4* o = (uint4*)(out + (size_t)t * 8);
+  o[0] = make_uint4(slot[0], slot[1], slot[2], slot[3]);
+  o[1] = make_uint4(slot[4], slot[5], slot[6], slot[7]);
 }
 
 // applyNonReflectiveOutletZ (include/boundaryConditions/applyNonReflectiveOutlet.h:9-73):
@@ -1005,6 +1042,32 @@ static int eso_io(psl_diad* d, float* buf) {
   return PSL_OK;
 }
 
+// The coarse level's table of child slot cells for the fine level f (256 B per
+// coarse-to-fine entry), built on first use and again whenever either level's
+// tables change.  Returns null where it cannot be allocated: the transfer then
+// reads the neighbour columns.
+static const uint32_t* c2f_child_slots(psl_diad* c, psl_diad* f) {
+  const uint64_t key = (c->rec_version * 0x9E3779B97F4A7C15ull) ^ (f->rec_version << 1) ^ (uint64_t)(uintptr_t)f;
+  if (c->c2f_slots && c->c2f_slots_key == key) return c->c2f_slots;
+  if (c->c2f_slots) (void)hipFree(c->c2f_slots);
+  c->c2f_slots = nullptr;
+  c->c2f_slots_key = 0;
+  if (hipMalloc(&c->c2f_slots, (size_t)c->n_c2f * 64 * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    c->c2f_slots = nullptr;
+    return nullptr;
+  }
+  k_eso_c2f_slots<<<(unsigned)((8 * c->n_c2f + 255) / 256), 256, 0, c->stream>>>(
+      eso_params(f, false), c->c2f_write, (uint32_t)c->n_c2f, c->c2f_slots);
+  if (hipGetLastError() != hipSuccess) {
+    (void)hipFree(c->c2f_slots);
+    c->c2f_slots = nullptr;
+    return nullptr;
+  }
+  c->c2f_slots_key = key;
+  return c->c2f_slots;
+}
+
 template <class St, bool FC, bool FF>
 static int eso_comm_t(psl_diad* c, psl_diad* f) {
   typedef typename St::T T;
@@ -1021,7 +1084,8 @@ static int eso_comm_t(psl_diad* c, psl_diad* f) {
   }
   if (c->n_c2f) {
     k_eso_to_fine<St, FC, FF><<<(unsigned)((8 * c->n_c2f + 255) / 256), 256, 0, c->stream>>>(
-        (const T*)c->f, pc, (T*)f->f, pf, c->c2f_write, c->c2f_read, (uint32_t)c->n_c2f, src_rec);
+        (const T*)c->f, pc, (T*)f->f, pf, c->c2f_write, c->c2f_read, (uint32_t)c->n_c2f, src_rec,
+        c2f_child_slots(c, f));
     PSL_LAUNCH_CHECK();
     records_stale(f);  // fine cells were rewritten after the fine update
   }
@@ -1178,6 +1242,7 @@ int psl_diad_destroy(psl_diad* d) {
   if (d->ijk) (void)hipFree(d->ijk);
   if (d->nbr) (void)hipFree(d->nbr);
   if (d->adj) (void)hipFree(d->adj);
+  if (d->c2f_slots) (void)hipFree(d->c2f_slots);
   if (d->flags) (void)hipFree(d->flags);
   if (d->bc_dev) (void)hipFree(d->bc_dev);
   for (int* a : {d->c2f_read, d->c2f_write, d->f2c_write, d->f2c_read, d->outlet, d->kmin, d->moving, d->forced})

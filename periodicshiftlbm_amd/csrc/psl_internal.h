@@ -155,6 +155,10 @@ struct psl_diad {
   // per cell: bit 0 set when the +i neighbour is the next cell in memory,
   // bits 1, 2, 3 when +ij, +ik, +ijk follow +j, +k, +jk (psl_esotwist.hip)
   uint8_t* adj = nullptr;
+  // per coarse-to-fine entry and child: the child's eight slot cells, read
+  // by the transfer to the finer level; built for one fine level (key)
+  uint32_t* c2f_slots = nullptr;
+  uint64_t c2f_slots_key = 0;
   uint8_t* flags = nullptr;
   float4* bc_dev = nullptr;
   float cs = 0.f;
