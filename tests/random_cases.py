@@ -15,9 +15,15 @@ from periodicshiftlbm_amd import cases
 FACE_KINDS = [cases.GIVEN_U, cases.GIVEN_RHO, cases.GIVEN_NONE, cases.GIVEN_RHOU, cases.BOUNCEBACK, cases.FLUID]
 
 
-def config(seed):
-    rng = np.random.default_rng(1000 + seed)
-    shape = (int(rng.integers(2, 21)), int(rng.integers(1, 13)), int(rng.integers(1, 13)))
+def config(seed, large=False):
+    """Small grids (2-20 x 1-12 x 1-12 cells), or with large=True 24-72 cells
+    per axis: many workgroups, wrap thresholds inside waves, several blocks of
+    staged boundary records."""
+    rng = np.random.default_rng((31000 if large else 1000) + seed)
+    if large:
+        shape = tuple(int(x) for x in rng.integers(24, 73, 3))
+    else:
+        shape = (int(rng.integers(2, 21)), int(rng.integers(1, 13)), int(rng.integers(1, 13)))
     nx, ny, nz = shape
     k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     coord = [i.reshape(-1), j.reshape(-1), k.reshape(-1)]
