@@ -385,7 +385,7 @@ def scalar_leg(P, cases, steps: int):
     return out
 
 
-def closure_leg(P, n: int, log, closure_every: int = 5):
+def closure_leg(P, n: int, log, closure_every: int = 5, sphere_steps: int = 20000):
     """Learned closure at the decaying-turbulence configuration: n^3 fine run
     (FP16 storage, Cs 0.1), filtered 8x to (n/8)^3 every 200 steps (a coarse
     grid on which an SGS model matters: without one the coarse spectrum error
@@ -422,6 +422,18 @@ def closure_leg(P, n: int, log, closure_every: int = 5):
     head = out[f"every_{closure_every}"]
     out.update(closure_every=closure_every, gain_vs_smagorinsky=head["gain_vs_smagorinsky"],
                gain_vs_none=head["gain_vs_none"], cost_ratio_vs_smagorinsky=head["cost_ratio_vs_smagorinsky"])
+    if sphere_steps:
+        # the same network on the Re = 1e5 sphere against a 2x finer Smagorinsky run
+        sph = les.sphere_comparison(P, wf.net, coarse_steps=sphere_steps, closure_every=closure_every, log=log)
+        out["sphere_re1e5"] = sph
+        parts = []
+        for name in ("none", "smagorinsky", "mlp"):
+            r = sph[name]
+            parts.append(f"{name} " + (f"diverged at step {r['diverged_at']}" if "diverged_at" in r else
+                                      f"C_D {r['cd_mean']:.3f} ({r['cd_rel_err'] * 100:.1f} %), wake u_x "
+                                      f"{r['wake_ux_rel_l2'] * 100:.1f} %"))
+        log(f"  sphere Re 1e5 against the 2x finer run (C_D {sph['fine'].get('cd_mean', float('nan')):.3f}): "
+            + "; ".join(parts))
     return out
 
 
@@ -472,6 +484,8 @@ def main():
     ap.add_argument("--diad-storages", default="f16,f32", help="storages of the DIAD sphere leg")
     ap.add_argument("--closure-n", type=int, default=512,
                     help="fine grid edge of the learned-closure leg (0: skip)")
+    ap.add_argument("--closure-sphere-steps", type=int, default=20000,
+                    help="coarse steps of the Re 1e5 sphere comparison of the closure (0: skip)")
     ap.add_argument("--closure-every", type=int, default=5,
                     help="network evaluated every this many steps in the learned-closure leg")
     ap.add_argument("--access-probes", action="store_true",
@@ -631,7 +645,8 @@ def main():
         out["variants"]["diad_sphere"] = dl
     if args.closure_n:
         log(f"[bench] learned closure, {args.closure_n}^3 decaying turbulence")
-        out["variants"]["closure"] = closure_leg(P, args.closure_n, log, args.closure_every)
+        out["variants"]["closure"] = closure_leg(P, args.closure_n, log, args.closure_every,
+                                                 args.closure_sphere_steps)
     if not args.no_cpu:
         log(f"[bench] cpu baseline (~{args.cpu_seconds:.0f} s)")
         out["cpu_baseline"] = cpu_baseline(cases, flags_keep, nu_keep, 0.17, shape, args.cpu_seconds)

@@ -320,3 +320,109 @@ class _Offset:
         self._buf = buf
         self.nbytes = buf.nbytes - offset
         self.ptr = ctypes.c_void_p(buf.ptr.value + offset)
+
+
+# ------------------------------------------------- sphere at Re = 1e5 with the closure
+def sphere_grid(P, scale: int, storage: str, reynolds: float, cs: float, u_inlet: float = 0.04):
+    """The sphere channel (256 x 128 x 128 cells, D = 32, flowAroundSphereMainSingle.cu's
+    layout: velocity inlet, pressure outlet, periodic sides) refined `scale` times:
+    cells of size 1/scale nested in the unrefined cells, the same icosphere
+    voxelized, the same lattice inlet velocity and so a lattice viscosity `scale`
+    times larger at the same Reynolds number.  dtPhys = res / 1000 makes the
+    section cuts report lattice velocities.  Starts from uniform flow."""
+    nx, ny, nz = (a * scale for a in cases.SPHERE_SHAPE)
+    res = 1.0 / scale
+    off = -0.5 + 0.5 * res
+    d = cases.SPHERE_DIAMETER * scale
+    info = P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=u_inlet * d / reynolds, res=res,
+                        ox=off, oy=off, oz=off, dtPhys=res / 1000.0)
+    stl = P.STLStruct.from_triangles(cases.icosphere(4, cases.SPHERE_DIAMETER / 2, cases.SPHERE_CENTER))
+    mask = P.BoolArray(nx * ny * nz)
+    P.applyMarkersInsideSTL(mask, stl, True, info)
+    solid = mask.get().astype(bool)
+    stl.close()
+    g = P.GridStruct(info, storage)
+    g.set_flags(cases.channel_flags((nx, ny, nz), solid))
+    g.set_given_values(cases.channel_given_values(u_inlet))
+    g.set_smagorinsky(cs)
+    g.set_periodic_copies(y=True)
+    P.fill_equilibrium(g, ux=np.where(solid, 0.0, u_inlet).astype(np.float32))
+    return g, d
+
+
+def _sphere_run(P, g, scale: int, d: float, steps: int, every: int, plane_every: int, log, label: str):
+    """Drag coefficient every `every` steps and the mid-plane (XY through the
+    sphere centre) streamwise velocity every `plane_every` steps over the
+    second half, averaged over the scale layers that make up the unrefined
+    mid-plane cell and over scale x scale in-plane blocks.  Stops at the first
+    sample with a non-finite or non-positive-density cell."""
+    lo, hi = cases.sphere_box()
+    lo = [a * scale for a in lo]
+    hi = [a * scale for a in hi]
+    k0 = int(cases.SPHERE_CENTER[2]) * scale
+    cds, planes, solid = [], [], None
+    t0 = time.perf_counter()
+    for s in range(every, steps + 1, every):
+        P.updateGrid([g], every)
+        if s % (10 * every) == 0 or s == steps:
+            bad = g.check_health()[0]
+            if bad:
+                log(f"[les-sphere] {label}: {bad} unhealthy cells after {s} steps")
+                return {"diverged_at": s, "cd_history": cds}
+        cds.append(cases.drag_coefficient(float(g.momentum_exchange(lo, hi)[0]), 0.04, d))
+        if s > steps // 2 and s % plane_every == 0:
+            cut = sum(P.section_cut(g, "XY", k0 + k) for k in range(scale)) / scale
+            ny, nx = cut.shape[0] // scale, cut.shape[1] // scale
+            blocks = cut.reshape(ny, scale, nx, scale, 5)
+            planes.append(blocks[..., 1].mean(axis=(1, 3)))
+            if solid is None:
+                solid = blocks[..., 4].max(axis=(1, 3)) > 0
+    el = time.perf_counter() - t0
+    half = cds[len(cds) // 2:]
+    log(f"[les-sphere] {label}: mean C_D {np.mean(half):.4f} over steps {steps // 2}-{steps} ({el:.1f} s)")
+    return {"cd_mean": float(np.mean(half)), "cd_std": float(np.std(half)), "cd_history": cds,
+            "ux_mean_plane": np.mean(planes, axis=0), "solid": solid, "seconds": el}
+
+
+def sphere_comparison(P, net, coarse_steps: int = 20000, closure_every: int = 5, cs: float = 0.1,
+                      reynolds: float = 1e5, storage: str = "f16", log=print):
+    """The Re = 1e5 sphere on the 256 x 128 x 128 grid with no SGS model, with
+    Smagorinsky (Cs = 0.1, the reference drivers' value) and with the learned
+    closure `net` (evaluated every `closure_every` steps, box-face cells on
+    Smagorinsky), judged against the same sphere on a 2x finer grid with
+    Smagorinsky over the same physical time (twice the steps).  Errors: mean
+    drag coefficient over the second half relative to the fine run's, and the
+    relative L2 error of the time-averaged mid-plane streamwise velocity in the
+    wake window (sphere centre to 4 D behind it, 1.5 D either side of the axis;
+    fluid cells of both grids) against the fine run's, averaged onto the
+    coarse cells."""
+    every = 100
+    runs = {}
+    g, d = sphere_grid(P, 2, storage, reynolds, cs)
+    runs["fine"] = _sphere_run(P, g, 2, d, 2 * coarse_steps, 2 * every, 1000, log, "fine (2x, Smagorinsky)")
+    g.close()
+    for name in ("none", "smagorinsky", "mlp"):
+        g, d = sphere_grid(P, 1, storage, reynolds, 0.0 if name == "none" else cs)
+        if name == "mlp":
+            g.set_closure(net)
+            g.set_closure_every(closure_every)
+        runs[name] = _sphere_run(P, g, 1, d, coarse_steps, every, 500, log, name)
+        g.close()
+    fine = runs["fine"]
+    cx, cy = int(cases.SPHERE_CENTER[0]), int(cases.SPHERE_CENTER[1])
+    D = int(cases.SPHERE_DIAMETER)
+    win = (slice(cy - 3 * D // 2, cy + 3 * D // 2), slice(cx, cx + 4 * D))
+    out = {"reynolds": reynolds, "coarse_steps": coarse_steps, "fine_steps": 2 * coarse_steps, "cs": cs,
+           "closure_every": closure_every, "storage": storage,
+           "fine": {k: fine[k] for k in ("cd_mean", "cd_std", "seconds") if k in fine}}
+    for name in ("none", "smagorinsky", "mlp"):
+        r = runs[name]
+        if "diverged_at" in r or "diverged_at" in fine:
+            out[name] = {"diverged_at": r.get("diverged_at"), "fine_diverged_at": fine.get("diverged_at")}
+            continue
+        keep = ~(r["solid"] | fine["solid"])[win]
+        a, b = r["ux_mean_plane"][win][keep], fine["ux_mean_plane"][win][keep]
+        out[name] = {"cd_mean": r["cd_mean"], "cd_std": r["cd_std"],
+                     "cd_rel_err": abs(r["cd_mean"] / fine["cd_mean"] - 1),
+                     "wake_ux_rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)), "seconds": r["seconds"]}
+    return out
