@@ -397,16 +397,18 @@ def sphere_comparison(P, net, coarse_steps: int = 20000, closure_every: int = 5,
     fluid cells of both grids) against the fine run's, averaged onto the
     coarse cells."""
     every = 100
+    plane_every = max(every, coarse_steps // 40 // every * every)  # 20 planes over the second half
     runs = {}
     g, d = sphere_grid(P, 2, storage, reynolds, cs)
-    runs["fine"] = _sphere_run(P, g, 2, d, 2 * coarse_steps, 2 * every, 1000, log, "fine (2x, Smagorinsky)")
+    runs["fine"] = _sphere_run(P, g, 2, d, 2 * coarse_steps, 2 * every, 2 * plane_every, log,
+                               "fine (2x, Smagorinsky)")
     g.close()
     for name in ("none", "smagorinsky", "mlp"):
         g, d = sphere_grid(P, 1, storage, reynolds, 0.0 if name == "none" else cs)
         if name == "mlp":
             g.set_closure(net)
             g.set_closure_every(closure_every)
-        runs[name] = _sphere_run(P, g, 1, d, coarse_steps, every, 500, log, name)
+        runs[name] = _sphere_run(P, g, 1, d, coarse_steps, every, plane_every, log, name)
         g.close()
     fine = runs["fine"]
     cx, cy = int(cases.SPHERE_CENTER[0]), int(cases.SPHERE_CENTER[1])

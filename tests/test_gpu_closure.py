@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import closure as C
-from periodicshiftlbm_amd import cases
+from periodicshiftlbm_amd import cases, les
 
 pytestmark = pytest.mark.gpu
 
@@ -310,3 +310,33 @@ def test_les_workflow_small(psl):
     held = wf.compare(start=100, cs_values=(0.1,), alpha_values=(1.0,))
     print(f"[les-small] network every 4 steps: {held['mlp']['evaluation_resolved']:.4f}", flush=True)
     assert abs(held["mlp"]["evaluation_resolved"] / res["mlp"]["evaluation_resolved"] - 1) < 0.02
+
+
+def test_sphere_comparison_grids_nest_and_run(psl):
+    """The Re = 1e5 sphere comparison (les.sphere_comparison): the 2x finer
+    channel voxelizes the same icosphere nested in the coarse cells (solid
+    volume within 3 % of 8x the coarse one, and every coarse cell whose eight
+    children are all solid is itself solid), and a short comparison returns
+    finite drag and wake errors for every model."""
+    P = psl
+    gc, dc = les.sphere_grid(P, 1, "f16", 1e5, 0.1)
+    gf, df = les.sphere_grid(P, 2, "f16", 1e5, 0.1)
+    sc = gc.get_flags().reshape(128, 128, 256) == cases.BOUNCEBACK
+    sf = gf.get_flags().reshape(256, 256, 512) == cases.BOUNCEBACK
+    gc.close()
+    gf.close()
+    assert dc == 32 and df == 64
+    ratio = sf.sum() / (8.0 * sc.sum())
+    full = sf.reshape(128, 2, 128, 2, 256, 2).all(axis=(1, 3, 5))
+    print(f"[sphere nest] coarse solid {int(sc.sum())}, fine solid {int(sf.sum())} (ratio to 8x {ratio:.4f}); "
+          f"coarse cells with all children solid {int(full.sum())}, of them solid {int((full & sc).sum())}",
+          flush=True)
+    assert abs(ratio - 1) < 0.03
+    assert np.all(sc[full])
+    net = P.MLP(les.glorot_params(7))
+    out = les.sphere_comparison(P, net, coarse_steps=400, log=print)
+    for name in ("none", "smagorinsky", "mlp"):
+        r = out[name]
+        assert "diverged_at" not in r, name
+        assert np.isfinite(r["cd_mean"]) and np.isfinite(r["wake_ux_rel_l2"]) and r["wake_ux_rel_l2"] < 1.0
+    assert np.isfinite(out["fine"]["cd_mean"])
