@@ -385,7 +385,7 @@ def scalar_leg(P, cases, steps: int):
     return out
 
 
-def closure_leg(P, n: int, log, closure_every: int = 5, sphere_steps: int = 20000):
+def closure_leg(P, n: int, log, closure_every: int = 8, sphere_steps: int = 20000):
     """Learned closure at the decaying-turbulence configuration: n^3 fine run
     (FP16 storage, Cs 0.1), filtered 8x to (n/8)^3 every 200 steps (a coarse
     grid on which an SGS model matters: without one the coarse spectrum error
@@ -486,7 +486,7 @@ def main():
                     help="fine grid edge of the learned-closure leg (0: skip)")
     ap.add_argument("--closure-sphere-steps", type=int, default=20000,
                     help="coarse steps of the Re 1e5 sphere comparison of the closure (0: skip)")
-    ap.add_argument("--closure-every", type=int, default=5,
+    ap.add_argument("--closure-every", type=int, default=8,
                     help="network evaluated every this many steps in the learned-closure leg")
     ap.add_argument("--access-probes", action="store_true",
                     help="only run the access-pattern probes on a 256^3 grid and print their JSON line")
