@@ -200,6 +200,44 @@ def access_probes(P, n: int = 256):
     return rows
 
 
+def occupancy_sweep(P, cases, n: int, steps: int = 200, repeats: int = 3):
+    """Bulk workgroups per CU capped at 6 to 2 (0: uncapped) on the sphere
+    256x128x128 and the periodic n^3 box, FP32 and FP16, with the variant the
+    autotuner picks uncapped: the bulk kernel alone at the shift reached (50
+    launches back to back) and whole steps, best of `repeats` passes over the
+    caps in alternating order."""
+    rows = []
+    caps = (0, 6, 5, 4, 3, 2)
+    for name in ("sphere", "box"):
+        for storage in ("f32", "f16"):
+            if name == "sphere":
+                g = build_sphere_grid(P, cases, storage)[0]
+            else:
+                nx, ny, nz = cases.tgv_shape(n)
+                g = P.GridStruct(P.InfoStruct(cellCountX=nx, cellCountY=ny, cellCountZ=nz, nu=0.005), storage)
+                g.set_periodic_copies(x=True, y=True)
+                P.fill_equilibrium(g, *cases.tgv_fields(n))
+            g.autotune(5)
+            g.occupancy = 0
+            best = {c: [float("inf"), float("inf")] for c in caps}
+            for r in range(repeats):
+                for c in (caps if r % 2 == 0 else caps[::-1]):
+                    g.occupancy = c
+                    ms_step = time_steps(P, g, steps, 20) * 1e3 / steps
+                    ms_kernel = g.time_update_kernel(50)[0]
+                    best[c][0] = min(best[c][0], ms_kernel)
+                    best[c][1] = min(best[c][1], ms_step)
+            for c in caps:
+                k, s = best[c]
+                rows.append({"grid": name, "storage": storage, "cells": g.n, "tuning": g.tuning,
+                             "workgroups_per_cu": c, "bulk_kernel_us": k * 1e3, "step_us": s * 1e3,
+                             "kernel_vs_uncapped": best[0][0] / k, "step_vs_uncapped": best[0][1] / s})
+                log(f"  {name} {storage} variant {g.tuning} cap {c or 'none':>4}: bulk kernel {k * 1e3:7.1f} us "
+                    f"({best[0][0] / k:5.3f}x), step {s * 1e3:7.1f} us ({best[0][1] / s:5.3f}x)")
+            g.close()
+    return rows
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -490,6 +528,9 @@ def main():
                     help="network evaluated every this many steps in the learned-closure leg")
     ap.add_argument("--access-probes", action="store_true",
                     help="only run the access-pattern probes on a 256^3 grid and print their JSON line")
+    ap.add_argument("--occupancy-sweep", action="store_true",
+                    help="only run the bulk occupancy-cap sweep (sphere and large box, FP32 and FP16) "
+                         "and print its JSON line")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (10 times) for counter calibration")
     args = ap.parse_args()
@@ -507,6 +548,10 @@ def main():
         cp = copy_probe(P)
         log(f"  device copy of 2 GiB: {cp['GBs']:.0f} GB/s")
         print(json.dumps({"access_probes": access_probes(P), "hbm_copy_probe": cp}), flush=True)
+        return
+    if args.occupancy_sweep:
+        log(f"[bench] bulk occupancy caps, sphere 256x128x128 and periodic box {args.large_n}^3")
+        print(json.dumps({"occupancy_sweep": occupancy_sweep(P, cases, args.large_n)}), flush=True)
         return
     if args.copy_calibration_mb:
         nbytes = args.copy_calibration_mb << 20

@@ -210,7 +210,8 @@ int psl_grid_get_occupancy(const psl_grid* grid, int* workgroups_per_cu);
  * steps each after one warm-up: streaming, the grid's periodic copies and the
  * local update, so the rows are misaligned as in a run; the state and the
  * shifter advance) and keeps the fastest;
- * then, for a variant whose bulk kernel is a launch of its own, times the
+ * then, for a variant whose bulk kernel is a launch of its own (every variant
+ * on a grid without listed boundary cells), times the
  * occupancy caps 5, 4 and 3 and keeps one if it is faster than no cap by more
  * than 0.5 % (psl_grid_get_occupancy tells which); best and
  * times_ms[PSL_TUNE_VARIANTS] (mean ms per step, no cap) are optional. */

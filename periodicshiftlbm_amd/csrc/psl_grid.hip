@@ -1334,9 +1334,12 @@ int psl_grid_autotune(psl_grid* g, int steps, int* best, double* times_ms) {
     }
   }
   // occupancy caps for the chosen variant when its bulk kernel has a launch of
-  // its own; a cap is kept only if it wins by more than the timing noise
+  // its own (always, unless the boundary cells are fused into it: a grid
+  // without listed cells runs the bulk launch under every variant); a cap is
+  // kept only if it wins by more than the timing noise
   int best_occ = 0;
-  if (rc == PSL_OK && !(best_v & PSL_TUNE_FUSED_BOUNDARY)) {
+  const bool fused_launch = (best_v & PSL_TUNE_FUSED_BOUNDARY) && g->flags && g->n_bc && !g->closure;
+  if (rc == PSL_OK && !fused_launch) {
     g->tuning = best_v;
     const double base_t = best_t;
     for (const int occ : {5, 4, 3}) {
