@@ -286,7 +286,9 @@ def large_box_leg(P, cases, n: int, steps: int):
                         "roofline_frac": gbs / HBM_PEAK_GBS, "tuning": g.tuning,
                         "occupancy": g.occupancy}
         g.close()
-        out[storage]["pattern_limit_GBs"] = P.probe_ps_pattern(nx, ny, nz, storage)
+        # the pattern's ceiling under the cap the bulk kernel ran with
+        out[storage]["pattern_limit_GBs"] = P.probe_ps_pattern(nx, ny, nz, storage,
+                                                               occupancy=out[storage]["occupancy"])
         out[storage]["frac_of_pattern_limit"] = gbs / out[storage]["pattern_limit_GBs"]
     out["f16_over_f32"] = out["f16"]["mlups"] / out["f32"]["mlups"]
     out["note"] = "working sets of 1.8 GB (FP32) and 0.9 GB (FP16): HBM-resident, beyond the 256 MB Infinity Cache"
@@ -591,7 +593,7 @@ def main():
         cells_per_launch = bulk + (nbc if kernel == "k_ps_fused" else 0)
         bytes_per_launch = BYTES_PER_UPDATE[storage] * cells_per_launch
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        pattern = P.probe_ps_pattern(*shape, storage)
+        pattern = P.probe_ps_pattern(*shape, storage, occupancy=g.occupancy)
         results[storage] = {
             "mlups": mlups, "ms_per_step": el * 1e3 / args.steps, "bulk_kernel_ms": avg_ms,
             "kernel": kernel, "cells_per_launch": cells_per_launch,

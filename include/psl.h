@@ -171,6 +171,11 @@ int psl_grid_profile_read(psl_grid* grid, double* total_ms, int* launches);
  * them back, no collision, `launches` times.  The data-movement ceiling of the
  * local update at that size and storage (no reference counterpart). */
 int psl_probe_ps_pattern(int nx, int ny, int nz, int storage, int launches, double* gbs);
+/* The same at a cap of `workgroups_per_cu` workgroups per CU (0: none; 2-6 as
+ * psl_grid_set_occupancy takes), the ceiling of a bulk kernel that runs under
+ * that cap (no reference counterpart). */
+int psl_probe_ps_pattern_capped(int nx, int ny, int nz, int storage, int workgroups_per_cu, int launches,
+                                double* gbs);
 /* The same access pattern taken apart (no collision, on a scratch allocation of
  * 27 rows of nx*ny*nz elements): `streams` populations (1, 2, 4, 8, 16 or 27); mode 0 reads then
  * writes in place, 1 only reads, 2 only writes; cells_per_lane 1, 2 or 4 consecutive

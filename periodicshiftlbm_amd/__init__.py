@@ -200,6 +200,7 @@ _SIGS = [
     ("psl_get_macroscopic_device", _I, [_P, _P]),
     ("psl_grid_time_update_kernel", _I, [_P, _I, _P, _P]),
     ("psl_probe_ps_pattern", _I, [_I, _I, _I, _I, _I, _P]),
+    ("psl_probe_ps_pattern_capped", _I, [_I, _I, _I, _I, _I, _I, _P]),
     ("psl_probe_ps_access", _I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _P]),
     ("psl_diad_create", _I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     ("psl_diad_destroy", _I, [_P]),
@@ -1174,13 +1175,15 @@ def les_targets(coarse: DeviceBuffer, m: int, nu: float):
     return feat, tgt
 
 
-def probe_ps_pattern(nx: int, ny: int, nz: int, storage="f32", launches: int = 20) -> float:
-    """HBM rate (GB/s, read + write) of the periodic-shift access pattern alone on this grid size."""
+def probe_ps_pattern(nx: int, ny: int, nz: int, storage="f32", launches: int = 20, occupancy: int = 0) -> float:
+    """HBM rate (GB/s, read + write) of the periodic-shift access pattern alone
+    on this grid size, at `occupancy` workgroups per CU (0: no cap; the cap a
+    grid's bulk kernel runs under, GridStruct.occupancy)."""
     if isinstance(storage, str):
         storage = STORAGE_NAMES[storage]
     require_device()
     g = ctypes.c_double()
-    _check(lib().psl_probe_ps_pattern(nx, ny, nz, storage, launches, ctypes.byref(g)))
+    _check(lib().psl_probe_ps_pattern_capped(nx, ny, nz, storage, int(occupancy), launches, ctypes.byref(g)))
     return g.value
 
 

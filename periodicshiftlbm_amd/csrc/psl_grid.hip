@@ -1145,13 +1145,43 @@ int psl_device_copy(void* dst, const void* src, size_t bytes) {
   return PSL_OK;
 }
 
+// Dynamic LDS per workgroup that leaves room for exactly `workgroups_per_cu`
+// workgroups on a CU of the current device (0: no cap, no LDS); the kernels
+// that take it do not use it.
+static int occupancy_lds(int workgroups_per_cu, size_t* lds_out) {
+  PSL_CHECK_ARG(workgroups_per_cu == 0 || (workgroups_per_cu >= 2 && workgroups_per_cu <= 6),
+                "workgroups per CU must be 0 (no cap) or 2-6");
+  size_t lds = 0;
+  if (workgroups_per_cu) {
+    int dev = 0, per_cu = 0, per_block = 0;
+    PSL_HIP_TRY(hipGetDevice(&dev));
+    PSL_HIP_TRY(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    PSL_HIP_TRY(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    // whole KiB below the CU's share: the allocation's rounding up then still
+    // leaves room for exactly that many workgroups
+    lds = ((size_t)per_cu / (size_t)workgroups_per_cu) & ~(size_t)1023;
+    PSL_CHECK_ARG(lds > 0 && lds <= (size_t)per_block && (size_t)per_cu / lds == (size_t)workgroups_per_cu,
+                  "this device's LDS cannot express that cap");
+  }
+  *lds_out = lds;
+  return PSL_OK;
+}
+
 int psl_probe_ps_pattern(int nx, int ny, int nz, int storage, int launches, double* gbs) {
+  return psl_probe_ps_pattern_capped(nx, ny, nz, storage, 0, launches, gbs);
+}
+
+int psl_probe_ps_pattern_capped(int nx, int ny, int nz, int storage, int workgroups_per_cu, int launches,
+                                double* gbs) {
   PSL_CHECK_ARG(nx > 0 && ny > 0 && nz > 0 && launches > 0 && gbs, "bad arguments");
   PSL_CHECK_ARG(storage == PSL_F32 || storage == PSL_F16DEV || storage == PSL_BF16DEV, "unknown storage");
   const uint64_t n64 = (uint64_t)nx * ny * nz;
   PSL_CHECK_ARG(n64 < (1ull << 31), "cellCount must be below 2^31");
   const uint32_t n = (uint32_t)n64;
   const size_t esz = storage_bytes(storage);
+  size_t lds = 0;
+  const int lrc = occupancy_lds(workgroups_per_cu, &lds);
+  if (lrc) return lrc;
   void* f = nullptr;
   PSL_HIP_TRY(hipMalloc(&f, 27ull * n * esz));
   // shifts after 1000 streaming steps of an nx x ny x nz grid
@@ -1165,8 +1195,8 @@ int psl_probe_ps_pattern(int nx, int ny, int nz, int storage, int launches, doub
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = PSL_OK;
   auto launch = [&]() {
-    if (esz == 4) k_ps_pattern<float><<<blocks_for(n), 256>>>((float*)f, sh, n);
-    else k_ps_pattern<uint16_t><<<blocks_for(n), 256>>>((uint16_t*)f, sh, n);
+    if (esz == 4) k_ps_pattern<float><<<blocks_for(n), 256, lds>>>((float*)f, sh, n);
+    else k_ps_pattern<uint16_t><<<blocks_for(n), 256, lds>>>((uint16_t*)f, sh, n);
   };
   if (hipMemset(f, 0, 27ull * n * esz) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess)
@@ -1263,20 +1293,9 @@ int psl_grid_get_tuning(const psl_grid* g, int* tuning) {
 // receives the mean bulk-kernel time of each variant.
 int psl_grid_set_occupancy(psl_grid* g, int workgroups_per_cu) {
   PSL_CHECK_ARG(g, "null grid");
-  PSL_CHECK_ARG(workgroups_per_cu == 0 || (workgroups_per_cu >= 2 && workgroups_per_cu <= 6),
-                "workgroups per CU must be 0 (no cap) or 2-6");
   size_t lds = 0;
-  if (workgroups_per_cu) {
-    int dev = 0, per_cu = 0, per_block = 0;
-    PSL_HIP_TRY(hipGetDevice(&dev));
-    PSL_HIP_TRY(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-    PSL_HIP_TRY(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    // whole KiB below the CU's share: the allocation's rounding up then still
-    // leaves room for exactly that many workgroups
-    lds = ((size_t)per_cu / (size_t)workgroups_per_cu) & ~(size_t)1023;
-    PSL_CHECK_ARG(lds > 0 && lds <= (size_t)per_block && (size_t)per_cu / lds == (size_t)workgroups_per_cu,
-                  "this device's LDS cannot express that cap");
-  }
+  const int rc = occupancy_lds(workgroups_per_cu, &lds);
+  if (rc) return rc;
   g->occupancy = workgroups_per_cu;
   g->bulk_lds = lds;
   return PSL_OK;

@@ -97,3 +97,7 @@ def test_size_and_argument_limits_without_gpu():
     gbs = ctypes.c_double()
     assert L.psl_probe_ps_access(64, 64, 64, P.F32, 5, 0, 1, 0, 1, ctypes.byref(gbs)) == E_ARG
     assert b"streams" in L.psl_last_error()
+    # so does the capped pattern probe: caps outside 0 and 2-6 are refused
+    for cap in (-1, 1, 7):
+        assert L.psl_probe_ps_pattern_capped(64, 64, 64, P.F32, cap, 5, ctypes.byref(gbs)) == E_ARG
+        assert b"workgroups per CU" in L.psl_last_error()

@@ -378,3 +378,15 @@ def test_occupancy_cap_arguments_and_autotune(psl):
     assert np.array_equal(g.get_f().view(np.uint32), ref.get_f().view(np.uint32))
     g.close()
     ref.close()
+
+
+def test_capped_pattern_probe(psl):
+    """The access-pattern probe at every cap the bulk kernel takes: a finite
+    positive rate, and no cap the same call as psl_probe_ps_pattern."""
+    P = psl
+    for occ in (0, 2, 3, 4, 5, 6):
+        gbs = P.probe_ps_pattern(64, 64, 64, "f32", launches=5, occupancy=occ)
+        print(f"[pattern probe] 64^3 f32 cap {occ}: {gbs:.0f} GB/s", flush=True)
+        assert np.isfinite(gbs) and gbs > 0
+    with pytest.raises(RuntimeError, match="workgroups per CU"):
+        P.probe_ps_pattern(64, 64, 64, "f16", launches=5, occupancy=7)
