@@ -214,11 +214,14 @@ int psl_grid_get_occupancy(const psl_grid* grid, int* workgroups_per_cu);
 /* Autotuner: times all PSL_TUNE_VARIANTS variants on this grid (`steps` whole
  * steps each after one warm-up: streaming, the grid's periodic copies and the
  * local update, so the rows are misaligned as in a run; the state and the
- * shifter advance) and keeps the fastest;
+ * shifter advance); the variants within 2 % of the fastest are timed again
+ * over max(steps, 256) steps (the time per step runs in phases tens of steps
+ * long as the shifts carry the rows across lines and channels) and the
+ * fastest of those is kept;
  * then, for a variant whose bulk kernel is a launch of its own (every variant
  * on a grid without listed boundary cells), times the
- * occupancy caps 5, 4 and 3 and keeps one if it is faster than no cap by more
- * than 0.5 % (psl_grid_get_occupancy tells which); best and
+ * occupancy caps 5, 4 and 3 over as many steps and keeps one if it is faster
+ * than no cap by more than 0.5 % (psl_grid_get_occupancy tells which); best and
  * times_ms[PSL_TUNE_VARIANTS] (mean ms per step, no cap) are optional. */
 int psl_grid_autotune(psl_grid* grid, int steps, int* best, double* times_ms);
 /* Mean duration of the grid's dominant update kernel (kernel = 0: k_ps_bulk,
@@ -227,6 +230,12 @@ int psl_grid_autotune(psl_grid* grid, int steps, int* best, double* times_ms);
  * kernel updates the distributions each time (no streaming in between), so
  * call it where the state no longer matters. */
 int psl_grid_time_update_kernel(psl_grid* grid, int launches, double* ms_per_launch, int* kernel);
+/* The same with psl_apply_streaming before every launch (a host-side shift
+ * update, no kernel): the launches see the shifts of consecutive steps, whose
+ * alignment to cache lines and memory channels changes the duration by up to
+ * ~10 % from step to step, so the mean is that of a run's update kernels.  The
+ * shifter advances by `launches` (no reference counterpart). */
+int psl_grid_time_update_kernel_streaming(psl_grid* grid, int launches, double* ms_per_launch, int* kernel);
 /* Cells handled by the bulk kernel (fluid + bounceback) and by the boundary list. */
 int psl_grid_cell_census(const psl_grid* grid, int64_t* bulk_cells, int64_t* boundary_cells, int64_t* skipped_cells);
 

@@ -199,6 +199,7 @@ _SIGS = [
     ("psl_grid_get_closure_y", _I, [_P, _P]),
     ("psl_get_macroscopic_device", _I, [_P, _P]),
     ("psl_grid_time_update_kernel", _I, [_P, _I, _P, _P]),
+    ("psl_grid_time_update_kernel_streaming", _I, [_P, _I, _P, _P]),
     ("psl_probe_ps_pattern", _I, [_I, _I, _I, _I, _I, _P]),
     ("psl_probe_ps_pattern_capped", _I, [_I, _I, _I, _I, _I, _I, _P]),
     ("psl_probe_ps_access", _I, [_I, _I, _I, _I, _I, _I, _I, _I, _I, _P]),
@@ -445,11 +446,14 @@ class GridStruct:
         """(cells with a non-finite population or rho <= 0, min rho, max rho of the others)."""
         return _health(lib().psl_grid_check_health, self._h)
 
-    def time_update_kernel(self, launches: int = 50):
-        """(mean ms, kernel name) of the dominant update kernel launched back to back."""
+    def time_update_kernel(self, launches: int = 50, streaming: bool = False):
+        """(mean ms, kernel name) of the dominant update kernel launched back to
+        back; with `streaming`, the shifter advances before every launch, so the
+        launches see the shifts of consecutive steps as in a run."""
         ms = ctypes.c_double()
         k = ctypes.c_int()
-        _check(lib().psl_grid_time_update_kernel(self._h, int(launches), ctypes.byref(ms), ctypes.byref(k)))
+        fn = lib().psl_grid_time_update_kernel_streaming if streaming else lib().psl_grid_time_update_kernel
+        _check(fn(self._h, int(launches), ctypes.byref(ms), ctypes.byref(k)))
         return ms.value, ("k_ps_bulk", "k_ps_fused", "k_ps_bulk_closure")[k.value]
 
     def set_closure(self, mlp):

@@ -390,3 +390,25 @@ def test_capped_pattern_probe(psl):
         assert np.isfinite(gbs) and gbs > 0
     with pytest.raises(RuntimeError, match="workgroups per CU"):
         P.probe_ps_pattern(64, 64, 64, "f16", launches=5, occupancy=7)
+
+
+def test_kernel_timer_with_streaming_advances_the_shifter(psl):
+    """psl_grid_time_update_kernel_streaming: a positive duration per launch,
+    the shifter moved on by one streaming step per launch (the warm-up launch
+    included none), and the fixed-shift timer leaves it where it was."""
+    P = psl
+    d = cases.batch_domains(count=1, seed=5, l_min=24, l_max=24)[0]
+    g, _ = cases.domain_grid(d, "f32")
+    ref, _ = cases.domain_grid(d, "f32")
+    s0 = g.shifter.tolist()
+    ms, kernel = g.time_update_kernel(5)
+    assert ms > 0 and kernel in ("k_ps_bulk", "k_ps_fused")
+    assert g.shifter.tolist() == s0
+    ms, _ = g.time_update_kernel(7, streaming=True)
+    for _ in range(7):
+        P.applyStreaming(ref)
+    print(f"[kernel timer] 24-cell domain, streaming: {ms * 1e3:.1f} us per launch", flush=True)
+    assert ms > 0
+    assert g.shifter.tolist() == ref.shifter.tolist() != s0
+    g.close()
+    ref.close()
